@@ -1,0 +1,32 @@
+"""clayhip -- MI355X (gfx950) fluid-to-field path of clay-engine, host side.
+
+A Python mirror of the reference's TypeScript surface for this path
+(FluidSim, SplatField, ParticleCloud, SdfVolume, TracedField and the field
+combinators), calling hand-written HIP kernels through the C ABI in
+include/clay_hip.h.  There is no CPU fallback: every call goes to the GPU.
+"""
+from ._lib import ClayError, LIB_PATH, lib
+from .brush import compile_brushes, make_brush
+from .field import TracedField, analyticField, unionField, volumeField
+from .fluid import FluidSim, fluid_desc
+from .splat import ParticleCloud, SplatField
+from .volume import Root, SdfVolume, derive
+
+__all__ = [
+    "ClayError",
+    "FluidSim",
+    "LIB_PATH",
+    "ParticleCloud",
+    "Root",
+    "SdfVolume",
+    "SplatField",
+    "TracedField",
+    "analyticField",
+    "compile_brushes",
+    "derive",
+    "fluid_desc",
+    "lib",
+    "make_brush",
+    "unionField",
+    "volumeField",
+]

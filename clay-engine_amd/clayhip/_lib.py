@@ -1,0 +1,287 @@
+"""ctypes binding of libclay_hip.so (include/clay_hip.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the in-tree build is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("CLAY_HIP_LIB", PKG_DIR / "libclay_hip.so"))
+
+MAX_MIPS = 16
+MAX_LEAVES = 4
+CONTACTS = 32
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_NODEVICE = -3
+ERR_STATE = -4
+
+LIVE_ALL, LIVE_PREFIX, LIVE_RANGE = 0, 1, 2
+SPLAT_PLAIN, SPLAT_TILED = 0, 1
+STATE_POS, STATE_VEL, STATE_DENSITY, STATE_PRESSURE, STATE_NVEL = range(5)
+DEBUG_KEYS, DEBUG_PERM = 0, 1
+
+
+class ClayError(RuntimeError):
+    """Raised with the library's message when a clay_* call fails."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Brush(C.Structure):
+    """`Brush` (lib/src/field/brush.ts:53-80), 80 bytes."""
+
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("scale", C.c_float),
+        ("rot", C.c_float * 4),
+        ("size", C.c_float * 3),
+        ("radius", C.c_float),
+        ("material", C.c_float),
+        ("blend", C.c_float),
+        ("bound", C.c_float),
+        ("mask", C.c_float),
+        ("kind", C.c_uint32),
+        ("op", C.c_uint32),
+        ("slot", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+class VolumeDesc(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_int32),
+        ("mip_levels", C.c_int32),
+        ("world_size", C.c_double),
+        ("band", C.c_double),
+        ("origin", C.c_double * 3),
+        ("gradient_voxels", C.c_double),
+    ]
+
+
+class VolumeInfo(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_int32),
+        ("mip_levels", C.c_int32),
+        ("tile_res", C.c_int32 * MAX_MIPS),
+        ("cell_offsets", C.c_int32 * MAX_MIPS),
+        ("total_cells", C.c_int32),
+        ("pad", C.c_int32),
+        ("world_size", C.c_double),
+        ("voxel_size", C.c_double),
+        ("band", C.c_double),
+        ("origin", C.c_double * 3),
+    ]
+
+
+class AnalyticOpts(C.Structure):
+    _fields_ = [("band", C.c_double), ("epsilon", C.c_double), ("material", C.c_double)]
+
+
+class CloudDesc(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32),
+        ("live_rule", C.c_int32),
+        ("live_first", C.c_uint32),
+        ("live_count", C.c_uint32),
+        ("pos", C.c_void_p * 3),
+        ("nrm", C.c_void_p * 3),
+    ]
+
+
+class SplatDesc(C.Structure):
+    _fields_ = [
+        ("radius", C.c_double),
+        ("material", C.c_double),
+        ("resolution", C.c_int32),
+        ("mip_levels", C.c_int32),
+        ("band", C.c_double),
+        ("origin", C.c_double * 3),
+        ("world_size", C.c_double),
+        ("mode", C.c_int32),
+        ("keep_codes", C.c_int32),
+    ]
+
+
+class FluidDesc(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32),
+        ("bucket", C.c_int32),
+        ("spacing", C.c_double),
+        ("smoothing", C.c_double),
+        ("density", C.c_double),
+        ("radius", C.c_double),
+        ("surface_radius", C.c_double),
+        ("material", C.c_double),
+        ("bake_resolution", C.c_int32),
+        ("bake_mip_levels", C.c_int32),
+        ("bake_band", C.c_double),
+        ("origin", C.c_double * 3),
+        ("world_size", C.c_double),
+        ("substeps", C.c_int32),
+        ("splat_mode", C.c_int32),
+        ("gravity", C.c_double * 3),
+        ("stiffness", C.c_double),
+        ("viscosity", C.c_double),
+        ("damping", C.c_double),
+        ("max_speed", C.c_double),
+    ]
+
+
+class FluidParams(C.Structure):
+    _fields_ = [
+        ("gravity", C.c_double * 3),
+        ("dt", C.c_double),
+        ("emit_pos", C.c_double * 3),
+        ("emit_radius", C.c_double),
+        ("emit_vel", C.c_double * 3),
+        ("max_speed", C.c_double),
+        ("stiffness", C.c_double),
+        ("viscosity", C.c_double),
+        ("damping", C.c_double),
+        ("contact_depth", C.c_double),
+    ]
+
+
+class FluidConsts(C.Structure):
+    _fields_ = [
+        ("h", C.c_double),
+        ("mass", C.c_double),
+        ("poly6", C.c_double),
+        ("spiky_grad", C.c_double),
+        ("rest_density", C.c_double),
+        ("dt", C.c_double),
+        ("radius", C.c_double),
+        ("contact_depth", C.c_double),
+        ("grid_origin", C.c_double * 3),
+        ("grid_res", C.c_int32),
+        ("bucket", C.c_int32),
+        ("substeps", C.c_int32),
+        ("pad", C.c_int32),
+        ("wall_lo", C.c_double * 3),
+        ("wall_hi", C.c_double * 3),
+    ]
+
+
+STRUCTS = {
+    "clay_brush": Brush,
+    "clay_volume_desc": VolumeDesc,
+    "clay_volume_info": VolumeInfo,
+    "clay_analytic_opts": AnalyticOpts,
+    "clay_cloud_desc": CloudDesc,
+    "clay_splat_desc": SplatDesc,
+    "clay_fluid_desc": FluidDesc,
+    "clay_fluid_params": FluidParams,
+    "clay_fluid_consts": FluidConsts,
+}
+
+P = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+I32 = C.c_int32
+U32 = C.c_uint32
+DP = C.POINTER(C.c_double)
+
+SIGNATURES = {
+    "clay_last_error": ([], C.c_char_p),
+    "clay_version": ([], C.c_int),
+    "clay_ctx_create": ([C.c_int, PP], C.c_int),
+    "clay_ctx_destroy": ([P], C.c_int),
+    "clay_ctx_synchronize": ([P], C.c_int),
+    "clay_ctx_set_timing": ([P, C.c_int], C.c_int),
+    "clay_ctx_kernel_stats": ([P, C.c_int, C.c_char_p, C.c_int, DP, C.POINTER(C.c_uint64)], C.c_int),
+    "clay_ctx_reset_stats": ([P], C.c_int),
+    "clay_volume_derive": ([I32, I32, C.POINTER(VolumeInfo)], C.c_int),
+    "clay_volume_create": ([P, C.POINTER(VolumeDesc), PP], C.c_int),
+    "clay_volume_destroy": ([P], C.c_int),
+    "clay_volume_get_info": ([P, C.POINTER(VolumeInfo)], C.c_int),
+    "clay_volume_set_origin": ([P, DP, P], C.c_int),
+    "clay_volume_set_origin_device": ([P, P, P], C.c_int),
+    "clay_volume_voxel_size_at": ([P, I32, DP], C.c_int),
+    "clay_volume_download": ([P, I32, P], C.c_int),
+    "clay_volume_upload": ([P, I32, P], C.c_int),
+    "clay_volume_bake_brushes": ([P, C.POINTER(Brush), I32, P], C.c_int),
+    "clay_field_analytic": ([P, C.POINTER(Brush), I32, C.POINTER(AnalyticOpts), PP], C.c_int),
+    "clay_field_volume": ([P, P, PP], C.c_int),
+    "clay_field_union": ([P, P, PP], C.c_int),
+    "clay_field_destroy": ([P], C.c_int),
+    "clay_field_max_mip": ([P, C.POINTER(I32)], C.c_int),
+    "clay_field_eval": ([P, P, I32, P, P], C.c_int),
+    "clay_splat_create": ([P, C.POINTER(CloudDesc), C.POINTER(SplatDesc), PP], C.c_int),
+    "clay_splat_destroy": ([P], C.c_int),
+    "clay_splat_set_live": ([P, U32, U32], C.c_int),
+    "clay_splat_bake": ([P, P], C.c_int),
+    "clay_splat_set_center": ([P, DP, P], C.c_int),
+    "clay_splat_volume": ([P, PP], C.c_int),
+    "clay_splat_field": ([P, PP], C.c_int),
+    "clay_splat_grid_download": ([P, I32, P], C.c_int),
+    "clay_splat_tiles_download": ([P, I32, P], C.c_int),
+    "clay_fluid_desc_default": ([C.POINTER(FluidDesc)], None),
+    "clay_fluid_create": ([P, P, C.POINTER(FluidDesc), PP], C.c_int),
+    "clay_fluid_destroy": ([P], C.c_int),
+    "clay_fluid_constants": ([P, C.POINTER(FluidConsts)], C.c_int),
+    "clay_fluid_set_state": ([P, P, P, U32], C.c_int),
+    "clay_fluid_get_state": ([P, I32, P], C.c_int),
+    "clay_fluid_spawn": ([P, U32, DP, DP, C.c_double], C.c_int),
+    "clay_fluid_configure": ([P, C.POINTER(FluidParams)], C.c_int),
+    "clay_fluid_get_params": ([P, C.POINTER(FluidParams)], C.c_int),
+    "clay_fluid_step": ([P, P], C.c_int),
+    "clay_fluid_substep": ([P, P], C.c_int),
+    "clay_fluid_read_contacts": ([P, P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    "clay_fluid_particle_count": ([P, C.POINTER(U32)], C.c_int),
+    "clay_fluid_surface": ([P, PP], C.c_int),
+    "clay_fluid_debug_download": ([P, I32, P], C.c_int),
+}
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Loads the in-tree library once.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ClayError(
+                ERR_STATE,
+                f"{LIB_PATH} is missing: build it with `python clay-engine_amd/build.py` "
+                "(there is no CPU fallback for the HIP path)",
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().clay_last_error()
+        raise ClayError(rc, msg.decode() if msg else f"clay error {rc}")
+
+
+def stream_handle(stream) -> int | None:
+    """Accepts None, an int hipStream_t, or a torch.cuda.Stream."""
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    handle = getattr(stream, "cuda_stream", None)
+    if handle is None:
+        raise TypeError(f"not a stream: {stream!r}")
+    return int(handle)
+
+
+def dvec(values) -> C.Array:
+    arr = (C.c_double * 3)()
+    for i in range(3):
+        arr[i] = float(values[i])
+    return arr

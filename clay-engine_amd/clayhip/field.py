@@ -1,0 +1,75 @@
+"""TracedField descriptors (lib/src/trace/field.ts).
+
+The reference composes shader closures; the C ABI composes descriptors that the
+integrate kernel interprets: an analytic brush fold, a baked volume, or a union
+of up to four of those (unionField folded left).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .brush import brush_array
+from .volume import Root, SdfVolume
+
+
+class TracedField:
+    def __init__(self, root: Root, handle, owned: bool = True, keep=()):
+        self.root = root
+        self._h = handle
+        self._owned = owned
+        self._keep = tuple(keep)  # operands that must outlive this field
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def maxMip(self) -> int:
+        out = C.c_int32()
+        L.check(L.lib().clay_field_max_mip(self._h, C.byref(out)))
+        return out.value
+
+    def evaluate(self, points) -> tuple[np.ndarray, np.ndarray]:
+        """`(sample(p, 0).x, normal(p))` at host points (n, 3), computed on the GPU."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        n = pts.shape[0]
+        dist = np.empty(n, dtype=np.float32)
+        nrm = np.empty((n, 3), dtype=np.float32)
+        L.check(L.lib().clay_field_eval(self._h, pts.ctypes.data, n, dist.ctypes.data, nrm.ctypes.data))
+        return dist, nrm
+
+    def destroy(self) -> None:
+        if self._owned and self._h is not None:
+            L.lib().clay_field_destroy(self._h)
+        self._h = None
+
+
+def analyticField(root: Root, brushes, options: dict | None = None) -> TracedField:
+    """analyticField (field.ts:196-236) over the fold of `brushes` (BrushDesc dicts
+    or Brush records) from (1e9, 0)."""
+    opts = options or {}
+    arr = brush_array(brushes)
+    o = L.AnalyticOpts()
+    o.band = float(opts.get("band", 64))
+    o.epsilon = float(opts.get("epsilon", 1e-3))
+    o.material = float(opts.get("material", 0))
+    h = C.c_void_p()
+    L.check(L.lib().clay_field_analytic(root.handle, arr, len(arr), C.byref(o), C.byref(h)))
+    return TracedField(root, h)
+
+
+def volumeField(volume: SdfVolume) -> TracedField:
+    """volumeField (field.ts:102-180)."""
+    h = C.c_void_p()
+    L.check(L.lib().clay_field_volume(volume.root.handle, volume.handle, C.byref(h)))
+    return TracedField(volume.root, h, keep=(volume,))
+
+
+def unionField(a: TracedField, b: TracedField) -> TracedField:
+    """unionField (field.ts:246-315)."""
+    h = C.c_void_p()
+    L.check(L.lib().clay_field_union(a.handle, b.handle, C.byref(h)))
+    return TracedField(a.root, h, keep=(a, b))

@@ -1,0 +1,1098 @@
+// api.hip -- the C ABI (include/clay_hip.h): handles, option defaults and the
+// orchestration of FluidSim.step / SplatField.bake on one HIP stream.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "clay_internal.h"
+
+namespace clay {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+// ---- timing ------------------------------------------------------------------
+int Ctx::stat_index(const char* name) {
+  for (size_t i = 0; i < stats.size(); i++)
+    if (stats[i].name == name) return (int)i;
+  KernelStat k;
+  k.name = name;
+  stats.push_back(k);
+  return (int)stats.size() - 1;
+}
+
+hipEvent_t Ctx::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void Ctx::collect() {
+  for (auto& p : pending) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(p.stop) == hipSuccess && hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      stats[p.stat].total_ms += ms;
+      stats[p.stat].launches += 1;
+    }
+    event_pool.push_back(p.start);
+    event_pool.push_back(p.stop);
+  }
+  pending.clear();
+}
+
+LaunchTimer::LaunchTimer(Ctx* ctx, hipStream_t s, const char* name) : ctx_(ctx), s_(s) {
+  if (!ctx_ || !ctx_->timing) return;
+  stat_ = ctx_->stat_index(name);
+  a_ = ctx_->take_event();
+  b_ = ctx_->take_event();
+  if (a_ && b_) (void)hipEventRecord(a_, s_);
+}
+
+LaunchTimer::~LaunchTimer() {
+  if (stat_ < 0 || !a_ || !b_) return;
+  (void)hipEventRecord(b_, s_);
+  ctx_->pending.push_back(PendingTiming{stat_, a_, b_});
+  if (ctx_->pending.size() > 4096) ctx_->collect();
+}
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+  if (e != hipSuccess)
+    return fail(CLAY_ERR_HIP, std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
+                                   " bytes): " + hipGetErrorString(e));
+  return CLAY_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int volume_geometry(int32_t resolution, int32_t mips_req, clay_volume_info* out) {
+  // volume.ts:139-179
+  memset(out, 0, sizeof(*out));
+  if (resolution <= 0) return fail(CLAY_ERR_INVALID, "SdfVolume: resolution must be positive");
+  const int TILE = kTile;
+  int halvings = 0;
+  double n = (double)resolution / TILE;
+  while (n != 0.0 && n == floor(n) && fmod(n, 2.0) == 0.0) {
+    halvings++;
+    n /= 2.0;
+  }
+  int mips = mips_req;
+  if (mips <= 0) {
+    const int li = (int)log2((double)resolution / TILE);
+    mips = std::max(1, std::min(li, halvings + 1));
+  }
+  if (mips > CLAY_MAX_MIPS)
+    return fail(CLAY_ERR_INVALID, "SdfVolume: at most " + std::to_string(CLAY_MAX_MIPS) + " mip levels");
+  const long long need = (long long)TILE << (mips - 1);
+  if (resolution % need != 0) {
+    char buf[160];
+    snprintf(buf, sizeof(buf), "SdfVolume: resolution %d must be a multiple of %lld for %d mip levels",
+             resolution, need, mips);
+    return fail(CLAY_ERR_INVALID, buf);
+  }
+  out->resolution = resolution;
+  out->mip_levels = mips;
+  const int tile_res = resolution / TILE;
+  int off = 0;
+  for (int m = 0; m < mips; m++) {
+    const int tr = tile_res >> m;
+    out->tile_res[m] = tr;
+    out->cell_offsets[m] = off;
+    off += tr * tr * tr;
+  }
+  out->total_cells = off;
+  return CLAY_OK;
+}
+
+void write_voldev(Volume* v) {
+  const double half = v->info.world_size / 2;
+  VolDev& d = v->host;
+  for (int a = 0; a < 3; a++) {
+    d.origin[a] = (float)v->origin[a];
+    d.center[a] = (float)(v->origin[a] + half);
+  }
+  d.voxel = (float)v->info.voxel_size;
+  d.extent = (float)v->info.world_size;
+  d.half = (float)half;
+  d.band = (float)v->info.band;
+  d.grad_h = d.voxel * (float)v->grad_voxels;
+  d.res = v->info.resolution;
+}
+
+__global__ void k_fill_u16(uint16_t* p, size_t n, uint16_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+int fill_u16(Ctx* c, hipStream_t s, uint16_t* p, size_t n, uint16_t v) {
+  if (!n) return CLAY_OK;
+  size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  LaunchTimer t(c, s, "fill_u16");
+  hipLaunchKernelGGL(k_fill_u16, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out) {
+  const int32_t res = d->resolution > 0 ? d->resolution : 128;
+  clay_volume_info info;
+  int rc = volume_geometry(res, d->mip_levels, &info);
+  if (rc) return rc;
+  Volume* v = new (std::nothrow) Volume();
+  if (!v) return fail(CLAY_ERR_INVALID, "out of host memory");
+  v->ctx = ctx;
+  v->info = info;
+  v->info.world_size = d->world_size > 0 ? d->world_size : 16.0;
+  v->info.band = d->band > 0 ? d->band : 4.0;
+  v->info.voxel_size = v->info.world_size / res;
+  for (int a = 0; a < 3; a++) v->origin[a] = v->info.origin[a] = d->origin[a];
+  v->grad_voxels = d->gradient_voxels > 0 ? d->gradient_voxels : 1.0;
+  size_t total = 0;
+  for (int m = 0; m < info.mip_levels; m++) {
+    const size_t r = (size_t)v->mip_res(m);
+    v->voxel_off.push_back(total);
+    total += r * r * r;
+  }
+  v->total_voxels = total;
+  write_voldev(v);
+  if ((rc = dalloc(&v->dev, 1)) || (rc = dalloc(&v->dist, total)) || (rc = dalloc(&v->mat, total))) {
+    dfree(v->dev);
+    dfree(v->dist);
+    dfree(v->mat);
+    delete v;
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  if (hipMemcpyAsync(v->dev, &v->host, sizeof(VolDev), hipMemcpyHostToDevice, s) != hipSuccess ||
+      fill_u16(ctx, s, v->dist, total, f2h_host(1.0f)) || fill_u16(ctx, s, v->mat, total, 0) ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    dfree(v->dev);
+    dfree(v->dist);
+    dfree(v->mat);
+    delete v;
+    return fail(CLAY_ERR_HIP, "SdfVolume: device initialisation failed");
+  }
+  *out = v;
+  return CLAY_OK;
+}
+
+void volume_destroy_impl(Volume* v) {
+  if (!v) return;
+  dfree(v->dev);
+  dfree(v->dist);
+  dfree(v->mat);
+  delete v;
+}
+
+int volume_set_origin_impl(Volume* v, const double o[3], hipStream_t s) {
+  for (int a = 0; a < 3; a++) v->origin[a] = v->info.origin[a] = o[a];
+  write_voldev(v);
+  // The record is small; a host->device copy on the stream keeps it ordered with
+  // the bakes that read it, as the reference's queue-ordered writePartial does.
+  VolDev* staged = new VolDev(v->host);
+  hipError_t e = hipMemcpyAsync(v->dev, staged, sizeof(VolDev), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  delete staged;
+  if (e != hipSuccess) return fail(CLAY_ERR_HIP, std::string("setOrigin: ") + hipGetErrorString(e));
+  return CLAY_OK;
+}
+
+Field* make_volume_field(Ctx* ctx, Volume* v) {
+  Field* f = new (std::nothrow) Field();
+  if (!f) return nullptr;
+  f->ctx = ctx;
+  f->max_mip = v->info.mip_levels - 1;
+  f->col.nleaf = 1;
+  LeafDev& l = f->col.leaf[0];
+  l.kind = 1;
+  l.vol = v->dev;
+  l.dist0 = v->dist;
+  return f;
+}
+
+void fluid_build_k(Fluid* f) {
+  const clay_fluid_consts& c = f->consts;
+  const clay_fluid_params& p = f->params;
+  FluidK& k = f->k;
+  for (int a = 0; a < 3; a++) {
+    k.go[a] = (float)c.grid_origin[a];
+    k.g[a] = (float)p.gravity[a];
+    k.lo[a] = (float)c.wall_lo[a];
+    k.hi[a] = (float)c.wall_hi[a];
+  }
+  k.inv_h = (float)f->inv_h;
+  k.R = c.grid_res;
+  k.bucket = c.bucket;
+  k.hh = (float)f->hh;
+  k.mass_poly6 = (float)f->mass_poly6;
+  k.rest = (float)c.rest_density;
+  k.rest_floor = (float)f->rest_floor;
+  k.h = (float)c.h;
+  k.h6 = (float)f->h6;
+  k.mass = (float)c.mass;
+  k.spiky = (float)c.spiky_grad;
+  k.dt = (float)p.dt;
+  k.vmax = (float)p.max_speed;
+  k.stiff = (float)p.stiffness;
+  k.visc = (float)p.viscosity;
+  k.damp = (float)p.damping;
+  k.cdepth = (float)p.contact_depth;
+  k.radius = (float)c.radius;
+}
+
+}  // namespace
+
+int launch_emit(Ctx* c, hipStream_t s, Fluid* f, uint32_t base, uint32_t count, uint32_t seed);
+
+}  // namespace clay
+
+using namespace clay;
+
+// ============================================================================
+// library / context
+// ============================================================================
+extern "C" const char* clay_last_error(void) { return g_error.c_str(); }
+extern "C" int clay_version(void) { return 1; }
+
+extern "C" int clay_ctx_create(int device, clay_ctx** out) {
+  if (!out) return fail(CLAY_ERR_INVALID, "clay_ctx_create: out is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(CLAY_ERR_NODEVICE, "clay_ctx_create: no HIP device is visible");
+  if (device < 0 || device >= n) return fail(CLAY_ERR_INVALID, "clay_ctx_create: bad device index");
+  CLAY_HIP_CHECK(hipSetDevice(device));
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return fail(CLAY_ERR_INVALID, "out of host memory");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(CLAY_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = reinterpret_cast<clay_ctx*>(c);
+  return CLAY_OK;
+}
+
+extern "C" int clay_ctx_destroy(clay_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return CLAY_OK;
+  (void)hipStreamSynchronize(c->stream);
+  c->collect();
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CLAY_OK;
+}
+
+extern "C" int clay_ctx_synchronize(clay_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(CLAY_ERR_INVALID, "clay_ctx_synchronize: NULL context");
+  CLAY_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return CLAY_OK;
+}
+
+extern "C" int clay_ctx_set_timing(clay_ctx* ctx, int enabled) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(CLAY_ERR_INVALID, "clay_ctx_set_timing: NULL context");
+  c->timing = enabled != 0;
+  return CLAY_OK;
+}
+
+extern "C" int clay_ctx_kernel_stats(clay_ctx* ctx, int index, char* name, int name_len,
+                                     double* total_ms, uint64_t* launches) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(CLAY_ERR_INVALID, "clay_ctx_kernel_stats: NULL context");
+  c->collect();
+  if (index < 0 || index >= (int)c->stats.size()) return fail(CLAY_ERR_INVALID, "no such kernel stat");
+  const KernelStat& k = c->stats[index];
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", k.name.c_str());
+  if (total_ms) *total_ms = k.total_ms;
+  if (launches) *launches = k.launches;
+  return CLAY_OK;
+}
+
+extern "C" int clay_ctx_reset_stats(clay_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return fail(CLAY_ERR_INVALID, "clay_ctx_reset_stats: NULL context");
+  c->collect();
+  c->stats.clear();
+  return CLAY_OK;
+}
+
+// ============================================================================
+// SdfVolume
+// ============================================================================
+extern "C" int clay_volume_derive(int32_t resolution, int32_t mip_levels, clay_volume_info* out) {
+  if (!out) return fail(CLAY_ERR_INVALID, "clay_volume_derive: out is NULL");
+  return volume_geometry(resolution, mip_levels, out);
+}
+
+extern "C" int clay_volume_create(clay_ctx* ctx, const clay_volume_desc* desc, clay_volume** out) {
+  if (!ctx || !desc || !out) return fail(CLAY_ERR_INVALID, "clay_volume_create: NULL argument");
+  Volume* v = nullptr;
+  int rc = volume_create_impl(reinterpret_cast<Ctx*>(ctx), desc, &v);
+  if (rc) return rc;
+  *out = reinterpret_cast<clay_volume*>(v);
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_destroy(clay_volume* vol) {
+  volume_destroy_impl(reinterpret_cast<Volume*>(vol));
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_get_info(const clay_volume* vol, clay_volume_info* out) {
+  if (!vol || !out) return fail(CLAY_ERR_INVALID, "clay_volume_get_info: NULL argument");
+  *out = reinterpret_cast<const Volume*>(vol)->info;
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_set_origin(clay_volume* vol, const double origin[3], clay_stream stream) {
+  if (!vol || !origin) return fail(CLAY_ERR_INVALID, "clay_volume_set_origin: NULL argument");
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  return volume_set_origin_impl(v, origin, pick(v->ctx, stream));
+}
+
+extern "C" int clay_volume_set_origin_device(clay_volume* vol, const float* origin3, clay_stream stream) {
+  if (!vol || !origin3) return fail(CLAY_ERR_INVALID, "clay_volume_set_origin_device: NULL argument");
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  return launch_set_origin_dev(v->ctx, pick(v->ctx, stream), v->dev, origin3, v->host.half);
+}
+
+extern "C" int clay_volume_voxel_size_at(const clay_volume* vol, int32_t mip, double* out) {
+  if (!vol || !out) return fail(CLAY_ERR_INVALID, "clay_volume_voxel_size_at: NULL argument");
+  const Volume* v = reinterpret_cast<const Volume*>(vol);
+  *out = v->info.voxel_size * pow(2.0, mip);  // volume.ts:336-338
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_download(clay_volume* vol, int32_t mip, uint16_t* rgba) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || !rgba) return fail(CLAY_ERR_INVALID, "clay_volume_download: NULL argument");
+  if (mip < 0 || mip >= v->info.mip_levels) return fail(CLAY_ERR_INVALID, "clay_volume_download: bad mip");
+  const size_t r = (size_t)v->mip_res(mip), n = r * r * r;
+  std::vector<uint16_t> d(n), m(n);
+  CLAY_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(d.data(), v->dist + v->voxel_off[mip], n * 2, hipMemcpyDeviceToHost));
+  CLAY_HIP_CHECK(hipMemcpy(m.data(), v->mat + v->voxel_off[mip], n * 2, hipMemcpyDeviceToHost));
+  const uint16_t one = f2h_host(1.0f);
+  for (size_t i = 0; i < n; i++) {
+    rgba[4 * i] = d[i];
+    rgba[4 * i + 1] = m[i];
+    rgba[4 * i + 2] = 0;
+    rgba[4 * i + 3] = one;
+  }
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_upload(clay_volume* vol, int32_t mip, const uint16_t* rgba) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || !rgba) return fail(CLAY_ERR_INVALID, "clay_volume_upload: NULL argument");
+  if (mip < 0 || mip >= v->info.mip_levels) return fail(CLAY_ERR_INVALID, "clay_volume_upload: bad mip");
+  const size_t r = (size_t)v->mip_res(mip), n = r * r * r;
+  std::vector<uint16_t> d(n), m(n);
+  for (size_t i = 0; i < n; i++) {
+    d[i] = rgba[4 * i];
+    m[i] = rgba[4 * i + 1];
+  }
+  CLAY_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(v->dist + v->voxel_off[mip], d.data(), n * 2, hipMemcpyHostToDevice));
+  CLAY_HIP_CHECK(hipMemcpy(v->mat + v->voxel_off[mip], m.data(), n * 2, hipMemcpyHostToDevice));
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_bake_brushes(clay_volume* vol, const clay_brush* brushes, int32_t n,
+                                        clay_stream stream) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || (n > 0 && !brushes)) return fail(CLAY_ERR_INVALID, "clay_volume_bake_brushes: NULL argument");
+  if (n < 0) return fail(CLAY_ERR_INVALID, "clay_volume_bake_brushes: negative brush count");
+  for (int i = 0; i < n; i++)
+    if (brushes[i].kind > 6)
+      return fail(CLAY_ERR_INVALID, "clay_volume_bake_brushes: brush kind " + std::to_string(brushes[i].kind) +
+                                        " is outside the supported set (sphere, box, roundBox, capsule, torus, cylinder, plane)");
+  hipStream_t s = pick(v->ctx, stream);
+  clay_brush* dev = nullptr;
+  int rc = dalloc(&dev, (size_t)std::max(n, 1));
+  if (rc) return rc;
+  if (n > 0) CLAY_HIP_CHECK(hipMemcpyAsync(dev, brushes, sizeof(clay_brush) * n, hipMemcpyHostToDevice, s));
+  rc = launch_bake_brushes(v->ctx, s, v, dev, n);
+  (void)hipStreamSynchronize(s);
+  dfree(dev);
+  return rc;
+}
+
+// ============================================================================
+// TracedField
+// ============================================================================
+extern "C" int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int32_t n,
+                                   const clay_analytic_opts* opts, clay_field** out) {
+  if (!ctx || !out || (n > 0 && !brushes)) return fail(CLAY_ERR_INVALID, "clay_field_analytic: NULL argument");
+  if (n <= 0) return fail(CLAY_ERR_INVALID, "analyticField: the brush list is empty");
+  for (int i = 0; i < n; i++)
+    if (brushes[i].kind > 6)
+      return fail(CLAY_ERR_INVALID, "analyticField: brush kind " + std::to_string(brushes[i].kind) +
+                                        " is outside the supported set (sphere, box, roundBox, capsule, torus, cylinder, plane)");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  Field* f = new (std::nothrow) Field();
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  f->ctx = c;
+  int rc = dalloc(&f->brushes_dev, (size_t)n);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  if (hipMemcpy(f->brushes_dev, brushes, sizeof(clay_brush) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    dfree(f->brushes_dev);
+    delete f;
+    return fail(CLAY_ERR_HIP, "analyticField: brush upload failed");
+  }
+  f->max_mip = 0;
+  f->col.nleaf = 1;
+  LeafDev& l = f->col.leaf[0];
+  l.kind = 0;
+  l.nbrush = n;
+  l.brushes = f->brushes_dev;
+  l.eps = (float)(opts && opts->epsilon > 0 ? opts->epsilon : 1e-3);
+  *out = reinterpret_cast<clay_field*>(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_volume(clay_ctx* ctx, clay_volume* vol, clay_field** out) {
+  if (!ctx || !vol || !out) return fail(CLAY_ERR_INVALID, "clay_field_volume: NULL argument");
+  Field* f = make_volume_field(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<Volume*>(vol));
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  *out = reinterpret_cast<clay_field*>(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_union(clay_field* a, clay_field* b, clay_field** out) {
+  Field* fa = reinterpret_cast<Field*>(a);
+  Field* fb = reinterpret_cast<Field*>(b);
+  if (!fa || !fb || !out) return fail(CLAY_ERR_INVALID, "clay_field_union: NULL argument");
+  if (fa->col.nleaf + fb->col.nleaf > CLAY_MAX_LEAVES)
+    return fail(CLAY_ERR_INVALID, "unionField: more than " + std::to_string(CLAY_MAX_LEAVES) + " leaves");
+  Field* f = new (std::nothrow) Field();
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  f->ctx = fa->ctx;
+  f->max_mip = std::max(fa->max_mip, fb->max_mip);
+  f->col.nleaf = 0;
+  for (int i = 0; i < fa->col.nleaf; i++) f->col.leaf[f->col.nleaf++] = fa->col.leaf[i];
+  for (int i = 0; i < fb->col.nleaf; i++) f->col.leaf[f->col.nleaf++] = fb->col.leaf[i];
+  f->children = {fa, fb};
+  *out = reinterpret_cast<clay_field*>(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_destroy(clay_field* fh) {
+  Field* f = reinterpret_cast<Field*>(fh);
+  if (!f) return CLAY_OK;
+  dfree(f->brushes_dev);
+  delete f;
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_max_mip(const clay_field* fh, int32_t* out) {
+  if (!fh || !out) return fail(CLAY_ERR_INVALID, "clay_field_max_mip: NULL argument");
+  *out = reinterpret_cast<const Field*>(fh)->max_mip;
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_eval(clay_field* fh, const float* pts, int32_t n, float* dist, float* nrm) {
+  Field* f = reinterpret_cast<Field*>(fh);
+  if (!f || (n > 0 && (!pts || !dist || !nrm))) return fail(CLAY_ERR_INVALID, "clay_field_eval: NULL argument");
+  if (n <= 0) return CLAY_OK;
+  float *dp = nullptr, *dd = nullptr, *dn = nullptr;
+  int rc;
+  if ((rc = dalloc(&dp, (size_t)n * 3)) || (rc = dalloc(&dd, (size_t)n)) || (rc = dalloc(&dn, (size_t)n * 3))) {
+    dfree(dp);
+    dfree(dd);
+    dfree(dn);
+    return rc;
+  }
+  hipStream_t s = f->ctx->stream;
+  hipError_t e = hipMemcpyAsync(dp, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) rc = launch_field_eval(f->ctx, s, f->col, n, dp, dd, dn);
+  if (e == hipSuccess && !rc) e = hipMemcpyAsync(dist, dd, sizeof(float) * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && !rc) e = hipMemcpyAsync(nrm, dn, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(dp);
+  dfree(dd);
+  dfree(dn);
+  if (e != hipSuccess) return fail(CLAY_ERR_HIP, std::string("clay_field_eval: ") + hipGetErrorString(e));
+  return rc;
+}
+
+// ============================================================================
+// SplatField
+// ============================================================================
+namespace {
+
+void splat_free(Splat* sp) {
+  if (!sp) return;
+  if (sp->field) delete sp->field;
+  volume_destroy_impl(sp->vol);
+  dfree(sp->grid);
+  dfree(sp->brick_count);
+  dfree(sp->brick_start);
+  dfree(sp->brick_cursor);
+  dfree(sp->brick_list);
+  dfree(sp->brick_total_dev);
+  dfree(sp->tile_live);
+  delete sp;
+}
+
+int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, Splat** out) {
+  if (!(desc->radius > 0)) return fail(CLAY_ERR_INVALID, "SplatField: radius must be positive");
+  if (desc->mode != CLAY_SPLAT_PLAIN && desc->mode != CLAY_SPLAT_TILED)
+    return fail(CLAY_ERR_INVALID, "SplatField: unknown splat mode");
+  if (!cloud->pos[0] || !cloud->pos[1] || !cloud->pos[2])
+    return fail(CLAY_ERR_INVALID, "SplatField: the cloud has no position arrays");
+  const bool n0 = cloud->nrm[0] != nullptr;
+  if ((cloud->nrm[1] != nullptr) != n0 || (cloud->nrm[2] != nullptr) != n0)
+    return fail(CLAY_ERR_INVALID, "SplatField: normals must be given for all three axes or none");
+  Splat* sp = new (std::nothrow) Splat();
+  if (!sp) return fail(CLAY_ERR_INVALID, "out of host memory");
+  sp->ctx = ctx;
+  sp->cloud = *cloud;
+  sp->desc = *desc;
+  // splat.ts:148-153
+  const int res = desc->resolution > 0 ? desc->resolution : 64;
+  const int mips = desc->mip_levels > 0 ? desc->mip_levels : std::max(1, (int)log2((double)res / kTile));
+  sp->desc.resolution = res;
+  sp->desc.mip_levels = mips;
+  sp->desc.band = desc->band > 0 ? desc->band : 2.0;
+  sp->desc.world_size = desc->world_size > 0 ? desc->world_size : 24.0;
+  clay_volume_desc vd{};
+  vd.resolution = res;
+  vd.mip_levels = mips;
+  vd.world_size = sp->desc.world_size;
+  vd.band = sp->desc.band;
+  for (int a = 0; a < 3; a++) vd.origin[a] = desc->origin[a];
+  int rc = volume_create_impl(ctx, &vd, &sp->vol);
+  if (rc) {
+    delete sp;
+    return rc;
+  }
+  sp->mips = sp->vol->info.mip_levels;
+  sp->field = make_volume_field(ctx, sp->vol);
+  size_t total = 0;
+  for (int m = 0; m < sp->mips; m++) {
+    const size_t r = (size_t)sp->vol->mip_res(m);
+    sp->grid_off.push_back(total);
+    total += r * r * r;
+  }
+  sp->grid_total = total;
+  hipStream_t s = ctx->stream;
+  if (desc->mode == CLAY_SPLAT_PLAIN || desc->keep_codes) {
+    if ((rc = dalloc(&sp->grid, total)) || (rc = launch_fill_u32(ctx, s, sp->grid, total, kEncSat))) {
+      splat_free(sp);
+      return rc;
+    }
+  }
+  if (desc->mode == CLAY_SPLAT_TILED) {
+    splat_brick_geometry(res, sp->mips, &sp->brick_res, &sp->brick_off, &sp->brick_total, &sp->tile_off,
+                         &sp->tile_total);
+    const size_t per = splat_max_bricks_per_particle(sp->desc.radius, sp->desc.band,
+                                                     sp->vol->info.voxel_size, sp->mips);
+    sp->brick_list_cap = (size_t)std::max<uint32_t>(cloud->capacity, 1) * per;
+    if ((rc = dalloc(&sp->brick_count, sp->brick_total)) || (rc = dalloc(&sp->brick_start, sp->brick_total)) ||
+        (rc = dalloc(&sp->brick_cursor, sp->brick_total)) || (rc = dalloc(&sp->brick_list, sp->brick_list_cap)) ||
+        (rc = dalloc(&sp->brick_total_dev, 1)) || (rc = dalloc(&sp->tile_live, sp->tile_total))) {
+      splat_free(sp);
+      return rc;
+    }
+    if (hipMemsetAsync(sp->tile_live, 0, sp->tile_total, s) != hipSuccess) {
+      splat_free(sp);
+      return fail(CLAY_ERR_HIP, "SplatField: tile mask initialisation failed");
+    }
+  }
+  // The material channel is constant for a splat (splat.ts:273): written once.
+  if ((rc = fill_u16(ctx, s, sp->vol->mat, sp->vol->total_voxels, f2h_host((float)desc->material)))) {
+    splat_free(sp);
+    return rc;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    splat_free(sp);
+    return fail(CLAY_ERR_HIP, "SplatField: initialisation failed");
+  }
+  *out = sp;
+  return CLAY_OK;
+}
+
+}  // namespace
+
+extern "C" int clay_splat_create(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                                 clay_splat** out) {
+  if (!ctx || !cloud || !desc || !out) return fail(CLAY_ERR_INVALID, "clay_splat_create: NULL argument");
+  Splat* sp = nullptr;
+  int rc = splat_create_impl(reinterpret_cast<Ctx*>(ctx), cloud, desc, &sp);
+  if (rc) return rc;
+  *out = reinterpret_cast<clay_splat*>(sp);
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_destroy(clay_splat* s) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (sp) (void)hipStreamSynchronize(sp->ctx->stream);
+  splat_free(sp);
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_set_live(clay_splat* s, uint32_t first, uint32_t count) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp) return fail(CLAY_ERR_INVALID, "clay_splat_set_live: NULL splat");
+  sp->cloud.live_first = first;
+  sp->cloud.live_count = count;
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_bake(clay_splat* s, clay_stream stream) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp) return fail(CLAY_ERR_INVALID, "clay_splat_bake: NULL splat");
+  return splat_bake(sp, pick(sp->ctx, stream));
+}
+
+extern "C" int clay_splat_set_center(clay_splat* s, const double c[3], clay_stream stream) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !c) return fail(CLAY_ERR_INVALID, "clay_splat_set_center: NULL argument");
+  const double half = sp->vol->info.world_size / 2;  // splat.ts:284-287
+  const double o[3] = {c[0] - half, c[1] - half, c[2] - half};
+  return volume_set_origin_impl(sp->vol, o, pick(sp->ctx, stream));
+}
+
+extern "C" int clay_splat_volume(clay_splat* s, clay_volume** out) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !out) return fail(CLAY_ERR_INVALID, "clay_splat_volume: NULL argument");
+  *out = reinterpret_cast<clay_volume*>(sp->vol);
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_field(clay_splat* s, clay_field** out) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !out) return fail(CLAY_ERR_INVALID, "clay_splat_field: NULL argument");
+  *out = reinterpret_cast<clay_field*>(sp->field);
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_grid_download(clay_splat* s, int32_t mip, uint32_t* host) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !host) return fail(CLAY_ERR_INVALID, "clay_splat_grid_download: NULL argument");
+  if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_splat_grid_download: bad mip");
+  if (!sp->grid) return fail(CLAY_ERR_STATE, "clay_splat_grid_download: tiled splat without keep_codes has no code grid");
+  const size_t r = (size_t)sp->vol->mip_res(mip), n = r * r * r;
+  CLAY_HIP_CHECK(hipStreamSynchronize(sp->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(host, sp->grid + sp->grid_off[mip], n * 4, hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* host) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !host) return fail(CLAY_ERR_INVALID, "clay_splat_tiles_download: NULL argument");
+  if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_splat_tiles_download: bad mip");
+  const int r = sp->vol->mip_res(mip), tr = r / kTile;
+  const size_t nt = (size_t)tr * tr * tr;
+  CLAY_HIP_CHECK(hipStreamSynchronize(sp->ctx->stream));
+  if (sp->desc.mode == CLAY_SPLAT_TILED) {
+    CLAY_HIP_CHECK(hipMemcpy(host, sp->tile_live + sp->tile_off[mip], nt, hipMemcpyDeviceToHost));
+    return CLAY_OK;
+  }
+  // Plain mode: derive the exact mask from the dense code grid.
+  std::vector<uint32_t> g((size_t)r * r * r);
+  CLAY_HIP_CHECK(hipMemcpy(g.data(), sp->grid + sp->grid_off[mip], g.size() * 4, hipMemcpyDeviceToHost));
+  memset(host, 0, nt);
+  for (int z = 0; z < r; z++)
+    for (int y = 0; y < r; y++)
+      for (int x = 0; x < r; x++)
+        if (g[(size_t)x + (size_t)r * ((size_t)y + (size_t)r * z)] != kEncSat)
+          host[(size_t)(x / kTile) + (size_t)tr * ((size_t)(y / kTile) + (size_t)tr * (z / kTile))] = 1;
+  return CLAY_OK;
+}
+
+// ============================================================================
+// FluidSim
+// ============================================================================
+extern "C" void clay_fluid_desc_default(clay_fluid_desc* d) {
+  if (!d) return;
+  memset(d, 0, sizeof(*d));
+  d->capacity = 8192;
+  d->bucket = 64;
+  d->spacing = 0.45;
+  d->smoothing = -1;
+  d->density = 1.0;
+  d->radius = -1;
+  d->surface_radius = -1;
+  d->material = 0;
+  d->bake_resolution = 64;
+  d->bake_mip_levels = 0;
+  d->bake_band = 2;
+  d->origin[0] = -12;
+  d->origin[1] = -4;
+  d->origin[2] = -12;
+  d->world_size = 24;
+  d->substeps = 3;
+  d->splat_mode = CLAY_SPLAT_PLAIN;
+  d->gravity[0] = 0;
+  d->gravity[1] = -9.81;
+  d->gravity[2] = 0;
+  d->stiffness = 220;
+  d->viscosity = 0.4;
+  d->damping = 0.995;
+  d->max_speed = 15;
+}
+
+namespace {
+
+void fluid_free(Fluid* f) {
+  if (!f) return;
+  splat_free(f->surface);
+  dfree(f->st);
+  dfree(f->ss);
+  dfree(f->keys);
+  dfree(f->vals);
+  dfree(f->keys2);
+  dfree(f->vals2);
+  dfree(f->hist);
+  dfree(f->cell_start);
+  dfree(f->cell_end);
+  dfree(f->contacts);
+  delete f;
+}
+
+int fluid_substep_impl(Fluid* f, hipStream_t s) {
+  Ctx* c = f->ctx;
+  const uint32_t n = f->live;
+  const uint32_t invalid = (uint32_t)f->grid_cells;
+  int rc;
+  // clearGrid's contacts reset (fluid.ts:302-304) rides on the key pass.
+  if ((rc = launch_fill_u32(c, s, f->contacts, 1, 0u))) return rc;
+  if (n == 0) return CLAY_OK;
+  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->keys, f->vals, invalid))) return rc;
+  if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
+  const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
+  float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
+  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, invalid, f->cell_start, f->cell_end, src, dst, 6)))
+    return rc;
+  if ((rc = launch_density(c, s, f->k, n, f->spx, f->spy, f->spz, f->keys, f->cell_start, f->cell_end,
+                           f->srho, f->spres)))
+    return rc;
+  const float* sorted[8] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz, f->srho, f->spres};
+  float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
+  ColliderDev none{};
+  const ColliderDev& col = f->world ? f->world->col : none;
+  return launch_forces_integrate(c, s, f->k, col, n, sorted, f->keys, f->vals, f->cell_start, f->cell_end,
+                                 out, f->contacts);
+}
+
+}  // namespace
+
+extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fluid_desc* d, clay_fluid** out) {
+  if (!ctx || !d || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_create: NULL argument");
+  if (d->capacity == 0) return fail(CLAY_ERR_INVALID, "FluidSim: capacity must be positive");
+  if (!(d->spacing > 0)) return fail(CLAY_ERR_INVALID, "FluidSim: spacing must be positive");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  Fluid* f = new (std::nothrow) Fluid();
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  f->ctx = c;
+  f->world = reinterpret_cast<Field*>(world);
+  f->desc = *d;
+  f->capacity = d->capacity;
+  // fluid.ts:141-203
+  const double spacing = d->spacing;
+  const double h = d->smoothing > 0 ? d->smoothing : spacing * 2.2;
+  const int bucket = d->bucket > 0 ? d->bucket : 64;
+  const double radius = d->radius >= 0 ? d->radius : spacing * 0.5;
+  const double surf_r = d->surface_radius >= 0 ? d->surface_radius : spacing * 1.1;
+  const int bake_res = d->bake_resolution > 0 ? d->bake_resolution : 64;
+  const int bake_mips = d->bake_mip_levels > 0 ? d->bake_mip_levels : std::max(1, (int)log2((double)bake_res / kTile));
+  const double band = d->bake_band > 0 ? d->bake_band : 2.0;
+  const int substeps = d->substeps > 0 ? d->substeps : 3;
+  const double density = d->density > 0 ? d->density : 1.0;
+  clay_fluid_consts& k = f->consts;
+  k.h = h;
+  k.bucket = bucket;
+  k.substeps = substeps;
+  k.radius = radius;
+  k.poly6 = 315.0 / (64.0 * M_PI * pow(h, 9));
+  k.spiky_grad = -45.0 / (M_PI * pow(h, 6));
+  k.mass = density * pow(spacing, 3);
+  double rest = 0;
+  const int reach = (int)ceil(h / spacing);
+  for (int z = -reach; z <= reach; z++)
+    for (int y = -reach; y <= reach; y++)
+      for (int x = -reach; x <= reach; x++) {
+        const double r2 = (double)(x * x + y * y + z * z) * spacing * spacing;
+        if (r2 < h * h) rest += k.mass * k.poly6 * pow(h * h - r2, 3);
+      }
+  k.rest_density = rest;
+  k.dt = 1.0 / 60.0 / substeps;
+  k.contact_depth = radius * 1.5;
+  f->hh = h * h;
+  f->mass_poly6 = k.mass * k.poly6;
+  f->h6 = pow(h, 6);
+  f->inv_h = 1.0 / h;
+  f->rest_floor = rest * 0.1;
+  clay_fluid_params& p = f->params;
+  for (int a = 0; a < 3; a++) {
+    p.gravity[a] = d->gravity[a];
+    p.emit_pos[a] = 0;
+    p.emit_vel[a] = 0;
+  }
+  p.dt = k.dt;
+  p.emit_radius = spacing;
+  p.max_speed = d->max_speed;
+  p.stiffness = d->stiffness;
+  p.viscosity = d->viscosity;
+  p.damping = d->damping;
+  p.contact_depth = k.contact_depth;
+
+  int rc;
+  const uint32_t P = f->capacity;
+  if ((rc = dalloc(&f->st, (size_t)P * 11)) || (rc = dalloc(&f->ss, (size_t)P * 8)) ||
+      (rc = dalloc(&f->keys, P)) || (rc = dalloc(&f->vals, P)) || (rc = dalloc(&f->keys2, P)) ||
+      (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, 4 + 4 * CLAY_CONTACTS))) {
+    fluid_free(f);
+    return rc;
+  }
+  float** sp_[11] = {&f->px, &f->py, &f->pz, &f->vx, &f->vy, &f->vz, &f->rho, &f->pres, &f->nvx, &f->nvy, &f->nvz};
+  for (int i = 0; i < 11; i++) *sp_[i] = f->st + (size_t)i * P;
+  float** ss_[8] = {&f->spx, &f->spy, &f->spz, &f->svx, &f->svy, &f->svz, &f->srho, &f->spres};
+  for (int i = 0; i < 8; i++) *ss_[i] = f->ss + (size_t)i * P;
+  f->hist_cap = radix_hist_size(P);
+  if ((rc = dalloc(&f->hist, f->hist_cap))) {
+    fluid_free(f);
+    return rc;
+  }
+  // The renderable surface (fluid.ts:215-234): a SplatField over the live prefix.
+  clay_cloud_desc cloud{};
+  cloud.capacity = P;
+  cloud.live_rule = CLAY_LIVE_PREFIX;
+  cloud.live_count = 0;
+  cloud.pos[0] = f->px;
+  cloud.pos[1] = f->py;
+  cloud.pos[2] = f->pz;
+  clay_splat_desc sd{};
+  sd.radius = surf_r;
+  sd.material = d->material;
+  sd.resolution = bake_res;
+  sd.mip_levels = bake_mips;
+  sd.band = band;
+  for (int a = 0; a < 3; a++) sd.origin[a] = d->origin[a];
+  sd.world_size = d->world_size > 0 ? d->world_size : 24.0;
+  sd.mode = d->splat_mode;
+  sd.keep_codes = 0;
+  if ((rc = splat_create_impl(c, &cloud, &sd, &f->surface))) {
+    fluid_free(f);
+    return rc;
+  }
+  // Hash grid (fluid.ts:239-247) and walls (fluid.ts:444-447) over the surface volume.
+  const Volume* vol = f->surface->vol;
+  const double ws = vol->info.world_size;
+  k.grid_res = (int)ceil(ws / h) + 2;
+  for (int a = 0; a < 3; a++) {
+    k.grid_origin[a] = vol->origin[a] - h;
+    k.wall_lo[a] = vol->origin[a] + radius;
+    k.wall_hi[a] = vol->origin[a] + ws - radius;
+  }
+  const uint64_t G = (uint64_t)k.grid_res * k.grid_res * k.grid_res;
+  if (G >= 0xFFFFFFFFull) {
+    fluid_free(f);
+    return fail(CLAY_ERR_INVALID, "FluidSim: worldSize / smoothing gives a hash grid beyond 2^32 cells");
+  }
+  f->grid_cells = (size_t)G;
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) <= G) bits++;
+  f->key_bits = bits;
+  if ((rc = dalloc(&f->cell_start, G)) || (rc = dalloc(&f->cell_end, G))) {
+    fluid_free(f);
+    return rc;
+  }
+  hipStream_t s = c->stream;
+  if ((rc = launch_fill_u32(c, s, f->cell_start, G, 0xFFFFFFFFu)) || (rc = launch_fill_u32(c, s, f->cell_end, G, 0u)) ||
+      (rc = launch_fill_u32(c, s, reinterpret_cast<uint32_t*>(f->st), (size_t)P * 11, 0u)) ||
+      (rc = launch_fill_u32(c, s, f->contacts, 4 + 4 * CLAY_CONTACTS, 0u))) {
+    fluid_free(f);
+    return rc;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    fluid_free(f);
+    return fail(CLAY_ERR_HIP, "FluidSim: initialisation failed");
+  }
+  fluid_build_k(f);
+  *out = reinterpret_cast<clay_fluid*>(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_destroy(clay_fluid* fh) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (f) (void)hipStreamSynchronize(f->ctx->stream);
+  fluid_free(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_constants(const clay_fluid* fh, clay_fluid_consts* out) {
+  if (!fh || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_constants: NULL argument");
+  *out = reinterpret_cast<const Fluid*>(fh)->consts;
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_set_state(clay_fluid* fh, const float* pos, const float* vel, uint32_t n) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || (n > 0 && !pos)) return fail(CLAY_ERR_INVALID, "clay_fluid_set_state: NULL argument");
+  if (n > f->capacity) return fail(CLAY_ERR_INVALID, "clay_fluid_set_state: more particles than capacity");
+  hipStream_t s = f->ctx->stream;
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  float* dp[3] = {f->px, f->py, f->pz};
+  float* dv[3] = {f->vx, f->vy, f->vz};
+  for (int a = 0; a < 3; a++) {
+    if (n) CLAY_HIP_CHECK(hipMemcpy(dp[a], pos + (size_t)a * n, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (n && vel) CLAY_HIP_CHECK(hipMemcpy(dv[a], vel + (size_t)a * n, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (n && !vel) CLAY_HIP_CHECK(hipMemset(dv[a], 0, sizeof(float) * n));
+  }
+  f->live = n;
+  f->cursor = n % f->capacity;
+  f->surface->cloud.live_count = n;
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_get_state(clay_fluid* fh, int32_t what, float* host) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !host) return fail(CLAY_ERR_INVALID, "clay_fluid_get_state: NULL argument");
+  const uint32_t n = f->live;
+  const float* planes[5][3] = {{f->px, f->py, f->pz},
+                               {f->vx, f->vy, f->vz},
+                               {f->rho, nullptr, nullptr},
+                               {f->pres, nullptr, nullptr},
+                               {f->nvx, f->nvy, f->nvz}};
+  if (what < 0 || what > 4) return fail(CLAY_ERR_INVALID, "clay_fluid_get_state: unknown field");
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  for (int a = 0; a < 3; a++) {
+    if (!planes[what][a] || !n) continue;
+    CLAY_HIP_CHECK(hipMemcpy(host + (size_t)a * n, planes[what][a], sizeof(float) * n, hipMemcpyDeviceToHost));
+  }
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_spawn(clay_fluid* fh, uint32_t n, const double pos[3], const double vel[3], double spread) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !pos || !vel) return fail(CLAY_ERR_INVALID, "clay_fluid_spawn: NULL argument");
+  // fluid.ts:511-525
+  if (spread < 0) spread = f->desc.spacing * cbrt((double)std::max<uint32_t>(n, 1)) * 0.62;
+  f->pending_emit = std::min(n, f->capacity);
+  for (int a = 0; a < 3; a++) {
+    f->params.emit_pos[a] = pos[a];
+    f->params.emit_vel[a] = vel[a];
+  }
+  f->params.emit_radius = spread;
+  f->seed++;
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_configure(clay_fluid* fh, const clay_fluid_params* p) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_fluid_configure: NULL argument");
+  f->params = *p;
+  fluid_build_k(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_get_params(const clay_fluid* fh, clay_fluid_params* out) {
+  if (!fh || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_get_params: NULL argument");
+  *out = reinterpret_cast<const Fluid*>(fh)->params;
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_substep(clay_fluid* fh, clay_stream stream) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_substep: NULL fluid");
+  fluid_build_k(f);
+  return fluid_substep_impl(f, pick(f->ctx, stream));
+}
+
+extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_step: NULL fluid");
+  hipStream_t s = pick(f->ctx, stream);
+  fluid_build_k(f);
+  // fluid.ts:536-558
+  const uint32_t emit_count = f->pending_emit;
+  f->pending_emit = 0;
+  const uint32_t emit_base = f->cursor;
+  if (emit_count > 0) {
+    f->cursor = (emit_base + emit_count) % f->capacity;
+    f->live = std::min<uint32_t>(f->capacity, std::max<uint32_t>(f->live, emit_base + emit_count));
+    int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->seed);
+    if (rc) return rc;
+  }
+  bool sorted = false;
+  for (int i = 0; i < f->consts.substeps && f->live > 0; i++) {
+    int rc = fluid_substep_impl(f, s);
+    if (rc) return rc;
+    sorted = true;
+  }
+  f->surface->cloud.live_count = f->live;
+  // The last substep's permutation is a spatially coherent visiting order for the
+  // splat; the codes do not depend on it.
+  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr);
+}
+
+extern "C" int clay_fluid_read_contacts(clay_fluid* fh, float* out4, uint32_t* n, uint32_t* raw) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_read_contacts: NULL fluid");
+  uint32_t buf[4 + 4 * CLAY_CONTACTS];
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(buf, f->contacts, sizeof(buf), hipMemcpyDeviceToHost));
+  const uint32_t cnt = std::min<uint32_t>(buf[0], CLAY_CONTACTS);
+  if (out4) memcpy(out4, buf + 4, sizeof(float) * 4 * cnt);
+  if (n) *n = cnt;
+  if (raw) *raw = buf[0];
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_particle_count(const clay_fluid* fh, uint32_t* out) {
+  if (!fh || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_particle_count: NULL argument");
+  *out = reinterpret_cast<const Fluid*>(fh)->live;
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_surface(clay_fluid* fh, clay_splat** out) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_surface: NULL argument");
+  *out = reinterpret_cast<clay_splat*>(f->surface);
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_debug_download(clay_fluid* fh, int32_t what, uint32_t* host) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !host) return fail(CLAY_ERR_INVALID, "clay_fluid_debug_download: NULL argument");
+  if (what != CLAY_DEBUG_KEYS && what != CLAY_DEBUG_PERM)
+    return fail(CLAY_ERR_INVALID, "clay_fluid_debug_download: unknown buffer");
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->live)
+    CLAY_HIP_CHECK(hipMemcpy(host, what == CLAY_DEBUG_KEYS ? f->keys : f->vals, sizeof(uint32_t) * f->live,
+                             hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}

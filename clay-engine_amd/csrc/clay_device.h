@@ -1,0 +1,306 @@
+// clay_device.h -- device-side data layout and inline math for the gfx950 kernels.
+//
+// Everything here is compiled with -ffp-contract=off: each f32 operation rounds
+// once, in the order the reference shader text gives, so the plain kernels agree
+// bit for bit with an IEEE-f32 evaluation of the same expressions.  min/max/clamp
+// are written as selects (not v_min/v_max) and float->int conversion saturates
+// explicitly, so signed zeros and out-of-range values behave identically on
+// every path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "clay_hip.h"
+
+namespace clay {
+
+constexpr int kTile = 8;             // voxels per tile edge (volume.ts:6)
+constexpr uint32_t kEncSat = 65535u; // splat.ts:110
+constexpr float kEncScale = 32767.5f;
+
+// ---------------------------------------------------------------------------
+// Device-resident records
+// ---------------------------------------------------------------------------
+
+// SdfVolume's `VolumeParams` uniform (volume.ts:10-23, 220-230), f32 as the shader
+// sees it.  Lives in device memory so setOrigin (host or device side) is picked up
+// by kernels already enqueued or captured.
+struct VolDev {
+  float origin[3];
+  float voxel;      // worldSize / resolution
+  float center[3];
+  float extent;     // worldSize
+  float half;       // worldSize / 2
+  float band;
+  float grad_h;     // voxel * gradientVoxels
+  int32_t res;
+  // host-side f64 origin is kept by the owner; these are what kernels read
+};
+
+// One collider leaf: analytic brush fold or volume (mip 0).
+struct LeafDev {
+  int32_t kind;  // 0 analytic, 1 volume
+  int32_t nbrush;
+  const clay_brush* brushes;
+  float eps;
+  int32_t pad;
+  const VolDev* vol;
+  const uint16_t* dist0;  // mip-0 distance plane (f16 bits), x fastest
+};
+
+// unionField folded left over up to CLAY_MAX_LEAVES leaves (field.ts:246-315).
+struct ColliderDev {
+  int32_t nleaf;
+  int32_t pad;
+  LeafDev leaf[CLAY_MAX_LEAVES];
+};
+
+// FluidSim constants as f32 shader literals, derived in f64 on the host.
+struct FluidK {
+  float go[3];        // grid origin (volume.origin - h)
+  float inv_h;
+  int32_t R;          // gridRes
+  int32_t bucket;
+  float hh, mass_poly6, rest, rest_floor;
+  float h, h6, mass, spiky;
+  float g[3], dt, vmax, stiff, visc, damp, cdepth;
+  float radius, lo[3], hi[3];
+};
+
+// ---------------------------------------------------------------------------
+// Scalar helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fmin_s(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmax_s(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float fclamp_s(float x, float lo, float hi) {
+  return fmin_s(fmax_s(x, lo), hi);
+}
+__device__ __forceinline__ float fmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// Truncating f32 -> i32 with saturation; NaN -> 0.
+__device__ __forceinline__ int32_t sat_i32(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return INT32_MAX;
+  if (x <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)x;
+}
+
+__device__ __forceinline__ float h2f(uint16_t h) {
+  return __half2float(__ushort_as_half(h));
+}
+__device__ __forceinline__ uint16_t f2h(float f) {
+  return __half_as_ushort(__float2half_rn(f));
+}
+
+// Host-side IEEE binary32 -> binary16, round to nearest even.
+inline uint16_t f2h_host(float f) {
+  uint32_t x;
+  __builtin_memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t exp = (x >> 23) & 0xffu;
+  uint32_t mant = x & 0x7fffffu;
+  if (exp == 0xffu) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+  const int e = (int)exp - 112;
+  if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (e <= 0) {
+    if (e < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    const int sh = 14 - e;
+    uint32_t hm = mant >> sh;
+    const uint32_t rem = mant & ((1u << sh) - 1u), half = 1u << (sh - 1);
+    if (rem > half || (rem == half && (hm & 1u))) hm++;
+    return (uint16_t)(sign | hm);
+  }
+  uint32_t h = sign | ((uint32_t)e << 10) | (mant >> 13);
+  const uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+  return (uint16_t)h;
+}
+
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 abs3(f3 a) { return mk(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+__device__ __forceinline__ f3 max30(f3 a) {
+  return mk(fmax_s(a.x, 0.0f), fmax_s(a.y, 0.0f), fmax_s(a.z, 0.0f));
+}
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// ---------------------------------------------------------------------------
+// Primitives and the brush fold (brush.ts:238-349, 466-511; math/gpu.ts:15-25,
+// 87-92, 132-137; @typegpu/sdf sdBox3d/sdRoundedBox3d/sdCapsule as Quilez's forms)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sd_box(f3 p, f3 s) {
+  f3 q = abs3(p) - s;
+  return len3(max30(q)) + fmin_s(fmax_s(q.x, fmax_s(q.y, q.z)), 0.0f);
+}
+__device__ __forceinline__ float sd_round_box(f3 p, f3 s, float r) {
+  f3 q = mk(fabsf(p.x) - s.x + r, fabsf(p.y) - s.y + r, fabsf(p.z) - s.z + r);
+  return len3(max30(q)) + fmin_s(fmax_s(q.x, fmax_s(q.y, q.z)), 0.0f) - r;
+}
+__device__ __forceinline__ float sd_capsule(f3 p, f3 a, f3 b, float r) {
+  f3 pa = p - a, ba = b - a;
+  float h = fclamp_s(dot3(pa, ba) / dot3(ba, ba), 0.0f, 1.0f);
+  return len3(pa - ba * h) - r;
+}
+
+__device__ inline float eval_local(uint32_t kind, f3 p, f3 size, float radius) {
+  switch (kind) {
+    case 0: return len3(p) - size.x;
+    case 1: return sd_box(p, size);
+    case 2: return sd_round_box(p, size, radius);
+    case 3: return sd_capsule(p, mk(0.0f, -size.y, 0.0f), mk(0.0f, size.y, 0.0f), radius);
+    case 4: {
+      float qx = sqrtf(p.x * p.x + p.z * p.z) - size.x, qy = p.y;
+      return sqrtf(qx * qx + qy * qy) - radius;
+    }
+    case 5: {
+      float dx = sqrtf(p.x * p.x + p.z * p.z) - size.x, dy = fabsf(p.y) - size.y;
+      float mx = fmax_s(dx, 0.0f), my = fmax_s(dy, 0.0f);
+      return fmin_s(fmax_s(dx, dy), 0.0f) + sqrtf(mx * mx + my * my);
+    }
+    case 6: return p.y - radius;
+    default: return 1e9f;
+  }
+}
+
+__device__ inline float eval_brush(const clay_brush& b, f3 p) {
+  float inv_scale = 1.0f / fmax_s(b.scale, 1e-5f);
+  f3 q = mk(-b.rot[0], -b.rot[1], -b.rot[2]);
+  float qw = b.rot[3];
+  f3 v = p - mk(b.pos[0], b.pos[1], b.pos[2]);
+  f3 t = cross3(q, v) * 2.0f;
+  f3 rot = (v + t * qw) + cross3(q, t);
+  f3 local = rot * inv_scale;
+  return eval_local(b.kind, local, mk(b.size[0], b.size[1], b.size[2]), b.radius) * b.scale;
+}
+
+__device__ __forceinline__ float smoothstep_s(float lo, float hi, float x) {
+  float t = fclamp_s((x - lo) / (hi - lo), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+__device__ inline void apply_brush(float acc[2], const clay_brush& b, f3 p) {
+  float dist = eval_brush(b, p);
+  float keep = b.mask >= 0.0f ? 1.0f - smoothstep_s(0.25f, 0.75f, fabsf(acc[1] - b.mask)) : 1.0f;
+  float kk = fmax_s(b.blend, 1e-6f);
+  if (b.op == 0) {
+    float h = fclamp_s(0.5f + 0.5f * (acc[0] - dist) / kk, 0.0f, 1.0f);
+    float rx = fmix(acc[0], dist, h) - kk * h * (1.0f - h);
+    float ny = fmix(acc[1], b.material, h);
+    acc[0] = fmix(acc[0], rx, keep);
+    acc[1] = fmix(acc[1], ny, keep);
+  } else if (b.op == 1) {
+    float a = -dist;
+    float h = fclamp_s(0.5f - 0.5f * (acc[0] - a) / kk, 0.0f, 1.0f);
+    float rx = fmix(acc[0], a, h) + kk * h * (1.0f - h);
+    float ny = fmix(acc[1], b.material, h * 0.85f);
+    acc[0] = fmix(acc[0], rx, keep);
+    acc[1] = fmix(acc[1], ny, keep);
+  } else {
+    float w = 1.0f - smoothstep_s(-fmax_s(b.blend, 1e-4f), 0.0f, dist);
+    acc[1] = fmix(acc[1], b.material, w * keep);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Collider: analyticField / volumeField / unionField (field.ts:102-315)
+// ---------------------------------------------------------------------------
+__device__ inline float analytic_dist(const LeafDev& l, f3 p) {
+  float acc[2] = {1e9f, 0.0f};
+  for (int i = 0; i < l.nbrush; i++) apply_brush(acc, l.brushes[i], p);
+  return acc[0];
+}
+
+__device__ __forceinline__ float texel0(const uint16_t* d, int n, int x, int y, int z) {
+  x = x < 0 ? 0 : (x >= n ? n - 1 : x);
+  y = y < 0 ? 0 : (y >= n ? n - 1 : y);
+  z = z < 0 ? 0 : (z >= n ? n - 1 : z);
+  return h2f(d[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z)]);
+}
+
+// textureSampleLevel(linear, clamp-to-edge) at mip 0 with f32 weights (volume.ts:262-267).
+__device__ inline float sample_raw0(const VolDev& v, const uint16_t* d, f3 p) {
+  float u = (p.x - v.origin[0]) / v.extent;
+  float w = (p.y - v.origin[1]) / v.extent;
+  float s = (p.z - v.origin[2]) / v.extent;
+  float fn = (float)v.res;
+  float tx = u * fn - 0.5f, ty = w * fn - 0.5f, tz = s * fn - 0.5f;
+  float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
+  int x0 = sat_i32(fx0), y0 = sat_i32(fy0), z0 = sat_i32(fz0);
+  float fx = tx - fx0, fy = ty - fy0, fz = tz - fz0;
+  int n = v.res;
+  float c00 = fmix(texel0(d, n, x0, y0, z0), texel0(d, n, x0 + 1, y0, z0), fx);
+  float c10 = fmix(texel0(d, n, x0, y0 + 1, z0), texel0(d, n, x0 + 1, y0 + 1, z0), fx);
+  float c01 = fmix(texel0(d, n, x0, y0, z0 + 1), texel0(d, n, x0 + 1, y0, z0 + 1), fx);
+  float c11 = fmix(texel0(d, n, x0, y0 + 1, z0 + 1), texel0(d, n, x0 + 1, y0 + 1, z0 + 1), fx);
+  float c0 = fmix(c00, c10, fy);
+  float c1 = fmix(c01, c11, fy);
+  return fmix(c0, c1, fz);
+}
+
+// SdfVolume.sampleDist at mip 0 (volume.ts:277-286) == volumeField.sample(p, 0).x.
+__device__ inline float vol_dist(const VolDev& v, const uint16_t* d, f3 p) {
+  f3 rel = mk(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
+  float dbox = sd_box(rel, mk(v.half, v.half, v.half));
+  if (dbox > 0.0f) return dbox;
+  float band_world = v.band * v.voxel * 1.0f;
+  return sample_raw0(v, d, p) * band_world;
+}
+
+__device__ inline float leaf_dist(const LeafDev& l, f3 p) {
+  if (l.kind == 0) return analytic_dist(l, p);
+  return vol_dist(*l.vol, l.dist0, p);
+}
+
+__device__ inline f3 leaf_normal(const LeafDev& l, f3 p) {
+  if (l.kind == 0) {
+    float e = l.eps;
+    f3 g = mk(analytic_dist(l, mk(p.x + e, p.y, p.z)) - analytic_dist(l, mk(p.x - e, p.y, p.z)),
+              analytic_dist(l, mk(p.x, p.y + e, p.z)) - analytic_dist(l, mk(p.x, p.y - e, p.z)),
+              analytic_dist(l, mk(p.x, p.y, p.z + e)) - analytic_dist(l, mk(p.x, p.y, p.z - e)));
+    float len = len3(g);
+    return len > 1e-9f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
+  }
+  const VolDev& v = *l.vol;
+  float h = v.grad_h;
+  f3 g = mk(vol_dist(v, l.dist0, mk(p.x + h, p.y, p.z)) - vol_dist(v, l.dist0, mk(p.x - h, p.y, p.z)),
+            vol_dist(v, l.dist0, mk(p.x, p.y + h, p.z)) - vol_dist(v, l.dist0, mk(p.x, p.y - h, p.z)),
+            vol_dist(v, l.dist0, mk(p.x, p.y, p.z + h)) - vol_dist(v, l.dist0, mk(p.x, p.y, p.z - h)));
+  float len = len3(g);
+  return len > 1e-8f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
+}
+
+// Returns the leaf the union selects (right child wins unless left is strictly nearer).
+__device__ inline int collider_pick(const ColliderDev& c, f3 p, float* dist) {
+  int best = 0;
+  float bd = leaf_dist(c.leaf[0], p);
+  for (int i = 1; i < c.nleaf; i++) {
+    float d = leaf_dist(c.leaf[i], p);
+    if (!(bd < d)) {
+      best = i;
+      bd = d;
+    }
+  }
+  *dist = bd;
+  return best;
+}
+
+__device__ inline f3 collider_normal(const ColliderDev& c, int leaf, f3 p) {
+  // Unrolled so the leaf record stays in registers/SGPRs.
+  for (int i = 0; i < CLAY_MAX_LEAVES; i++)
+    if (i == leaf) return leaf_normal(c.leaf[i], p);
+  return mk(0.0f, 1.0f, 0.0f);
+}
+
+}  // namespace clay

@@ -1,0 +1,182 @@
+// clay_internal.h -- host-side objects behind the C ABI handles, and the kernel
+// launchers each .hip translation unit exports to api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "clay_device.h"
+#include "clay_hip.h"
+
+namespace clay {
+
+// ---- errors ----------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define CLAY_HIP_CHECK(expr)                                                        \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return ::clay::fail(CLAY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- per-kernel timing -----------------------------------------------------
+struct KernelStat {
+  std::string name;
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+struct PendingTiming {
+  int stat;
+  hipEvent_t start, stop;
+};
+
+struct Ctx;
+
+// Brackets one launch with HIP events when the context's timing is on.
+class LaunchTimer {
+ public:
+  LaunchTimer(Ctx* ctx, hipStream_t s, const char* name);
+  ~LaunchTimer();
+
+ private:
+  Ctx* ctx_;
+  hipStream_t s_;
+  int stat_ = -1;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  std::vector<KernelStat> stats;
+  std::vector<PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  int stat_index(const char* name);
+  hipEvent_t take_event();
+  void collect();  // resolves pending timings (synchronises their events)
+};
+
+inline hipStream_t pick(Ctx* c, clay_stream s) {
+  return s ? reinterpret_cast<hipStream_t>(s) : c->stream;
+}
+
+// ---- objects ---------------------------------------------------------------
+struct Volume {
+  Ctx* ctx = nullptr;
+  clay_volume_info info{};
+  double origin[3] = {0, 0, 0};
+  double grad_voxels = 1.0;
+  VolDev host{};                 // mirror of the device record
+  VolDev* dev = nullptr;         // device record (kernels read this)
+  uint16_t* dist = nullptr;      // all mips, distance plane (f16 bits)
+  uint16_t* mat = nullptr;       // all mips, material plane (f16 bits)
+  std::vector<size_t> voxel_off; // per mip offset into the planes
+  size_t total_voxels = 0;
+  int mip_res(int m) const { return info.resolution >> m; }
+};
+
+struct Field {
+  Ctx* ctx = nullptr;
+  int max_mip = 0;
+  ColliderDev col{};             // leaves point at device memory owned below / by volumes
+  clay_brush* brushes_dev = nullptr;
+  std::vector<Field*> children;  // union operands (not owned)
+};
+
+struct Splat {
+  Ctx* ctx = nullptr;
+  clay_cloud_desc cloud{};
+  clay_splat_desc desc{};
+  Volume* vol = nullptr;         // owned
+  Field* field = nullptr;        // owned volumeField of `vol`
+  int mips = 0;
+  std::vector<size_t> grid_off;  // per mip offset into `grid`
+  size_t grid_total = 0;
+  uint32_t* grid = nullptr;      // u32 codes (plain mode, or tiled + keep_codes)
+  // tiled mode
+  std::vector<int> brick_res;    // bricks per axis per mip
+  std::vector<size_t> brick_off;
+  size_t brick_total = 0;
+  uint32_t* brick_count = nullptr;   // per brick
+  uint32_t* brick_start = nullptr;   // exclusive scan of counts
+  uint32_t* brick_cursor = nullptr;
+  uint32_t* brick_list = nullptr;    // particle ids per brick
+  size_t brick_list_cap = 0;
+  uint32_t* brick_total_dev = nullptr;  // [mips] total entries (overflow check)
+  uint8_t* tile_live = nullptr;      // per 8^3 tile, all mips: live after last bake
+  std::vector<size_t> tile_off;
+  size_t tile_total = 0;
+  bool has_baked = false;
+};
+
+struct Fluid {
+  Ctx* ctx = nullptr;
+  Field* world = nullptr;        // not owned
+  clay_fluid_desc desc{};
+  clay_fluid_consts consts{};
+  clay_fluid_params params{};
+  double rest_density = 0, mass_poly6 = 0, h6 = 0, inv_h = 0, rest_floor = 0, hh = 0;
+  uint32_t capacity = 0;
+  uint32_t live = 0;
+  uint32_t cursor = 0;
+  uint32_t pending_emit = 0;
+  uint32_t seed = 1;
+  // id-order state (SoA)
+  float* st = nullptr;           // 11 planes of capacity floats
+  float *px, *py, *pz, *vx, *vy, *vz, *rho, *pres, *nvx, *nvy, *nvz;
+  // sorted snapshot
+  float* ss = nullptr;           // 8 planes
+  float *spx, *spy, *spz, *svx, *svy, *svz, *srho, *spres;
+  // sort
+  uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
+  uint32_t* hist = nullptr;      // digit histograms
+  size_t hist_cap = 0;
+  uint32_t* cell_start = nullptr;  // dense over gridRes^3, validated by the sorted key
+  uint32_t* cell_end = nullptr;
+  size_t grid_cells = 0;
+  int key_bits = 0;
+  uint32_t* contacts = nullptr;  // [0] = count, then 32 x float4 (as bits)
+  Splat* surface = nullptr;      // owned
+  FluidK k{};
+};
+
+// ---- launchers (implemented in the .hip files) ------------------------------
+// fluid.hip
+int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px,
+                const float* py, const float* pz, uint32_t* keys, uint32_t* vals,
+                uint32_t invalid_key);
+int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
+                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cell_start,
+                   uint32_t* cell_end, const float* const* src, float* const* dst, int planes);
+int launch_density(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
+                   const float* spy, const float* spz, const uint32_t* skeys,
+                   const uint32_t* cell_start, const uint32_t* cell_end, float* srho,
+                   float* spres);
+int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
+                            uint32_t n, const float* const* sorted, const uint32_t* skeys,
+                            const uint32_t* svals, const uint32_t* cell_start,
+                            const uint32_t* cell_end, float* const* out, uint32_t* contacts);
+int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+// sort.hip
+size_t radix_hist_size(uint32_t n);
+int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
+                     uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist);
+// splat.hip
+int splat_bake(Splat* sp, hipStream_t s);
+int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order);
+int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
+                         std::vector<size_t>* brick_off, size_t* brick_total,
+                         std::vector<size_t>* tile_off, size_t* tile_total);
+size_t splat_max_bricks_per_particle(double radius, double band, double voxel0, int mips);
+// volume.hip
+int launch_bake_brushes(Ctx* c, hipStream_t s, Volume* v, const clay_brush* brushes_dev, int n);
+int launch_field_eval(Ctx* c, hipStream_t s, const ColliderDev& col, int n, const float* pts,
+                      float* dist, float* nrm);
+int launch_set_origin_dev(Ctx* c, hipStream_t s, VolDev* dev, const float* origin3, float half);
+
+}  // namespace clay

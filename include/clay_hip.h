@@ -1,0 +1,291 @@
+/*
+ * clay_hip.h -- C ABI of the MI355X (gfx950) fluid-to-field path.
+ *
+ * This is the drop-in boundary for clay-engine's FluidSim step and SplatField
+ * bake.  Each entry point replaces one member of the reference's TypeScript
+ * surface (paths relative to the reference root); the reference passes shader
+ * closures, which a C ABI cannot carry, so colliders and particle clouds are
+ * passed as descriptors instead.
+ *
+ * Conventions
+ *  - Every function returns an int status: CLAY_OK (0) or a negative code.  On
+ *    failure clay_last_error() returns a thread-local message; constructor
+ *    messages are the reference's own (e.g. volume.ts:156-160).
+ *  - Handles are opaque and owned by the caller; *_destroy frees them.  A
+ *    handle may be used from one host thread at a time.
+ *  - Host-side option values are doubles, as JS numbers are: values the
+ *    reference derives in JS (f64) and hands to the shaders as f32 are derived
+ *    here the same way and rounded once.
+ *  - clay_stream is a hipStream_t (NULL = the context's own stream).  step,
+ *    bake and set_origin_device only enqueue work; the download, upload,
+ *    get_state, set_state and read_contacts calls synchronise the stream.
+ *  - Device pointers in descriptors are plain `float*` into HBM, laid out as
+ *    structure-of-arrays (x[], y[], z[]).
+ */
+#ifndef CLAY_HIP_H
+#define CLAY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLAY_OK 0
+#define CLAY_ERR_INVALID (-1) /* bad argument or reference-rejected option */
+#define CLAY_ERR_HIP (-2)     /* HIP runtime error */
+#define CLAY_ERR_NODEVICE (-3)
+#define CLAY_ERR_STATE (-4)   /* call not valid in the object's current state */
+
+#define CLAY_MAX_MIPS 16
+#define CLAY_MAX_LEAVES 4
+#define CLAY_CONTACTS 32      /* ContactList capacity, fluid.ts:55-58 */
+
+typedef struct clay_ctx clay_ctx;
+typedef struct clay_volume clay_volume;
+typedef struct clay_field clay_field;
+typedef struct clay_splat clay_splat;
+typedef struct clay_fluid clay_fluid;
+typedef void* clay_stream;
+
+/* ---- library ----------------------------------------------------------- */
+const char* clay_last_error(void);
+int clay_version(void);
+
+/* The TgpuRoot equivalent: one device, one default stream (tgpu.init). */
+int clay_ctx_create(int device, clay_ctx** out);
+int clay_ctx_destroy(clay_ctx* ctx);
+int clay_ctx_synchronize(clay_ctx* ctx);
+/* Optional per-kernel HIP-event timing of everything the context launches. */
+int clay_ctx_set_timing(clay_ctx* ctx, int enabled);
+int clay_ctx_kernel_stats(clay_ctx* ctx, int index, char* name, int name_len,
+                          double* total_ms, uint64_t* launches);
+int clay_ctx_reset_stats(clay_ctx* ctx);
+
+/* ---- brushes ------------------------------------------------------------ */
+/* Same field order and 80-byte layout as `Brush` (lib/src/field/brush.ts:53-80). */
+typedef struct {
+  float pos[3];
+  float scale;
+  float rot[4];   /* xyzw unit quaternion */
+  float size[3];
+  float radius;
+  float material;
+  float blend;
+  float bound;
+  float mask;     /* material restriction, -1 = none */
+  uint32_t kind;  /* BuiltinBrushKind (brush.ts:19-35): 0..6 supported */
+  uint32_t op;    /* BrushOp (brush.ts:41-48): 0 add, 1 cut, 2 paint */
+  uint32_t slot;
+  uint32_t pad;
+} clay_brush;
+
+/* ---- SdfVolume (lib/src/field/volume.ts:91-343) ----------------------- */
+typedef struct {
+  int32_t resolution;     /* 0 -> 128 (volume.ts:140) */
+  int32_t mip_levels;     /* 0 -> derived (volume.ts:148-151) */
+  double world_size;      /* <= 0 -> 16 */
+  double band;            /* <= 0 -> 4 */
+  double origin[3];
+  double gradient_voxels; /* <= 0 -> 1 (volume.ts:299) */
+} clay_volume_desc;
+
+typedef struct {
+  int32_t resolution;
+  int32_t mip_levels;
+  int32_t tile_res[CLAY_MAX_MIPS];     /* tileResPerMip */
+  int32_t cell_offsets[CLAY_MAX_MIPS]; /* cellOffsets */
+  int32_t total_cells;                 /* totalCells */
+  int32_t pad;
+  double world_size;
+  double voxel_size;
+  double band;
+  double origin[3];
+} clay_volume_info;
+
+/* Geometry derivation and the reference's resolution check, host only (no GPU):
+ * volume.ts:139-179. */
+int clay_volume_derive(int32_t resolution, int32_t mip_levels, clay_volume_info* out);
+/* new SdfVolume(root, options): volume.ts:139. Storage starts saturated (+1, material 0). */
+int clay_volume_create(clay_ctx* ctx, const clay_volume_desc* desc, clay_volume** out);
+int clay_volume_destroy(clay_volume* vol);
+int clay_volume_get_info(const clay_volume* vol, clay_volume_info* out);
+/* SdfVolume.setOrigin: volume.ts:327-334 (origin and centre, f64 -> f32). */
+int clay_volume_set_origin(clay_volume* vol, const double origin[3], clay_stream stream);
+/* setOrigin from a device-resident vec3 (no host readback; graph-capturable). */
+int clay_volume_set_origin_device(clay_volume* vol, const float* origin3_device,
+                                  clay_stream stream);
+int clay_volume_voxel_size_at(const clay_volume* vol, int32_t mip, double* out);
+/* Texels of one mip as rgba16float (r = distance / band, g = material, b = 0, a = 1),
+ * x fastest, (res >> mip)^3 * 4 halves. */
+int clay_volume_download(clay_volume* vol, int32_t mip, uint16_t* rgba16_host);
+int clay_volume_upload(clay_volume* vol, int32_t mip, const uint16_t* rgba16_host);
+/* SdfBuilder.setBrushes + rebuild (lib/src/field/builder.ts:87-172): dense bake of
+ * a brush list into every mip. */
+int clay_volume_bake_brushes(clay_volume* vol, const clay_brush* brushes, int32_t n,
+                             clay_stream stream);
+
+/* ---- TracedField (lib/src/trace/field.ts:20-73) ------------------------ */
+typedef struct {
+  double band;     /* <= 0 -> 64 (field.ts:200) */
+  double epsilon;  /* <= 0 -> 1e-3 (field.ts:201) */
+  double material;
+} clay_analytic_opts;
+
+/* analyticField (field.ts:196-236) whose closure is the fold of `brushes` from
+ * (1e9, 0) with applyBrush (brush.ts:474-511). */
+int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int32_t n,
+                        const clay_analytic_opts* opts, clay_field** out);
+/* volumeField (field.ts:102-180). The volume must outlive the field. */
+int clay_field_volume(clay_ctx* ctx, clay_volume* vol, clay_field** out);
+/* unionField (field.ts:246-315); a and b must outlive the union. */
+int clay_field_union(clay_field* a, clay_field* b, clay_field** out);
+int clay_field_destroy(clay_field* f);
+int clay_field_max_mip(const clay_field* f, int32_t* out);
+/* sample(p, 0).x and normal(p) at n host points (xyz interleaved); runs on the GPU. */
+int clay_field_eval(clay_field* f, const float* points_host, int32_t n, float* dist_host,
+                    float* normal_host);
+
+/* ---- ParticleCloud + SplatField (lib/src/sim/splat.ts) ----------------- */
+#define CLAY_LIVE_ALL 0     /* occluder.gpu.ts:98-101 */
+#define CLAY_LIVE_PREFIX 1  /* i < count (fluid.ts:221-224) */
+#define CLAY_LIVE_RANGE 2   /* first <= i < first + count (splat.ts:56-60) */
+
+/* ParticleCloud (splat.ts:15-38) as device SoA pointers plus a live rule. */
+typedef struct {
+  uint32_t capacity;
+  int32_t live_rule;
+  uint32_t live_first;
+  uint32_t live_count;
+  const float* pos[3];
+  const float* nrm[3];  /* normalAt (splat.ts:37); all NULL for a union of spheres */
+} clay_cloud_desc;
+
+#define CLAY_SPLAT_PLAIN 0  /* dense u32 grid, global atomicMin: the reference's schedule */
+#define CLAY_SPLAT_TILED 1  /* particles binned to LDS bricks; only live 8^3 tiles written */
+
+typedef struct {
+  double radius;          /* required (splat.ts:84) */
+  double material;
+  int32_t resolution;     /* 0 -> 64 */
+  int32_t mip_levels;     /* 0 -> log2(res / 8) (splat.ts:149) */
+  double band;            /* <= 0 -> 2 */
+  double origin[3];       /* default (-12, -4, -12) is applied by the host layer */
+  double world_size;      /* <= 0 -> 24 */
+  int32_t mode;           /* CLAY_SPLAT_PLAIN | CLAY_SPLAT_TILED */
+  int32_t keep_codes;     /* tiled mode: also maintain the u32 code grid (tests) */
+} clay_splat_desc;
+
+/* new SplatField(root, cloud, options): splat.ts:147-278. */
+int clay_splat_create(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                      clay_splat** out);
+int clay_splat_destroy(clay_splat* s);
+/* Change the live count of a PREFIX/RANGE cloud (the reference's params.count). */
+int clay_splat_set_live(clay_splat* s, uint32_t first, uint32_t count);
+/* SplatField.bake: splat.ts:296-304. */
+int clay_splat_bake(clay_splat* s, clay_stream stream);
+/* SplatField.setCenter: splat.ts:284-287. */
+int clay_splat_set_center(clay_splat* s, const double center[3], clay_stream stream);
+int clay_splat_volume(clay_splat* s, clay_volume** out);
+int clay_splat_field(clay_splat* s, clay_field** out);
+/* The u32 code grid of one mip (plain mode, or tiled with keep_codes). */
+int clay_splat_grid_download(clay_splat* s, int32_t mip, uint32_t* codes_host);
+/* Exact live-tile mask of one mip after the last bake: 1 byte per 8^3 tile. */
+int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* live_host);
+
+/* ---- FluidSim (lib/src/sim/fluid.ts:111-582) --------------------------- */
+typedef struct {
+  uint32_t capacity;        /* 8192 */
+  int32_t bucket;           /* 64 */
+  double spacing;           /* 0.45 */
+  double smoothing;         /* <= 0 -> 2.2 * spacing */
+  double density;           /* mass = density * spacing^3; reference = 1 */
+  double radius;            /* < 0 -> 0.5 * spacing */
+  double surface_radius;    /* < 0 -> 1.1 * spacing */
+  double material;
+  int32_t bake_resolution;  /* 64 */
+  int32_t bake_mip_levels;  /* 0 -> max(1, log2(res / 8)) */
+  double bake_band;         /* 2 */
+  double origin[3];         /* (-12, -4, -12) */
+  double world_size;        /* 24 */
+  int32_t substeps;         /* 3 */
+  int32_t splat_mode;       /* CLAY_SPLAT_PLAIN */
+  double gravity[3];        /* (0, -9.81, 0) */
+  double stiffness;         /* 220 */
+  double viscosity;         /* 0.4 */
+  double damping;           /* 0.995 */
+  double max_speed;         /* 15 */
+} clay_fluid_desc;
+
+/* The reference's defaults (fluid.ts:141-203). */
+void clay_fluid_desc_default(clay_fluid_desc* d);
+
+/* FluidParams minus the emission cursor: what FluidSim.configure may patch (fluid.ts:528). */
+typedef struct {
+  double gravity[3];
+  double dt;
+  double emit_pos[3];
+  double emit_radius;
+  double emit_vel[3];
+  double max_speed;
+  double stiffness;
+  double viscosity;
+  double damping;
+  double contact_depth;
+} clay_fluid_params;
+
+/* Derived constants (fluid.ts:141-203, 239-247, 444-447), f64. */
+typedef struct {
+  double h, mass, poly6, spiky_grad, rest_density, dt, radius, contact_depth;
+  double grid_origin[3];
+  int32_t grid_res;
+  int32_t bucket;
+  int32_t substeps;
+  int32_t pad;
+  double wall_lo[3];
+  double wall_hi[3];
+} clay_fluid_consts;
+
+/* new FluidSim(root, world, options): fluid.ts:140-493. */
+int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fluid_desc* desc,
+                      clay_fluid** out);
+int clay_fluid_destroy(clay_fluid* f);
+int clay_fluid_constants(const clay_fluid* f, clay_fluid_consts* out);
+/* Replaces the first n particles (SoA host arrays: x[n], y[n], z[n]); the live count
+ * becomes n.  vel may be NULL (zero).  The reference fills particles only by emit;
+ * this is the seeded-lattice entry the configurations use. */
+int clay_fluid_set_state(clay_fluid* f, const float* pos_soa, const float* vel_soa, uint32_t n);
+#define CLAY_STATE_POS 0
+#define CLAY_STATE_VEL 1
+#define CLAY_STATE_DENSITY 2
+#define CLAY_STATE_PRESSURE 3
+#define CLAY_STATE_NVEL 4
+/* Reads `what` for the live particles: 3 * count floats (SoA) or count floats. */
+int clay_fluid_get_state(clay_fluid* f, int32_t what, float* host);
+/* FluidSim.spawn: fluid.ts:511-525 (spread < 0 -> spacing * cbrt(max(n,1)) * 0.62). */
+int clay_fluid_spawn(clay_fluid* f, uint32_t n, const double pos[3], const double vel[3],
+                     double spread);
+/* FluidSim.configure: fluid.ts:528-530. */
+int clay_fluid_configure(clay_fluid* f, const clay_fluid_params* p);
+int clay_fluid_get_params(const clay_fluid* f, clay_fluid_params* out);
+/* FluidSim.step: emit, substeps x (bin, density, forces, integrate), bake (fluid.ts:536-558). */
+int clay_fluid_step(clay_fluid* f, clay_stream stream);
+/* One substep without emit or bake (for per-substep parity checks). */
+int clay_fluid_substep(clay_fluid* f, clay_stream stream);
+/* FluidSim.readContacts: fluid.ts:565-569.  out4 holds min(raw, 32) points
+ * (x, y, z, speed); raw_count is the atomic counter, which may exceed 32. */
+int clay_fluid_read_contacts(clay_fluid* f, float* out4, uint32_t* n, uint32_t* raw_count);
+/* FluidSim.particleCount: fluid.ts:496-498. */
+int clay_fluid_particle_count(const clay_fluid* f, uint32_t* out);
+/* FluidSim.surface (the owned SplatField) and FluidSim.field. */
+int clay_fluid_surface(clay_fluid* f, clay_splat** out);
+/* Cell keys and the stable sort permutation of the last substep (count entries each). */
+#define CLAY_DEBUG_KEYS 0
+#define CLAY_DEBUG_PERM 1
+int clay_fluid_debug_download(clay_fluid* f, int32_t what, uint32_t* host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLAY_HIP_H */

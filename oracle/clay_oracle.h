@@ -1,0 +1,186 @@
+/*
+ * clay_oracle.h -- CPU oracle for clay-engine's fluid-to-field path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this
+ * code: it exists so that tests/, bench.py's cpu_baseline leg and
+ * __graft_entry__.smoke() can check the HIP library against an independent,
+ * plain-C restatement of the reference algorithm.
+ *
+ * Every entry point exists twice: `_f32` ("WGSL-f32" mode: IEEE binary32, no
+ * FMA contraction, operations in the reference shader's order) and `_f64`
+ * (the same algorithm in binary64, the accuracy yardstick).  Constants that
+ * the reference computes in JS (f64) and bakes into WGSL as f32 literals are
+ * derived here in double and rounded once to the working type.
+ *
+ * Reference files (paths relative to the reference root):
+ *   lib/src/sim/fluid.ts      FluidSim: constants, grid, density, forces, integrate
+ *   lib/src/sim/splat.ts      SplatField: clear, splat[m], resolve[m]
+ *   lib/src/field/volume.ts   SdfVolume: mip derivation, sampling, gradient
+ *   lib/src/trace/field.ts    volumeField / analyticField / unionField
+ *   lib/src/field/brush.ts    primitives, evalBrush, applyBrush
+ *   lib/src/field/builder.ts  SdfBuilder.gen (dense brush bake)
+ *   lib/src/field/tilegrid.ts packTile, compact, dilate
+ *   lib/src/field/mips.ts     MipRefiner (eikonal band re-expansion)
+ *   lib/src/field/modify.ts   SdfEditor bin/apply
+ *   lib/src/math/gpu.ts       hash1/hash3, smin/smax
+ *
+ * Parity status of each function is recorded in the comment above it in
+ * clay_oracle.c; a function with no independent check says "parity unpinned".
+ */
+#ifndef CLAY_ORACLE_H
+#define CLAY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_INVALID_KEY 0xFFFFFFFFu
+#define ORC_MAX_LEAVES 4
+#define ORC_CONTACTS 32
+
+/* Brush record: same field order and 80-byte size as the reference's `Brush`
+ * struct (lib/src/field/brush.ts:53-80). */
+typedef struct {
+  float pos[3];
+  float scale;
+  float rot[4];
+  float size[3];
+  float radius;
+  float material;
+  float blend;
+  float bound;
+  float mask;
+  uint32_t kind;
+  uint32_t op;
+  uint32_t slot;
+  uint32_t pad;
+} orc_brush;
+
+/* FluidSim options (lib/src/sim/fluid.ts:60-91, defaults :141-203).
+ * `density` scales the particle mass (mass = density * spacing^3); the
+ * reference has density 1. Fields set to a negative value take the default. */
+typedef struct {
+  double spacing;
+  double smoothing;
+  double density;
+  double radius;
+  int32_t bucket;
+  int32_t substeps;
+  double origin[3];
+  double world_size;
+  double gravity[3];
+  double stiffness;
+  double viscosity;
+  double damping;
+  double max_speed;
+} orc_fluid_opts;
+
+/* Derived FluidSim constants, in f64 as the reference's JS computes them
+ * (fluid.ts:141-203, 239-266, 444-447). Products that the shader sees as a
+ * single literal are stored pre-multiplied. */
+typedef struct {
+  double h, mass, poly6, spiky_grad, rest_density, dt, radius, contact_depth;
+  double grid_origin[3];
+  int32_t grid_res;
+  int32_t bucket;
+  int32_t substeps;
+  int32_t pad0;
+  double wall_lo[3], wall_hi[3];
+  double gravity[3];
+  double stiffness, viscosity, damping, max_speed;
+  double hh, mass_poly6, h6, inv_h, rest_floor;
+} orc_fluid_k;
+
+/* One collider leaf: an analytic brush fold or a baked volume (mip 0). */
+typedef struct {
+  int32_t kind;            /* 0 = analytic brush list, 1 = volume */
+  int32_t nbrush;
+  const orc_brush* brushes;
+  double eps;              /* analytic: central-difference epsilon (field.ts:201) */
+  /* volume (lib/src/field/volume.ts uniform VolumeParams) */
+  const uint16_t* dist16;  /* mip-0 `.r` channel as f16 bits, x-fastest, res^3 */
+  int32_t res;
+  int32_t pad1;
+  double origin[3];
+  double world_size;
+  double band;
+  double grad_voxels;
+} orc_leaf;
+
+/* unionField over leaves, folded left: union(union(a, b), c) (field.ts:246-315). */
+typedef struct {
+  int32_t nleaf;
+  int32_t pad;
+  orc_leaf leaf[ORC_MAX_LEAVES];
+} orc_collider;
+
+/* SplatField options (lib/src/sim/splat.ts:74-106, 147-162). `origin` is the
+ * volume origin at bake time (SdfVolume.setOrigin). */
+typedef struct {
+  int32_t res;
+  int32_t mips;
+  double world_size;
+  double origin[3];
+  double band;
+  double radius;
+  double material;
+} orc_splat_opts;
+
+/* Fluid state, structure-of-arrays, in the working precision. */
+typedef struct {
+  void* px; void* py; void* pz;
+  void* vx; void* vy; void* vz;
+  void* rho; void* pres;
+  void* nvx; void* nvy; void* nvz;
+} orc_state;
+
+typedef struct {
+  uint32_t count;        /* atomic counter value (may exceed 32) */
+  uint32_t pad;
+  double points[ORC_CONTACTS][4];
+} orc_contacts;
+
+/* ---- shared, precision independent ------------------------------------ */
+uint32_t orc_hash_u32(uint32_t x);
+float orc_hash1(uint32_t seed);
+int orc_volume_derive(int resolution, int mips_req, int* mips_out, int* tile_res_out,
+                      int* cell_off_out, int* total_cells_out, char* err, int errlen);
+void orc_fluid_derive(const orc_fluid_opts* o, orc_fluid_k* k);
+uint32_t orc_pack_tile(uint32_t x, uint32_t y, uint32_t z);
+void orc_unpack_tile(uint32_t v, uint32_t* xyz);
+uint16_t orc_f32_to_f16(float f);
+float orc_f16_to_f32(uint16_t h);
+uint32_t orc_enc_f32(float v);
+float orc_dec_f32(uint32_t e);
+void orc_set_threads(int n);
+int orc_get_threads(void);
+void orc_stable_sort_keys(int n, const uint32_t* keys, uint32_t* perm_out);
+
+#define ORC_DECLARE(T, S)                                                              \
+  void orc_keys_##S(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz, \
+                    uint32_t* keys);                                                   \
+  void orc_substep_##S(const orc_fluid_k* k, const orc_collider* col, int n,           \
+                       orc_state* st, orc_contacts* contacts, uint32_t* keys_out,      \
+                       uint32_t* perm_out);                                            \
+  void orc_density_##S(const orc_fluid_k* k, int n, orc_state* st);                    \
+  void orc_forces_##S(const orc_fluid_k* k, int n, orc_state* st);                     \
+  void orc_integrate_##S(const orc_fluid_k* k, const orc_collider* col, int n,         \
+                         orc_state* st, orc_contacts* contacts);                       \
+  void orc_collider_eval_##S(const orc_collider* col, int n, const T* p3, T* dist,     \
+                             T* nrm3);                                                 \
+  void orc_splat_bake_##S(const orc_splat_opts* o, int n, const T* px, const T* py,    \
+                          const T* pz, const T* nx, const T* ny, const T* nz,          \
+                          int live_first, int live_count, uint32_t* grid);             \
+  void orc_splat_resolve_##S(const orc_splat_opts* o, const uint32_t* grid,            \
+                             uint16_t* texels);                                        \
+  void orc_bake_brushes_##S(int res, int mips, double world_size, const double* origin,\
+                            double band, const orc_brush* b, int nb, uint16_t* texels);
+
+ORC_DECLARE(float, f32)
+ORC_DECLARE(double, f64)
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,705 @@
+/*
+ * clay_oracle_impl.c -- precision-generic body of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY (see clay_oracle.h).  Compiled twice by
+ * oracle/Makefile: -DORC_T=float -DORC_S=f32 and -DORC_T=double -DORC_S=f64,
+ * always with -ffp-contract=off so every a*b+c rounds twice, as a WGSL
+ * implementation without fused multiply-add does.
+ *
+ * Operation order follows the reference shader text left to right; vector
+ * `length`/`dot` sum x, then y, then z; `pow(t, 3)` is (t*t)*t and `mix(a,b,t)`
+ * is a*(1-t)+b*t (the WGSL definition).  Where the reference leaves a choice to
+ * the WGSL implementation (pow lowering, length, hardware filter weights) the
+ * choice made here is the one the HIP kernels make, and the functions carry
+ * "parity unpinned" for that choice.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "clay_oracle.h"
+
+#ifndef ORC_T
+#define ORC_T float
+#define ORC_S f32
+#endif
+
+typedef ORC_T T;
+#define CAT2(a, b) a##_##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, ORC_S)
+
+#define IS_F32 (sizeof(T) == 4)
+static inline T t_sqrt(T x) { return IS_F32 ? (T)sqrtf((float)x) : (T)sqrt((double)x); }
+static inline T t_floor(T x) { return IS_F32 ? (T)floorf((float)x) : (T)floor((double)x); }
+static inline T t_fabs(T x) { return (T)fabs((double)x); }
+static inline T t_min(T a, T b) { return b < a ? b : a; }
+static inline T t_max(T a, T b) { return a < b ? b : a; }
+static inline T t_clamp(T x, T lo, T hi) { return t_min(t_max(x, lo), hi); }
+static inline T t_mix(T a, T b, T t) { return a * ((T)1 - t) + b * t; }
+
+typedef struct { T x, y, z; } v3;
+static inline v3 V(T x, T y, T z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmuls(v3 a, T s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 vabs(v3 a) { return V(t_fabs(a.x), t_fabs(a.y), t_fabs(a.z)); }
+static inline v3 vmax0(v3 a) { return V(t_max(a.x, 0), t_max(a.y, 0), t_max(a.z, 0)); }
+static inline T vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline T vlen(v3 a) { return t_sqrt(vdot(a, a)); }
+static inline v3 vcross(v3 a, v3 b) {
+  return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* f32 -> i32 as the hardware converts: truncate, saturate, NaN -> 0. */
+static inline int32_t sat_i32(T x) {
+  if (x != x) return 0;
+  if (x >= (T)2147483648.0) return INT32_MAX;
+  if (x <= (T)-2147483648.0) return INT32_MIN;
+  return (int32_t)x;
+}
+static inline T rnd(double d) { return (T)d; }
+/* A shader literal: the f32 literal in f32 mode, the exact decimal in f64 mode. */
+#define KF(x) (IS_F32 ? (T)(x##f) : (T)(x))
+
+/* ---------------------------------------------------------------------------
+ * Primitives and the brush fold (lib/src/field/brush.ts:238-349, 474-511;
+ * lib/src/math/gpu.ts:21-25, 87-92, 132-137).  sdSphere/sdBox3d/sdCapsule are
+ * Quilez's formulas (@typegpu/sdf@0.12.0, not vendored); pinned by the
+ * reference's brush.gpu.ts:34-91 closed-form answers, ported in tests.
+ * sdRoundedBox3d: parity unpinned (no reference test; Quilez's
+ * q = |p| - b + r form is assumed).
+ * ------------------------------------------------------------------------- */
+static T sd_box(v3 p, v3 s) {
+  v3 q = vsub(vabs(p), s);
+  return vlen(vmax0(q)) + t_min(t_max(q.x, t_max(q.y, q.z)), 0);
+}
+static T sd_round_box(v3 p, v3 s, T r) {
+  v3 q = V(t_fabs(p.x) - s.x + r, t_fabs(p.y) - s.y + r, t_fabs(p.z) - s.z + r);
+  return vlen(vmax0(q)) + t_min(t_max(q.x, t_max(q.y, q.z)), 0) - r;
+}
+static T sd_capsule(v3 p, v3 a, v3 b, T r) {
+  v3 pa = vsub(p, a), ba = vsub(b, a);
+  T h = t_clamp(vdot(pa, ba) / vdot(ba, ba), 0, 1);
+  return vlen(vsub(pa, vmuls(ba, h))) - r;
+}
+
+static T eval_local(uint32_t kind, v3 p, v3 size, T radius) {
+  switch (kind) {
+    case 0: return vlen(p) - size.x;                                  /* sphere */
+    case 1: return sd_box(p, size);                                   /* box */
+    case 2: return sd_round_box(p, size, radius);                     /* roundBox */
+    case 3: return sd_capsule(p, V(0, -size.y, 0), V(0, size.y, 0), radius);
+    case 4: {                                                         /* torus */
+      T qx = t_sqrt(p.x * p.x + p.z * p.z) - size.x, qy = p.y;
+      return t_sqrt(qx * qx + qy * qy) - radius;
+    }
+    case 5: {                                                         /* cylinder */
+      T dx = t_sqrt(p.x * p.x + p.z * p.z) - size.x, dy = t_fabs(p.y) - size.y;
+      T mx = t_max(dx, 0), my = t_max(dy, 0);
+      return t_min(t_max(dx, dy), 0) + t_sqrt(mx * mx + my * my);
+    }
+    case 6: return p.y - radius;                                      /* plane */
+    default: return (T)1e9;  /* kinds outside the collider slice read as empty */
+  }
+}
+
+/* evalBrush (brush.ts:466-471) with quatRotate/quatConj (math/gpu.ts:15-25). */
+static T eval_brush(const orc_brush* b, v3 p) {
+  T inv_scale = (T)1 / t_max((T)b->scale, KF(1e-5));
+  v3 q = V(-(T)b->rot[0], -(T)b->rot[1], -(T)b->rot[2]);
+  T qw = (T)b->rot[3];
+  v3 v = vsub(p, V(b->pos[0], b->pos[1], b->pos[2]));
+  v3 t = vmuls(vcross(q, v), 2);
+  v3 rot = vadd(vadd(v, vmuls(t, qw)), vcross(q, t));
+  v3 local = vmuls(rot, inv_scale);
+  return eval_local(b->kind, local, V(b->size[0], b->size[1], b->size[2]), (T)b->radius) *
+         (T)b->scale;
+}
+
+static T smoothstep(T lo, T hi, T x) {
+  T t = t_clamp((x - lo) / (hi - lo), 0, 1);
+  return t * t * ((T)3 - (T)2 * t);
+}
+
+/* applyBrush (brush.ts:474-511), sminPolyWeighted/smaxPolyWeighted (math/gpu.ts:87-92,132-137). */
+static void apply_brush(T acc[2], const orc_brush* b, v3 p) {
+  T dist = eval_brush(b, p);
+  T keep = (T)b->mask >= 0 ? (T)1 - smoothstep((T)0.25, (T)0.75, t_fabs(acc[1] - (T)b->mask)) : (T)1;
+  T kk = t_max((T)b->blend, KF(1e-6));
+  if (b->op == 0) {
+    T h = t_clamp((T)0.5 + (T)0.5 * (acc[0] - dist) / kk, 0, 1);
+    T rx = t_mix(acc[0], dist, h) - kk * h * ((T)1 - h);
+    T ry = h;
+    T nx = rx, ny = t_mix(acc[1], (T)b->material, ry);
+    acc[0] = t_mix(acc[0], nx, keep);
+    acc[1] = t_mix(acc[1], ny, keep);
+  } else if (b->op == 1) {
+    T a = -dist;
+    T h = t_clamp((T)0.5 - (T)0.5 * (acc[0] - a) / kk, 0, 1);
+    T rx = t_mix(acc[0], a, h) + kk * h * ((T)1 - h);
+    T ry = h;
+    T nx = rx, ny = t_mix(acc[1], (T)b->material, ry * KF(0.85));
+    acc[0] = t_mix(acc[0], nx, keep);
+    acc[1] = t_mix(acc[1], ny, keep);
+  } else {
+    T w = (T)1 - smoothstep(-t_max((T)b->blend, KF(1e-4)), 0, dist);
+    acc[1] = t_mix(acc[1], (T)b->material, w * keep);
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * Collider (lib/src/trace/field.ts).  analyticField: sample/normal
+ * (field.ts:196-236), the closure being a brush fold from (1e9, 0).
+ * volumeField over SdfVolume (volume.ts:262-313, field.ts:102-156): trilinear
+ * over f16 texels with clamp-to-edge.  The fold's filter weights are full f32
+ * (hardware filter weights are implementation defined): parity unpinned
+ * against a WebGPU sampler; HIP parity is bit-exact by construction.
+ * ------------------------------------------------------------------------- */
+static T analytic_dist(const orc_leaf* l, v3 p) {
+  T acc[2] = {(T)1e9, 0};
+  for (int i = 0; i < l->nbrush; i++) apply_brush(acc, &l->brushes[i], p);
+  return acc[0];
+}
+
+typedef struct {
+  T origin[3], extent, voxel, band, center[3], half, grad_h;
+  int res;
+} volp;
+
+static volp vol_params(const orc_leaf* l) {
+  volp v;
+  double half = l->world_size / 2;
+  for (int c = 0; c < 3; c++) {
+    v.origin[c] = rnd(l->origin[c]);
+    v.center[c] = rnd(l->origin[c] + half);
+  }
+  v.extent = rnd(l->world_size);
+  v.voxel = rnd(l->world_size / l->res);
+  v.band = rnd(l->band);
+  v.half = rnd(half);
+  v.grad_h = v.voxel * rnd(l->grad_voxels);
+  v.res = l->res;
+  return v;
+}
+
+static T texel(const orc_leaf* l, int x, int y, int z) {
+  const int n = l->res;
+  x = x < 0 ? 0 : (x >= n ? n - 1 : x);
+  y = y < 0 ? 0 : (y >= n ? n - 1 : y);
+  z = z < 0 ? 0 : (z >= n ? n - 1 : z);
+  return (T)orc_f16_to_f32(l->dist16[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)]);
+}
+
+static T sample_raw(const orc_leaf* l, const volp* v, v3 p) {
+  T u = (p.x - v->origin[0]) / v->extent;
+  T w = (p.y - v->origin[1]) / v->extent;
+  T s = (p.z - v->origin[2]) / v->extent;
+  T fn = (T)v->res;
+  T tx = u * fn - (T)0.5, ty = w * fn - (T)0.5, tz = s * fn - (T)0.5;
+  T fx0 = t_floor(tx), fy0 = t_floor(ty), fz0 = t_floor(tz);
+  int x0 = sat_i32(fx0), y0 = sat_i32(fy0), z0 = sat_i32(fz0);
+  T fx = tx - fx0, fy = ty - fy0, fz = tz - fz0;
+  T c00 = t_mix(texel(l, x0, y0, z0), texel(l, x0 + 1, y0, z0), fx);
+  T c10 = t_mix(texel(l, x0, y0 + 1, z0), texel(l, x0 + 1, y0 + 1, z0), fx);
+  T c01 = t_mix(texel(l, x0, y0, z0 + 1), texel(l, x0 + 1, y0, z0 + 1), fx);
+  T c11 = t_mix(texel(l, x0, y0 + 1, z0 + 1), texel(l, x0 + 1, y0 + 1, z0 + 1), fx);
+  T c0 = t_mix(c00, c10, fy);
+  T c1 = t_mix(c01, c11, fy);
+  return t_mix(c0, c1, fz);
+}
+
+/* SdfVolume.sampleDist at mip 0 (volume.ts:277-286); equals volumeField.sample(p,0).x. */
+static T vol_dist(const orc_leaf* l, const volp* v, v3 p) {
+  v3 rel = V(p.x - v->center[0], p.y - v->center[1], p.z - v->center[2]);
+  T dbox = sd_box(rel, V(v->half, v->half, v->half));
+  if (dbox > 0) return dbox;
+  T band_world = v->band * v->voxel * (T)1;  /* band * voxelSize * exp2(0) */
+  return sample_raw(l, v, p) * band_world;
+}
+
+static T leaf_dist(const orc_leaf* l, v3 p) {
+  if (l->kind == 0) return analytic_dist(l, p);
+  volp v = vol_params(l);
+  return vol_dist(l, &v, p);
+}
+
+static v3 leaf_normal(const orc_leaf* l, v3 p) {
+  if (l->kind == 0) {
+    T e = rnd(l->eps);
+    v3 g = V(analytic_dist(l, V(p.x + e, p.y, p.z)) - analytic_dist(l, V(p.x - e, p.y, p.z)),
+             analytic_dist(l, V(p.x, p.y + e, p.z)) - analytic_dist(l, V(p.x, p.y - e, p.z)),
+             analytic_dist(l, V(p.x, p.y, p.z + e)) - analytic_dist(l, V(p.x, p.y, p.z - e)));
+    T len = vlen(g);
+    return len > KF(1e-9) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
+  }
+  volp v = vol_params(l);
+  T h = v.grad_h;
+  v3 g = V(vol_dist(l, &v, V(p.x + h, p.y, p.z)) - vol_dist(l, &v, V(p.x - h, p.y, p.z)),
+           vol_dist(l, &v, V(p.x, p.y + h, p.z)) - vol_dist(l, &v, V(p.x, p.y - h, p.z)),
+           vol_dist(l, &v, V(p.x, p.y, p.z + h)) - vol_dist(l, &v, V(p.x, p.y, p.z - h)));
+  T len = vlen(g);
+  return len > KF(1e-8) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
+}
+
+/* unionField fold: keep the left child unless the right one is strictly nearer
+ * (field.ts:264-313: select(sb, sa, sa.x < sb.x)). */
+static int collider_pick(const orc_collider* c, v3 p, T* dist) {
+  int best = 0;
+  T bd = leaf_dist(&c->leaf[0], p);
+  for (int i = 1; i < c->nleaf; i++) {
+    T d = leaf_dist(&c->leaf[i], p);
+    if (!(bd < d)) {
+      best = i;
+      bd = d;
+    }
+  }
+  *dist = bd;
+  return best;
+}
+
+void FN(orc_collider_eval)(const orc_collider* col, int n, const T* p3, T* dist, T* nrm3) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    v3 p = V(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
+    T d;
+    int leaf = collider_pick(col, p, &d);
+    v3 nn = leaf_normal(&col->leaf[leaf], p);
+    dist[i] = d;
+    nrm3[3 * i] = nn.x;
+    nrm3[3 * i + 1] = nn.y;
+    nrm3[3 * i + 2] = nn.z;
+  }
+}
+
+/* ---------------------------------------------------------------------------
+ * FluidSim hash grid (fluid.ts:239-266, 308-331).  cellOf = floor((p - o) * (1/h))
+ * as vec3i; inGrid = all in [0, R); key = x + R (y + R z).  The bucket keeps the
+ * first `bucket` ids of each cell in ascending id order (one admissible outcome
+ * of the reference's atomicAdd order).  Completeness of the 3x3x3 scan is pinned
+ * by fluid.test.mjs:30-65 (ported with its LCG in tests).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  T go[3], inv_h;
+  int R, bucket;
+  T hh, mass_poly6, rest, rest_floor;
+  T h, h6, mass, spiky;
+  T g[3], dt, vmax, stiff, visc, damp, cdepth;
+  T radius, lo[3], hi[3];
+} kk_t;
+
+static kk_t kconv(const orc_fluid_k* k) {
+  kk_t c;
+  for (int i = 0; i < 3; i++) {
+    c.go[i] = rnd(k->grid_origin[i]);
+    c.g[i] = rnd(k->gravity[i]);
+    c.lo[i] = rnd(k->wall_lo[i]);
+    c.hi[i] = rnd(k->wall_hi[i]);
+  }
+  c.inv_h = rnd(k->inv_h);
+  c.R = k->grid_res;
+  c.bucket = k->bucket;
+  c.hh = rnd(k->hh);
+  c.mass_poly6 = rnd(k->mass_poly6);
+  c.rest = rnd(k->rest_density);
+  c.rest_floor = rnd(k->rest_floor);
+  c.h = rnd(k->h);
+  c.h6 = rnd(k->h6);
+  c.mass = rnd(k->mass);
+  c.spiky = rnd(k->spiky_grad);
+  c.dt = rnd(k->dt);
+  c.vmax = rnd(k->max_speed);
+  c.stiff = rnd(k->stiffness);
+  c.visc = rnd(k->viscosity);
+  c.damp = rnd(k->damping);
+  c.cdepth = rnd(k->contact_depth);
+  c.radius = rnd(k->radius);
+  return c;
+}
+
+static inline void cell_of(const kk_t* c, v3 p, int32_t q[3]) {
+  q[0] = sat_i32(t_floor((p.x - c->go[0]) * c->inv_h));
+  q[1] = sat_i32(t_floor((p.y - c->go[1]) * c->inv_h));
+  q[2] = sat_i32(t_floor((p.z - c->go[2]) * c->inv_h));
+}
+static inline int in_grid(const kk_t* c, const int32_t q[3]) {
+  return q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < c->R && q[1] < c->R && q[2] < c->R;
+}
+static inline uint32_t cell_index(const kk_t* c, const int32_t q[3]) {
+  return (uint32_t)q[0] + (uint32_t)c->R * ((uint32_t)q[1] + (uint32_t)c->R * (uint32_t)q[2]);
+}
+
+typedef struct {
+  int n;
+  uint32_t* perm;   /* sorted slot -> particle id */
+  uint32_t* skeys;  /* sorted keys */
+} grid_t;
+
+static void grid_build(const kk_t* c, int n, const T* px, const T* py, const T* pz,
+                       uint32_t* keys, grid_t* g) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    int32_t q[3];
+    cell_of(c, V(px[i], py[i], pz[i]), q);
+    keys[i] = in_grid(c, q) ? cell_index(c, q) : ORC_INVALID_KEY;
+  }
+  g->n = n;
+  g->perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  g->skeys = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  orc_stable_sort_keys(n, keys, g->perm);
+  for (int i = 0; i < n; i++) g->skeys[i] = keys[g->perm[i]];
+}
+static void grid_free(grid_t* g) {
+  free(g->perm);
+  free(g->skeys);
+}
+/* First sorted slot with key >= k. */
+static int lower(const grid_t* g, uint32_t k) {
+  int lo = 0, hi = g->n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (g->skeys[mid] < k) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+static inline void cell_range(const kk_t* c, const grid_t* g, uint32_t key, int* s, int* e) {
+  int a = lower(g, key);
+  int b = a;
+  while (b < g->n && g->skeys[b] == key) b++;
+  if (b - a > c->bucket) b = a + c->bucket;
+  *s = a;
+  *e = b;
+}
+
+void FN(orc_keys)(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz,
+                  uint32_t* keys) {
+  kk_t c = kconv(k);
+  for (int i = 0; i < n; i++) {
+    int32_t q[3];
+    cell_of(&c, V(px[i], py[i], pz[i]), q);
+    keys[i] = in_grid(&c, q) ? cell_index(&c, q) : ORC_INVALID_KEY;
+  }
+}
+
+#define ST(f) ((T*)st->f)
+
+/* density pass (fluid.ts:333-368).  Summation order: dz, dy, dx ascending, then
+ * the cell's ids ascending.  pow(t,3) as (t*t)*t: parity unpinned against WGSL
+ * pow (exp2/log2 lowering); the lattice-at-rest identity rho = rho0 is checked. */
+static void density_core(const kk_t* c, const grid_t* g, int n, orc_state* st) {
+  const T* px = ST(px); const T* py = ST(py); const T* pz = ST(pz);
+  T* rho_out = ST(rho); T* pres_out = ST(pres);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; i++) {
+    v3 pi = V(px[i], py[i], pz[i]);
+    int32_t base[3];
+    cell_of(c, pi, base);
+    T rho = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int32_t q[3] = {(int32_t)((uint32_t)base[0] + (uint32_t)dx),
+                          (int32_t)((uint32_t)base[1] + (uint32_t)dy),
+                          (int32_t)((uint32_t)base[2] + (uint32_t)dz)};
+          if (!in_grid(c, q)) continue;
+          int s, e;
+          cell_range(c, g, cell_index(c, q), &s, &e);
+          for (int t = s; t < e; t++) {
+            uint32_t j = g->perm[t];
+            v3 d = V(pi.x - px[j], pi.y - py[j], pi.z - pz[j]);
+            T r2 = vdot(d, d);
+            if (r2 < c->hh) {
+              T w = c->hh - r2;
+              rho = rho + c->mass_poly6 * (w * w * w);
+            }
+          }
+        }
+    rho_out[i] = t_max(rho, c->rest_floor);
+    pres_out[i] = t_max(c->stiff * (rho - c->rest), 0);
+  }
+}
+
+/* forces pass (fluid.ts:370-439): spiky pressure gradient and XSPH blend. */
+static void forces_core(const kk_t* c, const grid_t* g, int n, orc_state* st) {
+  const T* px = ST(px); const T* py = ST(py); const T* pz = ST(pz);
+  const T* vx = ST(vx); const T* vy = ST(vy); const T* vz = ST(vz);
+  const T* rho = ST(rho); const T* pres = ST(pres);
+  T* nvx = ST(nvx); T* nvy = ST(nvy); T* nvz = ST(nvz);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; i++) {
+    v3 pi = V(px[i], py[i], pz[i]);
+    v3 vi = V(vx[i], vy[i], vz[i]);
+    T rhoi = rho[i], pri = pres[i];
+    int32_t base[3];
+    cell_of(c, pi, base);
+    v3 fp = V(0, 0, 0);
+    v3 vsum = vmuls(vi, c->h6);
+    T wsum = c->h6;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int32_t q[3] = {(int32_t)((uint32_t)base[0] + (uint32_t)dx),
+                          (int32_t)((uint32_t)base[1] + (uint32_t)dy),
+                          (int32_t)((uint32_t)base[2] + (uint32_t)dz)};
+          if (!in_grid(c, q)) continue;
+          int s, e;
+          cell_range(c, g, cell_index(c, q), &s, &e);
+          for (int t = s; t < e; t++) {
+            uint32_t j = g->perm[t];
+            if ((int)j == i) continue;
+            v3 dv = V(pi.x - px[j], pi.y - py[j], pi.z - pz[j]);
+            T r2 = vdot(dv, dv);
+            if (r2 >= c->hh) continue;
+            T inv = (T)1 / rho[j];
+            T w0 = c->hh - r2;
+            T w = w0 * w0 * w0;
+            vsum = V(vsum.x + vx[j] * w, vsum.y + vy[j] * w, vsum.z + vz[j] * w);
+            wsum = wsum + w;
+            T r = t_sqrt(r2);
+            if (r > KF(1e-6)) {
+              T hr = c->h - r;
+              T s2 = c->mass * (pri + pres[j]) * (T)0.5 * inv * c->spiky * (hr * hr);
+              T ir = (T)1 / r;
+              fp = V(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+            }
+          }
+        }
+    T irho = (T)1 / rhoi;
+    v3 acc = V(fp.x * irho + c->g[0], fp.y * irho + c->g[1], fp.z * irho + c->g[2]);
+    T t = t_clamp(c->visc, 0, 1);
+    T iw = (T)1 / wsum;
+    v3 vs = vmuls(vsum, iw);
+    v3 v = V(t_mix(vi.x, vs.x, t), t_mix(vi.y, vs.y, t), t_mix(vi.z, vs.z, t));
+    v = vmuls(V(v.x + acc.x * c->dt, v.y + acc.y * c->dt, v.z + acc.z * c->dt), c->damp);
+    T speed = vlen(v);
+    if (speed > c->vmax) v = vmuls(v, c->vmax / speed);
+    nvx[i] = v.x;
+    nvy[i] = v.y;
+    nvz[i] = v.z;
+  }
+}
+
+/* integrate pass (fluid.ts:449-491): advance, collide against the world field,
+ * record contacts (<= 32, counter keeps counting), clamp to the walls. */
+static void integrate_core(const kk_t* c, const orc_collider* col, int n, orc_state* st,
+                           orc_contacts* contacts) {
+  T* px = ST(px); T* py = ST(py); T* pz = ST(pz);
+  T* vx = ST(vx); T* vy = ST(vy); T* vz = ST(vz);
+  const T* nvx = ST(nvx); const T* nvy = ST(nvy); const T* nvz = ST(nvz);
+  unsigned char* hit = (unsigned char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  T* hp = (T*)malloc(sizeof(T) * 4 * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    v3 v = V(nvx[i], nvy[i], nvz[i]);
+    v3 p = V(px[i] + v.x * c->dt, py[i] + v.y * c->dt, pz[i] + v.z * c->dt);
+    if (col && col->nleaf > 0) {
+      T dw;
+      int leaf = collider_pick(col, p, &dw);
+      if (dw < c->radius) {
+        v3 nr = leaf_normal(&col->leaf[leaf], p);
+        T push = c->radius - dw;
+        p = V(p.x + nr.x * push, p.y + nr.y * push, p.z + nr.z * push);
+        T vn = t_min(vdot(v, nr), 0);
+        v = V(v.x - nr.x * vn, v.y - nr.y * vn, v.z - nr.z * vn);
+        if (dw < c->cdepth) {
+          hit[i] = 1;
+          hp[4 * i] = p.x;
+          hp[4 * i + 1] = p.y;
+          hp[4 * i + 2] = p.z;
+          hp[4 * i + 3] = vlen(v);
+        }
+      }
+    }
+    T cl[3] = {t_min(t_max(p.x, c->lo[0]), c->hi[0]), t_min(t_max(p.y, c->lo[1]), c->hi[1]),
+               t_min(t_max(p.z, c->lo[2]), c->hi[2])};
+    vx[i] = cl[0] == p.x ? v.x : 0;
+    vy[i] = cl[1] == p.y ? v.y : 0;
+    vz[i] = cl[2] == p.z ? v.z : 0;
+    px[i] = cl[0];
+    py[i] = cl[1];
+    pz[i] = cl[2];
+  }
+  for (int i = 0; i < n; i++) {
+    if (!hit[i]) continue;
+    uint32_t slot = contacts->count++;
+    if (slot < ORC_CONTACTS)
+      for (int q = 0; q < 4; q++) contacts->points[slot][q] = (double)hp[4 * i + q];
+  }
+  free(hit);
+  free(hp);
+}
+
+void FN(orc_density)(const orc_fluid_k* k, int n, orc_state* st) {
+  kk_t c = kconv(k);
+  grid_t g;
+  uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  grid_build(&c, n, ST(px), ST(py), ST(pz), keys, &g);
+  density_core(&c, &g, n, st);
+  grid_free(&g);
+  free(keys);
+}
+
+void FN(orc_forces)(const orc_fluid_k* k, int n, orc_state* st) {
+  kk_t c = kconv(k);
+  grid_t g;
+  uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  grid_build(&c, n, ST(px), ST(py), ST(pz), keys, &g);
+  forces_core(&c, &g, n, st);
+  grid_free(&g);
+  free(keys);
+}
+
+void FN(orc_integrate)(const orc_fluid_k* k, const orc_collider* col, int n, orc_state* st,
+                       orc_contacts* contacts) {
+  kk_t c = kconv(k);
+  contacts->count = 0;
+  integrate_core(&c, col, n, st, contacts);
+}
+
+/* One substep: clearGrid + insert + density + forces + integrate (fluid.ts:550-556). */
+void FN(orc_substep)(const orc_fluid_k* k, const orc_collider* col, int n, orc_state* st,
+                     orc_contacts* contacts, uint32_t* keys_out, uint32_t* perm_out) {
+  kk_t c = kconv(k);
+  grid_t g;
+  uint32_t* keys = keys_out ? keys_out : (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  grid_build(&c, n, ST(px), ST(py), ST(pz), keys, &g);
+  if (perm_out) memcpy(perm_out, g.perm, sizeof(uint32_t) * (size_t)n);
+  contacts->count = 0;
+  density_core(&c, &g, n, st);
+  forces_core(&c, &g, n, st);
+  integrate_core(&c, col, n, st, contacts);
+  grid_free(&g);
+  if (!keys_out) free(keys);
+}
+
+/* ---------------------------------------------------------------------------
+ * SplatField bake (lib/src/sim/splat.ts:186-304): clear to 65535, per mip splat
+ * with atomicMin of the monotone code, resolve to rgba16float.  The voxel box
+ * [floor((p-reach)/v), floor((p+reach)/v)] is pinned by fluid.test.mjs:67-91
+ * (ported); the grid against a brute-force min over particles in tests.
+ * ------------------------------------------------------------------------- */
+static void atomic_min_u32(uint32_t* a, uint32_t v) {
+  uint32_t cur = __atomic_load_n(a, __ATOMIC_RELAXED);
+  while (v < cur && !__atomic_compare_exchange_n(a, &cur, v, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+  }
+}
+
+void FN(orc_splat_bake)(const orc_splat_opts* o, int n, const T* px, const T* py, const T* pz,
+                        const T* nx, const T* ny, const T* nz, int live_first, int live_count,
+                        uint32_t* grid) {
+  size_t total = 0;
+  for (int m = 0; m < o->mips; m++) {
+    size_t r = (size_t)(o->res >> m);
+    total += r * r * r;
+  }
+  for (size_t i = 0; i < total; i++) grid[i] = 65535u;
+  const double voxel0 = o->world_size / o->res;
+  const T org[3] = {rnd(o->origin[0]), rnd(o->origin[1]), rnd(o->origin[2])};
+  const T rad = rnd(o->radius);
+  int first = live_first < 0 ? 0 : live_first;
+  int last = live_first + live_count;
+  if (last > n) last = n;
+  size_t off = 0;
+  for (int m = 0; m < o->mips; m++) {
+    const int r = o->res >> m;
+    const double voxel = voxel0 * (double)(1 << m);
+    const double bw = o->band * voxel;
+    const T tv = rnd(voxel), tbw = rnd(bw), reach = rnd(o->radius + bw), inv = rnd(1.0 / voxel);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = first; i < last; i++) {
+      v3 p = V(px[i] - org[0], py[i] - org[1], pz[i] - org[2]);
+      int32_t lo[3] = {sat_i32(t_floor((p.x - reach) * inv)), sat_i32(t_floor((p.y - reach) * inv)),
+                       sat_i32(t_floor((p.z - reach) * inv))};
+      int32_t hi[3] = {sat_i32(t_floor((p.x + reach) * inv)), sat_i32(t_floor((p.y + reach) * inv)),
+                       sat_i32(t_floor((p.z + reach) * inv))};
+      int32_t z0 = lo[2] > 0 ? lo[2] : 0, z1 = hi[2] < r - 1 ? hi[2] : r - 1;
+      int32_t y0 = lo[1] > 0 ? lo[1] : 0, y1 = hi[1] < r - 1 ? hi[1] : r - 1;
+      int32_t x0 = lo[0] > 0 ? lo[0] : 0, x1 = hi[0] < r - 1 ? hi[0] : r - 1;
+      for (int32_t z = z0; z <= z1; z++)
+        for (int32_t y = y0; y <= y1; y++)
+          for (int32_t x = x0; x <= x1; x++) {
+            v3 cc = V(((T)x + (T)0.5) * tv, ((T)y + (T)0.5) * tv, ((T)z + (T)0.5) * tv);
+            v3 d = vsub(cc, p);
+            T sd = vlen(d) - rad;
+            if (nx) {
+              T pl = d.x * nx[i] + d.y * ny[i] + d.z * nz[i];
+              sd = t_max(pl, sd);
+            }
+            if (sd < tbw) {
+              T q = (t_clamp(sd / tbw, -1, 1) + 1) * (T)32767.5;
+              uint32_t e = (uint32_t)q;
+              size_t idx = off + (size_t)x + (size_t)r * ((size_t)y + (size_t)r * (size_t)z);
+              atomic_min_u32(&grid[idx], e);
+            }
+          }
+    }
+    off += (size_t)r * r * r;
+  }
+}
+
+void FN(orc_splat_resolve)(const orc_splat_opts* o, const uint32_t* grid, uint16_t* texels) {
+  size_t off = 0;
+  const T scale = rnd(1.0 / 32767.5);
+  const uint16_t mat = orc_f32_to_f16((float)rnd(o->material));
+  const uint16_t one = orc_f32_to_f16(1.0f);
+  for (int m = 0; m < o->mips; m++) {
+    const size_t r = (size_t)(o->res >> m);
+    const size_t cells = r * r * r;
+    for (size_t i = 0; i < cells; i++) {
+      T v = (T)grid[off + i] * scale - (T)1;
+      texels[4 * (off + i)] = orc_f32_to_f16((float)v);
+      texels[4 * (off + i) + 1] = mat;
+      texels[4 * (off + i) + 2] = 0;
+      texels[4 * (off + i) + 3] = one;
+    }
+    off += cells;
+  }
+}
+
+/* SdfBuilder.gen (lib/src/field/builder.ts:87-142): per-tile brush cull (<= 64,
+ * in brush order) then the fold from (4 * bandWorld, 0) at every voxel centre,
+ * stored as clamp(d / bandWorld, -1, 1) and material. */
+void FN(orc_bake_brushes)(int res, int mips, double world_size, const double* origin,
+                          double band, const orc_brush* b, int nb, uint16_t* texels) {
+  const double voxel0 = world_size / res;
+  const uint16_t one = orc_f32_to_f16(1.0f);
+  size_t off = 0;
+  for (int m = 0; m < mips; m++) {
+    const int r = res >> m;
+    const int tiles = r / 8;
+    const double voxel = voxel0 * (double)(1 << m);
+    const double tile_size = voxel * 8;
+    const double band_world = band * voxel;
+    const T tv = rnd(voxel), tts = rnd(tile_size), thalf = rnd(tile_size / 2),
+            tbw = rnd(band_world), empty = rnd(band_world * 4);
+    const T org[3] = {rnd(origin[0]), rnd(origin[1]), rnd(origin[2])};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < tiles * tiles * tiles; t++) {
+      const int wx = t % tiles, wy = (t / tiles) % tiles, wz = t / (tiles * tiles);
+      v3 tmin = V(org[0] + (T)wx * tts, org[1] + (T)wy * tts, org[2] + (T)wz * tts);
+      v3 tc = V(tmin.x + thalf, tmin.y + thalf, tmin.z + thalf);
+      int ids[64];
+      int cnt = 0;
+      for (int i = 0; i < nb; i++) {
+        v3 bp = V(b[i].pos[0], b[i].pos[1], b[i].pos[2]);
+        v3 nr = vmax0(V(t_fabs(bp.x - tc.x) - thalf, t_fabs(bp.y - tc.y) - thalf, t_fabs(bp.z - tc.z) - thalf));
+        if (vlen(nr) < (T)b[i].bound + tbw && cnt < 64) ids[cnt++] = i;
+      }
+      for (int lz = 0; lz < 8; lz++)
+        for (int ly = 0; ly < 8; ly++)
+          for (int lx = 0; lx < 8; lx++) {
+            v3 p = V(tmin.x + ((T)lx + (T)0.5) * tv, tmin.y + ((T)ly + (T)0.5) * tv,
+                     tmin.z + ((T)lz + (T)0.5) * tv);
+            T acc[2] = {empty, 0};
+            for (int i = 0; i < cnt; i++) apply_brush(acc, &b[ids[i]], p);
+            const size_t x = (size_t)(wx * 8 + lx), y = (size_t)(wy * 8 + ly), z = (size_t)(wz * 8 + lz);
+            const size_t idx = off + x + (size_t)r * (y + (size_t)r * z);
+            texels[4 * idx] = orc_f32_to_f16((float)t_clamp(acc[0] / tbw, -1, 1));
+            texels[4 * idx + 1] = orc_f32_to_f16((float)acc[1]);
+            texels[4 * idx + 2] = 0;
+            texels[4 * idx + 3] = one;
+          }
+    }
+    off += (size_t)r * r * r;
+  }
+}

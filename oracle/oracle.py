@@ -1,0 +1,396 @@
+"""ctypes wrapper of the CPU oracle (oracle/libclay_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, bench.py's cpu_baseline leg and
+__graft_entry__.smoke() as the checker.  The product (clay-engine_amd/) never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libclay_oracle.so"
+INVALID_KEY = 0xFFFFFFFF
+CONTACTS = 32
+
+
+def build() -> Path:
+    res = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB
+
+
+class OrcBrush(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3), ("scale", C.c_float), ("rot", C.c_float * 4), ("size", C.c_float * 3),
+        ("radius", C.c_float), ("material", C.c_float), ("blend", C.c_float), ("bound", C.c_float),
+        ("mask", C.c_float), ("kind", C.c_uint32), ("op", C.c_uint32), ("slot", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+class OrcFluidOpts(C.Structure):
+    _fields_ = [
+        ("spacing", C.c_double), ("smoothing", C.c_double), ("density", C.c_double), ("radius", C.c_double),
+        ("bucket", C.c_int32), ("substeps", C.c_int32), ("origin", C.c_double * 3), ("world_size", C.c_double),
+        ("gravity", C.c_double * 3), ("stiffness", C.c_double), ("viscosity", C.c_double),
+        ("damping", C.c_double), ("max_speed", C.c_double),
+    ]
+
+
+class OrcFluidK(C.Structure):
+    _fields_ = [
+        ("h", C.c_double), ("mass", C.c_double), ("poly6", C.c_double), ("spiky_grad", C.c_double),
+        ("rest_density", C.c_double), ("dt", C.c_double), ("radius", C.c_double), ("contact_depth", C.c_double),
+        ("grid_origin", C.c_double * 3), ("grid_res", C.c_int32), ("bucket", C.c_int32), ("substeps", C.c_int32),
+        ("pad0", C.c_int32), ("wall_lo", C.c_double * 3), ("wall_hi", C.c_double * 3), ("gravity", C.c_double * 3),
+        ("stiffness", C.c_double), ("viscosity", C.c_double), ("damping", C.c_double), ("max_speed", C.c_double),
+        ("hh", C.c_double), ("mass_poly6", C.c_double), ("h6", C.c_double), ("inv_h", C.c_double),
+        ("rest_floor", C.c_double),
+    ]
+
+
+class OrcLeaf(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("nbrush", C.c_int32), ("brushes", C.c_void_p), ("eps", C.c_double),
+        ("dist16", C.c_void_p), ("res", C.c_int32), ("pad1", C.c_int32), ("origin", C.c_double * 3),
+        ("world_size", C.c_double), ("band", C.c_double), ("grad_voxels", C.c_double),
+    ]
+
+
+class OrcCollider(C.Structure):
+    _fields_ = [("nleaf", C.c_int32), ("pad", C.c_int32), ("leaf", OrcLeaf * 4)]
+
+
+class OrcSplatOpts(C.Structure):
+    _fields_ = [
+        ("res", C.c_int32), ("mips", C.c_int32), ("world_size", C.c_double), ("origin", C.c_double * 3),
+        ("band", C.c_double), ("radius", C.c_double), ("material", C.c_double),
+    ]
+
+
+class OrcState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("px", "py", "pz", "vx", "vy", "vz", "rho", "pres", "nvx", "nvy", "nvz")]
+
+
+class OrcContacts(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("pad", C.c_uint32), ("points", (C.c_double * 4) * CONTACTS)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.orc_hash_u32.restype = C.c_uint32
+        L.orc_hash_u32.argtypes = [C.c_uint32]
+        L.orc_hash1.restype = C.c_float
+        L.orc_hash1.argtypes = [C.c_uint32]
+        L.orc_f32_to_f16.restype = C.c_uint16
+        L.orc_f32_to_f16.argtypes = [C.c_float]
+        L.orc_f16_to_f32.restype = C.c_float
+        L.orc_f16_to_f32.argtypes = [C.c_uint16]
+        L.orc_enc_f32.restype = C.c_uint32
+        L.orc_enc_f32.argtypes = [C.c_float]
+        L.orc_dec_f32.restype = C.c_float
+        L.orc_dec_f32.argtypes = [C.c_uint32]
+        L.orc_pack_tile.restype = C.c_uint32
+        L.orc_pack_tile.argtypes = [C.c_uint32] * 3
+        L.orc_volume_derive.restype = C.c_int
+        L.orc_volume_derive.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 5 + [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_unpack_tile.argtypes = [C.c_uint32, C.c_void_p]
+        L.orc_fluid_derive.argtypes = [C.POINTER(OrcFluidOpts), C.POINTER(OrcFluidK)]
+        L.orc_stable_sort_keys.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        P, I, K = C.c_void_p, C.c_int, C.POINTER(OrcFluidK)
+        COL, ST, CT = C.POINTER(OrcCollider), C.POINTER(OrcState), C.POINTER(OrcContacts)
+        SO = C.POINTER(OrcSplatOpts)
+        for s in ("f32", "f64"):
+            getattr(L, f"orc_keys_{s}").argtypes = [K, I, P, P, P, P]
+            getattr(L, f"orc_substep_{s}").argtypes = [K, COL, I, ST, CT, P, P]
+            getattr(L, f"orc_density_{s}").argtypes = [K, I, ST]
+            getattr(L, f"orc_forces_{s}").argtypes = [K, I, ST]
+            getattr(L, f"orc_integrate_{s}").argtypes = [K, COL, I, ST, CT]
+            getattr(L, f"orc_collider_eval_{s}").argtypes = [COL, I, P, P, P]
+            getattr(L, f"orc_splat_bake_{s}").argtypes = [SO, I, P, P, P, P, P, P, I, I, P]
+            getattr(L, f"orc_splat_resolve_{s}").argtypes = [SO, P, P]
+            getattr(L, f"orc_bake_brushes_{s}").argtypes = [I, I, C.c_double, P, C.c_double, P, I, P]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def threads() -> int:
+    return lib().orc_get_threads()
+
+
+# ---- scalar helpers ------------------------------------------------------------
+def hash_u32(x: int) -> int:
+    return lib().orc_hash_u32(x)
+
+
+def hash1(x: int) -> float:
+    return lib().orc_hash1(x)
+
+
+def enc(v: float) -> int:
+    return lib().orc_enc_f32(v)
+
+
+def dec(e: int) -> float:
+    return lib().orc_dec_f32(e)
+
+
+def f32_to_f16(v: float) -> int:
+    return lib().orc_f32_to_f16(v)
+
+
+def f16_to_f32(h: int) -> float:
+    return lib().orc_f16_to_f32(h)
+
+
+def pack_tile(x: int, y: int, z: int) -> int:
+    return lib().orc_pack_tile(x, y, z)
+
+
+def unpack_tile(v: int) -> tuple[int, int, int]:
+    out = (C.c_uint32 * 3)()
+    lib().orc_unpack_tile(C.c_uint32(v), out)
+    return tuple(out)
+
+
+def volume_derive(resolution: int, mips: int = 0):
+    """-> (mips, tile_res list, cell offsets, total cells) or raises ValueError(message)."""
+    m = C.c_int()
+    tr = (C.c_int * 32)()
+    co = (C.c_int * 32)()
+    tot = C.c_int()
+    err = C.create_string_buffer(256)
+    rc = lib().orc_volume_derive(resolution, mips, C.byref(m), tr, co, C.byref(tot), err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return m.value, list(tr[: m.value]), list(co[: m.value]), tot.value
+
+
+def stable_sort(keys: np.ndarray) -> np.ndarray:
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    out = np.empty(k.size, dtype=np.uint32)
+    lib().orc_stable_sort_keys(k.size, k.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+# ---- fluid ---------------------------------------------------------------------
+DEFAULTS = {
+    "spacing": 0.45, "smoothing": -1.0, "density": 1.0, "radius": -1.0, "bucket": 64, "substeps": 3,
+    "origin": (-12.0, -4.0, -12.0), "worldSize": 24.0, "gravity": (0.0, -9.81, 0.0), "stiffness": 220.0,
+    "viscosity": 0.4, "damping": 0.995, "maxSpeed": 15.0,
+}
+
+
+def fluid_consts(options: dict) -> OrcFluidK:
+    o = dict(DEFAULTS)
+    o.update({k: v for k, v in options.items() if k in DEFAULTS})
+    op = OrcFluidOpts()
+    op.spacing = o["spacing"]
+    op.smoothing = o["smoothing"]
+    op.density = o["density"]
+    op.radius = o["radius"]
+    op.bucket = int(o["bucket"])
+    op.substeps = int(o["substeps"])
+    for i in range(3):
+        op.origin[i] = o["origin"][i]
+        op.gravity[i] = o["gravity"][i]
+    op.world_size = o["worldSize"]
+    op.stiffness = o["stiffness"]
+    op.viscosity = o["viscosity"]
+    op.damping = o["damping"]
+    op.max_speed = o["maxSpeed"]
+    k = OrcFluidK()
+    lib().orc_fluid_derive(C.byref(op), C.byref(k))
+    return k
+
+
+class Collider:
+    """Union (left fold) of analytic brush folds and mip-0 volume planes."""
+
+    def __init__(self):
+        self.c = OrcCollider()
+        self._keep = []
+
+    def add_analytic(self, brush_records, eps: float = 1e-3) -> "Collider":
+        arr = (OrcBrush * len(brush_records))()
+        for i, b in enumerate(brush_records):
+            C.memmove(C.byref(arr[i]), C.byref(b), C.sizeof(OrcBrush))
+        self._keep.append(arr)
+        leaf = self.c.leaf[self.c.nleaf]
+        leaf.kind = 0
+        leaf.nbrush = len(brush_records)
+        leaf.brushes = C.cast(arr, C.c_void_p)
+        leaf.eps = eps
+        self.c.nleaf += 1
+        return self
+
+    def add_volume(self, dist16: np.ndarray, res: int, origin, world_size: float, band: float,
+                   grad_voxels: float = 1.0) -> "Collider":
+        d = np.ascontiguousarray(dist16, dtype=np.uint16).reshape(-1)
+        assert d.size == res ** 3
+        self._keep.append(d)
+        leaf = self.c.leaf[self.c.nleaf]
+        leaf.kind = 1
+        leaf.dist16 = d.ctypes.data
+        leaf.res = res
+        for i in range(3):
+            leaf.origin[i] = origin[i]
+        leaf.world_size = world_size
+        leaf.band = band
+        leaf.grad_voxels = grad_voxels
+        self.c.nleaf += 1
+        return self
+
+    def eval(self, points: np.ndarray, precision: str = "f32"):
+        dt = np.float32 if precision == "f32" else np.float64
+        p = np.ascontiguousarray(points, dtype=dt).reshape(-1, 3)
+        n = p.shape[0]
+        dist = np.empty(n, dtype=dt)
+        nrm = np.empty((n, 3), dtype=dt)
+        getattr(lib(), f"orc_collider_eval_{precision}")(C.byref(self.c), n, p.ctypes.data_as(C.c_void_p),
+                                                         dist.ctypes.data_as(C.c_void_p),
+                                                         nrm.ctypes.data_as(C.c_void_p))
+        return dist, nrm
+
+
+class Fluid:
+    """Oracle state of one FluidSim: SoA arrays in f32 ("WGSL-f32") or f64."""
+
+    FIELDS = ("px", "py", "pz", "vx", "vy", "vz", "rho", "pres", "nvx", "nvy", "nvz")
+
+    def __init__(self, options: dict, pos: np.ndarray, vel: np.ndarray | None = None,
+                 collider: Collider | None = None, precision: str = "f32"):
+        self.precision = precision
+        self.dt = np.float32 if precision == "f32" else np.float64
+        self.k = fluid_consts(options)
+        self.collider = collider
+        self.n = pos.shape[0]
+        self.a = {f: np.zeros(self.n, dtype=self.dt) for f in self.FIELDS}
+        for i, ax in enumerate("xyz"):
+            self.a["p" + ax][:] = pos[:, i]
+            if vel is not None:
+                self.a["v" + ax][:] = vel[:, i]
+        self.contacts = OrcContacts()
+        self.keys = np.zeros(self.n, dtype=np.uint32)
+        self.perm = np.zeros(self.n, dtype=np.uint32)
+
+    def _state(self) -> OrcState:
+        st = OrcState()
+        for f in self.FIELDS:
+            setattr(st, f, self.a[f].ctypes.data)
+        return st
+
+    def substep(self) -> None:
+        st = self._state()
+        col = C.byref(self.collider.c) if self.collider else None
+        getattr(lib(), f"orc_substep_{self.precision}")(
+            C.byref(self.k), col, self.n, C.byref(st), C.byref(self.contacts),
+            self.keys.ctypes.data_as(C.c_void_p), self.perm.ctypes.data_as(C.c_void_p))
+
+    def get(self, what: str) -> np.ndarray:
+        if what in ("pos", "vel", "nvel"):
+            pre = {"pos": "p", "vel": "v", "nvel": "nv"}[what]
+            return np.stack([self.a[pre + ax] for ax in "xyz"], axis=-1)
+        return self.a[{"density": "rho", "pressure": "pres"}[what]].copy()
+
+    def read_contacts(self):
+        n = min(self.contacts.count, CONTACTS)
+        return self.contacts.count, [tuple(self.contacts.points[i]) for i in range(n)]
+
+
+# ---- splat ---------------------------------------------------------------------
+def splat_opts(res: int, mips: int, world_size: float, origin, band: float, radius: float,
+               material: float = 0.0) -> OrcSplatOpts:
+    o = OrcSplatOpts()
+    o.res, o.mips, o.world_size = res, mips, world_size
+    for i in range(3):
+        o.origin[i] = origin[i]
+    o.band, o.radius, o.material = band, radius, material
+    return o
+
+
+def grid_sizes(res: int, mips: int) -> list[int]:
+    return [(res >> m) ** 3 for m in range(mips)]
+
+
+def splat_bake(o: OrcSplatOpts, pos: np.ndarray, live_first: int = 0, live_count: int | None = None,
+               normals: np.ndarray | None = None, precision: str = "f32") -> list[np.ndarray]:
+    """Per-mip u32 code grids, each shaped (r, r, r) indexed [z, y, x]."""
+    dt = np.float32 if precision == "f32" else np.float64
+    p = np.ascontiguousarray(np.asarray(pos, dtype=dt).T)
+    n = p.shape[1]
+    if live_count is None:
+        live_count = n
+    total = sum(grid_sizes(o.res, o.mips))
+    grid = np.empty(total, dtype=np.uint32)
+    nv = [None, None, None]
+    if normals is not None:
+        nn = np.ascontiguousarray(np.asarray(normals, dtype=dt).T)
+        nv = [nn[i].ctypes.data for i in range(3)]
+    getattr(lib(), f"orc_splat_bake_{precision}")(
+        C.byref(o), n, p[0].ctypes.data, p[1].ctypes.data, p[2].ctypes.data,
+        C.c_void_p(nv[0]), C.c_void_p(nv[1]), C.c_void_p(nv[2]), live_first, live_count, grid.ctypes.data)
+    out, off = [], 0
+    for m in range(o.mips):
+        r = o.res >> m
+        out.append(grid[off:off + r ** 3].reshape(r, r, r))
+        off += r ** 3
+    return out
+
+
+def splat_resolve(o: OrcSplatOpts, grids: list[np.ndarray], precision: str = "f32") -> list[np.ndarray]:
+    flat = np.concatenate([g.reshape(-1) for g in grids]).astype(np.uint32)
+    tex = np.empty(flat.size * 4, dtype=np.uint16)
+    getattr(lib(), f"orc_splat_resolve_{precision}")(C.byref(o), flat.ctypes.data, tex.ctypes.data)
+    out, off = [], 0
+    for m in range(o.mips):
+        r = o.res >> m
+        out.append(tex[4 * off:4 * (off + r ** 3)].reshape(r, r, r, 4))
+        off += r ** 3
+    return out
+
+
+def bake_brushes(res: int, mips: int, world_size: float, origin, band: float, brush_records,
+                 precision: str = "f32") -> list[np.ndarray]:
+    arr = (OrcBrush * max(1, len(brush_records)))()
+    for i, b in enumerate(brush_records):
+        C.memmove(C.byref(arr[i]), C.byref(b), C.sizeof(OrcBrush))
+    total = sum(grid_sizes(res, mips))
+    tex = np.empty(total * 4, dtype=np.uint16)
+    org = (C.c_double * 3)(*origin)
+    getattr(lib(), f"orc_bake_brushes_{precision}")(res, mips, C.c_double(world_size), org, C.c_double(band),
+                                                    arr, len(brush_records), tex.ctypes.data)
+    out, off = [], 0
+    for m in range(mips):
+        r = res >> m
+        out.append(tex[4 * off:4 * (off + r ** 3)].reshape(r, r, r, 4))
+        off += r ** 3
+    return out
+
+
+def keys(options: dict, pos: np.ndarray) -> np.ndarray:
+    k = fluid_consts(options)
+    p = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).T)
+    out = np.empty(p.shape[1], dtype=np.uint32)
+    lib().orc_keys_f32(C.byref(k), p.shape[1], p[0].ctypes.data, p[1].ctypes.data, p[2].ctypes.data,
+                       out.ctypes.data)
+    return out
