@@ -253,6 +253,13 @@ def lib() -> C.CDLL:
                 f"{LIB_PATH} is missing: build it with `python clay-engine_amd/build.py` "
                 "(there is no CPU fallback for the HIP path)",
             )
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7 under the
+        # same soname, and whichever copy is loaded first serves both.  Loading torch's
+        # first keeps torch (streams, events, tensors) and this library on one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(str(LIB_PATH))
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -294,7 +294,14 @@ __global__ void __launch_bounds__(kThreads)
 
   const bool has_n = c.nrm[0] != nullptr;
   const uint32_t s0 = start[b];
-  for (uint32_t e = t; e < n; e += kThreads) {
+  // Blocked assignment: the list is in spatial order, so lane t takes the t-th
+  // contiguous chunk.  Neighbouring lanes then splat particles that lie far apart
+  // and their ds_min_u32 hit different addresses, instead of 64 lanes walking
+  // nearly the same voxels in lockstep.
+  const uint32_t chunk = (n + kThreads - 1) / kThreads;
+  const uint32_t e0 = (uint32_t)t * chunk;
+  const uint32_t e1 = e0 + chunk < n ? e0 + chunk : n;
+  for (uint32_t e = e0; e < e1; e++) {
     const uint32_t i = list[s0 + e];
     const f3 p = load_rel(c, vol, i);
     const f3 nr = load_nrm(c, i);

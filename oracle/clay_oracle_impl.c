@@ -381,6 +381,39 @@ void FN(orc_keys)(const orc_fluid_k* k, int n, const T* px, const T* py, const T
   }
 }
 
+/* Ids the 27-cell scan visits for particle i with r2 < h^2 (the set the density
+ * pass sums over); returns how many, writing at most `cap`. */
+int FN(orc_neighbors)(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz, int i,
+                      uint32_t* out, int cap) {
+  kk_t c = kconv(k);
+  grid_t g;
+  uint32_t* keys = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  grid_build(&c, n, px, py, pz, keys, &g);
+  v3 pi = V(px[i], py[i], pz[i]);
+  int32_t base[3];
+  cell_of(&c, pi, base);
+  int found = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        int32_t q[3] = {base[0] + dx, base[1] + dy, base[2] + dz};
+        if (!in_grid(&c, q)) continue;
+        int s, e;
+        cell_range(&c, &g, cell_index(&c, q), &s, &e);
+        for (int t = s; t < e; t++) {
+          uint32_t j = g.perm[t];
+          v3 d = V(pi.x - px[j], pi.y - py[j], pi.z - pz[j]);
+          if (vdot(d, d) < c.hh) {
+            if (found < cap) out[found] = j;
+            found++;
+          }
+        }
+      }
+  grid_free(&g);
+  free(keys);
+  return found;
+}
+
 #define ST(f) ((T*)st->f)
 
 /* density pass (fluid.ts:333-368).  Summation order: dz, dy, dx ascending, then

@@ -118,6 +118,8 @@ def lib() -> C.CDLL:
         for s in ("f32", "f64"):
             getattr(L, f"orc_keys_{s}").argtypes = [K, I, P, P, P, P]
             getattr(L, f"orc_substep_{s}").argtypes = [K, COL, I, ST, CT, P, P]
+            getattr(L, f"orc_neighbors_{s}").argtypes = [K, I, P, P, P, I, P, I]
+            getattr(L, f"orc_neighbors_{s}").restype = C.c_int
             getattr(L, f"orc_density_{s}").argtypes = [K, I, ST]
             getattr(L, f"orc_forces_{s}").argtypes = [K, I, ST]
             getattr(L, f"orc_integrate_{s}").argtypes = [K, COL, I, ST, CT]
@@ -385,6 +387,17 @@ def bake_brushes(res: int, mips: int, world_size: float, origin, band: float, br
         out.append(tex[4 * off:4 * (off + r ** 3)].reshape(r, r, r, 4))
         off += r ** 3
     return out
+
+
+def neighbors(options: dict, pos: np.ndarray, i: int, precision: str = "f64") -> list[int]:
+    """Ids the oracle's 27-cell scan finds within h of particle i."""
+    dt = np.float32 if precision == "f32" else np.float64
+    p = np.ascontiguousarray(np.asarray(pos, dtype=dt).T)
+    k = fluid_consts(options)
+    out = np.empty(p.shape[1], dtype=np.uint32)
+    n = getattr(lib(), f"orc_neighbors_{precision}")(C.byref(k), p.shape[1], p[0].ctypes.data, p[1].ctypes.data,
+                                                      p[2].ctypes.data, i, out.ctypes.data, out.size)
+    return sorted(int(v) for v in out[:n])
 
 
 def keys(options: dict, pos: np.ndarray) -> np.ndarray:
