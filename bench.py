@@ -37,10 +37,12 @@ def algorithmic_bytes(kernel: str, particles: int, mips: int, live_voxels: int) 
     }
     if kernel in per_particle:
         return float(per_particle[kernel] * particles)
-    if kernel in ("splat_brick", "splat_plain"):
-        # position read per mip + 16 B per live voxel (u32 clear/resolve + texel write)
-        per_launch_mips = mips if kernel == "splat_brick" else 1
-        return float(12 * particles * per_launch_mips + 16 * live_voxels / (1 if kernel == "splat_brick" else mips))
+    if kernel == "splat_brick":          # all mips in one launch: position reads
+        return float(12 * particles * mips)
+    if kernel == "splat_plain":          # one launch per mip
+        return float(12 * particles)
+    if kernel == "splat_brick_resolve":  # 16 B per live voxel (code clear/resolve + texel)
+        return float(16 * live_voxels)
     return None
 
 

@@ -288,6 +288,11 @@ extern "C" int clay_ctx_create(int device, clay_ctx** out) {
   Ctx* c = new (std::nothrow) Ctx();
   if (!c) return fail(CLAY_ERR_INVALID, "out of host memory");
   c->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->cus = cus;
+  // Eight 256-thread workgroups per CU fill the wave slots of a gfx950 CU.
+  c->persistent_blocks = c->cus * 8;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -561,6 +566,7 @@ void splat_free(Splat* sp) {
   dfree(sp->brick_cursor);
   dfree(sp->brick_list);
   dfree(sp->brick_total_dev);
+  dfree(sp->item_start);
   dfree(sp->tile_live);
   delete sp;
 }
@@ -607,11 +613,12 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
   }
   sp->grid_total = total;
   hipStream_t s = ctx->stream;
-  if (desc->mode == CLAY_SPLAT_PLAIN || desc->keep_codes) {
-    if ((rc = dalloc(&sp->grid, total)) || (rc = launch_fill_u32(ctx, s, sp->grid, total, kEncSat))) {
-      splat_free(sp);
-      return rc;
-    }
+  // Both schedules keep the u32 code grid: plain accumulates codes in it, tiled
+  // accumulates squared distances of the bricks it touches and resolves them to
+  // codes in place, so after every bake it holds the codes of every voxel.
+  if ((rc = dalloc(&sp->grid, total)) || (rc = launch_fill_u32(ctx, s, sp->grid, total, kEncSat))) {
+    splat_free(sp);
+    return rc;
   }
   if (desc->mode == CLAY_SPLAT_TILED) {
     splat_brick_geometry(res, sp->mips, &sp->brick_res, &sp->brick_off, &sp->brick_total, &sp->tile_off,
@@ -621,7 +628,8 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
     sp->brick_list_cap = (size_t)std::max<uint32_t>(cloud->capacity, 1) * per;
     if ((rc = dalloc(&sp->brick_count, sp->brick_total)) || (rc = dalloc(&sp->brick_start, sp->brick_total)) ||
         (rc = dalloc(&sp->brick_cursor, sp->brick_total)) || (rc = dalloc(&sp->brick_list, sp->brick_list_cap)) ||
-        (rc = dalloc(&sp->brick_total_dev, 1)) || (rc = dalloc(&sp->tile_live, sp->tile_total))) {
+        (rc = dalloc(&sp->brick_total_dev, 4)) || (rc = dalloc(&sp->item_start, sp->brick_total)) ||
+        (rc = dalloc(&sp->tile_live, sp->tile_total))) {
       splat_free(sp);
       return rc;
     }
@@ -1007,7 +1015,7 @@ extern "C" int clay_fluid_spawn(clay_fluid* fh, uint32_t n, const double pos[3],
     f->params.emit_vel[a] = vel[a];
   }
   f->params.emit_radius = spread;
-  f->seed++;
+  f->emit_seed = f->seed++;  // emitSeed: this.seed++ (fluid.ts:523)
   return CLAY_OK;
 }
 
@@ -1044,7 +1052,7 @@ extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
   if (emit_count > 0) {
     f->cursor = (emit_base + emit_count) % f->capacity;
     f->live = std::min<uint32_t>(f->capacity, std::max<uint32_t>(f->live, emit_base + emit_count));
-    int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->seed);
+    int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->emit_seed);
     if (rc) return rc;
   }
   bool sorted = false;

@@ -51,6 +51,8 @@ class LaunchTimer {
 
 struct Ctx {
   int device = 0;
+  int cus = 256;                 // compute units of the device
+  int persistent_blocks = 2048;  // grid of persistent work-queue kernels
   hipStream_t stream = nullptr;
   bool timing = false;
   std::vector<KernelStat> stats;
@@ -107,7 +109,8 @@ struct Splat {
   uint32_t* brick_cursor = nullptr;
   uint32_t* brick_list = nullptr;    // particle ids per brick
   size_t brick_list_cap = 0;
-  uint32_t* brick_total_dev = nullptr;  // [mips] total entries (overflow check)
+  uint32_t* brick_total_dev = nullptr;  // scan results: total entries, total work items
+  uint32_t* item_start = nullptr;       // first work item (chunk) of each brick
   uint8_t* tile_live = nullptr;      // per 8^3 tile, all mips: live after last bake
   std::vector<size_t> tile_off;
   size_t tile_total = 0;
@@ -126,6 +129,7 @@ struct Fluid {
   uint32_t cursor = 0;
   uint32_t pending_emit = 0;
   uint32_t seed = 1;
+  uint32_t emit_seed = 1;
   // id-order state (SoA)
   float* st = nullptr;           // 11 planes of capacity floats
   float *px, *py, *pz, *vx, *vy, *vz, *rho, *pres, *nvx, *nvy, *nvz;

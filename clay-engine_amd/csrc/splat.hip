@@ -7,19 +7,33 @@
 //   e = u32((clamp(sd / bandWorld, -1, 1) + 1) * 32767.5), then resolve every
 //   voxel to f16.
 //
-// TILED: particles are binned to 16^3-voxel bricks of every mip (a particle
-//   lands in each brick its voxel box overlaps; the per-brick counters are hit
-//   once per wave and distinct brick, not once per lane).  One workgroup per
-//   brick owns a 16 KB u32 LDS copy of the brick, does the atomicMin there
-//   (ds_min_u32), finds which 8^3 tiles came out live (some code < 65535) and
-//   writes only tiles that are live now or were live after the previous bake;
-//   every other tile already reads the saturated +1 the reference's clear would
-//   give it.  Voxels whose centre is farther than `reach` from the particle are
-//   skipped row by row, which cannot change a code: they fail sd < bandWorld.
+// TILED:
+//   1. k_bin<count>, k_brick_scan, k_bin<fill>: particles are binned to 16^3-voxel
+//      bricks of every mip (a particle lands in each brick its voxel box overlaps;
+//      the counters are hit once per wave and distinct brick).  The scan also
+//      cuts every brick's list into chunks of kChunk particles: a chunk is one
+//      work item, so a coarse-mip brick holding most of the cloud is spread over
+//      many workgroups instead of serialising on one.
+//   2. k_brick_clear: the grid of every brick that received particles is reset.
+//   3. k_brick_splat: a persistent grid walks the work items.  Each item keeps a
+//      16 KB LDS copy of its brick and, for every (particle, voxel) pair inside the
+//      reference's voxel box, takes ds_min_u32 of the f32 bits of the squared
+//      distance s = (dx*dx + dy*dy) + dz*dz.  Then it merges the brick into the
+//      grid (plain stores when the brick is one chunk, atomicMin otherwise).
+//   4. k_brick_resolve: per brick, code = enc(sqrt(s_min)) once per voxel, the exact
+//      live-tile mask, and f16 texels for tiles that are live now or were live
+//      after the previous bake.  Every other tile already reads the saturated +1.
 //
-// min is order independent, so both schedules give bit-identical codes whatever
-// the atomic order; the f32 arithmetic per voxel is the reference expression
-// evaluated in IEEE order (contraction off).
+// Why step 3 may keep only min s: the code is a non-decreasing function of s in
+// IEEE arithmetic (sqrt, subtracting the radius, dividing by the band, clamping,
+// adding, scaling and truncating are each monotone), so min over particles of
+// code(s_i) == code(min over particles of s_i), bit for bit.  Non-negative f32
+// values order like their bit patterns, so the min is an integer atomicMin.
+// Clouds with normals (sd = max(dot(d, n), |d| - R)) are not a function of s
+// alone; they take the min of the code itself, as the reference does.
+//
+// Voxels whose centre is farther than `reach` from the particle are skipped row
+// by row; they fail sd < bandWorld, so skipping them cannot change a code.
 #include "clay_internal.h"
 
 namespace clay {
@@ -29,6 +43,8 @@ namespace {
 constexpr int kBrick = 16;                  // voxels per brick edge
 constexpr int kBrickVox = kBrick * kBrick * kBrick;
 constexpr int kThreads = 256;
+constexpr uint32_t kChunk = 2048;           // particles per work item
+constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared distance
 
 struct SplatMipK {
   int32_t r;          // res >> m
@@ -54,6 +70,12 @@ struct MipsK {
   float inv_enc;          // f32(1 / 32767.5)
   uint16_t sat_half;      // f16(+1.0)
   SplatMipK m[CLAY_MAX_MIPS];
+};
+
+// Device-side results of the brick scan.
+struct ScanOut {
+  uint32_t entries;  // total brick-list entries
+  uint32_t items;    // total work items (chunks)
 };
 
 __device__ __forceinline__ uint32_t particle_at(const CloudK& c, uint32_t k) {
@@ -84,8 +106,17 @@ __device__ __forceinline__ bool voxel_box(const SplatMipK& m, f3 p, int32_t lo[3
   return ok;
 }
 
-// Code of voxel (x,y,z) for a particle at p (relative to the volume origin), or
-// 65535 when the voxel is outside the band.  splat.ts:244-250 (and :170-184 with normals).
+// enc of a signed distance (splat.ts:246-247), or 65535 outside the band.
+__device__ __forceinline__ uint32_t encode_sd(const SplatMipK& m, float sd) {
+  if (sd < m.bw) {
+    const float q = (fclamp_s(sd / m.bw, -1.0f, 1.0f) + 1.0f) * kEncScale;
+    return (uint32_t)q;
+  }
+  return kEncSat;
+}
+
+// Code of voxel (x,y,z) for a particle at p (relative to the volume origin).
+// splat.ts:244-250 (and :170-184 with normals).
 __device__ __forceinline__ uint32_t voxel_code(const SplatMipK& m, f3 p, bool has_n, f3 n,
                                                int32_t x, int32_t y, int32_t z) {
   const f3 cc = mk(((float)x + 0.5f) * m.voxel, ((float)y + 0.5f) * m.voxel,
@@ -93,11 +124,14 @@ __device__ __forceinline__ uint32_t voxel_code(const SplatMipK& m, f3 p, bool ha
   const f3 d = cc - p;
   float sd = len3(d) - m.radius;
   if (has_n) sd = fmax_s(dot3(d, n), sd);
-  if (sd < m.bw) {
-    const float q = (fclamp_s(sd / m.bw, -1.0f, 1.0f) + 1.0f) * kEncScale;
-    return (uint32_t)q;
-  }
-  return kEncSat;
+  return encode_sd(m, sd);
+}
+
+// Code of the nearest particle given the min squared distance's bits.
+__device__ __forceinline__ uint32_t code_from_s(const SplatMipK& m, uint32_t sbits) {
+  if (sbits == kSInit) return kEncSat;
+  const float sd = sqrtf(__uint_as_float(sbits)) - m.radius;
+  return encode_sd(m, sd);
 }
 
 __device__ __forceinline__ f3 load_rel(const CloudK& c, const VolDev* vol, uint32_t i) {
@@ -160,7 +194,7 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t* counters, uint32_t idx, b
   return result;
 }
 
-// Brick range of particle `i` at mip `m`; nb3 = number of bricks it overlaps (0 if none).
+// Brick range of a particle at mip `m`; returns the number of bricks it overlaps.
 __device__ __forceinline__ int brick_range(const SplatMipK& m, f3 p, int32_t blo[3], int32_t bext[3]) {
   int32_t lo[3], hi[3];
   if (!voxel_box(m, p, lo, hi)) return 0;
@@ -172,7 +206,7 @@ __device__ __forceinline__ int brick_range(const SplatMipK& m, f3 p, int32_t blo
   return bext[0] * bext[1] * bext[2];
 }
 
-// Pass 1 (count, fill=false) and pass 3 (fill=true) of the binning.
+// Binning pass 1 (count, kFill = false) and pass 3 (fill, kFill = true).
 template <bool kFill>
 __global__ void __launch_bounds__(kThreads)
     k_bin(MipsK mk_, CloudK c, const VolDev* __restrict__ vol, uint32_t* __restrict__ counters,
@@ -206,164 +240,245 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Pass 2: exclusive scan of brick counts -> start; cursor = start; total.
+// Binning pass 2: exclusive scans of the brick counts (-> list starts, cursors) and
+// of the per-brick chunk counts (-> first work item of each brick), one block.
 __global__ void __launch_bounds__(1024)
     k_brick_scan(const uint32_t* __restrict__ count, uint32_t nbricks, uint32_t* __restrict__ start,
-                 uint32_t* __restrict__ cursor, uint32_t* __restrict__ total) {
-  __shared__ uint32_t sh[1024];
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
+                 uint32_t* __restrict__ cursor, uint32_t* __restrict__ item_start,
+                 ScanOut* __restrict__ out) {
+  __shared__ uint32_t sa[1024], sb[1024];
+  __shared__ uint32_t carry_a, carry_b;
+  if (threadIdx.x == 0) {
+    carry_a = 0;
+    carry_b = 0;
+  }
   __syncthreads();
   for (uint32_t c0 = 0; c0 < nbricks; c0 += 1024) {
     const uint32_t i = c0 + threadIdx.x;
-    const uint32_t v = i < nbricks ? count[i] : 0u;
-    sh[threadIdx.x] = v;
+    const uint32_t va = i < nbricks ? count[i] : 0u;
+    const uint32_t vb = (va + kChunk - 1) / kChunk;
+    sa[threadIdx.x] = va;
+    sb[threadIdx.x] = vb;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0u;
+      const uint32_t xa = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0u;
+      const uint32_t xb = threadIdx.x >= (unsigned)off ? sb[threadIdx.x - off] : 0u;
       __syncthreads();
-      sh[threadIdx.x] += add;
+      sa[threadIdx.x] += xa;
+      sb[threadIdx.x] += xb;
       __syncthreads();
     }
-    const uint32_t incl = sh[threadIdx.x];
+    const uint32_t ia = sa[threadIdx.x], ib = sb[threadIdx.x];
     if (i < nbricks) {
-      start[i] = carry + incl - v;
-      cursor[i] = carry + incl - v;
+      start[i] = carry_a + ia - va;
+      cursor[i] = carry_a + ia - va;
+      item_start[i] = carry_b + ib - vb;
     }
     __syncthreads();
-    if (threadIdx.x == 1023) carry += incl;
+    if (threadIdx.x == 1023) {
+      carry_a += ia;
+      carry_b += ib;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    out->entries = carry_a;
+    out->items = carry_b;
+  }
 }
 
-// Pass 4: one workgroup per brick of every mip.
-__global__ void __launch_bounds__(kThreads)
-    k_brick_splat(MipsK mk_, CloudK c, const VolDev* __restrict__ vol,
-                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ count,
-                  const uint32_t* __restrict__ list, uint8_t* __restrict__ tile_live,
-                  uint16_t* __restrict__ dist, uint32_t* __restrict__ codes) {
-  __shared__ uint32_t lds[kBrickVox];
-  __shared__ uint32_t tflag;
-  const uint32_t b = blockIdx.x;
+struct BrickPos {
+  int32_t mi, ox, oy, oz, bx, by, bz;
+};
+
+__device__ __forceinline__ BrickPos brick_pos(const MipsK& mk_, uint32_t b) {
+  BrickPos q;
   int mi = 0;
   while (mi + 1 < mk_.mips && b >= mk_.m[mi + 1].boff) mi++;
   const SplatMipK& m = mk_.m[mi];
   const uint32_t lb = b - m.boff;
-  const int32_t bx = (int32_t)(lb % (uint32_t)m.nb);
-  const int32_t by = (int32_t)((lb / (uint32_t)m.nb) % (uint32_t)m.nb);
-  const int32_t bz = (int32_t)(lb / ((uint32_t)m.nb * (uint32_t)m.nb));
-  const int32_t ox = bx * kBrick, oy = by * kBrick, oz = bz * kBrick;
-  const int32_t tres = m.r / kTile;  // tiles per axis at this mip
-  const uint32_t n = count[b];
-  const int t = threadIdx.x;
+  q.mi = mi;
+  q.bx = (int32_t)(lb % (uint32_t)m.nb);
+  q.by = (int32_t)((lb / (uint32_t)m.nb) % (uint32_t)m.nb);
+  q.bz = (int32_t)(lb / ((uint32_t)m.nb * (uint32_t)m.nb));
+  q.ox = q.bx * kBrick;
+  q.oy = q.by * kBrick;
+  q.oz = q.bz * kBrick;
+  return q;
+}
 
-  // Which of the brick's 2x2x2 tiles exist (a brick may hang over the grid edge).
-  auto tile_index = [&](int tx, int ty, int tz) -> int32_t {
-    const int32_t gx = bx * 2 + tx, gy = by * 2 + ty, gz = bz * 2 + tz;
-    if (gx >= tres || gy >= tres || gz >= tres) return -1;
-    return (int32_t)(m.toff + (uint32_t)gx + (uint32_t)tres * ((uint32_t)gy + (uint32_t)tres * (uint32_t)gz));
-  };
-
-  if (n == 0) {
-    // Nothing lands here: only tiles still holding last bake's surface need the +1.
-    uint32_t stale = 0;
-    for (int q = 0; q < 8; q++) {
-      const int32_t ti = tile_index(q & 1, (q >> 1) & 1, q >> 2);
-      if (ti >= 0 && tile_live[ti]) stale |= 1u << q;
-    }
-    if (!stale) return;
-    __syncthreads();
-    for (int v = t; v < kBrickVox; v += kThreads) {
-      const int lx = v & 15, ly = (v >> 4) & 15, lz = v >> 8;
-      const int q = (lx >> 3) | ((ly >> 3) << 1) | ((lz >> 3) << 2);
-      if (!((stale >> q) & 1u)) continue;
-      const uint32_t x = (uint32_t)(ox + lx), y = (uint32_t)(oy + ly), z = (uint32_t)(oz + lz);
-      const uint32_t gi = x + (uint32_t)m.r * (y + (uint32_t)m.r * z);
-      dist[m.voff + gi] = mk_.sat_half;
-      if (codes) codes[m.off + gi] = kEncSat;
-    }
-    __syncthreads();
-    if (t < 8 && ((stale >> t) & 1u)) tile_live[tile_index(t & 1, (t >> 1) & 1, t >> 2)] = 0;
-    return;
+// Step 2: reset the grid of bricks that received particles.
+template <bool kNormals>
+__global__ void __launch_bounds__(kThreads)
+    k_brick_clear(MipsK mk_, const uint32_t* __restrict__ count, uint32_t* __restrict__ grid) {
+  const uint32_t b = blockIdx.x;
+  if (count[b] == 0) return;
+  const BrickPos q = brick_pos(mk_, b);
+  const SplatMipK& m = mk_.m[q.mi];
+  const uint32_t init = kNormals ? kEncSat : kSInit;
+  for (int v = threadIdx.x; v < kBrickVox; v += kThreads) {
+    const int32_t x = q.ox + (v & 15), y = q.oy + ((v >> 4) & 15), z = q.oz + (v >> 8);
+    if (x >= m.r || y >= m.r || z >= m.r) continue;
+    grid[m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z)] = init;
   }
+}
 
-  for (int v = t; v < kBrickVox; v += kThreads) lds[v] = kEncSat;
-  if (t == 0) tflag = 0;
-  __syncthreads();
-
-  const bool has_n = c.nrm[0] != nullptr;
-  const uint32_t s0 = start[b];
-  // Blocked assignment: the list is in spatial order, so lane t takes the t-th
-  // contiguous chunk.  Neighbouring lanes then splat particles that lie far apart
-  // and their ds_min_u32 hit different addresses, instead of 64 lanes walking
-  // nearly the same voxels in lockstep.
-  const uint32_t chunk = (n + kThreads - 1) / kThreads;
-  const uint32_t e0 = (uint32_t)t * chunk;
-  const uint32_t e1 = e0 + chunk < n ? e0 + chunk : n;
-  for (uint32_t e = e0; e < e1; e++) {
-    const uint32_t i = list[s0 + e];
-    const f3 p = load_rel(c, vol, i);
-    const f3 nr = load_nrm(c, i);
-    int32_t lo[3], hi[3];
-    voxel_box(m, p, lo, hi);  // non-empty: the particle was binned here
-    const int32_t x0 = lo[0] > ox ? lo[0] : ox, x1 = hi[0] < ox + kBrick - 1 ? hi[0] : ox + kBrick - 1;
-    const int32_t y0 = lo[1] > oy ? lo[1] : oy, y1 = hi[1] < oy + kBrick - 1 ? hi[1] : oy + kBrick - 1;
-    const int32_t z0 = lo[2] > oz ? lo[2] : oz, z1 = hi[2] < oz + kBrick - 1 ? hi[2] : oz + kBrick - 1;
-    for (int32_t z = z0; z <= z1; z++) {
-      const float dz = ((float)z + 0.5f) * m.voxel - p.z;
-      const float dz2 = dz * dz;
-      if (dz2 > m.cull2) continue;
-      for (int32_t y = y0; y <= y1; y++) {
-        const float dy = ((float)y + 0.5f) * m.voxel - p.y;
-        const float rem = m.cull2 - dz2 - dy * dy;
-        if (rem < 0.0f) continue;
-        // x-span of the reach ball on this row, widened by one voxel each side.
-        const float s = sqrtf(rem);
-        int32_t xa = sat_i32(floorf((p.x - s) * m.inv - 0.5f)) - 1;
-        int32_t xb = sat_i32(floorf((p.x + s) * m.inv - 0.5f)) + 1;
-        xa = xa > x0 ? xa : x0;
-        xb = xb < x1 ? xb : x1;
-        for (int32_t x = xa; x <= xb; x++) {
-          const uint32_t code = voxel_code(m, p, has_n, nr, x, y, z);
-          if (code != kEncSat)
-            atomicMin(&lds[(x - ox) + kBrick * ((y - oy) + kBrick * (z - oz))], code);
+// Step 3: persistent walk over (brick, chunk) work items.
+template <bool kNormals>
+__global__ void __launch_bounds__(kThreads)
+    k_brick_splat(MipsK mk_, CloudK c, const VolDev* __restrict__ vol,
+                  const uint32_t* __restrict__ start, const uint32_t* __restrict__ count,
+                  const uint32_t* __restrict__ item_start, const ScanOut* __restrict__ scan,
+                  uint32_t nbricks, const uint32_t* __restrict__ list, uint32_t* __restrict__ grid) {
+  __shared__ uint32_t lds[kBrickVox];
+  __shared__ uint32_t sh_brick;
+  const int t = threadIdx.x;
+  const uint32_t init = kNormals ? kEncSat : kSInit;
+  // Lane order inside a pass: the 8-bit reversal of the thread index, so that
+  // neighbouring lanes take list entries 128 apart (spatially apart) and their
+  // ds_min_u32 rarely meet on one address.
+  const uint32_t rev = __brev((uint32_t)t) >> 24;
+  const uint32_t items = scan->items;
+  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+    if (t == 0) {
+      // Last brick whose first item is <= item (a brick with no items shares its
+      // first item with the next brick, so the last one always has items).
+      uint32_t lo = 0, hi = nbricks;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (item_start[mid] <= item) lo = mid; else hi = mid;
+      }
+      sh_brick = lo;
+    }
+    for (int v = t; v < kBrickVox; v += kThreads) lds[v] = init;
+    __syncthreads();
+    const uint32_t b = sh_brick;
+    const BrickPos q = brick_pos(mk_, b);
+    const SplatMipK& m = mk_.m[q.mi];
+    const uint32_t n_b = count[b];
+    const uint32_t chunk = item - item_start[b];
+    const uint32_t e_lo = chunk * kChunk;
+    const uint32_t n_e = (n_b - e_lo) < kChunk ? (n_b - e_lo) : kChunk;
+    const uint32_t base = start[b] + e_lo;
+    const int32_t ox = q.ox, oy = q.oy, oz = q.oz;
+    for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
+      const uint32_t e = pass * kThreads + rev;
+      if (e >= n_e) continue;
+      const uint32_t i = list[base + e];
+      const f3 p = load_rel(c, vol, i);
+      const f3 nr = kNormals ? load_nrm(c, i) : mk(0.0f, 0.0f, 0.0f);
+      int32_t lo[3], hi[3];
+      voxel_box(m, p, lo, hi);  // non-empty: the particle was binned here
+      const int32_t x0 = lo[0] > ox ? lo[0] : ox, x1 = hi[0] < ox + kBrick - 1 ? hi[0] : ox + kBrick - 1;
+      const int32_t y0 = lo[1] > oy ? lo[1] : oy, y1 = hi[1] < oy + kBrick - 1 ? hi[1] : oy + kBrick - 1;
+      const int32_t z0 = lo[2] > oz ? lo[2] : oz, z1 = hi[2] < oz + kBrick - 1 ? hi[2] : oz + kBrick - 1;
+      for (int32_t z = z0; z <= z1; z++) {
+        const float dz = ((float)z + 0.5f) * m.voxel - p.z;
+        const float dz2 = dz * dz;
+        if (dz2 > m.cull2) continue;
+        for (int32_t y = y0; y <= y1; y++) {
+          const float dy = ((float)y + 0.5f) * m.voxel - p.y;
+          const float dy2 = dy * dy;
+          const float rem = m.cull2 - dz2 - dy2;
+          if (rem < 0.0f) continue;
+          // x-span of the reach ball on this row, widened by one voxel each side.
+          const float s = sqrtf(rem);
+          int32_t xa = sat_i32(floorf((p.x - s) * m.inv - 0.5f)) - 1;
+          int32_t xb = sat_i32(floorf((p.x + s) * m.inv - 0.5f)) + 1;
+          xa = xa > x0 ? xa : x0;
+          xb = xb < x1 ? xb : x1;
+          uint32_t* row = lds + (y - oy) * kBrick + (z - oz) * kBrick * kBrick - ox;
+          for (int32_t x = xa; x <= xb; x++) {
+            const float dx = ((float)x + 0.5f) * m.voxel - p.x;
+            if (kNormals) {
+              const float len = sqrtf(dx * dx + dy2 + dz2);
+              float sd = len - m.radius;
+              sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
+              const uint32_t code = encode_sd(m, sd);
+              if (code != kEncSat) atomicMin(&row[x], code);
+            } else {
+              const float s2 = dx * dx + dy2 + dz2;
+              atomicMin(&row[x], __float_as_uint(s2));
+            }
+          }
         }
       }
     }
+    __syncthreads();
+    // Merge the brick into the grid.
+    const bool single = n_b <= kChunk;
+    for (int v = t; v < kBrickVox; v += kThreads) {
+      const int32_t x = ox + (v & 15), y = oy + ((v >> 4) & 15), z = oz + (v >> 8);
+      if (x >= m.r || y >= m.r || z >= m.r) continue;
+      const uint32_t val = lds[v];
+      uint32_t* g = grid + m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z);
+      if (single) {
+        *g = val;
+      } else if (val != init) {
+        atomicMin(g, val);
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
+}
 
-  // Exact live-tile mask of this brick.
+// Step 4: codes, exact live-tile mask and texels, one workgroup per brick.
+template <bool kNormals>
+__global__ void __launch_bounds__(kThreads)
+    k_brick_resolve(MipsK mk_, const uint32_t* __restrict__ count, uint8_t* __restrict__ tile_live,
+                    uint16_t* __restrict__ dist, uint32_t* __restrict__ grid) {
+  __shared__ uint32_t tflag;
+  const uint32_t b = blockIdx.x;
+  const BrickPos q = brick_pos(mk_, b);
+  const SplatMipK& m = mk_.m[q.mi];
+  const int32_t tres = m.r / kTile;  // tiles per axis at this mip
+  const int t = threadIdx.x;
+  auto tile_index = [&](int tx, int ty, int tz) -> int32_t {
+    const int32_t gx = q.bx * 2 + tx, gy = q.by * 2 + ty, gz = q.bz * 2 + tz;
+    if (gx >= tres || gy >= tres || gz >= tres) return -1;
+    return (int32_t)(m.toff + (uint32_t)gx + (uint32_t)tres * ((uint32_t)gy + (uint32_t)tres * (uint32_t)gz));
+  };
+  // Tiles of this brick that were live after the previous bake.
+  uint32_t stale = 0;
+  for (int k = 0; k < 8; k++) {
+    const int32_t ti = tile_index(k & 1, (k >> 1) & 1, k >> 2);
+    if (ti >= 0 && tile_live[ti]) stale |= 1u << k;
+  }
+  const bool has = count[b] != 0;
+  if (!has && !stale) return;
+  if (t == 0) tflag = 0;
+  __syncthreads();
+  uint32_t codes[kBrickVox / kThreads];
   uint32_t mask = 0;
-  for (int v = t; v < kBrickVox; v += kThreads) {
-    const int lx = v & 15, ly = (v >> 4) & 15, lz = v >> 8;
-    if (lds[v] != kEncSat) mask |= 1u << ((lx >> 3) | ((ly >> 3) << 1) | ((lz >> 3) << 2));
+#pragma unroll
+  for (int j = 0; j < kBrickVox / kThreads; j++) {
+    const int v = t + j * kThreads;
+    const int32_t x = q.ox + (v & 15), y = q.oy + ((v >> 4) & 15), z = q.oz + (v >> 8);
+    uint32_t code = kEncSat;
+    if (has && x < m.r && y < m.r && z < m.r) {
+      uint32_t* g = grid + m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z);
+      const uint32_t acc = *g;
+      code = kNormals ? acc : code_from_s(m, acc);
+      *g = code;
+      if (code != kEncSat) mask |= 1u << (((v & 15) >> 3) | ((((v >> 4) & 15) >> 3) << 1) | (((v >> 8) >> 3) << 2));
+    }
+    codes[j] = code;
   }
   if (mask) atomicOr(&tflag, mask);
   __syncthreads();
   const uint32_t live = tflag;
-  uint32_t write = live;
-  for (int q = 0; q < 8; q++) {
-    const int32_t ti = tile_index(q & 1, (q >> 1) & 1, q >> 2);
-    if (ti < 0) {
-      write &= ~(1u << q);
-    } else if (tile_live[ti]) {
-      write |= 1u << q;
-    }
-  }
-  __syncthreads();
-  if (write) {
-    for (int v = t; v < kBrickVox; v += kThreads) {
-      const int lx = v & 15, ly = (v >> 4) & 15, lz = v >> 8;
-      const int q = (lx >> 3) | ((ly >> 3) << 1) | ((lz >> 3) << 2);
-      if (!((write >> q) & 1u)) continue;
-      const uint32_t x = (uint32_t)(ox + lx), y = (uint32_t)(oy + ly), z = (uint32_t)(oz + lz);
-      const uint32_t gi = x + (uint32_t)m.r * (y + (uint32_t)m.r * z);
-      const uint32_t code = lds[v];
-      dist[m.voff + gi] = f2h((float)code * mk_.inv_enc - 1.0f);
-      if (codes) codes[m.off + gi] = code;
-    }
+  const uint32_t write = live | stale;
+#pragma unroll
+  for (int j = 0; j < kBrickVox / kThreads; j++) {
+    const int v = t + j * kThreads;
+    const int lx = v & 15, ly = (v >> 4) & 15, lz = v >> 8;
+    const int k = (lx >> 3) | ((ly >> 3) << 1) | ((lz >> 3) << 2);
+    if (!((write >> k) & 1u)) continue;
+    const int32_t x = q.ox + lx, y = q.oy + ly, z = q.oz + lz;
+    if (x >= m.r || y >= m.r || z >= m.r) continue;
+    const uint32_t gi = (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z);
+    dist[m.voff + gi] = codes[j] == kEncSat ? mk_.sat_half : f2h((float)codes[j] * mk_.inv_enc - 1.0f);
+    if (!has) grid[m.off + gi] = kEncSat;
   }
   if (t < 8) {
     const int32_t ti = tile_index(t & 1, (t >> 1) & 1, t >> 2);
@@ -421,6 +536,54 @@ static CloudK cloud_k(const Splat* sp, const uint32_t* order) {
   return c;
 }
 
+template <bool kNormals>
+static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
+  Ctx* ctx = sp->ctx;
+  const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
+  const unsigned nbricks = (unsigned)sp->brick_total;
+  ScanOut* scan = reinterpret_cast<ScanOut*>(sp->brick_total_dev);
+  int rc = launch_fill_u32(ctx, s, sp->brick_count, sp->brick_total, 0u);
+  if (rc) return rc;
+  if (c.count) {
+    LaunchTimer t(ctx, s, "splat_bin_count");
+    hipLaunchKernelGGL(k_bin<false>, dim3(pgrid), dim3(kThreads), 0, s, mk_, c, sp->vol->dev,
+                       sp->brick_count, nullptr);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  {
+    LaunchTimer t(ctx, s, "splat_brick_scan");
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_count, (uint32_t)nbricks,
+                       sp->brick_start, sp->brick_cursor, sp->item_start, scan);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  if (c.count) {
+    LaunchTimer t(ctx, s, "splat_bin_fill");
+    hipLaunchKernelGGL(k_bin<true>, dim3(pgrid), dim3(kThreads), 0, s, mk_, c, sp->vol->dev,
+                       sp->brick_cursor, sp->brick_list);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  {
+    LaunchTimer t(ctx, s, "splat_brick_clear");
+    hipLaunchKernelGGL(k_brick_clear<kNormals>, dim3(nbricks), dim3(kThreads), 0, s, mk_, sp->brick_count,
+                       sp->grid);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  if (c.count) {
+    LaunchTimer t(ctx, s, "splat_brick");
+    hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(ctx->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
+                       sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
+                       (uint32_t)nbricks, sp->brick_list, sp->grid);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  {
+    LaunchTimer t(ctx, s, "splat_brick_resolve");
+    hipLaunchKernelGGL(k_brick_resolve<kNormals>, dim3(nbricks), dim3(kThreads), 0, s, mk_, sp->brick_count,
+                       sp->tile_live, sp->vol->dist, sp->grid);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
+  return CLAY_OK;
+}
+
 int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
   Ctx* ctx = sp->ctx;
   MipsK mk_{};
@@ -448,35 +611,8 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
     sp->has_baked = true;
     return CLAY_OK;
   }
-  // TILED
-  int rc = launch_fill_u32(ctx, s, sp->brick_count, sp->brick_total, 0u);
+  const int rc = c.nrm[0] ? tiled_bake<true>(sp, s, mk_, c) : tiled_bake<false>(sp, s, mk_, c);
   if (rc) return rc;
-  if (c.count) {
-    LaunchTimer t(ctx, s, "splat_bin_count");
-    hipLaunchKernelGGL(k_bin<false>, dim3(pgrid), dim3(kThreads), 0, s, mk_, c, sp->vol->dev,
-                       sp->brick_count, nullptr);
-    CLAY_HIP_CHECK(hipGetLastError());
-  }
-  {
-    LaunchTimer t(ctx, s, "splat_brick_scan");
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_count,
-                       (uint32_t)sp->brick_total, sp->brick_start, sp->brick_cursor,
-                       sp->brick_total_dev);
-    CLAY_HIP_CHECK(hipGetLastError());
-  }
-  if (c.count) {
-    LaunchTimer t(ctx, s, "splat_bin_fill");
-    hipLaunchKernelGGL(k_bin<true>, dim3(pgrid), dim3(kThreads), 0, s, mk_, c, sp->vol->dev,
-                       sp->brick_cursor, sp->brick_list);
-    CLAY_HIP_CHECK(hipGetLastError());
-  }
-  {
-    LaunchTimer t(ctx, s, "splat_brick");
-    hipLaunchKernelGGL(k_brick_splat, dim3((unsigned)sp->brick_total), dim3(kThreads), 0, s, mk_, c,
-                       sp->vol->dev, sp->brick_start, sp->brick_count, sp->brick_list,
-                       sp->tile_live, sp->vol->dist, sp->desc.keep_codes ? sp->grid : nullptr);
-    CLAY_HIP_CHECK(hipGetLastError());
-  }
   sp->has_baked = true;
   return CLAY_OK;
 }

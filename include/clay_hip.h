@@ -174,7 +174,7 @@ typedef struct {
   double origin[3];       /* default (-12, -4, -12) is applied by the host layer */
   double world_size;      /* <= 0 -> 24 */
   int32_t mode;           /* CLAY_SPLAT_PLAIN | CLAY_SPLAT_TILED */
-  int32_t keep_codes;     /* tiled mode: also maintain the u32 code grid (tests) */
+  int32_t keep_codes;     /* reserved: both schedules keep the u32 code grid */
 } clay_splat_desc;
 
 /* new SplatField(root, cloud, options): splat.ts:147-278. */
@@ -189,7 +189,7 @@ int clay_splat_bake(clay_splat* s, clay_stream stream);
 int clay_splat_set_center(clay_splat* s, const double center[3], clay_stream stream);
 int clay_splat_volume(clay_splat* s, clay_volume** out);
 int clay_splat_field(clay_splat* s, clay_field** out);
-/* The u32 code grid of one mip (plain mode, or tiled with keep_codes). */
+/* The u32 code grid of one mip after the last bake. */
 int clay_splat_grid_download(clay_splat* s, int32_t mip, uint32_t* codes_host);
 /* Exact live-tile mask of one mip after the last bake: 1 byte per 8^3 tile. */
 int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* live_host);

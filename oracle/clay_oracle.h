@@ -165,6 +165,8 @@ void orc_stable_sort_keys(int n, const uint32_t* keys, uint32_t* perm_out);
                        uint32_t* perm_out);                                            \
   int orc_neighbors_##S(const orc_fluid_k* k, int n, const T* px, const T* py,         \
                         const T* pz, int i, uint32_t* out, int cap);                   \
+  void orc_emit_##S(int cap, int base, int count, uint32_t seed, const double* pos3,    \
+                    double radius, const double* vel3, orc_state* st);                 \
   void orc_density_##S(const orc_fluid_k* k, int n, orc_state* st);                    \
   void orc_forces_##S(const orc_fluid_k* k, int n, orc_state* st);                     \
   void orc_integrate_##S(const orc_fluid_k* k, const orc_collider* col, int n,         \

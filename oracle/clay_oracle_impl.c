@@ -589,6 +589,37 @@ void FN(orc_integrate)(const orc_fluid_k* k, const orc_collider* col, int n, orc
   integrate_core(&c, col, n, st, contacts);
 }
 
+/* emit (fluid.ts:268-291): `count` particles on a jittered ball around `pos` into
+ * the ring at (base + i) % cap, velocity and staged velocity `vel`.  pow(x, 1/3)
+ * goes through the C library: parity unpinned against WGSL pow (tolerance-graded). */
+void FN(orc_emit)(int cap, int base, int count, uint32_t seed, const double* pos3, double radius,
+                  const double* vel3, orc_state* st) {
+  const T ep[3] = {rnd(pos3[0]), rnd(pos3[1]), rnd(pos3[2])};
+  const T ev[3] = {rnd(vel3[0]), rnd(vel3[1]), rnd(vel3[2])};
+  const T er = rnd(radius);
+  const T third = IS_F32 ? (T)(1.0f / 3.0f) : (T)(1.0 / 3.0);
+  for (int i = 0; i < count; i++) {
+    const int slot = (int)(((uint32_t)base + (uint32_t)i) % (uint32_t)cap);
+    const uint32_t s = seed * 9781u + (uint32_t)i;
+    const T r[3] = {(T)orc_hash1(s * 3u + 0u), (T)orc_hash1(s * 3u + 1u), (T)orc_hash1(s * 3u + 2u)};
+    v3 v = V(r[0] * 2 - 1 + KF(1e-4), r[1] * 2 - 1 + KF(1e-4), r[2] * 2 - 1 + KF(1e-4));
+    const T len = vlen(v);
+    const v3 dir = V(v.x / len, v.y / len, v.z / len);
+    const T sc = er * (IS_F32 ? (T)powf((float)r[0], (float)third) : (T)pow((double)r[0], (double)third));
+    ST(px)[slot] = ep[0] + dir.x * sc;
+    ST(py)[slot] = ep[1] + dir.y * sc;
+    ST(pz)[slot] = ep[2] + dir.z * sc;
+    ST(vx)[slot] = ev[0];
+    ST(vy)[slot] = ev[1];
+    ST(vz)[slot] = ev[2];
+    ST(rho)[slot] = 0;
+    ST(pres)[slot] = 0;
+    ST(nvx)[slot] = ev[0];
+    ST(nvy)[slot] = ev[1];
+    ST(nvz)[slot] = ev[2];
+  }
+}
+
 /* One substep: clearGrid + insert + density + forces + integrate (fluid.ts:550-556). */
 void FN(orc_substep)(const orc_fluid_k* k, const orc_collider* col, int n, orc_state* st,
                      orc_contacts* contacts, uint32_t* keys_out, uint32_t* perm_out) {

@@ -43,8 +43,11 @@ def test_one_substep_bit_exact(root, small):
     sim.destroy()
 
 
-def test_step_and_bake_bit_exact(root, small):
-    sim, orc, _ = make_pair(root, small)
+@pytest.mark.parametrize("mode", ["plain", "tiled"])
+def test_step_and_bake_bit_exact(root, small, mode):
+    # Tiled: at mips 2-3 a brick spans 12-48 world units and holds all 16k particles,
+    # so its list is cut into several chunks merged with atomicMin.
+    sim, orc, _ = make_pair(root, small, splat_mode=mode)
     sim.step()
     for _ in range(small["options"]["substeps"]):
         orc.substep()
