@@ -46,6 +46,20 @@ def algorithmic_bytes(kernel: str, particles: int, mips: int, live_voxels: int) 
     return None
 
 
+def committed_traffic(kernel: str, workload: str) -> float | None:
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/counters_<workload>.md), already
+    scaled by the copy-kernel calibration; None when there is none."""
+    path = ROOT / "profiles" / f"counters_{workload.replace('-', '_')}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def build_workload(root, name: str, splat: str):
     import clayhip
     from clayhip import scenes
@@ -119,7 +133,19 @@ def main() -> int:
     ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}))
+        return 0
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -166,8 +192,9 @@ def main() -> int:
     roofline = None
     if dom_bytes is not None:
         achieved = dom_bytes / dom_avg_s / 1e9
+        traffic = committed_traffic(dom_name, args.workload) if args.splat == "tiled" else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                     "kernel": dom_name, "avg_ms": round(dom_avg_s * 1e3, 4), "bytes_per_launch": dom_bytes}
     step_bytes = n * (2 * 124 + 12 * mips) + 16 * live_voxels
     line = {

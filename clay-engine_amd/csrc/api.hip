@@ -564,6 +564,7 @@ void splat_free(Splat* sp) {
   dfree(sp->brick_count);
   dfree(sp->brick_start);
   dfree(sp->brick_cursor);
+  dfree(sp->brick_sub);
   dfree(sp->brick_list);
   dfree(sp->brick_total_dev);
   dfree(sp->item_start);
@@ -627,7 +628,9 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
                                                      sp->vol->info.voxel_size, sp->mips);
     sp->brick_list_cap = (size_t)std::max<uint32_t>(cloud->capacity, 1) * per;
     if ((rc = dalloc(&sp->brick_count, sp->brick_total)) || (rc = dalloc(&sp->brick_start, sp->brick_total)) ||
-        (rc = dalloc(&sp->brick_cursor, sp->brick_total)) || (rc = dalloc(&sp->brick_list, sp->brick_list_cap)) ||
+        (rc = dalloc(&sp->brick_cursor, sp->brick_total * splat_bin_slices())) ||
+        (rc = dalloc(&sp->brick_sub, sp->brick_total * splat_bin_slices())) ||
+        (rc = dalloc(&sp->brick_list, sp->brick_list_cap)) ||
         (rc = dalloc(&sp->brick_total_dev, 4)) || (rc = dalloc(&sp->item_start, sp->brick_total)) ||
         (rc = dalloc(&sp->tile_live, sp->tile_total))) {
       splat_free(sp);
@@ -784,8 +787,7 @@ void fluid_free(Fluid* f) {
   dfree(f->keys2);
   dfree(f->vals2);
   dfree(f->hist);
-  dfree(f->cell_start);
-  dfree(f->cell_end);
+  dfree(f->cells);
   dfree(f->contacts);
   delete f;
 }
@@ -795,24 +797,28 @@ int fluid_substep_impl(Fluid* f, hipStream_t s) {
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
   int rc;
-  // clearGrid's contacts reset (fluid.ts:302-304) rides on the key pass.
+  // clearGrid's contacts reset (fluid.ts:302-304).
   if ((rc = launch_fill_u32(c, s, f->contacts, 1, 0u))) return rc;
   if (n == 0) return CLAY_OK;
+  // A new stamp per substep makes every older cell record stale at once.  After
+  // 2^32 - 1 substeps the stamps restart, and the table is wiped once so that no
+  // record from the previous cycle can carry a reused stamp.
+  if (++f->epoch == 0) {
+    if ((rc = launch_fill_u32(c, s, f->cells, 4 * f->grid_cells, 0u))) return rc;
+    f->epoch = 1;
+  }
   if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->keys, f->vals, invalid))) return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
-  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, invalid, f->cell_start, f->cell_end, src, dst, 6)))
-    return rc;
-  if ((rc = launch_density(c, s, f->k, n, f->spx, f->spy, f->spz, f->keys, f->cell_start, f->cell_end,
-                           f->srho, f->spres)))
+  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, invalid, f->cells, f->epoch, src, dst, 6))) return rc;
+  if ((rc = launch_density(c, s, f->k, n, f->spx, f->spy, f->spz, f->cells, f->epoch, f->srho, f->spres)))
     return rc;
   const float* sorted[8] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz, f->srho, f->spres};
   float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
   ColliderDev none{};
   const ColliderDev& col = f->world ? f->world->col : none;
-  return launch_forces_integrate(c, s, f->k, col, n, sorted, f->keys, f->vals, f->cell_start, f->cell_end,
-                                 out, f->contacts);
+  return launch_forces_integrate(c, s, f->k, col, n, sorted, f->vals, f->cells, f->epoch, out, f->contacts);
 }
 
 }  // namespace
@@ -934,12 +940,12 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
   int bits = 1;
   while (bits < 32 && (1ull << bits) <= G) bits++;
   f->key_bits = bits;
-  if ((rc = dalloc(&f->cell_start, G)) || (rc = dalloc(&f->cell_end, G))) {
+  if ((rc = dalloc(&f->cells, 4 * G))) {
     fluid_free(f);
     return rc;
   }
   hipStream_t s = c->stream;
-  if ((rc = launch_fill_u32(c, s, f->cell_start, G, 0xFFFFFFFFu)) || (rc = launch_fill_u32(c, s, f->cell_end, G, 0u)) ||
+  if ((rc = launch_fill_u32(c, s, f->cells, 4 * G, 0u)) ||
       (rc = launch_fill_u32(c, s, reinterpret_cast<uint32_t*>(f->st), (size_t)P * 11, 0u)) ||
       (rc = launch_fill_u32(c, s, f->contacts, 4 + 4 * CLAY_CONTACTS, 0u))) {
     fluid_free(f);

@@ -106,7 +106,8 @@ struct Splat {
   size_t brick_total = 0;
   uint32_t* brick_count = nullptr;   // per brick
   uint32_t* brick_start = nullptr;   // exclusive scan of counts
-  uint32_t* brick_cursor = nullptr;
+  uint32_t* brick_cursor = nullptr;  // per brick slice (binning fill cursors)
+  uint32_t* brick_sub = nullptr;     // per brick slice (binning counts)
   uint32_t* brick_list = nullptr;    // particle ids per brick
   size_t brick_list_cap = 0;
   uint32_t* brick_total_dev = nullptr;  // scan results: total entries, total work items
@@ -140,8 +141,8 @@ struct Fluid {
   uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
   uint32_t* hist = nullptr;      // digit histograms
   size_t hist_cap = 0;
-  uint32_t* cell_start = nullptr;  // dense over gridRes^3, validated by the sorted key
-  uint32_t* cell_end = nullptr;
+  uint32_t* cells = nullptr;     // dense over gridRes^3: {start, end, epoch, 0} per cell
+  uint32_t epoch = 0;            // stamp of the current substep's cell records
   size_t grid_cells = 0;
   int key_bits = 0;
   uint32_t* contacts = nullptr;  // [0] = count, then 32 x float4 (as bits)
@@ -155,16 +156,15 @@ int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float*
                 const float* py, const float* pz, uint32_t* keys, uint32_t* vals,
                 uint32_t invalid_key);
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
-                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cell_start,
-                   uint32_t* cell_end, const float* const* src, float* const* dst, int planes);
+                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cells, uint32_t epoch,
+                   const float* const* src, float* const* dst, int planes);
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                   const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cell_start, const uint32_t* cell_end, float* srho,
-                   float* spres);
+                   const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
+                   float* srho, float* spres);
 int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                            uint32_t n, const float* const* sorted, const uint32_t* skeys,
-                            const uint32_t* svals, const uint32_t* cell_start,
-                            const uint32_t* cell_end, float* const* out, uint32_t* contacts);
+                            uint32_t n, const float* const* sorted, const uint32_t* svals,
+                            const uint32_t* cells, uint32_t epoch, float* const* out,
+                            uint32_t* contacts);
 int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 // sort.hip
 size_t radix_hist_size(uint32_t n);
@@ -177,6 +177,7 @@ int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
                          std::vector<size_t>* brick_off, size_t* brick_total,
                          std::vector<size_t>* tile_off, size_t* tile_total);
 size_t splat_max_bricks_per_particle(double radius, double band, double voxel0, int mips);
+size_t splat_bin_slices();
 // volume.hip
 int launch_bake_brushes(Ctx* c, hipStream_t s, Volume* v, const clay_brush* brushes_dev, int n);
 int launch_field_eval(Ctx* c, hipStream_t s, const ColliderDev& col, int n, const float* pts,

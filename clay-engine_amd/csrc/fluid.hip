@@ -9,9 +9,9 @@
 //   k_forces_integrate  pressure + XSPH -> nvel, then advance, collide, contain
 //                     (fluid.ts:370-439 fused with :449-491)
 // The dense per-cell count grid of the reference (cleared every substep) is gone:
-// cell_start/cell_end are written only for occupied cells and an entry is trusted
-// only if the sorted key at `start` equals the cell's key, so stale entries from
-// earlier substeps are never read and nothing grid-sized is cleared per substep.
+// a cell record {start, end, stamp} is written only for occupied cells, stamped
+// with the substep's epoch, and trusted only when its stamp is the current epoch,
+// so stale records are never read and nothing grid-sized is cleared per substep.
 //
 // Forces and integrate can share one launch because the neighbour reads all hit
 // the sorted snapshot while the writes go to the id-ordered state: the snapshot is
@@ -39,19 +39,56 @@ __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) {
   return (int32_t)((uint32_t)a + (uint32_t)b);
 }
 
-// Looks up one cell: returns false for an empty (or out-of-grid) cell.
-__device__ __forceinline__ bool cell_span(const FluidK& k, uint32_t n, const uint32_t* skeys,
-                                          const uint32_t* cs, const uint32_t* ce, int32_t x,
-                                          int32_t y, int32_t z, uint32_t* s, uint32_t* e) {
-  if (!in_grid(k, x, y, z)) return false;
-  const uint32_t key = cell_index(k, x, y, z);
-  const uint32_t st = cs[key];
-  if (st >= n || skeys[st] != key) return false;
-  uint32_t en = ce[key];
-  const uint32_t cap = st + (uint32_t)k.bucket;
-  *s = st;
-  *e = en < cap ? en : cap;
-  return true;
+// Walks the 27-cell neighbourhood of cell b in the reference's order (dz, dy, dx
+// ascending, ids ascending inside a cell), calling visit(t) for the sorted slot t
+// of each of the first `bucket` particles of every cell.
+//
+// A cell record is {start, end, stamp}: it describes the current substep only if
+// its stamp equals this substep's epoch, so the table is never cleared.  The three
+// cells of one grid row have consecutive keys, so when none of them holds more
+// than `bucket` particles their particles form one contiguous run of the sorted
+// order and are walked as one loop (same order, same result).
+template <typename F>
+__device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __restrict__ cells,
+                                               uint32_t epoch, const int32_t b[3], F&& visit) {
+  for (int dz = -1; dz <= 1; dz++) {
+    const int32_t z = wrap_add(b[2], dz);
+    for (int dy = -1; dy <= 1; dy++) {
+      const int32_t y = wrap_add(b[1], dy);
+      uint32_t st[3], en[3];
+      bool ok[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const int32_t x = wrap_add(b[0], j - 1);
+        const bool in = in_grid(k, x, y, z);
+        uint4 c = make_uint4(0u, 0u, 0u, 0u);
+        if (in) c = cells[cell_index(k, x, y, z)];
+        ok[j] = in && c.z == epoch;
+        st[j] = c.x;
+        en[j] = c.y;
+      }
+      uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+      bool capped = false;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        if (!ok[j]) continue;
+        lo = st[j] < lo ? st[j] : lo;
+        hi = en[j] > hi ? en[j] : hi;
+        capped = capped || (en[j] - st[j] > (uint32_t)k.bucket);
+      }
+      if (!capped) {
+        for (uint32_t t = lo; t < hi; t++) visit(t);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          if (!ok[j]) continue;
+          const uint32_t cap = st[j] + (uint32_t)k.bucket;
+          const uint32_t e = en[j] < cap ? en[j] : cap;
+          for (uint32_t t = st[j]; t < e; t++) visit(t);
+        }
+      }
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const float* __restrict__ px,
@@ -72,17 +109,21 @@ struct Planes8 {
   float* dst[8];
 };
 
+// Cell records of this substep (run boundaries of the sorted keys) and the sorted
+// snapshot of the state planes.
 __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals,
-                                                    uint32_t invalid, uint32_t* __restrict__ cs,
-                                                    uint32_t* __restrict__ ce, Planes8 pl,
-                                                    int planes) {
+                                                    uint32_t invalid, uint32_t* __restrict__ cells,
+                                                    uint32_t epoch, Planes8 pl, int planes) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t key = skeys[s];
   if (key != invalid) {
-    if (s == 0 || skeys[s - 1] != key) cs[key] = s;
-    if (s == n - 1 || skeys[s + 1] != key) ce[key] = s + 1;
+    if (s == 0 || skeys[s - 1] != key) {
+      cells[4 * (size_t)key + 0] = s;
+      cells[4 * (size_t)key + 2] = epoch;
+    }
+    if (s == n - 1 || skeys[s + 1] != key) cells[4 * (size_t)key + 1] = s + 1;
   }
   const uint32_t j = svals[s];
 #pragma unroll
@@ -92,8 +133,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
 
 __global__ void __launch_bounds__(kBlock)
     k_density(FluidK k, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
-              const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
-              const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+              const float* __restrict__ spz, const uint4* __restrict__ cells, uint32_t epoch,
               float* __restrict__ srho, float* __restrict__ spres) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
@@ -101,22 +141,14 @@ __global__ void __launch_bounds__(kBlock)
   int32_t b[3];
   cell_of(k, pi, b);
   float rho = 0.0f;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        uint32_t st, en;
-        if (!cell_span(k, n, skeys, cs, ce, wrap_add(b[0], dx), wrap_add(b[1], dy),
-                       wrap_add(b[2], dz), &st, &en))
-          continue;
-        for (uint32_t t = st; t < en; t++) {
-          const f3 d = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
-          const float r2 = dot3(d, d);
-          if (r2 < k.hh) {
-            const float w = k.hh - r2;
-            rho = rho + k.mass_poly6 * (w * w * w);
-          }
-        }
-      }
+  for_neighbours(k, cells, epoch, b, [&](uint32_t t) {
+    const f3 d = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
+    const float r2 = dot3(d, d);
+    if (r2 < k.hh) {
+      const float w = k.hh - r2;
+      rho = rho + k.mass_poly6 * (w * w * w);
+    }
+  });
   srho[s] = fmax_s(rho, k.rest_floor);
   spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
 }
@@ -130,9 +162,8 @@ struct StateOut {
 
 __global__ void __launch_bounds__(kBlock)
     k_forces_integrate(FluidK k, ColliderDev col, uint32_t n, SortedIn in,
-                       const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
-                       const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
-                       StateOut out, uint32_t* __restrict__ contacts) {
+                       const uint32_t* __restrict__ svals, const uint4* __restrict__ cells,
+                       uint32_t epoch, StateOut out, uint32_t* __restrict__ contacts) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const float* __restrict__ spx = in.p[0];
@@ -152,32 +183,24 @@ __global__ void __launch_bounds__(kBlock)
   f3 fp = mk(0.0f, 0.0f, 0.0f);
   f3 vsum = vi * k.h6;
   float wsum = k.h6;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        uint32_t st, en;
-        if (!cell_span(k, n, skeys, cs, ce, wrap_add(b[0], dx), wrap_add(b[1], dy),
-                       wrap_add(b[2], dz), &st, &en))
-          continue;
-        for (uint32_t t = st; t < en; t++) {
-          if (t == s) continue;
-          const f3 dv = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
-          const float r2 = dot3(dv, dv);
-          if (r2 >= k.hh) continue;
-          const float inv = 1.0f / srho[t];
-          const float w0 = k.hh - r2;
-          const float w = w0 * w0 * w0;
-          vsum = mk(vsum.x + svx[t] * w, vsum.y + svy[t] * w, vsum.z + svz[t] * w);
-          wsum = wsum + w;
-          const float r = sqrtf(r2);
-          if (r > 1e-6f) {
-            const float hr = k.h - r;
-            const float s2 = k.mass * (pri + spres[t]) * 0.5f * inv * k.spiky * (hr * hr);
-            const float ir = 1.0f / r;
-            fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
-          }
-        }
-      }
+  for_neighbours(k, cells, epoch, b, [&](uint32_t t) {
+    if (t == s) return;
+    const f3 dv = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
+    const float r2 = dot3(dv, dv);
+    if (r2 >= k.hh) return;
+    const float inv = 1.0f / srho[t];
+    const float w0 = k.hh - r2;
+    const float w = w0 * w0 * w0;
+    vsum = mk(vsum.x + svx[t] * w, vsum.y + svy[t] * w, vsum.z + svz[t] * w);
+    wsum = wsum + w;
+    const float r = sqrtf(r2);
+    if (r > 1e-6f) {
+      const float hr = k.h - r;
+      const float s2 = k.mass * (pri + spres[t]) * 0.5f * inv * k.spiky * (hr * hr);
+      const float ir = 1.0f / r;
+      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+    }
+  });
   const float irho = 1.0f / rhoi;
   const f3 acc = mk(fp.x * irho + k.g[0], fp.y * irho + k.g[1], fp.z * irho + k.g[2]);
   const float tv = fclamp_s(k.visc, 0.0f, 1.0f);
@@ -287,8 +310,8 @@ int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float*
 }
 
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
-                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cell_start,
-                   uint32_t* cell_end, const float* const* src, float* const* dst, int planes) {
+                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cells, uint32_t epoch,
+                   const float* const* src, float* const* dst, int planes) {
   if (n == 0) return CLAY_OK;
   Planes8 pl{};
   for (int p = 0; p < planes && p < 8; p++) {
@@ -297,27 +320,26 @@ int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
   }
   LaunchTimer t(c, s, "fluid_reorder");
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, skeys, svals,
-                     invalid_key, cell_start, cell_end, pl, planes);
+                     invalid_key, cells, epoch, pl, planes);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                   const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cell_start, const uint32_t* cell_end, float* srho,
-                   float* spres) {
+                   const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
+                   float* srho, float* spres) {
   if (n == 0) return CLAY_OK;
   LaunchTimer t(c, s, "fluid_density");
-  hipLaunchKernelGGL(k_density, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, spx, spy, spz, skeys,
-                     cell_start, cell_end, srho, spres);
+  hipLaunchKernelGGL(k_density, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, spx, spy, spz,
+                     reinterpret_cast<const uint4*>(cells), epoch, srho, spres);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
 int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                            uint32_t n, const float* const* sorted, const uint32_t* skeys,
-                            const uint32_t* svals, const uint32_t* cell_start,
-                            const uint32_t* cell_end, float* const* out, uint32_t* contacts) {
+                            uint32_t n, const float* const* sorted, const uint32_t* svals,
+                            const uint32_t* cells, uint32_t epoch, float* const* out,
+                            uint32_t* contacts) {
   if (n == 0) return CLAY_OK;
   SortedIn in{};
   for (int p = 0; p < 8; p++) in.p[p] = sorted[p];
@@ -325,7 +347,7 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const Collid
   for (int p = 0; p < 11; p++) o.p[p] = out[p];
   LaunchTimer t(c, s, "fluid_forces_integrate");
   hipLaunchKernelGGL(k_forces_integrate, dim3(grid_for(n)), dim3(kBlock), 0, s, k, col, n, in,
-                     skeys, svals, cell_start, cell_end, o, contacts);
+                     svals, reinterpret_cast<const uint4*>(cells), epoch, o, contacts);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

@@ -44,6 +44,7 @@ constexpr int kBrick = 16;                  // voxels per brick edge
 constexpr int kBrickVox = kBrick * kBrick * kBrick;
 constexpr int kThreads = 256;
 constexpr uint32_t kChunk = 2048;           // particles per work item
+constexpr int kSlices = 32;                 // binning counters per brick
 constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared distance
 
 struct SplatMipK {
@@ -234,18 +235,22 @@ __global__ void __launch_bounds__(kThreads)
         const int32_t bz = blo[2] + q / (bext[0] * bext[1]);
         b = m.boff + (uint32_t)bx + (uint32_t)m.nb * ((uint32_t)by + (uint32_t)m.nb * (uint32_t)bz);
       }
-      const uint32_t slot = wave_add(counters, b, act);
+      // Each brick has kSlices counters; a workgroup uses slice blockIdx % kSlices,
+      // which spreads a coarse-mip brick's tens of thousands of wave atomics over
+      // kSlices addresses instead of serialising them on one.
+      const uint32_t slot = wave_add(counters, b * kSlices + blockIdx.x % kSlices, act);
       if (kFill && act) list[slot] = i;
     }
   }
 }
 
-// Binning pass 2: exclusive scans of the brick counts (-> list starts, cursors) and
-// of the per-brick chunk counts (-> first work item of each brick), one block.
+// Binning pass 2: per brick, the total of its slice counters (-> count) and, over all
+// bricks, exclusive scans of the totals (-> list starts; slice cursors) and of the
+// per-brick chunk counts (-> first work item of each brick).  One block.
 __global__ void __launch_bounds__(1024)
-    k_brick_scan(const uint32_t* __restrict__ count, uint32_t nbricks, uint32_t* __restrict__ start,
-                 uint32_t* __restrict__ cursor, uint32_t* __restrict__ item_start,
-                 ScanOut* __restrict__ out) {
+    k_brick_scan(const uint32_t* __restrict__ sub, uint32_t nbricks, uint32_t* __restrict__ count,
+                 uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
+                 uint32_t* __restrict__ item_start, ScanOut* __restrict__ out) {
   __shared__ uint32_t sa[1024], sb[1024];
   __shared__ uint32_t carry_a, carry_b;
   if (threadIdx.x == 0) {
@@ -255,7 +260,9 @@ __global__ void __launch_bounds__(1024)
   __syncthreads();
   for (uint32_t c0 = 0; c0 < nbricks; c0 += 1024) {
     const uint32_t i = c0 + threadIdx.x;
-    const uint32_t va = i < nbricks ? count[i] : 0u;
+    uint32_t va = 0;
+    if (i < nbricks)
+      for (int j = 0; j < kSlices; j++) va += sub[i * kSlices + j];
     const uint32_t vb = (va + kChunk - 1) / kChunk;
     sa[threadIdx.x] = va;
     sb[threadIdx.x] = vb;
@@ -270,8 +277,14 @@ __global__ void __launch_bounds__(1024)
     }
     const uint32_t ia = sa[threadIdx.x], ib = sb[threadIdx.x];
     if (i < nbricks) {
-      start[i] = carry_a + ia - va;
-      cursor[i] = carry_a + ia - va;
+      const uint32_t s0 = carry_a + ia - va;
+      count[i] = va;
+      start[i] = s0;
+      uint32_t run = s0;
+      for (int j = 0; j < kSlices; j++) {
+        cursor[i * kSlices + j] = run;
+        run += sub[i * kSlices + j];
+      }
       item_start[i] = carry_b + ib - vb;
     }
     __syncthreads();
@@ -382,7 +395,9 @@ __global__ void __launch_bounds__(kThreads)
           const float rem = m.cull2 - dz2 - dy2;
           if (rem < 0.0f) continue;
           // x-span of the reach ball on this row, widened by one voxel each side.
-          const float s = sqrtf(rem);
+          // The hardware square root (within 1 ulp) is enough here: the span is
+          // widened by a whole voxel on each side.
+          const float s = __builtin_amdgcn_sqrtf(rem);
           int32_t xa = sat_i32(floorf((p.x - s) * m.inv - 0.5f)) - 1;
           int32_t xb = sat_i32(floorf((p.x + s) * m.inv - 0.5f)) + 1;
           xa = xa > x0 ? xa : x0;
@@ -542,18 +557,18 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
   const unsigned nbricks = (unsigned)sp->brick_total;
   ScanOut* scan = reinterpret_cast<ScanOut*>(sp->brick_total_dev);
-  int rc = launch_fill_u32(ctx, s, sp->brick_count, sp->brick_total, 0u);
+  int rc = launch_fill_u32(ctx, s, sp->brick_sub, sp->brick_total * kSlices, 0u);
   if (rc) return rc;
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_bin_count");
     hipLaunchKernelGGL(k_bin<false>, dim3(pgrid), dim3(kThreads), 0, s, mk_, c, sp->vol->dev,
-                       sp->brick_count, nullptr);
+                       sp->brick_sub, nullptr);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   {
     LaunchTimer t(ctx, s, "splat_brick_scan");
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_count, (uint32_t)nbricks,
-                       sp->brick_start, sp->brick_cursor, sp->item_start, scan);
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_sub, (uint32_t)nbricks,
+                       sp->brick_count, sp->brick_start, sp->brick_cursor, sp->item_start, scan);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   if (c.count) {
@@ -636,6 +651,8 @@ int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res, std::ve
   *tile_total = to;
   return CLAY_OK;
 }
+
+size_t splat_bin_slices() { return kSlices; }
 
 size_t splat_max_bricks_per_particle(double radius, double band, double voxel0, int mips) {
   size_t worst = 1;

@@ -120,6 +120,7 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_substep_{s}").argtypes = [K, COL, I, ST, CT, P, P]
             getattr(L, f"orc_neighbors_{s}").argtypes = [K, I, P, P, P, I, P, I]
             getattr(L, f"orc_neighbors_{s}").restype = C.c_int
+            getattr(L, f"orc_emit_{s}").argtypes = [I, I, I, C.c_uint32, P, C.c_double, P, ST]
             getattr(L, f"orc_density_{s}").argtypes = [K, I, ST]
             getattr(L, f"orc_forces_{s}").argtypes = [K, I, ST]
             getattr(L, f"orc_integrate_{s}").argtypes = [K, COL, I, ST, CT]
@@ -275,25 +276,34 @@ class Collider:
 
 
 class Fluid:
-    """Oracle state of one FluidSim: SoA arrays in f32 ("WGSL-f32") or f64."""
+    """Oracle state of one FluidSim: SoA arrays of `capacity` entries in f32
+    ("WGSL-f32") or f64, the live prefix `n`, and the emission ring bookkeeping of
+    FluidSim.spawn/step (fluid.ts:511-549)."""
 
     FIELDS = ("px", "py", "pz", "vx", "vy", "vz", "rho", "pres", "nvx", "nvy", "nvz")
 
-    def __init__(self, options: dict, pos: np.ndarray, vel: np.ndarray | None = None,
-                 collider: Collider | None = None, precision: str = "f32"):
+    def __init__(self, options: dict, pos: np.ndarray | None = None, vel: np.ndarray | None = None,
+                 collider: Collider | None = None, precision: str = "f32", capacity: int | None = None):
         self.precision = precision
         self.dt = np.float32 if precision == "f32" else np.float64
+        self.options = dict(options)
         self.k = fluid_consts(options)
         self.collider = collider
-        self.n = pos.shape[0]
-        self.a = {f: np.zeros(self.n, dtype=self.dt) for f in self.FIELDS}
+        self.n = 0 if pos is None else pos.shape[0]
+        self.capacity = capacity if capacity is not None else max(self.n, 1)
+        self.a = {f: np.zeros(self.capacity, dtype=self.dt) for f in self.FIELDS}
         for i, ax in enumerate("xyz"):
-            self.a["p" + ax][:] = pos[:, i]
+            if pos is not None:
+                self.a["p" + ax][: self.n] = pos[:, i]
             if vel is not None:
-                self.a["v" + ax][:] = vel[:, i]
+                self.a["v" + ax][: self.n] = vel[:, i]
         self.contacts = OrcContacts()
-        self.keys = np.zeros(self.n, dtype=np.uint32)
-        self.perm = np.zeros(self.n, dtype=np.uint32)
+        self.keys = np.zeros(self.capacity, dtype=np.uint32)
+        self.perm = np.zeros(self.capacity, dtype=np.uint32)
+        self.cursor = self.n % self.capacity
+        self.pending = 0
+        self.seed = 1
+        self.emit_args = None
 
     def _state(self) -> OrcState:
         st = OrcState()
@@ -308,11 +318,35 @@ class Fluid:
             C.byref(self.k), col, self.n, C.byref(st), C.byref(self.contacts),
             self.keys.ctypes.data_as(C.c_void_p), self.perm.ctypes.data_as(C.c_void_p))
 
+    def spawn(self, count: int, pos, vel, spread: float | None = None) -> None:
+        """FluidSim.spawn (fluid.ts:511-525)."""
+        spacing = self.options.get("spacing", DEFAULTS["spacing"])
+        if spread is None:
+            spread = spacing * max(count, 1) ** (1.0 / 3.0) * 0.62
+        self.pending = min(count, self.capacity)
+        self.emit_args = (tuple(map(float, pos)), float(spread), tuple(map(float, vel)), self.seed)
+        self.seed += 1
+
+    def emit_pending(self) -> None:
+        """The emission half of FluidSim.step (fluid.ts:537-549)."""
+        count, self.pending = self.pending, 0
+        if count <= 0:
+            return
+        base = self.cursor
+        self.cursor = (base + count) % self.capacity
+        self.n = min(self.capacity, max(self.n, base + count))
+        pos, spread, vel, seed = self.emit_args
+        st = self._state()
+        getattr(lib(), f"orc_emit_{self.precision}")(
+            self.capacity, base, count, seed, (C.c_double * 3)(*pos), C.c_double(spread),
+            (C.c_double * 3)(*vel), C.byref(st))
+
     def get(self, what: str) -> np.ndarray:
+        n = self.n
         if what in ("pos", "vel", "nvel"):
             pre = {"pos": "p", "vel": "v", "nvel": "nv"}[what]
-            return np.stack([self.a[pre + ax] for ax in "xyz"], axis=-1)
-        return self.a[{"density": "rho", "pressure": "pres"}[what]].copy()
+            return np.stack([self.a[pre + ax][:n] for ax in "xyz"], axis=-1)
+        return self.a[{"density": "rho", "pressure": "pres"}[what]][:n].copy()
 
     def read_contacts(self):
         n = min(self.contacts.count, CONTACTS)

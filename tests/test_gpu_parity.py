@@ -241,6 +241,30 @@ def test_set_center_follows_and_far_origin_saturates(root):
         s.destroy()
 
 
+def test_spawn_emits_into_the_ring_like_the_oracle(root, small):
+    """FluidSim.spawn + step (fluid.ts:268-291, 511-549): emission slots, live count,
+    ring wrap.  Emission uses pow(x, 1/3), whose last bit is the math library's
+    (parity unpinned), so positions are compared to 1e-5 after the step's substeps."""
+    opts = dict(small["options"], capacity=1500)
+    brushes = small["collider"]["brushes"]
+    world = clayhip.analyticField(root, brushes)
+    sim = clayhip.FluidSim(root, world, opts)
+    orc = O.Fluid(opts, None, collider=O.Collider().add_analytic([clayhip.make_brush(b) for b in brushes]),
+                  capacity=1500)
+    for rep, (pos, vel) in enumerate([((0.0, 2.0, 0.0), (0.0, -1.0, 0.0)), ((0.3, 2.5, -0.2), (1.0, 0.0, 0.0))]):
+        sim.spawn(1000, pos, vel)
+        orc.spawn(1000, pos, vel)
+        sim.step()
+        orc.emit_pending()
+        for _ in range(opts["substeps"]):
+            orc.substep()
+        assert sim.particleCount == orc.n == (1000 if rep == 0 else 1500)
+        got, want = sim.get_state("pos"), orc.get("pos")
+        assert np.abs(got - want).max() < 1e-5, describe_diff("pos", got, want)
+    sim.destroy()
+    world.destroy()
+
+
 def test_empty_fluid_bakes_saturated(root, small):
     world = clayhip.analyticField(root, small["collider"]["brushes"])
     sim = clayhip.FluidSim(root, world, dict(small["options"], splatMode="tiled"))
