@@ -60,7 +60,7 @@ def committed_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
-def build_workload(root, name: str, splat: str):
+def build_workload(root, name: str, splat: str, precision: str = "f32"):
     import clayhip
     from clayhip import scenes
 
@@ -74,7 +74,7 @@ def build_workload(root, name: str, splat: str):
         vol.bakeBrushes(spec["brushes"])
         keep.append(vol)
         world = clayhip.volumeField(vol)
-    opts = dict(scene["options"], splatMode=splat)
+    opts = dict(scene["options"], splatMode=splat, precision=precision)
     sim = clayhip.FluidSim(root, world, opts)
     sim.set_state(scene["pos"])
     return scene, sim, world, keep
@@ -131,6 +131,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="dam-break-1m", choices=["dam-break-1m", "dam-break-16k"])
     ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"])
+    ap.add_argument("--precision", default="f32", choices=["f32", "fp16lds"],
+                    help="fluid kernels: IEEE f32 (bit-exact) or FP16-LDS staged neighbours")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -158,7 +160,7 @@ def main() -> int:
     import clayhip
 
     root = clayhip.Root(0)
-    scene, sim, world, keep = build_workload(root, args.workload, args.splat)
+    scene, sim, world, keep = build_workload(root, args.workload, args.splat, args.precision)
     n = scene["pos"].shape[0]
     mips = scene["options"]["bakeMipLevels"]
     stream = torch.cuda.Stream()
@@ -216,6 +218,7 @@ def main() -> int:
             "substeps": scene["options"]["substeps"],
             "bake": f"{scene['options']['bakeResolution']}^3 x {mips} mips",
             "splat": args.splat,
+            "precision": args.precision,
             "live_voxels": live_voxels,
         },
         "step_roofline": {"bytes_per_step": step_bytes,

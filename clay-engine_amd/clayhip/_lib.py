@@ -133,7 +133,12 @@ class FluidDesc(C.Structure):
         ("viscosity", C.c_double),
         ("damping", C.c_double),
         ("max_speed", C.c_double),
+        ("precision", C.c_int32),
+        ("pad", C.c_int32),
     ]
+
+
+PRECISION_F32, PRECISION_FP16LDS = 0, 1
 
 
 class FluidParams(C.Structure):
@@ -235,6 +240,7 @@ SIGNATURES = {
     "clay_fluid_step": ([P, P], C.c_int),
     "clay_fluid_substep": ([P, P], C.c_int),
     "clay_fluid_read_contacts": ([P, P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    "clay_fluid_conserved": ([P, DP], C.c_int),
     "clay_fluid_particle_count": ([P, C.POINTER(U32)], C.c_int),
     "clay_fluid_surface": ([P, PP], C.c_int),
     "clay_fluid_debug_download": ([P, I32, P], C.c_int),

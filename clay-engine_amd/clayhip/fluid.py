@@ -59,6 +59,8 @@ def fluid_desc(options: dict | None = None) -> L.FluidDesc:
                 arr[i] = float(value[i])
         elif key == "splatMode":
             d.splat_mode = {"plain": L.SPLAT_PLAIN, "tiled": L.SPLAT_TILED}[value]
+        elif key == "precision":
+            d.precision = {"f32": L.PRECISION_F32, "fp16lds": L.PRECISION_FP16LDS}[value]
         elif key == "label":
             pass
         else:
@@ -148,6 +150,12 @@ class FluidSim:
         L.check(L.lib().clay_fluid_read_contacts(self._h, pts.ctypes.data, C.byref(n), C.byref(raw)))
         self.lastContactCount = raw.value
         return [tuple(float(v) for v in pts[i]) for i in range(n.value)]
+
+    def conserved(self) -> dict:
+        """Total mass, momentum and centre of mass of the live particles, reduced in FP64."""
+        out = (C.c_double * 7)()
+        L.check(L.lib().clay_fluid_conserved(self._h, out))
+        return {"mass": out[0], "momentum": tuple(out[1:4]), "com": tuple(out[4:7])}
 
     # ---- state (the build's seeded-lattice entry and readback) ----
     def set_state(self, pos: np.ndarray, vel: np.ndarray | None = None) -> None:

@@ -789,6 +789,7 @@ void fluid_free(Fluid* f) {
   dfree(f->hist);
   dfree(f->cells);
   dfree(f->contacts);
+  dfree(f->reduce);
   delete f;
 }
 
@@ -812,13 +813,16 @@ int fluid_substep_impl(Fluid* f, hipStream_t s) {
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
   if ((rc = launch_reorder(c, s, n, f->keys, f->vals, invalid, f->cells, f->epoch, src, dst, 6))) return rc;
-  if ((rc = launch_density(c, s, f->k, n, f->spx, f->spy, f->spz, f->cells, f->epoch, f->srho, f->spres)))
+  const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
+  if ((rc = (lds ? launch_density_lds : launch_density)(c, s, f->k, n, f->spx, f->spy, f->spz, f->cells,
+                                                        f->epoch, f->srho, f->spres)))
     return rc;
   const float* sorted[8] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz, f->srho, f->spres};
   float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
   ColliderDev none{};
   const ColliderDev& col = f->world ? f->world->col : none;
-  return launch_forces_integrate(c, s, f->k, col, n, sorted, f->vals, f->cells, f->epoch, out, f->contacts);
+  return (lds ? launch_forces_integrate_lds : launch_forces_integrate)(c, s, f->k, col, n, sorted, f->vals,
+                                                                       f->cells, f->epoch, out, f->contacts);
 }
 
 }  // namespace
@@ -827,6 +831,8 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
   if (!ctx || !d || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_create: NULL argument");
   if (d->capacity == 0) return fail(CLAY_ERR_INVALID, "FluidSim: capacity must be positive");
   if (!(d->spacing > 0)) return fail(CLAY_ERR_INVALID, "FluidSim: spacing must be positive");
+  if (d->precision != CLAY_PRECISION_F32 && d->precision != CLAY_PRECISION_FP16LDS)
+    return fail(CLAY_ERR_INVALID, "FluidSim: unknown precision mode");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   Fluid* f = new (std::nothrow) Fluid();
   if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
@@ -888,6 +894,11 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
   if ((rc = dalloc(&f->st, (size_t)P * 11)) || (rc = dalloc(&f->ss, (size_t)P * 8)) ||
       (rc = dalloc(&f->keys, P)) || (rc = dalloc(&f->vals, P)) || (rc = dalloc(&f->keys2, P)) ||
       (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, 4 + 4 * CLAY_CONTACTS))) {
+    fluid_free(f);
+    return rc;
+  }
+  f->reduce_blocks = std::min<uint32_t>(1024, std::max<uint32_t>(1, (P + 255) / 256));
+  if ((rc = dalloc(&f->reduce, (size_t)f->reduce_blocks * 6 + 8))) {
     fluid_free(f);
     return rc;
   }
@@ -1083,6 +1094,19 @@ extern "C" int clay_fluid_read_contacts(clay_fluid* fh, float* out4, uint32_t* n
   if (out4) memcpy(out4, buf + 4, sizeof(float) * 4 * cnt);
   if (n) *n = cnt;
   if (raw) *raw = buf[0];
+  return CLAY_OK;
+}
+
+extern "C" int clay_fluid_conserved(clay_fluid* fh, double out[7]) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_conserved: NULL argument");
+  hipStream_t s = f->ctx->stream;
+  const float* pv[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
+  double* res = f->reduce + (size_t)f->reduce_blocks * 6;
+  int rc = launch_conserved(f->ctx, s, f->live, pv, f->consts.mass, f->reduce, f->reduce_blocks, res);
+  if (rc) return rc;
+  CLAY_HIP_CHECK(hipMemcpyAsync(out, res, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
   return CLAY_OK;
 }
 

@@ -146,6 +146,8 @@ struct Fluid {
   size_t grid_cells = 0;
   int key_bits = 0;
   uint32_t* contacts = nullptr;  // [0] = count, then 32 x float4 (as bits)
+  double* reduce = nullptr;      // FP64 reduction partials (+ 8 results at the end)
+  uint32_t reduce_blocks = 0;
   Splat* surface = nullptr;      // owned
   FluidK k{};
 };
@@ -166,6 +168,15 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const Collid
                             const uint32_t* cells, uint32_t epoch, float* const* out,
                             uint32_t* contacts);
 int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+int launch_density_lds(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
+                       const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
+                       float* srho, float* spres);
+int launch_forces_integrate_lds(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
+                                uint32_t n, const float* const* sorted, const uint32_t* svals,
+                                const uint32_t* cells, uint32_t epoch, float* const* out,
+                                uint32_t* contacts);
+int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, double mass,
+                     double* partial, uint32_t partial_blocks, double* out);
 // sort.hip
 size_t radix_hist_size(uint32_t n);
 int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,

@@ -160,6 +160,11 @@ struct StateOut {
   float* p[11];  // px py pz vx vy vz rho pres nvx nvy nvz
 };
 
+__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t s, f3 pi,
+                                                f3 vi, float rhoi, float pri, f3 fp, f3 vsum, float wsum,
+                                                const uint32_t* __restrict__ svals, const StateOut& out,
+                                                uint32_t* __restrict__ contacts);
+
 __global__ void __launch_bounds__(kBlock)
     k_forces_integrate(FluidK k, ColliderDev col, uint32_t n, SortedIn in,
                        const uint32_t* __restrict__ svals, const uint4* __restrict__ cells,
@@ -201,6 +206,15 @@ __global__ void __launch_bounds__(kBlock)
       fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
     }
   });
+  finish_particle(k, col, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out, contacts);
+}
+
+// The end of the force pass (fluid.ts:430-437) and the integrate pass
+// (fluid.ts:449-491) for one particle, writing its id-ordered state.
+__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t s, f3 pi,
+                                                f3 vi, float rhoi, float pri, f3 fp, f3 vsum, float wsum,
+                                                const uint32_t* __restrict__ svals, const StateOut& out,
+                                                uint32_t* __restrict__ contacts) {
   const float irho = 1.0f / rhoi;
   const f3 acc = mk(fp.x * irho + k.g[0], fp.y * irho + k.g[1], fp.z * irho + k.g[2]);
   const float tv = fclamp_s(k.visc, 0.0f, 1.0f);
@@ -247,6 +261,325 @@ __global__ void __launch_bounds__(kBlock)
   out.p[8][j] = nv.x;
   out.p[9][j] = nv.y;
   out.p[10][j] = nv.z;
+}
+
+// ---------------------------------------------------------------------------
+// Precision variant: FP16 cell-relative offsets staged in LDS, compensated
+// density sums.  Tolerance-graded against the f64 oracle; FMA contraction is
+// allowed in the neighbour loops.
+//
+// For each of the 9 rows of the 27-cell neighbourhood, a wave takes the union of
+// its lanes' neighbour runs (consecutive sorted particles share most of them),
+// stages that run in its LDS slice -- positions as f16 offsets from the centre of
+// each particle's own cell (|offset| <= h/2), the other fields as f32 -- and every
+// lane walks its three cells out of LDS.  A lane knows which of its cells a staged
+// index falls in, so it rebuilds p_i - p_j = (o_i - o_j) - (c_j - c_i) h with o_i
+// in full f32.  A row whose union is wider than the slice, or whose cells
+// overflow the bucket, is walked from global memory instead.
+// ---------------------------------------------------------------------------
+constexpr int kStage = 128;  // staged neighbours per wave per row
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Neumaier's compensated sum.
+struct KahanSum {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+      c += (s - t) + x;
+    else
+      c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// One row (dz, dy) of a particle's neighbourhood: its three cells' run [lo, hi)
+// when no cell overflows the bucket (capped = false), else the per-cell spans.
+struct RowRun {
+  uint32_t st[3], en[3];
+  bool ok[3];
+  uint32_t lo, hi;
+  bool capped;
+};
+
+__device__ __forceinline__ RowRun row_run(const FluidK& k, const uint4* __restrict__ cells, uint32_t epoch,
+                                          const int32_t b[3], int32_t y, int32_t z, bool active) {
+  RowRun r;
+  r.lo = 0xFFFFFFFFu;
+  r.hi = 0u;
+  r.capped = false;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int32_t x = wrap_add(b[0], j - 1);
+    const bool in = active && in_grid(k, x, y, z);
+    uint4 c = make_uint4(0u, 0u, 0u, 0u);
+    if (in) c = cells[cell_index(k, x, y, z)];
+    r.ok[j] = in && c.z == epoch;
+    r.st[j] = c.x;
+    r.en[j] = c.y;
+    if (r.ok[j]) {
+      r.lo = c.x < r.lo ? c.x : r.lo;
+      r.hi = c.y > r.hi ? c.y : r.hi;
+      r.capped = r.capped || (c.y - c.x > (uint32_t)k.bucket);
+    }
+  }
+  return r;
+}
+
+template <typename F>
+__device__ __forceinline__ void walk_row(const FluidK& k, const RowRun& r, F&& visit) {
+  if (!r.capped) {
+    for (uint32_t t = r.lo; t < r.hi; t++) visit(t);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (!r.ok[j]) continue;
+      const uint32_t cap = r.st[j] + (uint32_t)k.bucket;
+      const uint32_t e = r.en[j] < cap ? r.en[j] : cap;
+      for (uint32_t t = r.st[j]; t < e; t++) visit(t);
+    }
+  }
+}
+
+// The same visits as walk_row, cell by cell, passing the cell's x step (0..2).
+template <typename F>
+__device__ __forceinline__ void walk_cells(const FluidK& k, const RowRun& r, F&& visit) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    if (!r.ok[j]) continue;
+    const uint32_t cap = r.st[j] + (uint32_t)k.bucket;
+    const uint32_t e = r.en[j] < cap ? r.en[j] : cap;
+    for (uint32_t t = r.st[j]; t < e; t++) visit(t, j);
+  }
+}
+
+// Offset of p from the centre of its own cell b.
+__device__ __forceinline__ f3 cell_rel(const FluidK& k, f3 p, const int32_t b[3]) {
+  return mk(p.x - (k.go[0] + ((float)b[0] + 0.5f) * k.h), p.y - (k.go[1] + ((float)b[1] + 0.5f) * k.h),
+            p.z - (k.go[2] + ((float)b[2] + 0.5f) * k.h));
+}
+__device__ __forceinline__ f3 own_cell_rel(const FluidK& k, f3 p) {
+  int32_t c[3];
+  cell_of(k, p, c);
+  return cell_rel(k, p, c);
+}
+
+__device__ __forceinline__ uint2 pack_off(f3 d) {
+  return make_uint2((uint32_t)f2h(d.x) | ((uint32_t)f2h(d.y) << 16), (uint32_t)f2h(d.z));
+}
+__device__ __forceinline__ f3 unpack_off(uint2 u) {
+  return mk(h2f((uint16_t)(u.x & 0xffffu)), h2f((uint16_t)(u.x >> 16)), h2f((uint16_t)(u.y & 0xffffu)));
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_density_lds(FluidK k, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
+                  const float* __restrict__ spz, const uint4* __restrict__ cells, uint32_t epoch,
+                  float* __restrict__ srho, float* __restrict__ spres) {
+#pragma clang fp contract(fast)
+  __shared__ uint2 stage[kWaves][kStage];
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = s < n;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  f3 pi = mk(0.0f, 0.0f, 0.0f);
+  if (active) pi = mk(spx[s], spy[s], spz[s]);
+  int32_t b[3] = {0, 0, 0};
+  if (active) cell_of(k, pi, b);
+  const f3 oi = cell_rel(k, pi, b);
+  KahanSum rho;
+  auto term = [&](f3 d) {
+    const float r2 = dot3(d, d);
+    if (r2 < k.hh) {
+      const float w = k.hh - r2;
+      rho.add(k.mass_poly6 * (w * w * w));
+    }
+  };
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const RowRun r = row_run(k, cells, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const bool any = r.hi > r.lo;
+      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kStage;
+      if (staged)
+        for (uint32_t t = ulo + lane; t < uhi; t += 64)
+          stage[wave][t - ulo] = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
+      __syncthreads();
+      if (staged) {
+        const float ey = (float)dy * k.h, ez = (float)dz * k.h;
+        walk_cells(k, r, [&](uint32_t t, int j) {
+          const f3 d = oi - unpack_off(stage[wave][t - ulo]);
+          term(mk(d.x - (float)(j - 1) * k.h, d.y - ey, d.z - ez));
+        });
+      } else {
+        walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
+      }
+      __syncthreads();
+    }
+  }
+  if (!active) return;
+  const float rv = rho.value();
+  srho[s] = fmax_s(rv, k.rest_floor);
+  spres[s] = fmax_s(k.stiff * (rv - k.rest), 0.0f);
+}
+
+struct StagedNbr {
+  uint2 off;   // f16 position offset from the centre of the particle's cell
+  float4 vr;   // velocity and density
+  float pres;
+};
+
+__global__ void __launch_bounds__(kBlock)
+    k_forces_integrate_lds(FluidK k, ColliderDev col, uint32_t n, SortedIn in,
+                           const uint32_t* __restrict__ svals, const uint4* __restrict__ cells,
+                           uint32_t epoch, StateOut out, uint32_t* __restrict__ contacts) {
+#pragma clang fp contract(fast)
+  __shared__ StagedNbr stage[kWaves][kStage];
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = s < n;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float* __restrict__ spx = in.p[0];
+  const float* __restrict__ spy = in.p[1];
+  const float* __restrict__ spz = in.p[2];
+  const float* __restrict__ svx = in.p[3];
+  const float* __restrict__ svy = in.p[4];
+  const float* __restrict__ svz = in.p[5];
+  const float* __restrict__ srho = in.p[6];
+  const float* __restrict__ spres = in.p[7];
+  f3 pi = mk(0.0f, 0.0f, 0.0f), vi = mk(0.0f, 0.0f, 0.0f);
+  float rhoi = 1.0f, pri = 0.0f;
+  int32_t b[3] = {0, 0, 0};
+  if (active) {
+    pi = mk(spx[s], spy[s], spz[s]);
+    vi = mk(svx[s], svy[s], svz[s]);
+    rhoi = srho[s];
+    pri = spres[s];
+    cell_of(k, pi, b);
+  }
+  const f3 oi = cell_rel(k, pi, b);
+  f3 fp = mk(0.0f, 0.0f, 0.0f);
+  f3 vsum = vi * k.h6;
+  float wsum = k.h6;
+  auto term = [&](uint32_t t, f3 dv, f3 vj, float rhoj, float prj) {
+    if (t == s) return;
+    const float r2 = dot3(dv, dv);
+    if (r2 >= k.hh) return;
+    const float inv = 1.0f / rhoj;
+    const float w0 = k.hh - r2;
+    const float w = w0 * w0 * w0;
+    vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
+    wsum = wsum + w;
+    const float r = sqrtf(r2);
+    if (r > 1e-6f) {
+      const float hr = k.h - r;
+      const float s2 = k.mass * (pri + prj) * 0.5f * inv * k.spiky * (hr * hr);
+      const float ir = 1.0f / r;
+      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+    }
+  };
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const RowRun r = row_run(k, cells, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const bool any = r.hi > r.lo;
+      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kStage;
+      if (staged)
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) {
+          StagedNbr e;
+          e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
+          e.vr = make_float4(svx[t], svy[t], svz[t], srho[t]);
+          e.pres = spres[t];
+          stage[wave][t - ulo] = e;
+        }
+      __syncthreads();
+      if (staged) {
+        const float ey = (float)dy * k.h, ez = (float)dz * k.h;
+        walk_cells(k, r, [&](uint32_t t, int j) {
+          const StagedNbr& e = stage[wave][t - ulo];
+          const f3 d = oi - unpack_off(e.off);
+          term(t, mk(d.x - (float)(j - 1) * k.h, d.y - ey, d.z - ez), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w,
+               e.pres);
+        });
+      } else {
+        walk_row(k, r, [&](uint32_t t) {
+          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), srho[t], spres[t]);
+        });
+      }
+      __syncthreads();
+    }
+  }
+  if (!active) return;
+  finish_particle(k, col, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out, contacts);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 conserved quantities: total mass, momentum and centre of mass, reduced in
+// a fixed order (per-block partials, then one block) so the result is the same
+// on every run.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+    k_conserved_partial(uint32_t n, const float* __restrict__ px, const float* __restrict__ py,
+                        const float* __restrict__ pz, const float* __restrict__ vx,
+                        const float* __restrict__ vy, const float* __restrict__ vz,
+                        double* __restrict__ partial) {
+  __shared__ double sh[6][kBlock];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    acc[0] += (double)vx[i];
+    acc[1] += (double)vy[i];
+    acc[2] += (double)vz[i];
+    acc[3] += (double)px[i];
+    acc[4] += (double)py[i];
+    acc[5] += (double)pz[i];
+  }
+  for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int q = 0; q < 6; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_conserved_final(uint32_t blocks, uint32_t n, double mass, const double* __restrict__ partial,
+                      double* __restrict__ out) {
+  __shared__ double sh[6][kBlock];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t i = threadIdx.x; i < blocks; i += kBlock)
+    for (int q = 0; q < 6; q++) acc[q] += partial[i * 6 + q];
+  for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int q = 0; q < 6; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = mass * (double)n;
+    out[0] = m;                                  // total mass
+    for (int q = 0; q < 3; q++) out[1 + q] = mass * sh[q][0];           // momentum
+    for (int q = 0; q < 3; q++) out[4 + q] = n ? sh[3 + q][0] / n : 0;  // centre of mass
+  }
 }
 
 // hash1 / hash3 (math/gpu.ts:103-117): lowbias32, low 24 bits scaled to [0,1).
@@ -379,6 +712,47 @@ int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
   if (blocks > 65536) blocks = 65536;
   LaunchTimer t(c, s, "fill_u32");
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+}  // namespace clay
+
+namespace clay {
+
+int launch_density_lds(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
+                       const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
+                       float* srho, float* spres) {
+  if (n == 0) return CLAY_OK;
+  LaunchTimer t(c, s, "fluid_density_lds");
+  hipLaunchKernelGGL(k_density_lds, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, spx, spy, spz,
+                     reinterpret_cast<const uint4*>(cells), epoch, srho, spres);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int launch_forces_integrate_lds(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
+                                uint32_t n, const float* const* sorted, const uint32_t* svals,
+                                const uint32_t* cells, uint32_t epoch, float* const* out,
+                                uint32_t* contacts) {
+  if (n == 0) return CLAY_OK;
+  SortedIn in{};
+  for (int p = 0; p < 8; p++) in.p[p] = sorted[p];
+  StateOut o{};
+  for (int p = 0; p < 11; p++) o.p[p] = out[p];
+  LaunchTimer t(c, s, "fluid_forces_integrate_lds");
+  hipLaunchKernelGGL(k_forces_integrate_lds, dim3(grid_for(n)), dim3(kBlock), 0, s, k, col, n, in, svals,
+                     reinterpret_cast<const uint4*>(cells), epoch, o, contacts);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, double mass,
+                     double* partial, uint32_t partial_blocks, double* out) {
+  LaunchTimer t(c, s, "fluid_conserved_fp64");
+  hipLaunchKernelGGL(k_conserved_partial, dim3(partial_blocks), dim3(kBlock), 0, s, n, pv[0], pv[1], pv[2],
+                     pv[3], pv[4], pv[5], partial);
+  hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(kBlock), 0, s, partial_blocks, n, mass, partial, out);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

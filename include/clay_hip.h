@@ -216,7 +216,16 @@ typedef struct {
   double viscosity;         /* 0.4 */
   double damping;           /* 0.995 */
   double max_speed;         /* 15 */
+  int32_t precision;        /* CLAY_PRECISION_F32 (bit-exact) | CLAY_PRECISION_FP16LDS */
+  int32_t pad;
 } clay_fluid_desc;
+
+/* Arithmetic of the density and force passes.  F32: every operation rounds once
+ * in the reference's order (bit-exact against an IEEE-f32 evaluation).  FP16LDS:
+ * neighbour positions staged in LDS as f16 offsets from a cell corner,
+ * compensated f32 density sums, fused multiply-add allowed. */
+#define CLAY_PRECISION_F32 0
+#define CLAY_PRECISION_FP16LDS 1
 
 /* The reference's defaults (fluid.ts:141-203). */
 void clay_fluid_desc_default(clay_fluid_desc* d);
@@ -276,6 +285,11 @@ int clay_fluid_substep(clay_fluid* f, clay_stream stream);
 /* FluidSim.readContacts: fluid.ts:565-569.  out4 holds min(raw, 32) points
  * (x, y, z, speed); raw_count is the atomic counter, which may exceed 32. */
 int clay_fluid_read_contacts(clay_fluid* f, float* out4, uint32_t* n, uint32_t* raw_count);
+/* Conserved quantities of the live particles in FP64, reduced in a fixed order:
+ * out = {total mass, momentum x, y, z, centre of mass x, y, z}.  The reference
+ * reduces its body centre of mass in f32 (pbd.ts:240-297); this is the FP64 form
+ * for any particle set.  Synchronises the stream. */
+int clay_fluid_conserved(clay_fluid* f, double out[7]);
 /* FluidSim.particleCount: fluid.ts:496-498. */
 int clay_fluid_particle_count(const clay_fluid* f, uint32_t* out);
 /* FluidSim.surface (the owned SplatField) and FluidSim.field. */
