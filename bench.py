@@ -1,14 +1,23 @@
 """Particle-steps/sec including the SDF bake, on one MI355X.
 
-A step is one FluidSim.step(): 2 substeps (bin + counting sort, density, forces +
-integrate + collide) and the splat bake of every mip.  The default workload is
-the 1M-particle dam break: 128 x 64 x 128 particles at spacing 0.05 over
-[-3.2,3.2] x [0,3.2] x [-3.2,3.2], colliding with a 288^3 baked world volume,
-baked every step into a 288^3 volume with 3 mips.  Inputs are resident in HBM
-before the timed region starts.
+A step is one FluidSim.step(): 2 substeps (bin + sort, density, forces +
+integrate + collide) and the splat bake of every mip.  Inputs are resident in
+HBM before the timed region starts.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this build
-measures one GPU.
+Default workload (`batched-256`): 256 independent dam-break scenes with particle
+counts log-uniform in [1024, 65536] (3 532 015 in total) and drop heights in
+[0.5, 3.0], each in a 6.4-unit world over the roundBox floor, each baking a
+32^3 x 3-mip volume 3.2 units wide that follows its particles.  The measured
+path is the batch with FP16-LDS staging, one captured graph per frame and the
+autotuned launch shapes.  Beside it, in the same run:
+  * the same scenes stepped one FluidSim at a time (plain f32 path),
+  * the single-scene 1M dam break (`dam-break-1m`, plain f32 path),
+  * the position error of the FP16-LDS batch against the f32 batch over one step
+    from the same state,
+  * the CPU oracle on a bounded sample of the scenes.
+`--workload dam-break-1m` / `dam-break-16k` measure a single FluidSim instead.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build measures one GPU.
 """
 from __future__ import annotations
 
@@ -23,10 +32,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "clay-engine_amd"))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/sec incl. SDF bake (1 MI355X); batched+FP16-LDS vs plain at stated err"
 
 
 def algorithmic_bytes(kernel: str, particles: int, mips: int, live_voxels: int) -> float | None:
     """Compulsory HBM bytes of one launch of `kernel` (per-unit figures x units)."""
+    kernel = kernel.removesuffix("_lds")
     per_particle = {
         "fluid_keys": 20,              # read pos 12, write key + id 8
         "fluid_reorder": 56,           # read key/id 8 + pos/vel 24, write sorted pos/vel 24
@@ -60,7 +71,65 @@ def committed_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
-def build_workload(root, name: str, splat: str, precision: str = "f32"):
+def roofline_of(stats: dict, particles: int, mips: int, live_voxels: int, workload: str) -> dict | None:
+    """Roofline object of the dominant kernel of a timed pass."""
+    if not stats:
+        return None
+    name, (tot_ms, launches) = max(stats.items(), key=lambda kv: kv[1][0])
+    avg_s = tot_ms / launches / 1e3
+    nbytes = algorithmic_bytes(name, particles, mips, live_voxels)
+    if nbytes is None:
+        return None
+    achieved = nbytes / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": committed_traffic(name, workload),
+            "kernel": name, "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes}
+
+
+def step_roofline(particles: int, substeps: int, mips: int, live_voxels: int, ms_per_step: float) -> dict:
+    b = particles * (substeps * 124 + 12 * mips) + 16 * live_voxels
+    gbs = b / (ms_per_step / 1e3) / 1e9
+    return {"bytes_per_step": b, "achieved_GBs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
+def timed(step, steps: int, warmup: int, stream) -> float:
+    """ms per step of `step(stream)` over `steps` steps after `warmup`, device-bracketed."""
+    import torch
+
+    for _ in range(warmup):
+        step(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        step(stream)
+    stop.record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def kernel_stats(root, step, steps: int, stream) -> dict:
+    """Per-kernel HIP-event totals over `steps` eager steps: {name: (ms, launches)}."""
+    root.synchronize()
+    root.reset_stats()
+    root.set_timing(True)
+    for _ in range(steps):
+        step(stream)
+    stream.synchronize()
+    root.synchronize()
+    root.set_timing(False)
+    return root.kernel_stats()
+
+
+def per_step(stats: dict, steps: int) -> dict:
+    return {k: round(v[0] / steps, 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])}
+
+
+# ---- single FluidSim workloads ------------------------------------------------
+def build_single(root, name: str, splat: str, precision: str = "f32"):
     import clayhip
     from clayhip import scenes
 
@@ -80,12 +149,32 @@ def build_workload(root, name: str, splat: str, precision: str = "f32"):
     return scene, sim, world, keep
 
 
-def cpu_baseline(scene, budget_s: float, threads: int) -> dict:
+def measure_single(root, name: str, splat: str, precision: str, steps: int, warmup: int, stream) -> dict:
+    scene, sim, world, keep = build_single(root, name, splat, precision)
+    n = scene["pos"].shape[0]
+    mips = scene["options"]["bakeMipLevels"]
+    ms = timed(sim.step, steps, warmup, stream)
+    ks = min(steps, 5)
+    stats = kernel_stats(root, sim.step, ks, stream)
+    live = sum(int(sim.surface.liveTiles(m).sum()) * 512 for m in range(mips))
+    out = {
+        "workload": scene["name"], "particles": n, "ms_per_step": ms, "value": n / (ms / 1e3),
+        "substeps": scene["options"]["substeps"], "mips": mips, "live_voxels": live,
+        "kernels_ms_per_step": per_step(stats, ks),
+        "roofline": roofline_of(stats, n, mips, live, name) if splat == "tiled" else None,
+        "scene": scene,
+    }
+    sim.destroy()
+    world.destroy()
+    for k in keep:
+        k.destroy()
+    return out
+
+
+def cpu_baseline_single(scene, budget_s: float, threads: int) -> dict:
     """The oracle (plain C, f32 mode, OpenMP) on this host's cores: whole steps of
     the same workload until `budget_s` has elapsed (at least one)."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
-
     import oracle as O
     from clayhip import make_brush
 
@@ -115,13 +204,138 @@ def cpu_baseline(scene, budget_s: float, threads: int) -> dict:
     dt = time.perf_counter() - t0
     n = scene["pos"].shape[0]
     return {
-        "value": n * steps / dt,
-        "unit": "particle-steps/s",
-        "cores": O.threads(),
-        "kind": "port",
+        "value": n * steps / dt, "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
         "sample": f"{steps} full step(s) of the same workload ({n} particles, 2 substeps + "
                   f"{opts['bakeMipLevels']}-mip bake), {dt:.1f} s wall, oracle f32 mode with OpenMP",
     }
+
+
+# ---- batched workload ----------------------------------------------------------
+def cpu_baseline_batched(scene, budget_s: float, threads: int) -> dict:
+    """The oracle on whole scenes of the batch, one after another, each taking one
+    step (2 substeps, then the splat of its following volume centred as the device
+    centres it), until `budget_s` has elapsed."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+
+    import oracle as O
+    from clayhip import make_brush
+
+    O.set_threads(threads)
+    opts = scene["options"]
+    col = O.Collider().add_analytic([make_brush(b) for b in scene["collider"]["brushes"]])
+    counts = scene["counts"]
+    base = np.concatenate([np.zeros(1, np.int64), np.cumsum(counts, dtype=np.int64)])
+    half = opts["followSize"] / 2
+    done = 0
+    used = 0
+    t0 = time.perf_counter()
+    for s in range(len(counts)):
+        pos = scene["pos"][base[s]:base[s + 1]]
+        fl = O.Fluid(dict(opts, capacity=int(counts[s])), pos, collider=col)
+        for _ in range(opts["substeps"]):
+            fl.substep()
+        p = fl.get("pos")
+        centre = (p.min(axis=0) + p.max(axis=0)) * np.float32(0.5)
+        origin = [float(np.float32(c - half)) for c in centre.astype(np.float64)]
+        so = O.splat_opts(opts["bakeResolution"], opts["bakeMipLevels"], opts["followSize"], origin,
+                          opts["bakeBand"], opts["surfaceRadius"])
+        O.splat_resolve(so, O.splat_bake(so, p))
+        done += int(counts[s])
+        used += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": done / dt, "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
+        "sample": f"one full step of each of the first {used} scenes ({done} particles, 2 substeps + "
+                  f"{opts['bakeMipLevels']}-mip following bake), {dt:.1f} s wall, oracle f32 mode with OpenMP",
+    }
+
+
+def measure_per_scene(root, world, scene, steps: int, stream) -> dict:
+    """The same scenes, one f32 FluidSim each, stepped one after another."""
+    import numpy as np
+
+    import clayhip
+
+    counts = scene["counts"]
+    base = np.concatenate([np.zeros(1, np.int64), np.cumsum(counts, dtype=np.int64)])
+    sims = []
+    for s in range(len(counts)):
+        sim = clayhip.FluidSim(root, world, dict(scene["options"], capacity=int(counts[s])))
+        sim.set_state(scene["pos"][base[s]:base[s + 1]])
+        sims.append(sim)
+
+    def frame(st):
+        for sim in sims:
+            sim.step(st)
+
+    ms = timed(frame, steps, 1, stream)
+    for sim in sims:
+        sim.destroy()
+    n = int(counts.sum())
+    return {"path": "one f32 FluidSim per scene, same scenes", "ms_per_step": ms, "value": n / (ms / 1e3),
+            "steps": steps}
+
+
+def measure_batched(root, args, stream) -> dict:
+    import numpy as np
+
+    import clayhip
+    from clayhip import autotune, scenes
+
+    scene = scenes.batched(256)
+    world = clayhip.analyticField(root, scene["collider"]["brushes"])
+    opts = dict(scene["options"], precision=args.precision)
+    b = clayhip.FluidBatch(root, world, opts, scene["counts"])
+    b.set_graph(not args.no_graph)
+    b.set_state(scene["pos"])
+    key = autotune.cache_key("batched-256", args.precision)
+    tune_src = "default"
+    if args.autotune:
+        autotune.autotune(b, key, frames=3, state=lambda: b.set_state(scene["pos"]))
+        tune_src = "autotuned in this run"
+    else:
+        cached = autotune.load(autotune.DEFAULT_CACHE, key)
+        if cached:
+            b.tuning = cached
+            tune_src = "autotune cache"
+    b.set_state(scene["pos"])
+    n = int(scene["counts"].sum())
+    mips = opts["bakeMipLevels"]
+    ms = timed(b.step, args.steps, args.warmup, stream)
+    # Per-kernel breakdown from eager frames (a graph replay has no per-launch events).
+    b.set_graph(False)
+    stats = kernel_stats(root, b.step, 3, stream)
+    live = sum(int(b.live_tiles(s, m).sum()) * 512 for s in range(b.scenes) for m in range(mips))
+    # Error of the measured path against the f32 batch over one step from its current state.
+    err = None
+    if args.precision != "f32":
+        pos, vel = b.get_state("pos"), b.get_state("vel")
+        ref = clayhip.FluidBatch(root, world, dict(opts, precision="f32"), scene["counts"])
+        ref.set_graph(False)
+        for x in (b, ref):
+            x.set_state(pos, vel)
+            x.step(stream)
+        stream.synchronize()
+        d = np.linalg.norm(b.get_state("pos").astype(np.float64) - ref.get_state("pos"), axis=1) / b.constants.h
+        err = {"vs": "f32 batch, one step from the same state", "position_over_h_rms": float(np.sqrt(np.mean(d * d))),
+               "position_over_h_p999": float(np.quantile(d, 0.999)), "position_over_h_max": float(d.max())}
+        ref.destroy()
+    out = {
+        "workload": scene["name"], "particles": n, "scenes": b.scenes, "ms_per_step": ms,
+        "value": n / (ms / 1e3), "substeps": opts["substeps"], "mips": mips, "live_voxels": live,
+        "tuning": dict(b.tuning, source=tune_src), "graph": not args.no_graph,
+        "kernels_ms_per_step": per_step(stats, 3),
+        "roofline": roofline_of(stats, n, mips, live, "batched-256"),
+        "error": err, "scene": scene,
+    }
+    b.destroy()
+    if not args.no_compare:
+        out["plain_per_scene"] = measure_per_scene(root, world, scene, min(args.steps, 5), stream)
+    world.destroy()
+    return out
 
 
 def main() -> int:
@@ -129,10 +343,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dam-break-1m", choices=["dam-break-1m", "dam-break-16k"])
-    ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"])
-    ap.add_argument("--precision", default="f32", choices=["f32", "fp16lds"],
-                    help="fluid kernels: IEEE f32 (bit-exact) or FP16-LDS staged neighbours")
+    ap.add_argument("--workload", default="batched-256", choices=["batched-256", "dam-break-1m", "dam-break-16k"])
+    ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"], help="single-scene workloads only")
+    ap.add_argument("--precision", default=None, choices=["f32", "fp16lds"],
+                    help="fluid kernels (default: fp16lds for the batch, f32 for a single scene)")
+    ap.add_argument("--no-graph", action="store_true", help="batch: eager frames instead of a captured graph")
+    ap.add_argument("--autotune", action="store_true", help="batch: re-tune launch shapes (else the cache)")
+    ap.add_argument("--no-compare", action="store_true", help="batch: skip the plain comparisons")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -153,89 +370,72 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         return 2
+    batched = args.workload == "batched-256"
+    if args.precision is None:
+        args.precision = "fp16lds" if batched else "f32"
 
-    import numpy as np
     import torch
 
     import clayhip
 
     root = clayhip.Root(0)
-    scene, sim, world, keep = build_workload(root, args.workload, args.splat, args.precision)
-    n = scene["pos"].shape[0]
-    mips = scene["options"]["bakeMipLevels"]
     stream = torch.cuda.Stream()
-    for _ in range(args.warmup):
-        sim.step(stream)
-    stream.synchronize()
-    root.synchronize()
-
-    root.reset_stats()
-    root.set_timing(True)
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        sim.step(stream)
-    stop.record(stream)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    root.set_timing(False)
-    ms = start.elapsed_time(stop)
-    ms_per_step = ms / args.steps
-    stats = root.kernel_stats()
-
-    live_voxels = sum(int(sim.surface.liveTiles(m).sum()) * 512 for m in range(mips))
-    dom_name, (dom_ms, dom_n) = max(stats.items(), key=lambda kv: kv[1][0])
-    dom_avg_s = dom_ms / dom_n / 1e3
-    dom_bytes = algorithmic_bytes(dom_name, n, mips, live_voxels)
-    roofline = None
-    if dom_bytes is not None:
-        achieved = dom_bytes / dom_avg_s / 1e9
-        traffic = committed_traffic(dom_name, args.workload) if args.splat == "tiled" else None
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                    "kernel": dom_name, "avg_ms": round(dom_avg_s * 1e3, 4), "bytes_per_launch": dom_bytes}
-    step_bytes = n * (2 * 124 + 12 * mips) + 16 * live_voxels
+    t_wall = time.perf_counter()
+    comparisons = None
+    if batched:
+        r = measure_batched(root, args, stream)
+        if not args.no_compare:
+            single = measure_single(root, "dam-break-1m", "tiled", "f32", args.steps, args.warmup, stream)
+            comparisons = {
+                "plain_per_scene": r["plain_per_scene"],
+                "plain_single_1m": {"path": "one f32 FluidSim, " + single["workload"],
+                                    "ms_per_step": single["ms_per_step"], "value": single["value"]},
+                "batched_over_plain_per_scene": r["value"] / r["plain_per_scene"]["value"],
+                "batched_over_plain_single_1m": r["value"] / single["value"],
+            }
+    else:
+        r = measure_single(root, args.workload, args.splat, args.precision, args.steps, args.warmup, stream)
+    wall = time.perf_counter() - t_wall
+    scene = r["scene"]
     line = {
-        "metric": "particle-steps/sec incl. SDF bake (1 MI355X)",
-        "value": n * args.steps / (ms / 1e3),
+        "metric": METRIC,
+        "value": r["value"] * 1.0,
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded jittered lattice)",
+        "data": "synthetic (seeded jittered lattices)",
         "config": {
-            "workload": scene["name"],
-            "particles": n,
-            "substeps": scene["options"]["substeps"],
-            "bake": f"{scene['options']['bakeResolution']}^3 x {mips} mips",
-            "splat": args.splat,
+            "workload": r["workload"],
+            "particles": r["particles"],
+            "substeps": r["substeps"],
+            "bake": (f"{scene['options']['bakeResolution']}^3 x {r['mips']} mips"
+                     + (" following, per scene" if batched else "")),
             "precision": args.precision,
-            "live_voxels": live_voxels,
+            "live_voxels": r["live_voxels"],
         },
-        "step_roofline": {"bytes_per_step": step_bytes,
-                          "achieved_GBs": round(step_bytes / (ms_per_step / 1e3) / 1e9, 2),
-                          "frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)},
-        "roofline": roofline,
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])},
-        "wall_s": round(wall, 3),
+        "step_roofline": step_roofline(r["particles"], r["substeps"], r["mips"], r["live_voxels"], r["ms_per_step"]),
+        "roofline": r["roofline"],
+        "kernels_ms_per_step": r["kernels_ms_per_step"],
     }
+    if batched:
+        line["config"].update({"scenes": r["scenes"], "graph": r["graph"], "tuning": r["tuning"]})
+        line["error"] = r["error"]
+        line["comparisons"] = comparisons
+    else:
+        line["config"]["splat"] = args.splat
+    line["wall_s"] = round(wall, 3)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, min(16, os.cpu_count() or 1))
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = (cpu_baseline_batched(scene, args.cpu_budget, threads) if batched
+                                else cpu_baseline_single(scene, args.cpu_budget, threads))
     else:
         line["cpu_baseline"] = None
-    sim.destroy()
-    world.destroy()
-    for k in keep:
-        k.destroy()
     root.destroy()
     print(json.dumps(line), flush=True)
     return 0

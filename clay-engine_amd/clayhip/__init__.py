@@ -6,6 +6,7 @@ combinators), calling hand-written HIP kernels through the C ABI in
 include/clay_hip.h.  There is no CPU fallback: every call goes to the GPU.
 """
 from ._lib import ClayError, LIB_PATH, lib
+from .batch import FluidBatch
 from .brush import compile_brushes, make_brush
 from .field import TracedField, analyticField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
@@ -14,6 +15,7 @@ from .volume import Root, SdfVolume, derive
 
 __all__ = [
     "ClayError",
+    "FluidBatch",
     "FluidSim",
     "LIB_PATH",
     "ParticleCloud",

@@ -135,10 +135,29 @@ class FluidDesc(C.Structure):
         ("max_speed", C.c_double),
         ("precision", C.c_int32),
         ("pad", C.c_int32),
+        ("follow_size", C.c_double),
     ]
 
 
 PRECISION_F32, PRECISION_FP16LDS = 0, 1
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("block", C.c_int32),
+        ("stage", C.c_int32),
+        ("chunk", C.c_int32),
+        ("groups_per_cu", C.c_int32),
+    ]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("scenes", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("counts", C.POINTER(C.c_uint32)),
+        ("fluid", FluidDesc),
+    ]
 
 
 class FluidParams(C.Structure):
@@ -186,6 +205,8 @@ STRUCTS = {
     "clay_fluid_desc": FluidDesc,
     "clay_fluid_params": FluidParams,
     "clay_fluid_consts": FluidConsts,
+    "clay_tuning": Tuning,
+    "clay_batch_desc": BatchDesc,
 }
 
 P = C.c_void_p
@@ -244,6 +265,25 @@ SIGNATURES = {
     "clay_fluid_particle_count": ([P, C.POINTER(U32)], C.c_int),
     "clay_fluid_surface": ([P, PP], C.c_int),
     "clay_fluid_debug_download": ([P, I32, P], C.c_int),
+    "clay_fluid_set_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+    "clay_fluid_get_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+    "clay_batch_create": ([P, P, C.POINTER(BatchDesc), PP], C.c_int),
+    "clay_batch_destroy": ([P], C.c_int),
+    "clay_batch_set_state": ([P, P, P], C.c_int),
+    "clay_batch_get_state": ([P, I32, P], C.c_int),
+    "clay_batch_step": ([P, P], C.c_int),
+    "clay_batch_set_graph": ([P, I32], C.c_int),
+    "clay_batch_configure": ([P, C.POINTER(FluidParams)], C.c_int),
+    "clay_batch_set_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+    "clay_batch_get_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+    "clay_batch_constants": ([P, C.POINTER(FluidConsts)], C.c_int),
+    "clay_batch_scene": ([P, U32, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    "clay_batch_read_contacts": ([P, U32, P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
+    "clay_batch_volume_download": ([P, U32, I32, P], C.c_int),
+    "clay_batch_grid_download": ([P, U32, I32, P], C.c_int),
+    "clay_batch_tiles_download": ([P, U32, I32, P], C.c_int),
+    "clay_batch_origin": ([P, U32, C.POINTER(C.c_float)], C.c_int),
+    "clay_batch_conserved": ([P, U32, DP], C.c_int),
 }
 
 _lib: C.CDLL | None = None

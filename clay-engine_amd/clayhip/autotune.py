@@ -1,0 +1,99 @@
+"""Launch-shape autotuner for a FluidBatch (or FluidSim).
+
+Tries each candidate clay_tuning on the live workload -- a few captured frames
+per candidate, timed with HIP events on one stream -- keeps the fastest, and
+caches the choice in a JSON file keyed by device and workload.  The knobs only
+change launch shapes (workgroup size, LDS staging slice, splat work-item size,
+persistent grid), never results; the cell size is not a knob because it would
+change every neighbour's summation order.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+from pathlib import Path
+
+DEFAULT_CACHE = Path(__file__).resolve().parent / "tuned.json"
+
+SPACE = {
+    "block": (128, 256),
+    "stage": (64, 128, 256),
+    "chunk": (1024, 2048, 4096),
+    "groups_per_cu": (4, 8, 16),
+}
+
+
+def candidates(space: dict = SPACE) -> list[dict]:
+    keys = list(space)
+    return [dict(zip(keys, vals)) for vals in itertools.product(*(space[k] for k in keys))]
+
+
+def _device_name() -> str:
+    import torch
+
+    return torch.cuda.get_device_name(0)
+
+
+def time_frames(obj, frames: int, stream) -> float:
+    """Milliseconds per step of `obj` over `frames` steps on `stream` (one warm step first)."""
+    import torch
+
+    obj.step(stream)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(frames):
+        obj.step(stream)
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / frames
+
+
+def autotune(obj, key: str, frames: int = 3, space: dict = SPACE, cache: Path | None = DEFAULT_CACHE,
+             state=None) -> dict:
+    """Tunes `obj` in place and returns {"tuning", "ms", "table"}.  `state`, if
+    given, is a callable that restores the workload's initial state before each
+    candidate, so every candidate times the same frames."""
+    import torch
+
+    stream = torch.cuda.Stream()
+    table = []
+    for cand in candidates(space):
+        if state is not None:
+            state()
+        obj.tuning = cand
+        table.append({"tuning": cand, "ms": time_frames(obj, frames, stream)})
+    best = min(table, key=lambda r: r["ms"])
+    obj.tuning = best["tuning"]
+    if state is not None:
+        state()
+    result = {"tuning": best["tuning"], "ms": best["ms"], "table": table}
+    if cache is not None:
+        store(cache, key, {"tuning": best["tuning"], "ms": best["ms"]})
+    return result
+
+
+def cache_key(workload: str, precision: str) -> str:
+    return f"{_device_name()}|{workload}|{precision}"
+
+
+def load(cache: Path, key: str) -> dict | None:
+    try:
+        data = json.loads(Path(cache).read_text())
+    except (OSError, ValueError):
+        return None
+    entry = data.get(key)
+    return entry.get("tuning") if isinstance(entry, dict) else None
+
+
+def store(cache: Path, key: str, entry: dict) -> None:
+    cache = Path(cache)
+    try:
+        data = json.loads(cache.read_text())
+    except (OSError, ValueError):
+        data = {}
+    data[key] = entry
+    tmp = cache.with_suffix(".tmp")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+    os.replace(tmp, cache)

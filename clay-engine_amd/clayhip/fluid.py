@@ -29,6 +29,7 @@ _OPTION_FIELDS = {
     "viscosity": "viscosity",
     "damping": "damping",
     "maxSpeed": "max_speed",
+    "followSize": "follow_size",
 }
 # FluidParams members configure() may patch (fluid.ts:528-530)
 _PARAM_FIELDS = {
@@ -150,6 +151,19 @@ class FluidSim:
         L.check(L.lib().clay_fluid_read_contacts(self._h, pts.ctypes.data, C.byref(n), C.byref(raw)))
         self.lastContactCount = raw.value
         return [tuple(float(v) for v in pts[i]) for i in range(n.value)]
+
+    @property
+    def tuning(self) -> dict:
+        t = L.Tuning()
+        L.check(L.lib().clay_fluid_get_tuning(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
+
+    @tuning.setter
+    def tuning(self, values: dict) -> None:
+        cur = self.tuning
+        cur.update(values)
+        t = L.Tuning(**cur)
+        L.check(L.lib().clay_fluid_set_tuning(self._h, C.byref(t)))
 
     def conserved(self) -> dict:
         """Total mass, momentum and centre of mass of the live particles, reduced in FP64."""

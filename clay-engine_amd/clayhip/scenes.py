@@ -125,3 +125,48 @@ def batched_counts(scenes: int = 256, lo: int = 1024, hi: int = 65536, seed: int
     """Particle counts drawn log-uniformly in [lo, hi] from the reference hash."""
     u = hash1(np.arange(scenes, dtype=np.uint32) + np.uint32(seed * 7919)).astype(np.float64)
     return np.floor(np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))).astype(np.int64)
+
+
+BATCH_ORIGIN = (-3.2, -0.8, -3.2)
+BATCH_WORLD = 6.4
+BATCH_FOLLOW = 3.2
+
+
+def batch_options(bake_res: int = 32, bake_mips: int = 3, follow: float = BATCH_FOLLOW) -> dict:
+    """Options of every scene of the batched workload: the dam-break constants in a
+    6.4-unit world over the same roundBox floor, each scene baking a 32^3 x 3-mip
+    volume 3.2 units wide that follows its particles."""
+    opts = dam_break_options(1, bake_res, bake_mips)
+    opts.update({"origin": BATCH_ORIGIN, "worldSize": BATCH_WORLD, "followSize": follow,
+                 "splatMode": "tiled"})
+    return opts
+
+
+def drop_block(count: int, drop: float, seed: int, spacing: float = 0.05) -> np.ndarray:
+    """`count` particles of a jittered lattice block (nx = nz = ceil(cbrt(count)),
+    filled layer by layer) centred on x = z = 0 with its base at height `drop`."""
+    nx = int(math.ceil(round(count ** (1.0 / 3.0), 9)))
+    while nx ** 3 < count:
+        nx += 1
+    ny = int(math.ceil(count / (nx * nx)))
+    lo = (-nx * spacing / 2, drop, -nx * spacing / 2)
+    return lattice(nx, ny, nx, spacing, lo, seed=seed)[:count]
+
+
+def batched(scenes: int = 256, seed: int = 7) -> dict:
+    """Independent scenes with particle counts log-uniform in [1024, 65536] and drop
+    heights uniform in [0.5, 3.0], all from the reference hash; positions are the
+    scenes' blocks concatenated scene-major."""
+    counts = batched_counts(scenes, seed=seed)
+    u = hash1(np.arange(scenes, dtype=np.uint32) + np.uint32(seed * 7919 + 104729)).astype(np.float64)
+    drops = 0.5 + 2.5 * u
+    blocks = [drop_block(int(counts[i]), float(drops[i]), seed=seed * 1000 + i) for i in range(scenes)]
+    return {
+        "name": f"batched {scenes} scenes, {int(counts.sum())} particles, following 32^3 x 3 mips each",
+        "options": batch_options(),
+        "collider": {"kind": "analytic", "brushes": [floor_brush()]},
+        "counts": counts.astype(np.uint32),
+        "drops": drops,
+        "pos": np.concatenate(blocks, axis=0),
+        "steps": 100,
+    }

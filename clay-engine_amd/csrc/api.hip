@@ -56,7 +56,7 @@ void Ctx::collect() {
 }
 
 LaunchTimer::LaunchTimer(Ctx* ctx, hipStream_t s, const char* name) : ctx_(ctx), s_(s) {
-  if (!ctx_ || !ctx_->timing) return;
+  if (!ctx_ || !ctx_->timing || ctx_->capturing) return;
   stat_ = ctx_->stat_index(name);
   a_ = ctx_->take_event();
   b_ = ctx_->take_event();
@@ -157,7 +157,7 @@ int fill_u16(Ctx* c, hipStream_t s, uint16_t* p, size_t n, uint16_t v) {
   return CLAY_OK;
 }
 
-int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out) {
+int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out, int instances = 1) {
   const int32_t res = d->resolution > 0 ? d->resolution : 128;
   clay_volume_info info;
   int rc = volume_geometry(res, d->mip_levels, &info);
@@ -165,6 +165,7 @@ int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out) {
   Volume* v = new (std::nothrow) Volume();
   if (!v) return fail(CLAY_ERR_INVALID, "out of host memory");
   v->ctx = ctx;
+  v->instances = instances;
   v->info = info;
   v->info.world_size = d->world_size > 0 ? d->world_size : 16.0;
   v->info.band = d->band > 0 ? d->band : 4.0;
@@ -179,7 +180,9 @@ int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out) {
   }
   v->total_voxels = total;
   write_voldev(v);
-  if ((rc = dalloc(&v->dev, 1)) || (rc = dalloc(&v->dist, total)) || (rc = dalloc(&v->mat, total))) {
+  const size_t planes = total * (size_t)instances;
+  if ((rc = dalloc(&v->dev, (size_t)instances)) || (rc = dalloc(&v->dist, planes)) ||
+      (rc = dalloc(&v->mat, planes))) {
     dfree(v->dev);
     dfree(v->dist);
     dfree(v->mat);
@@ -187,8 +190,9 @@ int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out) {
     return rc;
   }
   hipStream_t s = ctx->stream;
-  if (hipMemcpyAsync(v->dev, &v->host, sizeof(VolDev), hipMemcpyHostToDevice, s) != hipSuccess ||
-      fill_u16(ctx, s, v->dist, total, f2h_host(1.0f)) || fill_u16(ctx, s, v->mat, total, 0) ||
+  const std::vector<VolDev> recs((size_t)instances, v->host);
+  if (hipMemcpyAsync(v->dev, recs.data(), sizeof(VolDev) * instances, hipMemcpyHostToDevice, s) != hipSuccess ||
+      fill_u16(ctx, s, v->dist, planes, f2h_host(1.0f)) || fill_u16(ctx, s, v->mat, planes, 0) ||
       hipStreamSynchronize(s) != hipSuccess) {
     dfree(v->dev);
     dfree(v->dist);
@@ -385,7 +389,7 @@ extern "C" int clay_volume_set_origin(clay_volume* vol, const double origin[3], 
 extern "C" int clay_volume_set_origin_device(clay_volume* vol, const float* origin3, clay_stream stream) {
   if (!vol || !origin3) return fail(CLAY_ERR_INVALID, "clay_volume_set_origin_device: NULL argument");
   Volume* v = reinterpret_cast<Volume*>(vol);
-  return launch_set_origin_dev(v->ctx, pick(v->ctx, stream), v->dev, origin3, v->host.half);
+  return launch_set_origin_dev(v->ctx, pick(v->ctx, stream), v->dev, origin3, v->info.world_size / 2);
 }
 
 extern "C" int clay_volume_voxel_size_at(const clay_volume* vol, int32_t mip, double* out) {
@@ -572,8 +576,20 @@ void splat_free(Splat* sp) {
   delete sp;
 }
 
-int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, Splat** out) {
+// One SplatField, or (scenes > 1, or a follow splat) one volume instance per scene
+// of a batch whose particle scenes are `scene_of` with prefix offsets `scene_base`.
+struct SplatScenes {
+  int scenes = 1;
+  const uint32_t* scene_of = nullptr;
+  const uint32_t* scene_base = nullptr;
+  bool follow = false;
+};
+
+int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, Splat** out,
+                      const SplatScenes& sc = SplatScenes()) {
   if (!(desc->radius > 0)) return fail(CLAY_ERR_INVALID, "SplatField: radius must be positive");
+  if (sc.scenes > 1 && desc->mode != CLAY_SPLAT_TILED)
+    return fail(CLAY_ERR_INVALID, "SplatField: a batch of scenes needs the tiled schedule");
   if (desc->mode != CLAY_SPLAT_PLAIN && desc->mode != CLAY_SPLAT_TILED)
     return fail(CLAY_ERR_INVALID, "SplatField: unknown splat mode");
   if (!cloud->pos[0] || !cloud->pos[1] || !cloud->pos[2])
@@ -586,6 +602,11 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
   sp->ctx = ctx;
   sp->cloud = *cloud;
   sp->desc = *desc;
+  sp->scenes = sc.scenes;
+  sp->scene_of = sc.scene_of;
+  sp->scene_base = sc.scene_base;
+  sp->follow = sc.follow;
+  sp->persistent_blocks = ctx->persistent_blocks;
   // splat.ts:148-153
   const int res = desc->resolution > 0 ? desc->resolution : 64;
   const int mips = desc->mip_levels > 0 ? desc->mip_levels : std::max(1, (int)log2((double)res / kTile));
@@ -599,7 +620,7 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
   vd.world_size = sp->desc.world_size;
   vd.band = sp->desc.band;
   for (int a = 0; a < 3; a++) vd.origin[a] = desc->origin[a];
-  int rc = volume_create_impl(ctx, &vd, &sp->vol);
+  int rc = volume_create_impl(ctx, &vd, &sp->vol, sc.scenes);
   if (rc) {
     delete sp;
     return rc;
@@ -617,7 +638,8 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
   // Both schedules keep the u32 code grid: plain accumulates codes in it, tiled
   // accumulates squared distances of the bricks it touches and resolves them to
   // codes in place, so after every bake it holds the codes of every voxel.
-  if ((rc = dalloc(&sp->grid, total)) || (rc = launch_fill_u32(ctx, s, sp->grid, total, kEncSat))) {
+  const size_t S = (size_t)sc.scenes;
+  if ((rc = dalloc(&sp->grid, total * S)) || (rc = launch_fill_u32(ctx, s, sp->grid, total * S, kEncSat))) {
     splat_free(sp);
     return rc;
   }
@@ -627,22 +649,23 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
     const size_t per = splat_max_bricks_per_particle(sp->desc.radius, sp->desc.band,
                                                      sp->vol->info.voxel_size, sp->mips);
     sp->brick_list_cap = (size_t)std::max<uint32_t>(cloud->capacity, 1) * per;
-    if ((rc = dalloc(&sp->brick_count, sp->brick_total)) || (rc = dalloc(&sp->brick_start, sp->brick_total)) ||
-        (rc = dalloc(&sp->brick_cursor, sp->brick_total * splat_bin_slices())) ||
-        (rc = dalloc(&sp->brick_sub, sp->brick_total * splat_bin_slices())) ||
+    const size_t nb = sp->brick_total * S;
+    if ((rc = dalloc(&sp->brick_count, nb)) || (rc = dalloc(&sp->brick_start, nb)) ||
+        (rc = dalloc(&sp->brick_cursor, nb * splat_bin_slices())) ||
+        (rc = dalloc(&sp->brick_sub, nb * splat_bin_slices())) ||
         (rc = dalloc(&sp->brick_list, sp->brick_list_cap)) ||
-        (rc = dalloc(&sp->brick_total_dev, 4)) || (rc = dalloc(&sp->item_start, sp->brick_total)) ||
-        (rc = dalloc(&sp->tile_live, sp->tile_total))) {
+        (rc = dalloc(&sp->brick_total_dev, 4)) || (rc = dalloc(&sp->item_start, nb)) ||
+        (rc = dalloc(&sp->tile_live, sp->tile_total * S))) {
       splat_free(sp);
       return rc;
     }
-    if (hipMemsetAsync(sp->tile_live, 0, sp->tile_total, s) != hipSuccess) {
+    if (hipMemsetAsync(sp->tile_live, 0, sp->tile_total * S, s) != hipSuccess) {
       splat_free(sp);
       return fail(CLAY_ERR_HIP, "SplatField: tile mask initialisation failed");
     }
   }
   // The material channel is constant for a splat (splat.ts:273): written once.
-  if ((rc = fill_u16(ctx, s, sp->vol->mat, sp->vol->total_voxels, f2h_host((float)desc->material)))) {
+  if ((rc = fill_u16(ctx, s, sp->vol->mat, sp->vol->total_voxels * S, f2h_host((float)desc->material)))) {
     splat_free(sp);
     return rc;
   }
@@ -790,56 +813,100 @@ void fluid_free(Fluid* f) {
   dfree(f->cells);
   dfree(f->contacts);
   dfree(f->reduce);
+  dfree(f->epoch_dev);
+  dfree(f->scene_of);
+  dfree(f->scene_base_dev);
+  if (f->graph) (void)hipGraphExecDestroy(f->graph);
   delete f;
 }
+
+size_t contact_words(const Fluid* f) { return (size_t)f->scenes * (4 + 4 * CLAY_CONTACTS); }
 
 int fluid_substep_impl(Fluid* f, hipStream_t s) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
   int rc;
-  // clearGrid's contacts reset (fluid.ts:302-304).
-  if ((rc = launch_fill_u32(c, s, f->contacts, 1, 0u))) return rc;
+  // clearGrid's contacts reset (fluid.ts:302-304), for every scene.
+  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u))) return rc;
   if (n == 0) return CLAY_OK;
   // A new stamp per substep makes every older cell record stale at once.  After
   // 2^32 - 1 substeps the stamps restart, and the table is wiped once so that no
   // record from the previous cycle can carry a reused stamp.
   if (++f->epoch == 0) {
-    if ((rc = launch_fill_u32(c, s, f->cells, 4 * f->grid_cells, 0u))) return rc;
+    if ((rc = launch_fill_u32(c, s, f->cells, 4 * f->grid_cells * f->scenes, 0u)) ||
+        (rc = launch_fill_u32(c, s, f->epoch_dev, 1, 1u)))
+      return rc;
     f->epoch = 1;
+  } else if ((rc = launch_bump(c, s, f->epoch_dev))) {
+    return rc;
   }
-  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->keys, f->vals, invalid))) return rc;
+  const SceneK sk = f->scene_k();
+  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, f->keys, f->vals, invalid))) return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
-  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, invalid, f->cells, f->epoch, src, dst, 6))) return rc;
+  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6))) return rc;
   const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
-  if ((rc = (lds ? launch_density_lds : launch_density)(c, s, f->k, n, f->spx, f->spy, f->spz, f->cells,
-                                                        f->epoch, f->srho, f->spres)))
+  if ((rc = launch_density(c, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells, f->srho,
+                           f->spres)))
     return rc;
   const float* sorted[8] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz, f->srho, f->spres};
   float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
   ColliderDev none{};
   const ColliderDev& col = f->world ? f->world->col : none;
-  return (lds ? launch_forces_integrate_lds : launch_forces_integrate)(c, s, f->k, col, n, sorted, f->vals,
-                                                                       f->cells, f->epoch, out, f->contacts);
+  return launch_forces_integrate(c, s, f->k, sk, f->tune, lds, col, n, sorted, f->vals, f->keys, f->cells, out,
+                                 f->contacts);
 }
 
-}  // namespace
+// FluidSim.step (fluid.ts:536-558): emit, `substeps` substeps, bake every mip.
+int fluid_frame_impl(Fluid* f, hipStream_t s) {
+  const uint32_t emit_count = f->pending_emit;
+  f->pending_emit = 0;
+  const uint32_t emit_base = f->cursor;
+  if (emit_count > 0) {
+    f->cursor = (emit_base + emit_count) % f->capacity;
+    f->live = std::min<uint32_t>(f->capacity, std::max<uint32_t>(f->live, emit_base + emit_count));
+    int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->emit_seed);
+    if (rc) return rc;
+  }
+  bool sorted = false;
+  for (int i = 0; i < f->consts.substeps && f->live > 0; i++) {
+    int rc = fluid_substep_impl(f, s);
+    if (rc) return rc;
+    sorted = true;
+  }
+  f->surface->cloud.live_count = f->live;
+  // The last substep's permutation is a spatially coherent visiting order for the
+  // splat; the codes do not depend on it.
+  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr);
+}
 
-extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fluid_desc* d, clay_fluid** out) {
-  if (!ctx || !d || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_create: NULL argument");
-  if (d->capacity == 0) return fail(CLAY_ERR_INVALID, "FluidSim: capacity must be positive");
+// Shared by FluidSim (one scene, `counts` null) and a batch of `scenes` scenes
+// holding counts[i] particles each, scene-major.
+int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t scenes, const uint32_t* counts,
+                      Fluid** out) {
   if (!(d->spacing > 0)) return fail(CLAY_ERR_INVALID, "FluidSim: spacing must be positive");
   if (d->precision != CLAY_PRECISION_F32 && d->precision != CLAY_PRECISION_FP16LDS)
     return fail(CLAY_ERR_INVALID, "FluidSim: unknown precision mode");
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (d->follow_size < 0) return fail(CLAY_ERR_INVALID, "FluidSim: followSize must not be negative");
+  const bool batch = counts != nullptr;
+  uint64_t total = d->capacity;
+  if (batch) {
+    total = 0;
+    for (uint32_t i = 0; i < scenes; i++) total += counts[i];
+    if (total == 0) return fail(CLAY_ERR_INVALID, "FluidBatch: the scenes hold no particles");
+    if (total > 0x7FFFFFFFull) return fail(CLAY_ERR_INVALID, "FluidBatch: more than 2^31 particles");
+  }
+  if (total == 0) return fail(CLAY_ERR_INVALID, "FluidSim: capacity must be positive");
   Fluid* f = new (std::nothrow) Fluid();
   if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
   f->ctx = c;
-  f->world = reinterpret_cast<Field*>(world);
+  f->world = world;
   f->desc = *d;
-  f->capacity = d->capacity;
+  f->capacity = (uint32_t)total;
+  f->scenes = batch ? scenes : 1;
+  f->use_graph = batch;
   // fluid.ts:141-203
   const double spacing = d->spacing;
   const double h = d->smoothing > 0 ? d->smoothing : spacing * 2.2;
@@ -891,11 +958,32 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
 
   int rc;
   const uint32_t P = f->capacity;
+  f->scene_base.assign(f->scenes + 1, 0);
+  if (batch)
+    for (uint32_t i = 0; i < scenes; i++) f->scene_base[i + 1] = f->scene_base[i] + counts[i];
+  else
+    f->scene_base[1] = P;
   if ((rc = dalloc(&f->st, (size_t)P * 11)) || (rc = dalloc(&f->ss, (size_t)P * 8)) ||
       (rc = dalloc(&f->keys, P)) || (rc = dalloc(&f->vals, P)) || (rc = dalloc(&f->keys2, P)) ||
-      (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, 4 + 4 * CLAY_CONTACTS))) {
+      (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, contact_words(f))) ||
+      (rc = dalloc(&f->epoch_dev, 1))) {
     fluid_free(f);
     return rc;
+  }
+  if (batch) {
+    std::vector<uint32_t> so(P);
+    for (uint32_t i = 0; i < scenes; i++)
+      for (uint32_t j = f->scene_base[i]; j < f->scene_base[i + 1]; j++) so[j] = i;
+    if ((rc = dalloc(&f->scene_of, P)) || (rc = dalloc(&f->scene_base_dev, (size_t)scenes + 1))) {
+      fluid_free(f);
+      return rc;
+    }
+    if (hipMemcpy(f->scene_of, so.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->scene_base_dev, f->scene_base.data(), sizeof(uint32_t) * (scenes + 1),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      fluid_free(f);
+      return fail(CLAY_ERR_HIP, "FluidBatch: scene table upload failed");
+    }
   }
   f->reduce_blocks = std::min<uint32_t>(1024, std::max<uint32_t>(1, (P + 255) / 256));
   if ((rc = dalloc(&f->reduce, (size_t)f->reduce_blocks * 6 + 8))) {
@@ -912,10 +1000,13 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
     return rc;
   }
   // The renderable surface (fluid.ts:215-234): a SplatField over the live prefix.
+  // With followSize set it is a volume of that size centred on the particles'
+  // bounds before every bake (one per scene in a batch), as SoftBody does for a body
+  // (softbody.ts:424-428).
   clay_cloud_desc cloud{};
   cloud.capacity = P;
   cloud.live_rule = CLAY_LIVE_PREFIX;
-  cloud.live_count = 0;
+  cloud.live_count = batch ? P : 0;
   cloud.pos[0] = f->px;
   cloud.pos[1] = f->py;
   cloud.pos[2] = f->pz;
@@ -927,38 +1018,49 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
   sd.band = band;
   for (int a = 0; a < 3; a++) sd.origin[a] = d->origin[a];
   sd.world_size = d->world_size > 0 ? d->world_size : 24.0;
+  if (d->follow_size > 0) sd.world_size = d->follow_size;
   sd.mode = d->splat_mode;
   sd.keep_codes = 0;
-  if ((rc = splat_create_impl(c, &cloud, &sd, &f->surface))) {
+  SplatScenes sc;
+  sc.scenes = (int)f->scenes;
+  sc.scene_of = f->scene_of;
+  sc.scene_base = f->scene_base_dev;
+  sc.follow = d->follow_size > 0;
+  if ((rc = splat_create_impl(c, &cloud, &sd, &f->surface, sc))) {
     fluid_free(f);
     return rc;
   }
-  // Hash grid (fluid.ts:239-247) and walls (fluid.ts:444-447) over the surface volume.
-  const Volume* vol = f->surface->vol;
-  const double ws = vol->info.world_size;
+  // Hash grid (fluid.ts:239-247) and walls (fluid.ts:444-447) over the world box.
+  const double ws = d->world_size > 0 ? d->world_size : 24.0;
   k.grid_res = (int)ceil(ws / h) + 2;
   for (int a = 0; a < 3; a++) {
-    k.grid_origin[a] = vol->origin[a] - h;
-    k.wall_lo[a] = vol->origin[a] + radius;
-    k.wall_hi[a] = vol->origin[a] + ws - radius;
+    k.grid_origin[a] = d->origin[a] - h;
+    k.wall_lo[a] = d->origin[a] + radius;
+    k.wall_hi[a] = d->origin[a] + ws - radius;
   }
   const uint64_t G = (uint64_t)k.grid_res * k.grid_res * k.grid_res;
   if (G >= 0xFFFFFFFFull) {
     fluid_free(f);
     return fail(CLAY_ERR_INVALID, "FluidSim: worldSize / smoothing gives a hash grid beyond 2^32 cells");
   }
+  if (batch && G > (uint64_t)kCellMask) {
+    fluid_free(f);
+    return fail(CLAY_ERR_INVALID, "FluidBatch: worldSize / smoothing gives a hash grid beyond 2^24 - 1 cells");
+  }
   f->grid_cells = (size_t)G;
+  const uint64_t max_key = batch ? ((uint64_t)(scenes - 1) << kSceneShift) | kCellMask : G;
   int bits = 1;
-  while (bits < 32 && (1ull << bits) <= G) bits++;
+  while (bits < 32 && (1ull << bits) <= max_key) bits++;
   f->key_bits = bits;
-  if ((rc = dalloc(&f->cells, 4 * G))) {
+  if ((rc = dalloc(&f->cells, 4 * G * f->scenes))) {
     fluid_free(f);
     return rc;
   }
   hipStream_t s = c->stream;
-  if ((rc = launch_fill_u32(c, s, f->cells, 4 * G, 0u)) ||
+  if ((rc = launch_fill_u32(c, s, f->cells, 4 * G * f->scenes, 0u)) ||
       (rc = launch_fill_u32(c, s, reinterpret_cast<uint32_t*>(f->st), (size_t)P * 11, 0u)) ||
-      (rc = launch_fill_u32(c, s, f->contacts, 4 + 4 * CLAY_CONTACTS, 0u))) {
+      (rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u)) ||
+      (rc = launch_fill_u32(c, s, f->epoch_dev, 1, 0u))) {
     fluid_free(f);
     return rc;
   }
@@ -966,7 +1068,20 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
     fluid_free(f);
     return fail(CLAY_ERR_HIP, "FluidSim: initialisation failed");
   }
+  if (batch) f->live = P;
   fluid_build_k(f);
+  *out = f;
+  return CLAY_OK;
+}
+
+}  // namespace
+
+extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fluid_desc* d, clay_fluid** out) {
+  if (!ctx || !d || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_create: NULL argument");
+  if (d->capacity == 0) return fail(CLAY_ERR_INVALID, "FluidSim: capacity must be positive");
+  Fluid* f = nullptr;
+  int rc = fluid_create_impl(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<Field*>(world), d, 1, nullptr, &f);
+  if (rc) return rc;
   *out = reinterpret_cast<clay_fluid*>(f);
   return CLAY_OK;
 }
@@ -1060,28 +1175,8 @@ extern "C" int clay_fluid_substep(clay_fluid* fh, clay_stream stream) {
 extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_step: NULL fluid");
-  hipStream_t s = pick(f->ctx, stream);
   fluid_build_k(f);
-  // fluid.ts:536-558
-  const uint32_t emit_count = f->pending_emit;
-  f->pending_emit = 0;
-  const uint32_t emit_base = f->cursor;
-  if (emit_count > 0) {
-    f->cursor = (emit_base + emit_count) % f->capacity;
-    f->live = std::min<uint32_t>(f->capacity, std::max<uint32_t>(f->live, emit_base + emit_count));
-    int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->emit_seed);
-    if (rc) return rc;
-  }
-  bool sorted = false;
-  for (int i = 0; i < f->consts.substeps && f->live > 0; i++) {
-    int rc = fluid_substep_impl(f, s);
-    if (rc) return rc;
-    sorted = true;
-  }
-  f->surface->cloud.live_count = f->live;
-  // The last substep's permutation is a spatially coherent visiting order for the
-  // splat; the codes do not depend on it.
-  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr);
+  return fluid_frame_impl(f, pick(f->ctx, stream));
 }
 
 extern "C" int clay_fluid_read_contacts(clay_fluid* fh, float* out4, uint32_t* n, uint32_t* raw) {
@@ -1132,5 +1227,310 @@ extern "C" int clay_fluid_debug_download(clay_fluid* fh, int32_t what, uint32_t*
   if (f->live)
     CLAY_HIP_CHECK(hipMemcpy(host, what == CLAY_DEBUG_KEYS ? f->keys : f->vals, sizeof(uint32_t) * f->live,
                              hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+// ============================================================================
+// Launch tuning
+// ============================================================================
+namespace {
+
+int apply_tuning(Fluid* f, const clay_tuning* t) {
+  FluidTune ft;
+  ft.block = t->block;
+  ft.stage = t->stage;
+  if (!fluid_tune_valid(ft))
+    return fail(CLAY_ERR_INVALID, "clay_tuning: block must be 128 or 256 and stage 64, 128 or 256");
+  if (t->chunk < 256 || t->chunk > 65536) return fail(CLAY_ERR_INVALID, "clay_tuning: chunk must be in [256, 65536]");
+  if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
+    return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
+  f->tune = ft;
+  f->surface->chunk = (uint32_t)t->chunk;
+  f->surface->persistent_blocks = f->ctx->cus * t->groups_per_cu;
+  if (f->graph) {
+    (void)hipGraphExecDestroy(f->graph);
+    f->graph = nullptr;
+  }
+  return CLAY_OK;
+}
+
+void read_tuning(const Fluid* f, clay_tuning* out) {
+  out->block = f->tune.block;
+  out->stage = f->tune.stage;
+  out->chunk = (int32_t)f->surface->chunk;
+  out->groups_per_cu = f->surface->persistent_blocks / std::max(1, f->ctx->cus);
+}
+
+}  // namespace
+
+extern "C" int clay_fluid_set_tuning(clay_fluid* fh, const clay_tuning* t) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f || !t) return fail(CLAY_ERR_INVALID, "clay_fluid_set_tuning: NULL argument");
+  return apply_tuning(f, t);
+}
+
+extern "C" int clay_fluid_get_tuning(const clay_fluid* fh, clay_tuning* out) {
+  if (!fh || !out) return fail(CLAY_ERR_INVALID, "clay_fluid_get_tuning: NULL argument");
+  read_tuning(reinterpret_cast<const Fluid*>(fh), out);
+  return CLAY_OK;
+}
+
+// ============================================================================
+// Batched scenes
+// ============================================================================
+namespace {
+
+Fluid* as_batch(clay_batch* b) { return reinterpret_cast<Fluid*>(b); }
+const Fluid* as_batch(const clay_batch* b) { return reinterpret_cast<const Fluid*>(b); }
+
+void drop_graph(Fluid* f) {
+  if (f->graph) (void)hipGraphExecDestroy(f->graph);
+  f->graph = nullptr;
+  f->graph_stream = nullptr;
+}
+
+// One frame of every scene.  The frame is captured once per stream and replayed:
+// every per-substep value it needs (the cell-record stamp, the scan totals, the
+// following origins) lives in device memory, and the fills are kernels, so a
+// replay repeats exactly what an eager frame enqueues.
+int batch_frame(Fluid* f, hipStream_t s) {
+  fluid_build_k(f);
+  const uint32_t subs = (uint32_t)f->consts.substeps;
+  // A stamp wrap wipes the cell table, which only an eager frame does.
+  if (!f->use_graph || f->epoch > 0xFFFFFFFFu - subs - 1u) return fluid_frame_impl(f, s);
+  if (!f->graph || f->graph_stream != s) {
+    drop_graph(f);
+    const uint32_t epoch0 = f->epoch;
+    f->ctx->capturing = true;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      f->ctx->capturing = false;
+      return fail(CLAY_ERR_HIP, std::string("FluidBatch: graph capture: ") + hipGetErrorString(e));
+    }
+    const int rc = fluid_frame_impl(f, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    f->ctx->capturing = false;
+    f->epoch = epoch0;
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(CLAY_ERR_HIP, std::string("FluidBatch: graph capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&f->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      f->graph = nullptr;
+      return fail(CLAY_ERR_HIP, std::string("FluidBatch: graph instantiation: ") + hipGetErrorString(e));
+    }
+    f->graph_stream = s;
+  }
+  CLAY_HIP_CHECK(hipGraphLaunch(f->graph, s));
+  f->epoch += subs;
+  return CLAY_OK;
+}
+
+int check_scene(const Fluid* f, uint32_t scene, const char* who) {
+  if (scene >= f->scenes) return fail(CLAY_ERR_INVALID, std::string(who) + ": scene out of range");
+  return CLAY_OK;
+}
+
+}  // namespace
+
+extern "C" int clay_batch_create(clay_ctx* ctx, clay_field* world, const clay_batch_desc* d, clay_batch** out) {
+  if (!ctx || !d || !out || !d->counts) return fail(CLAY_ERR_INVALID, "clay_batch_create: NULL argument");
+  if (d->scenes < 1 || d->scenes > 256) return fail(CLAY_ERR_INVALID, "FluidBatch: scenes must be in [1, 256]");
+  if (!(d->fluid.follow_size > 0))
+    return fail(CLAY_ERR_INVALID, "FluidBatch: every scene bakes a following volume (follow_size > 0)");
+  if (d->fluid.splat_mode != CLAY_SPLAT_TILED)
+    return fail(CLAY_ERR_INVALID, "FluidBatch: the batched bake uses the tiled schedule");
+  Fluid* f = nullptr;
+  int rc = fluid_create_impl(reinterpret_cast<Ctx*>(ctx), reinterpret_cast<Field*>(world), &d->fluid, d->scenes,
+                             d->counts, &f);
+  if (rc) return rc;
+  *out = reinterpret_cast<clay_batch*>(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_destroy(clay_batch* b) {
+  Fluid* f = as_batch(b);
+  if (f) (void)hipStreamSynchronize(f->ctx->stream);
+  if (f && f->graph_stream) (void)hipStreamSynchronize(f->graph_stream);
+  fluid_free(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_set_state(clay_batch* b, const float* pos, const float* vel) {
+  Fluid* f = as_batch(b);
+  if (!f || !pos) return fail(CLAY_ERR_INVALID, "clay_batch_set_state: NULL argument");
+  const uint32_t n = f->capacity;
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  float* dp[3] = {f->px, f->py, f->pz};
+  float* dv[3] = {f->vx, f->vy, f->vz};
+  for (int a = 0; a < 3; a++) {
+    CLAY_HIP_CHECK(hipMemcpy(dp[a], pos + (size_t)a * n, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (vel)
+      CLAY_HIP_CHECK(hipMemcpy(dv[a], vel + (size_t)a * n, sizeof(float) * n, hipMemcpyHostToDevice));
+    else
+      CLAY_HIP_CHECK(hipMemset(dv[a], 0, sizeof(float) * n));
+  }
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_get_state(clay_batch* b, int32_t what, float* host) {
+  Fluid* f = as_batch(b);
+  if (f && f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  return clay_fluid_get_state(reinterpret_cast<clay_fluid*>(b), what, host);
+}
+
+extern "C" int clay_batch_step(clay_batch* b, clay_stream stream) {
+  Fluid* f = as_batch(b);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_step: NULL batch");
+  return batch_frame(f, pick(f->ctx, stream));
+}
+
+extern "C" int clay_batch_set_graph(clay_batch* b, int32_t enabled) {
+  Fluid* f = as_batch(b);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_set_graph: NULL batch");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);
+  f->use_graph = enabled != 0;
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_configure(clay_batch* b, const clay_fluid_params* p) {
+  Fluid* f = as_batch(b);
+  if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_batch_configure: NULL argument");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);
+  f->params = *p;
+  fluid_build_k(f);
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_set_tuning(clay_batch* b, const clay_tuning* t) {
+  Fluid* f = as_batch(b);
+  if (!f || !t) return fail(CLAY_ERR_INVALID, "clay_batch_set_tuning: NULL argument");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);
+  return apply_tuning(f, t);
+}
+
+extern "C" int clay_batch_get_tuning(const clay_batch* b, clay_tuning* out) {
+  if (!b || !out) return fail(CLAY_ERR_INVALID, "clay_batch_get_tuning: NULL argument");
+  read_tuning(as_batch(b), out);
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_constants(const clay_batch* b, clay_fluid_consts* out) {
+  if (!b || !out) return fail(CLAY_ERR_INVALID, "clay_batch_constants: NULL argument");
+  *out = as_batch(b)->consts;
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_scene(const clay_batch* b, uint32_t scene, uint32_t* first, uint32_t* count) {
+  const Fluid* f = as_batch(b);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_scene: NULL batch");
+  if (int rc = check_scene(f, scene, "clay_batch_scene")) return rc;
+  if (first) *first = f->scene_base[scene];
+  if (count) *count = f->scene_base[scene + 1] - f->scene_base[scene];
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_read_contacts(clay_batch* b, uint32_t scene, float* out4, uint32_t* n,
+                                        uint32_t* raw) {
+  Fluid* f = as_batch(b);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_read_contacts: NULL batch");
+  if (int rc = check_scene(f, scene, "clay_batch_read_contacts")) return rc;
+  uint32_t buf[4 + 4 * CLAY_CONTACTS];
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(hipMemcpy(buf, f->contacts + (size_t)scene * (4 + 4 * CLAY_CONTACTS), sizeof(buf),
+                           hipMemcpyDeviceToHost));
+  const uint32_t cnt = std::min<uint32_t>(buf[0], CLAY_CONTACTS);
+  if (out4) memcpy(out4, buf + 4, sizeof(float) * 4 * cnt);
+  if (n) *n = cnt;
+  if (raw) *raw = buf[0];
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_volume_download(clay_batch* b, uint32_t scene, int32_t mip, uint16_t* rgba) {
+  Fluid* f = as_batch(b);
+  if (!f || !rgba) return fail(CLAY_ERR_INVALID, "clay_batch_volume_download: NULL argument");
+  if (int rc = check_scene(f, scene, "clay_batch_volume_download")) return rc;
+  Volume* v = f->surface->vol;
+  if (mip < 0 || mip >= v->info.mip_levels) return fail(CLAY_ERR_INVALID, "clay_batch_volume_download: bad mip");
+  const size_t r = (size_t)v->mip_res(mip), n = r * r * r;
+  const size_t off = (size_t)scene * v->total_voxels + v->voxel_off[mip];
+  std::vector<uint16_t> d(n), m(n);
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(hipMemcpy(d.data(), v->dist + off, n * 2, hipMemcpyDeviceToHost));
+  CLAY_HIP_CHECK(hipMemcpy(m.data(), v->mat + off, n * 2, hipMemcpyDeviceToHost));
+  const uint16_t one = f2h_host(1.0f);
+  for (size_t i = 0; i < n; i++) {
+    rgba[4 * i] = d[i];
+    rgba[4 * i + 1] = m[i];
+    rgba[4 * i + 2] = 0;
+    rgba[4 * i + 3] = one;
+  }
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_grid_download(clay_batch* b, uint32_t scene, int32_t mip, uint32_t* host) {
+  Fluid* f = as_batch(b);
+  if (!f || !host) return fail(CLAY_ERR_INVALID, "clay_batch_grid_download: NULL argument");
+  if (int rc = check_scene(f, scene, "clay_batch_grid_download")) return rc;
+  Splat* sp = f->surface;
+  if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_batch_grid_download: bad mip");
+  const size_t r = (size_t)sp->vol->mip_res(mip), n = r * r * r;
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(hipMemcpy(host, sp->grid + (size_t)scene * sp->grid_total + sp->grid_off[mip], n * 4,
+                           hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_tiles_download(clay_batch* b, uint32_t scene, int32_t mip, uint8_t* host) {
+  Fluid* f = as_batch(b);
+  if (!f || !host) return fail(CLAY_ERR_INVALID, "clay_batch_tiles_download: NULL argument");
+  if (int rc = check_scene(f, scene, "clay_batch_tiles_download")) return rc;
+  Splat* sp = f->surface;
+  if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_batch_tiles_download: bad mip");
+  const int tr = sp->vol->mip_res(mip) / kTile;
+  const size_t nt = (size_t)tr * tr * tr;
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(hipMemcpy(host, sp->tile_live + (size_t)scene * sp->tile_total + sp->tile_off[mip], nt,
+                           hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_origin(clay_batch* b, uint32_t scene, float origin[3]) {
+  Fluid* f = as_batch(b);
+  if (!f || !origin) return fail(CLAY_ERR_INVALID, "clay_batch_origin: NULL argument");
+  if (int rc = check_scene(f, scene, "clay_batch_origin")) return rc;
+  VolDev rec;
+  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(hipMemcpy(&rec, f->surface->vol->dev + scene, sizeof(VolDev), hipMemcpyDeviceToHost));
+  for (int a = 0; a < 3; a++) origin[a] = rec.origin[a];
+  return CLAY_OK;
+}
+
+extern "C" int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]) {
+  Fluid* f = as_batch(b);
+  if (!f || !out) return fail(CLAY_ERR_INVALID, "clay_batch_conserved: NULL argument");
+  if (int rc = check_scene(f, scene, "clay_batch_conserved")) return rc;
+  hipStream_t s = f->ctx->stream;
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  const size_t first = f->scene_base[scene];
+  const uint32_t n = f->scene_base[scene + 1] - f->scene_base[scene];
+  const float* pv[6] = {f->px + first, f->py + first, f->pz + first, f->vx + first, f->vy + first, f->vz + first};
+  double* res = f->reduce + (size_t)f->reduce_blocks * 6;
+  int rc = launch_conserved(f->ctx, s, n, pv, f->consts.mass, f->reduce, f->reduce_blocks, res);
+  if (rc) return rc;
+  CLAY_HIP_CHECK(hipMemcpyAsync(out, res, sizeof(double) * 7, hipMemcpyDeviceToHost, s));
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
   return CLAY_OK;
 }

@@ -45,8 +45,8 @@ struct LeafDev {
   int32_t nbrush;
   const clay_brush* brushes;
   float eps;
-  int32_t pad;
-  const VolDev* vol;
+  uint32_t scene_stride;  // volume leaf of a batch: one instance per scene, planes this far apart
+  const VolDev* vol;      // scene_stride != 0: an array of one record per scene
   const uint16_t* dist0;  // mip-0 distance plane (f16 bits), x fastest
 };
 
@@ -55,6 +55,21 @@ struct ColliderDev {
   int32_t nleaf;
   int32_t pad;
   LeafDev leaf[CLAY_MAX_LEAVES];
+};
+
+// How one substep launch finds its scenes.  A batch sorts keys (scene << 24 | cell),
+// so the sorted key of a particle names its scene; a single simulation is scene 0.
+// Each scene owns `cells_per_scene` cell records and `contact_stride` words of
+// contact list.  The substep's stamp is read from device memory so that a frame
+// captured once in a graph advances it on every replay.
+constexpr int kSceneShift = 24;
+constexpr uint32_t kCellMask = (1u << kSceneShift) - 1u;  // cell bits; all ones = outside the grid
+struct SceneK {
+  const uint32_t* epoch;
+  uint32_t batched;
+  uint32_t cells_per_scene;
+  uint32_t contact_stride;
+  uint32_t pad;
 };
 
 // FluidSim constants as f32 shader literals, derived in f64 on the host.
@@ -258,12 +273,19 @@ __device__ inline float vol_dist(const VolDev& v, const uint16_t* d, f3 p) {
   return sample_raw0(v, d, p) * band_world;
 }
 
-__device__ inline float leaf_dist(const LeafDev& l, f3 p) {
-  if (l.kind == 0) return analytic_dist(l, p);
-  return vol_dist(*l.vol, l.dist0, p);
+__device__ __forceinline__ const VolDev& leaf_vol(const LeafDev& l, uint32_t scene) {
+  return l.scene_stride ? l.vol[scene] : *l.vol;
+}
+__device__ __forceinline__ const uint16_t* leaf_plane(const LeafDev& l, uint32_t scene) {
+  return l.dist0 + (size_t)l.scene_stride * scene;
 }
 
-__device__ inline f3 leaf_normal(const LeafDev& l, f3 p) {
+__device__ inline float leaf_dist(const LeafDev& l, uint32_t scene, f3 p) {
+  if (l.kind == 0) return analytic_dist(l, p);
+  return vol_dist(leaf_vol(l, scene), leaf_plane(l, scene), p);
+}
+
+__device__ inline f3 leaf_normal(const LeafDev& l, uint32_t scene, f3 p) {
   if (l.kind == 0) {
     float e = l.eps;
     f3 g = mk(analytic_dist(l, mk(p.x + e, p.y, p.z)) - analytic_dist(l, mk(p.x - e, p.y, p.z)),
@@ -272,21 +294,22 @@ __device__ inline f3 leaf_normal(const LeafDev& l, f3 p) {
     float len = len3(g);
     return len > 1e-9f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
   }
-  const VolDev& v = *l.vol;
+  const VolDev& v = leaf_vol(l, scene);
+  const uint16_t* d = leaf_plane(l, scene);
   float h = v.grad_h;
-  f3 g = mk(vol_dist(v, l.dist0, mk(p.x + h, p.y, p.z)) - vol_dist(v, l.dist0, mk(p.x - h, p.y, p.z)),
-            vol_dist(v, l.dist0, mk(p.x, p.y + h, p.z)) - vol_dist(v, l.dist0, mk(p.x, p.y - h, p.z)),
-            vol_dist(v, l.dist0, mk(p.x, p.y, p.z + h)) - vol_dist(v, l.dist0, mk(p.x, p.y, p.z - h)));
+  f3 g = mk(vol_dist(v, d, mk(p.x + h, p.y, p.z)) - vol_dist(v, d, mk(p.x - h, p.y, p.z)),
+            vol_dist(v, d, mk(p.x, p.y + h, p.z)) - vol_dist(v, d, mk(p.x, p.y - h, p.z)),
+            vol_dist(v, d, mk(p.x, p.y, p.z + h)) - vol_dist(v, d, mk(p.x, p.y, p.z - h)));
   float len = len3(g);
   return len > 1e-8f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
 }
 
 // Returns the leaf the union selects (right child wins unless left is strictly nearer).
-__device__ inline int collider_pick(const ColliderDev& c, f3 p, float* dist) {
+__device__ inline int collider_pick(const ColliderDev& c, uint32_t scene, f3 p, float* dist) {
   int best = 0;
-  float bd = leaf_dist(c.leaf[0], p);
+  float bd = leaf_dist(c.leaf[0], scene, p);
   for (int i = 1; i < c.nleaf; i++) {
-    float d = leaf_dist(c.leaf[i], p);
+    float d = leaf_dist(c.leaf[i], scene, p);
     if (!(bd < d)) {
       best = i;
       bd = d;
@@ -296,10 +319,10 @@ __device__ inline int collider_pick(const ColliderDev& c, f3 p, float* dist) {
   return best;
 }
 
-__device__ inline f3 collider_normal(const ColliderDev& c, int leaf, f3 p) {
+__device__ inline f3 collider_normal(const ColliderDev& c, uint32_t scene, int leaf, f3 p) {
   // Unrolled so the leaf record stays in registers/SGPRs.
   for (int i = 0; i < CLAY_MAX_LEAVES; i++)
-    if (i == leaf) return leaf_normal(c.leaf[i], p);
+    if (i == leaf) return leaf_normal(c.leaf[i], scene, p);
   return mk(0.0f, 1.0f, 0.0f);
 }
 

@@ -55,6 +55,7 @@ struct Ctx {
   int persistent_blocks = 2048;  // grid of persistent work-queue kernels
   hipStream_t stream = nullptr;
   bool timing = false;
+  bool capturing = false;        // a graph capture is open: launch timers stay off
   std::vector<KernelStat> stats;
   std::vector<PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
@@ -70,13 +71,14 @@ inline hipStream_t pick(Ctx* c, clay_stream s) {
 // ---- objects ---------------------------------------------------------------
 struct Volume {
   Ctx* ctx = nullptr;
+  int instances = 1;             // a batch keeps one instance per scene, planes back to back
   clay_volume_info info{};
   double origin[3] = {0, 0, 0};
   double grad_voxels = 1.0;
-  VolDev host{};                 // mirror of the device record
-  VolDev* dev = nullptr;         // device record (kernels read this)
-  uint16_t* dist = nullptr;      // all mips, distance plane (f16 bits)
-  uint16_t* mat = nullptr;       // all mips, material plane (f16 bits)
+  VolDev host{};                 // mirror of the device record (of every instance at creation)
+  VolDev* dev = nullptr;         // device records, one per instance (kernels read these)
+  uint16_t* dist = nullptr;      // all mips of every instance, distance plane (f16 bits)
+  uint16_t* mat = nullptr;       // all mips of every instance, material plane (f16 bits)
   std::vector<size_t> voxel_off; // per mip offset into the planes
   size_t total_voxels = 0;
   int mip_res(int m) const { return info.resolution >> m; }
@@ -114,9 +116,24 @@ struct Splat {
   uint32_t* item_start = nullptr;       // first work item (chunk) of each brick
   uint8_t* tile_live = nullptr;      // per 8^3 tile, all mips: live after last bake
   std::vector<size_t> tile_off;
-  size_t tile_total = 0;
+  size_t tile_total = 0;             // per scene
   bool has_baked = false;
+  // scenes (a batch): brick, grid, tile and texel blocks per scene, scene-major
+  int scenes = 1;
+  const uint32_t* scene_of = nullptr;    // device, per particle id (not owned)
+  const uint32_t* scene_base = nullptr;  // device, scenes + 1 prefix offsets (not owned)
+  bool follow = false;                   // centre each volume on its scene's AABB before a bake
+  // tuning
+  uint32_t chunk = 2048;                 // particles per splat work item
+  int persistent_blocks = 2048;          // grid of the splat work-item kernel
 };
+
+// Tunable launch shapes of the neighbour kernels (results do not depend on them).
+struct FluidTune {
+  int block = 256;   // threads per workgroup: 128 or 256
+  int stage = 128;   // FP16-LDS staging slice per wave: 64, 128 or 256 neighbours
+};
+bool fluid_tune_valid(const FluidTune& t);
 
 struct Fluid {
   Ctx* ctx = nullptr;
@@ -141,40 +158,52 @@ struct Fluid {
   uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
   uint32_t* hist = nullptr;      // digit histograms
   size_t hist_cap = 0;
-  uint32_t* cells = nullptr;     // dense over gridRes^3: {start, end, epoch, 0} per cell
-  uint32_t epoch = 0;            // stamp of the current substep's cell records
+  uint32_t* cells = nullptr;     // dense over gridRes^3 per scene: {start, end, epoch, 0} per cell
+  uint32_t epoch = 0;            // host count of the device stamp (wrap bookkeeping)
+  uint32_t* epoch_dev = nullptr; // stamp of the current substep's cell records
   size_t grid_cells = 0;
   int key_bits = 0;
-  uint32_t* contacts = nullptr;  // [0] = count, then 32 x float4 (as bits)
+  uint32_t* contacts = nullptr;  // per scene: [0] = count, then 32 x float4 (as bits)
   double* reduce = nullptr;      // FP64 reduction partials (+ 8 results at the end)
   uint32_t reduce_blocks = 0;
   Splat* surface = nullptr;      // owned
   FluidK k{};
+  FluidTune tune{};
+  // batch of independent scenes (one scene for a FluidSim)
+  uint32_t scenes = 1;
+  std::vector<uint32_t> scene_base;  // host, scenes + 1 prefix offsets
+  uint32_t* scene_of = nullptr;      // device, per particle (batches only)
+  uint32_t* scene_base_dev = nullptr;
+  // one frame captured as a graph and replayed (batches)
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
+  SceneK scene_k() const {
+    SceneK sk{};
+    sk.epoch = epoch_dev;
+    sk.batched = scene_of != nullptr;
+    sk.cells_per_scene = (uint32_t)grid_cells;
+    sk.contact_stride = 4 + 4 * CLAY_CONTACTS;
+    return sk;
+  }
 };
 
 // ---- launchers (implemented in the .hip files) ------------------------------
 // fluid.hip
-int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px,
-                const float* py, const float* pz, uint32_t* keys, uint32_t* vals,
-                uint32_t invalid_key);
-int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
-                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cells, uint32_t epoch,
-                   const float* const* src, float* const* dst, int planes);
-int launch_density(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                   const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
-                   float* srho, float* spres);
-int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                            uint32_t n, const float* const* sorted, const uint32_t* svals,
-                            const uint32_t* cells, uint32_t epoch, float* const* out,
-                            uint32_t* contacts);
+int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
+                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key);
+int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
+                   const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
+                   float* const* dst, int planes);
+int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
+                   uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
+                   const uint32_t* cells, float* srho, float* spres);
+int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune,
+                            bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
+                            const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
+                            float* const* out, uint32_t* contacts);
+int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch);
 int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
-int launch_density_lds(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                       const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
-                       float* srho, float* spres);
-int launch_forces_integrate_lds(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                                uint32_t n, const float* const* sorted, const uint32_t* svals,
-                                const uint32_t* cells, uint32_t epoch, float* const* out,
-                                uint32_t* contacts);
 int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, double mass,
                      double* partial, uint32_t partial_blocks, double* out);
 // sort.hip
@@ -193,6 +222,6 @@ size_t splat_bin_slices();
 int launch_bake_brushes(Ctx* c, hipStream_t s, Volume* v, const clay_brush* brushes_dev, int n);
 int launch_field_eval(Ctx* c, hipStream_t s, const ColliderDev& col, int n, const float* pts,
                       float* dist, float* nrm);
-int launch_set_origin_dev(Ctx* c, hipStream_t s, VolDev* dev, const float* origin3, float half);
+int launch_set_origin_dev(Ctx* c, hipStream_t s, VolDev* dev, const float* origin3, double half);
 
 }  // namespace clay

@@ -16,13 +16,18 @@
 // Forces and integrate can share one launch because the neighbour reads all hit
 // the sorted snapshot while the writes go to the id-ordered state: the snapshot is
 // the reference's `nvel` staging double buffer (fluid.ts:14-19) turned around.
+//
+// The same launches serve a batch of independent scenes: keys carry the scene in
+// their top bits, so one sort orders every scene's particles by (scene, cell, id),
+// each scene has its own block of cell records and contact list, and a scene's
+// result is the one its own single simulation would compute.
 #include "clay_internal.h"
 
 namespace clay {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // kernels that are not tuned
 
 __device__ __forceinline__ void cell_of(const FluidK& k, f3 p, int32_t q[3]) {
   q[0] = sat_i32(floorf((p.x - k.go[0]) * k.inv_h));
@@ -37,6 +42,15 @@ __device__ __forceinline__ uint32_t cell_index(const FluidK& k, int32_t x, int32
 }
 __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) {
   return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+
+// Scene of sorted slot s and the first of its cell records.
+__device__ __forceinline__ uint32_t scene_at(const SceneK& sk, const uint32_t* __restrict__ skeys, uint32_t s) {
+  return sk.batched ? skeys[s] >> kSceneShift : 0u;
+}
+__device__ __forceinline__ const uint4* scene_cells(const SceneK& sk, const uint4* __restrict__ cells,
+                                                    uint32_t scene) {
+  return cells + (size_t)scene * sk.cells_per_scene;
 }
 
 // Walks the 27-cell neighbourhood of cell b in the reference's order (dz, dy, dx
@@ -91,16 +105,24 @@ __device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __r
   }
 }
 
+// Cell key per particle.  One scene: the cell index, or `invalid` outside the
+// grid.  A batch: (scene << 24) | cell, with all cell bits set outside the grid,
+// so a scene's outside particles sort after its cells.
 __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const float* __restrict__ px,
                                                  const float* __restrict__ py,
                                                  const float* __restrict__ pz,
+                                                 const uint32_t* __restrict__ scene_of,
                                                  uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals, uint32_t invalid) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   int32_t q[3];
   cell_of(k, mk(px[i], py[i], pz[i]), q);
-  keys[i] = in_grid(k, q[0], q[1], q[2]) ? cell_index(k, q[0], q[1], q[2]) : invalid;
+  const bool in = in_grid(k, q[0], q[1], q[2]);
+  if (scene_of)
+    keys[i] = (scene_of[i] << kSceneShift) | (in ? cell_index(k, q[0], q[1], q[2]) : kCellMask);
+  else
+    keys[i] = in ? cell_index(k, q[0], q[1], q[2]) : invalid;
   vals[i] = i;
 }
 
@@ -112,18 +134,21 @@ struct Planes8 {
 // Cell records of this substep (run boundaries of the sorted keys) and the sorted
 // snapshot of the state planes.
 __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* __restrict__ skeys,
-                                                    const uint32_t* __restrict__ svals,
+                                                    const uint32_t* __restrict__ svals, SceneK sk,
                                                     uint32_t invalid, uint32_t* __restrict__ cells,
-                                                    uint32_t epoch, Planes8 pl, int planes) {
+                                                    Planes8 pl, int planes) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t key = skeys[s];
-  if (key != invalid) {
+  const uint32_t cell = sk.batched ? (key & kCellMask) : key;
+  const bool inside = sk.batched ? cell != kCellMask : key != invalid;
+  if (inside) {
+    const size_t rec = 4 * ((size_t)(sk.batched ? key >> kSceneShift : 0u) * sk.cells_per_scene + cell);
     if (s == 0 || skeys[s - 1] != key) {
-      cells[4 * (size_t)key + 0] = s;
-      cells[4 * (size_t)key + 2] = epoch;
+      cells[rec + 0] = s;
+      cells[rec + 2] = *sk.epoch;
     }
-    if (s == n - 1 || skeys[s + 1] != key) cells[4 * (size_t)key + 1] = s + 1;
+    if (s == n - 1 || skeys[s + 1] != key) cells[rec + 1] = s + 1;
   }
   const uint32_t j = svals[s];
 #pragma unroll
@@ -131,17 +156,20 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
     if (p < planes) pl.dst[p][s] = pl.src[p][j];
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_density(FluidK k, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
-              const float* __restrict__ spz, const uint4* __restrict__ cells, uint32_t epoch,
-              float* __restrict__ srho, float* __restrict__ spres) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+template <int kB>
+__global__ void __launch_bounds__(kB)
+    k_density(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
+              const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
+              const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
+  const uint32_t s = blockIdx.x * kB + threadIdx.x;
   if (s >= n) return;
+  const uint32_t epoch = *sk.epoch;
+  const uint4* cs = scene_cells(sk, cells, scene_at(sk, skeys, s));
   const f3 pi = mk(spx[s], spy[s], spz[s]);
   int32_t b[3];
   cell_of(k, pi, b);
   float rho = 0.0f;
-  for_neighbours(k, cells, epoch, b, [&](uint32_t t) {
+  for_neighbours(k, cs, epoch, b, [&](uint32_t t) {
     const f3 d = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
     const float r2 = dot3(d, d);
     if (r2 < k.hh) {
@@ -160,61 +188,12 @@ struct StateOut {
   float* p[11];  // px py pz vx vy vz rho pres nvx nvy nvz
 };
 
-__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t s, f3 pi,
-                                                f3 vi, float rhoi, float pri, f3 fp, f3 vsum, float wsum,
-                                                const uint32_t* __restrict__ svals, const StateOut& out,
-                                                uint32_t* __restrict__ contacts);
-
-__global__ void __launch_bounds__(kBlock)
-    k_forces_integrate(FluidK k, ColliderDev col, uint32_t n, SortedIn in,
-                       const uint32_t* __restrict__ svals, const uint4* __restrict__ cells,
-                       uint32_t epoch, StateOut out, uint32_t* __restrict__ contacts) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= n) return;
-  const float* __restrict__ spx = in.p[0];
-  const float* __restrict__ spy = in.p[1];
-  const float* __restrict__ spz = in.p[2];
-  const float* __restrict__ svx = in.p[3];
-  const float* __restrict__ svy = in.p[4];
-  const float* __restrict__ svz = in.p[5];
-  const float* __restrict__ srho = in.p[6];
-  const float* __restrict__ spres = in.p[7];
-
-  const f3 pi = mk(spx[s], spy[s], spz[s]);
-  const f3 vi = mk(svx[s], svy[s], svz[s]);
-  const float rhoi = srho[s], pri = spres[s];
-  int32_t b[3];
-  cell_of(k, pi, b);
-  f3 fp = mk(0.0f, 0.0f, 0.0f);
-  f3 vsum = vi * k.h6;
-  float wsum = k.h6;
-  for_neighbours(k, cells, epoch, b, [&](uint32_t t) {
-    if (t == s) return;
-    const f3 dv = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
-    const float r2 = dot3(dv, dv);
-    if (r2 >= k.hh) return;
-    const float inv = 1.0f / srho[t];
-    const float w0 = k.hh - r2;
-    const float w = w0 * w0 * w0;
-    vsum = mk(vsum.x + svx[t] * w, vsum.y + svy[t] * w, vsum.z + svz[t] * w);
-    wsum = wsum + w;
-    const float r = sqrtf(r2);
-    if (r > 1e-6f) {
-      const float hr = k.h - r;
-      const float s2 = k.mass * (pri + spres[t]) * 0.5f * inv * k.spiky * (hr * hr);
-      const float ir = 1.0f / r;
-      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
-    }
-  });
-  finish_particle(k, col, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out, contacts);
-}
-
 // The end of the force pass (fluid.ts:430-437) and the integrate pass
 // (fluid.ts:449-491) for one particle, writing its id-ordered state.
-__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t s, f3 pi,
-                                                f3 vi, float rhoi, float pri, f3 fp, f3 vsum, float wsum,
-                                                const uint32_t* __restrict__ svals, const StateOut& out,
-                                                uint32_t* __restrict__ contacts) {
+__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t scene,
+                                                uint32_t s, f3 pi, f3 vi, float rhoi, float pri, f3 fp,
+                                                f3 vsum, float wsum, const uint32_t* __restrict__ svals,
+                                                const StateOut& out, uint32_t* __restrict__ contacts) {
   const float irho = 1.0f / rhoi;
   const f3 acc = mk(fp.x * irho + k.g[0], fp.y * irho + k.g[1], fp.z * irho + k.g[2]);
   const float tv = fclamp_s(k.visc, 0.0f, 1.0f);
@@ -230,9 +209,9 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderD
   f3 p = mk(pi.x + v.x * k.dt, pi.y + v.y * k.dt, pi.z + v.z * k.dt);
   if (col.nleaf > 0) {
     float dw;
-    const int leaf = collider_pick(col, p, &dw);
+    const int leaf = collider_pick(col, scene, p, &dw);
     if (dw < k.radius) {
-      const f3 nr = collider_normal(col, leaf, p);
+      const f3 nr = collider_normal(col, scene, leaf, p);
       const float push = k.radius - dw;
       p = mk(p.x + nr.x * push, p.y + nr.y * push, p.z + nr.z * push);
       const float vn = fmin_s(dot3(v, nr), 0.0f);
@@ -263,6 +242,55 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderD
   out.p[10][j] = nv.z;
 }
 
+template <int kB>
+__global__ void __launch_bounds__(kB)
+    k_forces_integrate(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
+                       const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
+                       const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
+  const uint32_t s = blockIdx.x * kB + threadIdx.x;
+  if (s >= n) return;
+  const float* __restrict__ spx = in.p[0];
+  const float* __restrict__ spy = in.p[1];
+  const float* __restrict__ spz = in.p[2];
+  const float* __restrict__ svx = in.p[3];
+  const float* __restrict__ svy = in.p[4];
+  const float* __restrict__ svz = in.p[5];
+  const float* __restrict__ srho = in.p[6];
+  const float* __restrict__ spres = in.p[7];
+  const uint32_t epoch = *sk.epoch;
+  const uint32_t scene = scene_at(sk, skeys, s);
+  const uint4* cs = scene_cells(sk, cells, scene);
+
+  const f3 pi = mk(spx[s], spy[s], spz[s]);
+  const f3 vi = mk(svx[s], svy[s], svz[s]);
+  const float rhoi = srho[s], pri = spres[s];
+  int32_t b[3];
+  cell_of(k, pi, b);
+  f3 fp = mk(0.0f, 0.0f, 0.0f);
+  f3 vsum = vi * k.h6;
+  float wsum = k.h6;
+  for_neighbours(k, cs, epoch, b, [&](uint32_t t) {
+    if (t == s) return;
+    const f3 dv = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
+    const float r2 = dot3(dv, dv);
+    if (r2 >= k.hh) return;
+    const float inv = 1.0f / srho[t];
+    const float w0 = k.hh - r2;
+    const float w = w0 * w0 * w0;
+    vsum = mk(vsum.x + svx[t] * w, vsum.y + svy[t] * w, vsum.z + svz[t] * w);
+    wsum = wsum + w;
+    const float r = sqrtf(r2);
+    if (r > 1e-6f) {
+      const float hr = k.h - r;
+      const float s2 = k.mass * (pri + spres[t]) * 0.5f * inv * k.spiky * (hr * hr);
+      const float ir = 1.0f / r;
+      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+    }
+  });
+  finish_particle(k, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
+                  contacts + (size_t)scene * sk.contact_stride);
+}
+
 // ---------------------------------------------------------------------------
 // Precision variant: FP16 cell-relative offsets staged in LDS, compensated
 // density sums.  Tolerance-graded against the f64 oracle; FMA contraction is
@@ -275,11 +303,10 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderD
 // lane walks its three cells out of LDS.  A lane knows which of its cells a staged
 // index falls in, so it rebuilds p_i - p_j = (o_i - o_j) - (c_j - c_i) h with o_i
 // in full f32.  A row whose union is wider than the slice, or whose cells
-// overflow the bucket, is walked from global memory instead.
+// overflow the bucket, is walked from global memory instead.  In a batch a wave
+// may straddle two scenes: each lane reads its own scene's cell records, and the
+// union of runs is still one span of the sorted order.
 // ---------------------------------------------------------------------------
-constexpr int kStage = 128;  // staged neighbours per wave per row
-constexpr int kWaves = kBlock / 64;
-
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -389,15 +416,18 @@ __device__ __forceinline__ f3 unpack_off(uint2 u) {
   return mk(h2f((uint16_t)(u.x & 0xffffu)), h2f((uint16_t)(u.x >> 16)), h2f((uint16_t)(u.y & 0xffffu)));
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_density_lds(FluidK k, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
-                  const float* __restrict__ spz, const uint4* __restrict__ cells, uint32_t epoch,
-                  float* __restrict__ srho, float* __restrict__ spres) {
+template <int kB, int kS>
+__global__ void __launch_bounds__(kB)
+    k_density_lds(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
+                  const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
+                  const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
 #pragma clang fp contract(fast)
-  __shared__ uint2 stage[kWaves][kStage];
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ uint2 stage[kB / 64][kS];
+  const uint32_t s = blockIdx.x * kB + threadIdx.x;
   const bool active = s < n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t epoch = *sk.epoch;
+  const uint4* cs = scene_cells(sk, cells, active ? scene_at(sk, skeys, s) : 0u);
   f3 pi = mk(0.0f, 0.0f, 0.0f);
   if (active) pi = mk(spx[s], spy[s], spz[s]);
   int32_t b[3] = {0, 0, 0};
@@ -413,11 +443,11 @@ __global__ void __launch_bounds__(kBlock)
   };
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
-      const RowRun r = row_run(k, cells, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
       const bool any = r.hi > r.lo;
       const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
       const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
-      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kStage;
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64)
           stage[wave][t - ulo] = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
@@ -441,18 +471,20 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 struct StagedNbr {
-  uint2 off;   // f16 position offset from the centre of the particle's cell
   float4 vr;   // velocity and density
+  uint2 off;   // f16 position offset from the centre of the particle's cell
   float pres;
+  float pad;
 };
 
-__global__ void __launch_bounds__(kBlock)
-    k_forces_integrate_lds(FluidK k, ColliderDev col, uint32_t n, SortedIn in,
-                           const uint32_t* __restrict__ svals, const uint4* __restrict__ cells,
-                           uint32_t epoch, StateOut out, uint32_t* __restrict__ contacts) {
+template <int kB, int kS>
+__global__ void __launch_bounds__(kB)
+    k_forces_integrate_lds(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
+                           const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
+                           const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
 #pragma clang fp contract(fast)
-  __shared__ StagedNbr stage[kWaves][kStage];
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ StagedNbr stage[kB / 64][kS];
+  const uint32_t s = blockIdx.x * kB + threadIdx.x;
   const bool active = s < n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float* __restrict__ spx = in.p[0];
@@ -463,6 +495,9 @@ __global__ void __launch_bounds__(kBlock)
   const float* __restrict__ svz = in.p[5];
   const float* __restrict__ srho = in.p[6];
   const float* __restrict__ spres = in.p[7];
+  const uint32_t epoch = *sk.epoch;
+  const uint32_t scene = active ? scene_at(sk, skeys, s) : 0u;
+  const uint4* cs = scene_cells(sk, cells, scene);
   f3 pi = mk(0.0f, 0.0f, 0.0f), vi = mk(0.0f, 0.0f, 0.0f);
   float rhoi = 1.0f, pri = 0.0f;
   int32_t b[3] = {0, 0, 0};
@@ -496,17 +531,18 @@ __global__ void __launch_bounds__(kBlock)
   };
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
-      const RowRun r = row_run(k, cells, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
       const bool any = r.hi > r.lo;
       const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
       const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
-      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kStage;
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
           StagedNbr e;
-          e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
           e.vr = make_float4(svx[t], svy[t], svz[t], srho[t]);
+          e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
           e.pres = spres[t];
+          e.pad = 0.0f;
           stage[wave][t - ulo] = e;
         }
       __syncthreads();
@@ -527,7 +563,8 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   if (!active) return;
-  finish_particle(k, col, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out, contacts);
+  finish_particle(k, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
+                  contacts + (size_t)scene * sk.contact_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,24 +664,70 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-inline unsigned grid_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+// Advances the device-resident substep stamp.
+__global__ void k_bump(uint32_t* epoch) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) epoch[0] = epoch[0] + 1u;
+}
+
+inline unsigned grid_for(uint32_t n, int block = kBlock) { return (n + block - 1) / block; }
+
+template <int kB>
+void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t n, const float* spx,
+                      const float* spy, const float* spz, const uint32_t* skeys, const uint4* cells,
+                      float* srho, float* spres, int stage) {
+  if (stage == 0) {
+    hipLaunchKernelGGL(k_density<kB>, dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz, skeys,
+                       cells, srho, spres);
+  } else if (stage == 64) {
+    hipLaunchKernelGGL((k_density_lds<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz,
+                       skeys, cells, srho, spres);
+  } else if (stage == 256) {
+    hipLaunchKernelGGL((k_density_lds<kB, 256>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy,
+                       spz, skeys, cells, srho, spres);
+  } else {
+    hipLaunchKernelGGL((k_density_lds<kB, 128>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy,
+                       spz, skeys, cells, srho, spres);
+  }
+}
+
+template <int kB>
+void launch_forces_t(hipStream_t s, const FluidK& k, const SceneK& sk, const ColliderDev& col, uint32_t n,
+                     const SortedIn& in, const uint32_t* svals, const uint32_t* skeys, const uint4* cells,
+                     const StateOut& o, uint32_t* contacts, int stage) {
+  if (stage == 0) {
+    hipLaunchKernelGGL(k_forces_integrate<kB>, dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n, in, svals,
+                       skeys, cells, o, contacts);
+  } else if (stage == 64) {
+    hipLaunchKernelGGL((k_forces_integrate_lds<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n,
+                       in, svals, skeys, cells, o, contacts);
+  } else if (stage == 256) {
+    hipLaunchKernelGGL((k_forces_integrate_lds<kB, 256>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col,
+                       n, in, svals, skeys, cells, o, contacts);
+  } else {
+    hipLaunchKernelGGL((k_forces_integrate_lds<kB, 128>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col,
+                       n, in, svals, skeys, cells, o, contacts);
+  }
+}
 
 }  // namespace
 
-int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px,
-                const float* py, const float* pz, uint32_t* keys, uint32_t* vals,
-                uint32_t invalid_key) {
+bool fluid_tune_valid(const FluidTune& t) {
+  return (t.block == 128 || t.block == 256) && (t.stage == 64 || t.stage == 128 || t.stage == 256);
+}
+
+int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
+                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key) {
   if (n == 0) return CLAY_OK;
   LaunchTimer t(c, s, "fluid_keys");
-  hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, px, py, pz, keys, vals,
+  hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, px, py, pz, scene_of, keys, vals,
                      invalid_key);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
-int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
-                   const uint32_t* svals, uint32_t invalid_key, uint32_t* cells, uint32_t epoch,
-                   const float* const* src, float* const* dst, int planes) {
+int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
+                   const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
+                   float* const* dst, int planes) {
   if (n == 0) return CLAY_OK;
   Planes8 pl{};
   for (int p = 0; p < planes && p < 8; p++) {
@@ -652,35 +735,50 @@ int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys,
     pl.dst[p] = dst[p];
   }
   LaunchTimer t(c, s, "fluid_reorder");
-  hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, skeys, svals,
-                     invalid_key, cells, epoch, pl, planes);
+  hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, skeys, svals, sk, invalid_key, cells,
+                     pl, planes);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
-int launch_density(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                   const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
-                   float* srho, float* spres) {
+int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
+                   uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
+                   const uint32_t* cells, float* srho, float* spres) {
   if (n == 0) return CLAY_OK;
-  LaunchTimer t(c, s, "fluid_density");
-  hipLaunchKernelGGL(k_density, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, spx, spy, spz,
-                     reinterpret_cast<const uint4*>(cells), epoch, srho, spres);
+  const uint4* cl = reinterpret_cast<const uint4*>(cells);
+  const int stage = lds ? tune.stage : 0;
+  LaunchTimer t(c, s, lds ? "fluid_density_lds" : "fluid_density");
+  if (tune.block == 128)
+    launch_density_t<128>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage);
+  else
+    launch_density_t<256>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
-int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                            uint32_t n, const float* const* sorted, const uint32_t* svals,
-                            const uint32_t* cells, uint32_t epoch, float* const* out,
-                            uint32_t* contacts) {
+int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune,
+                            bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
+                            const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
+                            float* const* out, uint32_t* contacts) {
   if (n == 0) return CLAY_OK;
   SortedIn in{};
   for (int p = 0; p < 8; p++) in.p[p] = sorted[p];
   StateOut o{};
   for (int p = 0; p < 11; p++) o.p[p] = out[p];
-  LaunchTimer t(c, s, "fluid_forces_integrate");
-  hipLaunchKernelGGL(k_forces_integrate, dim3(grid_for(n)), dim3(kBlock), 0, s, k, col, n, in,
-                     svals, reinterpret_cast<const uint4*>(cells), epoch, o, contacts);
+  const uint4* cl = reinterpret_cast<const uint4*>(cells);
+  const int stage = lds ? tune.stage : 0;
+  LaunchTimer t(c, s, lds ? "fluid_forces_integrate_lds" : "fluid_forces_integrate");
+  if (tune.block == 128)
+    launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
+  else
+    launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch) {
+  LaunchTimer t(c, s, "fluid_epoch");
+  hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, epoch);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
@@ -712,37 +810,6 @@ int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
   if (blocks > 65536) blocks = 65536;
   LaunchTimer t(c, s, "fill_u32");
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v);
-  CLAY_HIP_CHECK(hipGetLastError());
-  return CLAY_OK;
-}
-
-}  // namespace clay
-
-namespace clay {
-
-int launch_density_lds(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* spx,
-                       const float* spy, const float* spz, const uint32_t* cells, uint32_t epoch,
-                       float* srho, float* spres) {
-  if (n == 0) return CLAY_OK;
-  LaunchTimer t(c, s, "fluid_density_lds");
-  hipLaunchKernelGGL(k_density_lds, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, spx, spy, spz,
-                     reinterpret_cast<const uint4*>(cells), epoch, srho, spres);
-  CLAY_HIP_CHECK(hipGetLastError());
-  return CLAY_OK;
-}
-
-int launch_forces_integrate_lds(Ctx* c, hipStream_t s, const FluidK& k, const ColliderDev& col,
-                                uint32_t n, const float* const* sorted, const uint32_t* svals,
-                                const uint32_t* cells, uint32_t epoch, float* const* out,
-                                uint32_t* contacts) {
-  if (n == 0) return CLAY_OK;
-  SortedIn in{};
-  for (int p = 0; p < 8; p++) in.p[p] = sorted[p];
-  StateOut o{};
-  for (int p = 0; p < 11; p++) o.p[p] = out[p];
-  LaunchTimer t(c, s, "fluid_forces_integrate_lds");
-  hipLaunchKernelGGL(k_forces_integrate_lds, dim3(grid_for(n)), dim3(kBlock), 0, s, k, col, n, in, svals,
-                     reinterpret_cast<const uint4*>(cells), epoch, o, contacts);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

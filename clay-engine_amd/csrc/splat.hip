@@ -11,9 +11,9 @@
 //   1. k_bin<count>, k_brick_scan, k_bin<fill>: particles are binned to 16^3-voxel
 //      bricks of every mip (a particle lands in each brick its voxel box overlaps;
 //      the counters are hit once per wave and distinct brick).  The scan also
-//      cuts every brick's list into chunks of kChunk particles: a chunk is one
-//      work item, so a coarse-mip brick holding most of the cloud is spread over
-//      many workgroups instead of serialising on one.
+//      cuts every brick's list into chunks (2048 particles unless tuned): a chunk
+//      is one work item, so a coarse-mip brick holding most of the cloud is spread
+//      over many workgroups instead of serialising on one.
 //   2. k_brick_clear: the grid of every brick that received particles is reset.
 //   3. k_brick_splat: a persistent grid walks the work items.  Each item keeps a
 //      16 KB LDS copy of its brick and, for every (particle, voxel) pair inside the
@@ -34,6 +34,11 @@
 //
 // Voxels whose centre is farther than `reach` from the particle are skipped row
 // by row; they fail sd < bandWorld, so skipping them cannot change a code.
+//
+// A batch bakes one volume instance per scene in the same launches: brick ids run
+// scene-major (scene * bricks-per-scene + brick), a particle's scene comes from a
+// per-particle table, and each scene has its own origin record, code grid block,
+// texel planes and tile mask.  A one-volume splat is the batch of one scene.
 #include "clay_internal.h"
 
 namespace clay {
@@ -43,7 +48,6 @@ namespace {
 constexpr int kBrick = 16;                  // voxels per brick edge
 constexpr int kBrickVox = kBrick * kBrick * kBrick;
 constexpr int kThreads = 256;
-constexpr uint32_t kChunk = 2048;           // particles per work item
 constexpr int kSlices = 32;                 // binning counters per brick
 constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared distance
 
@@ -61,15 +65,21 @@ struct SplatMipK {
 struct CloudK {
   const float* pos[3];
   const float* nrm[3];
-  const uint32_t* order;  // optional visiting order (any permutation gives the same codes)
-  uint32_t first, end;    // live particles: ids in [first, end)
-  uint32_t count;         // end - first
+  const uint32_t* order;     // optional visiting order (any permutation gives the same codes)
+  const uint32_t* scene_of;  // batch: scene of each particle id (else every particle is scene 0)
+  uint32_t first, end;       // live particles: ids in [first, end)
+  uint32_t count;            // end - first
 };
 
 struct MipsK {
   int32_t mips;
   float inv_enc;          // f32(1 / 32767.5)
   uint16_t sat_half;      // f16(+1.0)
+  uint32_t chunk;         // particles per work item
+  uint32_t bps;           // bricks per scene (all mips)
+  uint32_t gps;           // code-grid cells per scene
+  uint32_t vps;           // texels per scene (all mips)
+  uint32_t tps;           // 8^3 tiles per scene (all mips)
   SplatMipK m[CLAY_MAX_MIPS];
 };
 
@@ -219,8 +229,11 @@ __global__ void __launch_bounds__(kThreads)
     i = particle_at(c, k);
     valid = live_id(c, i);
   }
+  uint32_t scene = 0;
+  if (valid && c.scene_of) scene = c.scene_of[i];
   f3 p = mk(0.0f, 0.0f, 0.0f);
-  if (valid) p = load_rel(c, vol, i);
+  if (valid) p = load_rel(c, vol + scene, i);
+  const uint32_t sb = scene * mk_.bps;
   for (int mi = 0; mi < mk_.mips; mi++) {
     const SplatMipK& m = mk_.m[mi];
     int32_t blo[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
@@ -233,7 +246,7 @@ __global__ void __launch_bounds__(kThreads)
         const int32_t bx = blo[0] + q % bext[0];
         const int32_t by = blo[1] + (q / bext[0]) % bext[1];
         const int32_t bz = blo[2] + q / (bext[0] * bext[1]);
-        b = m.boff + (uint32_t)bx + (uint32_t)m.nb * ((uint32_t)by + (uint32_t)m.nb * (uint32_t)bz);
+        b = sb + m.boff + (uint32_t)bx + (uint32_t)m.nb * ((uint32_t)by + (uint32_t)m.nb * (uint32_t)bz);
       }
       // Each brick has kSlices counters; a workgroup uses slice blockIdx % kSlices,
       // which spreads a coarse-mip brick's tens of thousands of wave atomics over
@@ -248,7 +261,7 @@ __global__ void __launch_bounds__(kThreads)
 // bricks, exclusive scans of the totals (-> list starts; slice cursors) and of the
 // per-brick chunk counts (-> first work item of each brick).  One block.
 __global__ void __launch_bounds__(1024)
-    k_brick_scan(const uint32_t* __restrict__ sub, uint32_t nbricks, uint32_t* __restrict__ count,
+    k_brick_scan(const uint32_t* __restrict__ sub, uint32_t nbricks, uint32_t chunk, uint32_t* __restrict__ count,
                  uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
                  uint32_t* __restrict__ item_start, ScanOut* __restrict__ out) {
   __shared__ uint32_t sa[1024], sb[1024];
@@ -263,7 +276,7 @@ __global__ void __launch_bounds__(1024)
     uint32_t va = 0;
     if (i < nbricks)
       for (int j = 0; j < kSlices; j++) va += sub[i * kSlices + j];
-    const uint32_t vb = (va + kChunk - 1) / kChunk;
+    const uint32_t vb = (va + chunk - 1) / chunk;
     sa[threadIdx.x] = va;
     sb[threadIdx.x] = vb;
     __syncthreads();
@@ -326,13 +339,15 @@ __global__ void __launch_bounds__(kThreads)
     k_brick_clear(MipsK mk_, const uint32_t* __restrict__ count, uint32_t* __restrict__ grid) {
   const uint32_t b = blockIdx.x;
   if (count[b] == 0) return;
-  const BrickPos q = brick_pos(mk_, b);
+  const uint32_t scene = b / mk_.bps;
+  const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
   const SplatMipK& m = mk_.m[q.mi];
   const uint32_t init = kNormals ? kEncSat : kSInit;
+  uint32_t* g = grid + (size_t)scene * mk_.gps + m.off;
   for (int v = threadIdx.x; v < kBrickVox; v += kThreads) {
     const int32_t x = q.ox + (v & 15), y = q.oy + ((v >> 4) & 15), z = q.oz + (v >> 8);
     if (x >= m.r || y >= m.r || z >= m.r) continue;
-    grid[m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z)] = init;
+    g[(uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z)] = init;
   }
 }
 
@@ -366,19 +381,22 @@ __global__ void __launch_bounds__(kThreads)
     for (int v = t; v < kBrickVox; v += kThreads) lds[v] = init;
     __syncthreads();
     const uint32_t b = sh_brick;
-    const BrickPos q = brick_pos(mk_, b);
+    const uint32_t scene = b / mk_.bps;
+    const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
     const SplatMipK& m = mk_.m[q.mi];
+    const VolDev* sv = vol + scene;
+    uint32_t* sgrid = grid + (size_t)scene * mk_.gps;
     const uint32_t n_b = count[b];
     const uint32_t chunk = item - item_start[b];
-    const uint32_t e_lo = chunk * kChunk;
-    const uint32_t n_e = (n_b - e_lo) < kChunk ? (n_b - e_lo) : kChunk;
+    const uint32_t e_lo = chunk * mk_.chunk;
+    const uint32_t n_e = (n_b - e_lo) < mk_.chunk ? (n_b - e_lo) : mk_.chunk;
     const uint32_t base = start[b] + e_lo;
     const int32_t ox = q.ox, oy = q.oy, oz = q.oz;
     for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
       const uint32_t e = pass * kThreads + rev;
       if (e >= n_e) continue;
       const uint32_t i = list[base + e];
-      const f3 p = load_rel(c, vol, i);
+      const f3 p = load_rel(c, sv, i);
       const f3 nr = kNormals ? load_nrm(c, i) : mk(0.0f, 0.0f, 0.0f);
       int32_t lo[3], hi[3];
       voxel_box(m, p, lo, hi);  // non-empty: the particle was binned here
@@ -421,12 +439,12 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
     // Merge the brick into the grid.
-    const bool single = n_b <= kChunk;
+    const bool single = n_b <= mk_.chunk;
     for (int v = t; v < kBrickVox; v += kThreads) {
       const int32_t x = ox + (v & 15), y = oy + ((v >> 4) & 15), z = oz + (v >> 8);
       if (x >= m.r || y >= m.r || z >= m.r) continue;
       const uint32_t val = lds[v];
-      uint32_t* g = grid + m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z);
+      uint32_t* g = sgrid + m.off + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z);
       if (single) {
         *g = val;
       } else if (val != init) {
@@ -444,8 +462,12 @@ __global__ void __launch_bounds__(kThreads)
                     uint16_t* __restrict__ dist, uint32_t* __restrict__ grid) {
   __shared__ uint32_t tflag;
   const uint32_t b = blockIdx.x;
-  const BrickPos q = brick_pos(mk_, b);
+  const uint32_t scene = b / mk_.bps;
+  const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
   const SplatMipK& m = mk_.m[q.mi];
+  tile_live += (size_t)scene * mk_.tps;
+  dist += (size_t)scene * mk_.vps;
+  grid += (size_t)scene * mk_.gps;
   const int32_t tres = m.r / kTile;  // tiles per axis at this mip
   const int t = threadIdx.x;
   auto tile_index = [&](int tx, int ty, int tz) -> int32_t {
@@ -501,6 +523,50 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Follows each scene's particles: origin = centre of their AABB - worldSize / 2,
+// as SplatField.setCenter followed by SdfVolume.setOrigin computes it
+// (splat.ts:284-287, volume.ts:327-334: f64 arithmetic, stored as f32).  The
+// centre is the f32 midpoint of the f32 bounds.  A scene with no particles keeps
+// its origin.  One workgroup per scene; `base` holds scenes + 1 prefix offsets,
+// or is null for one scene of `n` particles.
+__global__ void __launch_bounds__(kThreads)
+    k_follow(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+             const uint32_t* __restrict__ base, uint32_t n, VolDev* __restrict__ vols, double half) {
+  __shared__ float sh[6][kThreads];
+  const uint32_t sc = blockIdx.x;
+  const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
+  if (lo >= hi) return;
+  const int t = threadIdx.x;
+  const float inf = __uint_as_float(0x7f800000u);
+  float a[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (uint32_t i = lo + t; i < hi; i += kThreads) {
+    const float x = px[i], y = py[i], z = pz[i];
+    a[0] = fmin_s(a[0], x);
+    a[1] = fmin_s(a[1], y);
+    a[2] = fmin_s(a[2], z);
+    a[3] = fmax_s(a[3], x);
+    a[4] = fmax_s(a[4], y);
+    a[5] = fmax_s(a[5], z);
+  }
+  for (int q = 0; q < 6; q++) sh[q][t] = a[q];
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      for (int q = 0; q < 3; q++) sh[q][t] = fmin_s(sh[q][t], sh[q][t + off]);
+      for (int q = 3; q < 6; q++) sh[q][t] = fmax_s(sh[q][t], sh[q][t + off]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int q = 0; q < 3; q++) {
+      const float c = (sh[q][0] + sh[q + 3][0]) * 0.5f;
+      const double o = (double)c - half;
+      vols[sc].origin[q] = (float)o;
+      vols[sc].center[q] = (float)(o + half);
+    }
+  }
+}
+
 }  // namespace
 
 static void fill_mips(Splat* sp, MipsK* mk_) {
@@ -509,6 +575,11 @@ static void fill_mips(Splat* sp, MipsK* mk_) {
   mk_->mips = sp->mips;
   mk_->inv_enc = (float)(1.0 / 32767.5);
   mk_->sat_half = f2h_host(1.0f);
+  mk_->chunk = sp->chunk;
+  mk_->bps = (uint32_t)sp->brick_total;
+  mk_->gps = (uint32_t)sp->grid_total;
+  mk_->vps = (uint32_t)v->total_voxels;
+  mk_->tps = (uint32_t)sp->tile_total;
   for (int m = 0; m < sp->mips; m++) {
     SplatMipK& k = mk_->m[m];
     k.r = v->mip_res(m);
@@ -548,6 +619,7 @@ static CloudK cloud_k(const Splat* sp, const uint32_t* order) {
   c.end = first + count;
   c.count = count;
   c.order = order;
+  c.scene_of = sp->scene_of;
   return c;
 }
 
@@ -555,9 +627,9 @@ template <bool kNormals>
 static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
   Ctx* ctx = sp->ctx;
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
-  const unsigned nbricks = (unsigned)sp->brick_total;
+  const unsigned nbricks = (unsigned)(sp->brick_total * sp->scenes);
   ScanOut* scan = reinterpret_cast<ScanOut*>(sp->brick_total_dev);
-  int rc = launch_fill_u32(ctx, s, sp->brick_sub, sp->brick_total * kSlices, 0u);
+  int rc = launch_fill_u32(ctx, s, sp->brick_sub, (size_t)nbricks * kSlices, 0u);
   if (rc) return rc;
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_bin_count");
@@ -567,7 +639,7 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   }
   {
     LaunchTimer t(ctx, s, "splat_brick_scan");
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_sub, (uint32_t)nbricks,
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_sub, (uint32_t)nbricks, sp->chunk,
                        sp->brick_count, sp->brick_start, sp->brick_cursor, sp->item_start, scan);
     CLAY_HIP_CHECK(hipGetLastError());
   }
@@ -585,7 +657,7 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   }
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_brick");
-    hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(ctx->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
+    hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
                        sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
                        (uint32_t)nbricks, sp->brick_list, sp->grid);
     CLAY_HIP_CHECK(hipGetLastError());
@@ -605,6 +677,12 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
   fill_mips(sp, &mk_);
   const CloudK c = cloud_k(sp, order);
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
+  if (sp->follow) {
+    LaunchTimer t(ctx, s, "splat_follow");
+    hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
+                       sp->scene_base, c.end, sp->vol->dev, sp->vol->info.world_size / 2);
+    CLAY_HIP_CHECK(hipGetLastError());
+  }
   if (sp->desc.mode == CLAY_SPLAT_PLAIN) {
     int rc = launch_fill_u32(ctx, s, sp->grid, sp->grid_total, kEncSat);
     if (rc) return rc;

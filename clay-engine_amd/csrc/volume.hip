@@ -56,20 +56,21 @@ __global__ void k_field_eval(ColliderDev col, int n, const float* __restrict__ p
   if (i >= n) return;
   const f3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   float d;
-  const int leaf = collider_pick(col, p, &d);
-  const f3 g = collider_normal(col, leaf, p);
+  const int leaf = collider_pick(col, 0u, p, &d);
+  const f3 g = collider_normal(col, 0u, leaf, p);
   dist[i] = d;
   nrm[3 * i] = g.x;
   nrm[3 * i + 1] = g.y;
   nrm[3 * i + 2] = g.z;
 }
 
-__global__ void k_set_origin(VolDev* v, const float* __restrict__ o, float half) {
+// volume.ts:327-334: centre = origin + worldSize / 2 in f64, stored as f32.
+__global__ void k_set_origin(VolDev* v, const float* __restrict__ o, double half) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     for (int c = 0; c < 3; c++) {
       const float oc = o[c];
       v->origin[c] = oc;
-      v->center[c] = oc + half;
+      v->center[c] = (float)((double)oc + half);
     }
   }
 }
@@ -108,7 +109,7 @@ int launch_field_eval(Ctx* c, hipStream_t s, const ColliderDev& col, int n, cons
   return CLAY_OK;
 }
 
-int launch_set_origin_dev(Ctx* c, hipStream_t s, VolDev* dev, const float* origin3, float half) {
+int launch_set_origin_dev(Ctx* c, hipStream_t s, VolDev* dev, const float* origin3, double half) {
   LaunchTimer t(c, s, "volume_set_origin");
   hipLaunchKernelGGL(k_set_origin, dim3(1), dim3(64), 0, s, dev, origin3, half);
   CLAY_HIP_CHECK(hipGetLastError());
