@@ -8,6 +8,7 @@ include/clay_hip.h.  There is no CPU fallback: every call goes to the GPU.
 from ._lib import ClayError, LIB_PATH, lib
 from .batch import FluidBatch
 from .brush import compile_brushes, make_brush
+from .edit import SdfEditor, body_centroids
 from .field import TracedField, analyticField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
 from .splat import ParticleCloud, SplatField
@@ -20,10 +21,12 @@ __all__ = [
     "LIB_PATH",
     "ParticleCloud",
     "Root",
+    "SdfEditor",
     "SdfVolume",
     "SplatField",
     "TracedField",
     "analyticField",
+    "body_centroids",
     "compile_brushes",
     "derive",
     "fluid_desc",

@@ -195,6 +195,12 @@ class FluidConsts(C.Structure):
     ]
 
 
+class EditorDesc(C.Structure):
+    _fields_ = [("max_edits", C.c_int32), ("tile_capacity", C.c_int32), ("safety", C.c_double)]
+
+
+REDUCE_F32_TREE, REDUCE_F64 = 0, 1
+
 STRUCTS = {
     "clay_brush": Brush,
     "clay_volume_desc": VolumeDesc,
@@ -207,6 +213,7 @@ STRUCTS = {
     "clay_fluid_consts": FluidConsts,
     "clay_tuning": Tuning,
     "clay_batch_desc": BatchDesc,
+    "clay_editor_desc": EditorDesc,
 }
 
 P = C.c_void_p
@@ -284,6 +291,17 @@ SIGNATURES = {
     "clay_batch_tiles_download": ([P, U32, I32, P], C.c_int),
     "clay_batch_origin": ([P, U32, C.POINTER(C.c_float)], C.c_int),
     "clay_batch_conserved": ([P, U32, DP], C.c_int),
+    "clay_editor_create": ([P, C.POINTER(EditorDesc), PP], C.c_int),
+    "clay_editor_destroy": ([P], C.c_int),
+    "clay_editor_push": ([P, C.POINTER(Brush), I32, C.POINTER(I32)], C.c_int),
+    "clay_editor_pending": ([P, C.POINTER(I32)], C.c_int),
+    "clay_editor_flush": ([P, P, C.POINTER(I32)], C.c_int),
+    "clay_editor_erode": ([P, P, C.c_double, C.c_double, C.c_double, P], C.c_int),
+    "clay_editor_erode_scene": ([P, P, U32, C.c_double, C.c_double, C.c_double, P], C.c_int),
+    "clay_editor_refresh_all": ([P, P], C.c_int),
+    "clay_editor_tile_counts": ([P, P], C.c_int),
+    "clay_editor_cells_download": ([P, P], C.c_int),
+    "clay_body_centroids": ([P, P, P, P, U32, I32, P, P], C.c_int),
 }
 
 _lib: C.CDLL | None = None

@@ -1534,3 +1534,209 @@ extern "C" int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]
   CLAY_HIP_CHECK(hipStreamSynchronize(s));
   return CLAY_OK;
 }
+
+// ============================================================================
+// SdfEditor
+// ============================================================================
+namespace {
+
+void editor_free(Editor* e) {
+  if (!e) return;
+  dfree(e->edits_dev);
+  dfree(e->nedit_dev);
+  dfree(e->counts);
+  dfree(e->ids);
+  dfree(e->list);
+  dfree(e->args);
+  delete e;
+}
+
+// Uploads the host queue as the device edit list (count included) and clears it.
+// With nothing queued it only resets the device count (a kernel), so an erosion
+// flush can be captured in a graph.
+int editor_stage(Editor* e, hipStream_t s) {
+  const uint32_t n = (uint32_t)e->pending.size();
+  if (n) CLAY_HIP_CHECK(hipMemcpyAsync(e->edits_dev, e->pending.data(), sizeof(clay_brush) * n,
+                                       hipMemcpyHostToDevice, s));
+  int rc = launch_fill_u32(e->ctx, s, e->nedit_dev, 1, n);
+  if (rc) return rc;
+  if (n) {
+    // The queue's storage must outlive the copy: wait for it before reusing the vector.
+    CLAY_HIP_CHECK(hipStreamSynchronize(s));
+    e->pending.clear();
+  }
+  return CLAY_OK;
+}
+
+int editor_erode_impl(Editor* e, Fluid* f, uint32_t scene, double min_speed, double radius, double blend,
+                      hipStream_t s) {
+  if (scene >= f->scenes) return fail(CLAY_ERR_INVALID, "clay_editor_erode: scene out of range");
+  if (!(radius > 0)) return fail(CLAY_ERR_INVALID, "clay_editor_erode: radius must be positive");
+  if (f->graph_stream && f->graph_stream != s) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  int rc = editor_stage(e, s);
+  if (rc) return rc;
+  rc = launch_contacts_to_edits(e->ctx, s, e, f->contacts + (size_t)scene * (4 + 4 * CLAY_CONTACTS), min_speed,
+                                radius, blend);
+  if (rc) return rc;
+  return editor_flush_device(e, s, false);
+}
+
+}  // namespace
+
+extern "C" int clay_editor_create(clay_volume* vol, const clay_editor_desc* desc, clay_editor** out) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || !out) return fail(CLAY_ERR_INVALID, "clay_editor_create: NULL argument");
+  if (v->instances != 1) return fail(CLAY_ERR_INVALID, "SdfEditor: the volume holds several instances");
+  Editor* e = new (std::nothrow) Editor();
+  if (!e) return fail(CLAY_ERR_INVALID, "out of host memory");
+  e->ctx = v->ctx;
+  e->vol = v;
+  if (desc) {
+    if (desc->max_edits > 0) e->max_edits = desc->max_edits;
+    if (desc->tile_capacity > 0) e->capacity = desc->tile_capacity;
+    if (desc->safety > 0) e->safety = desc->safety;
+  }
+  const size_t total = (size_t)v->info.total_cells;
+  const size_t t0 = (size_t)v->info.tile_res[0] * v->info.tile_res[0] * v->info.tile_res[0];
+  int rc;
+  if ((rc = dalloc(&e->edits_dev, (size_t)e->max_edits)) || (rc = dalloc(&e->nedit_dev, 1)) ||
+      (rc = dalloc(&e->counts, total)) || (rc = dalloc(&e->ids, t0 * (size_t)e->capacity)) ||
+      (rc = dalloc(&e->list, total)) || (rc = dalloc(&e->args, 4 * (size_t)v->info.mip_levels))) {
+    editor_free(e);
+    return rc;
+  }
+  hipStream_t s = e->ctx->stream;
+  if ((rc = launch_fill_u32(e->ctx, s, e->counts, total, 0u)) || (rc = launch_fill_u32(e->ctx, s, e->nedit_dev, 1, 0u)) ||
+      (rc = launch_fill_u32(e->ctx, s, e->args, 4 * (size_t)v->info.mip_levels, 0u))) {
+    editor_free(e);
+    return rc;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    editor_free(e);
+    return fail(CLAY_ERR_HIP, "SdfEditor: initialisation failed");
+  }
+  *out = reinterpret_cast<clay_editor*>(e);
+  return CLAY_OK;
+}
+
+extern "C" int clay_editor_destroy(clay_editor* eh) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (e) (void)hipStreamSynchronize(e->ctx->stream);
+  editor_free(e);
+  return CLAY_OK;
+}
+
+extern "C" int clay_editor_push(clay_editor* eh, const clay_brush* edits, int32_t n, int32_t* accepted) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (!e || (n > 0 && !edits)) return fail(CLAY_ERR_INVALID, "clay_editor_push: NULL argument");
+  int32_t k = 0;
+  for (int32_t i = 0; i < n; i++) {
+    if (edits[i].kind > 6)
+      return fail(CLAY_ERR_INVALID, "SdfEditor: brush kind " + std::to_string(edits[i].kind) +
+                                        " is outside the supported set (sphere, box, roundBox, capsule, torus, cylinder, plane)");
+    if ((int)e->pending.size() >= e->max_edits) break;  // modify.ts:203-209 drops the rest
+    e->pending.push_back(edits[i]);
+    k++;
+  }
+  if (accepted) *accepted = k;
+  if (k < n)
+    fprintf(stderr, "SdfEditor: dropping %d edit(s), %d already queued this frame\n", n - k, e->max_edits);
+  return CLAY_OK;
+}
+
+extern "C" int clay_editor_pending(const clay_editor* eh, int32_t* n) {
+  if (!eh || !n) return fail(CLAY_ERR_INVALID, "clay_editor_pending: NULL argument");
+  *n = (int32_t)reinterpret_cast<const Editor*>(eh)->pending.size();
+  return CLAY_OK;
+}
+
+extern "C" int clay_editor_flush(clay_editor* eh, clay_stream stream, int32_t* flushed) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (!e) return fail(CLAY_ERR_INVALID, "clay_editor_flush: NULL editor");
+  if (flushed) *flushed = 0;
+  if (e->pending.empty()) return CLAY_OK;  // modify.ts:215-217
+  hipStream_t s = pick(e->ctx, stream);
+  int rc = editor_stage(e, s);
+  if (rc) return rc;
+  rc = editor_flush_device(e, s, false);
+  if (!rc && flushed) *flushed = 1;
+  return rc;
+}
+
+extern "C" int clay_editor_erode(clay_editor* eh, clay_fluid* fh, double min_speed, double radius, double blend,
+                                 clay_stream stream) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!e || !f) return fail(CLAY_ERR_INVALID, "clay_editor_erode: NULL argument");
+  return editor_erode_impl(e, f, 0, min_speed, radius, blend, pick(e->ctx, stream));
+}
+
+extern "C" int clay_editor_erode_scene(clay_editor* eh, clay_batch* bh, uint32_t scene, double min_speed,
+                                       double radius, double blend, clay_stream stream) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  Fluid* f = reinterpret_cast<Fluid*>(bh);
+  if (!e || !f) return fail(CLAY_ERR_INVALID, "clay_editor_erode_scene: NULL argument");
+  return editor_erode_impl(e, f, scene, min_speed, radius, blend, pick(e->ctx, stream));
+}
+
+extern "C" int clay_editor_refresh_all(clay_editor* eh, clay_stream stream) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (!e) return fail(CLAY_ERR_INVALID, "clay_editor_refresh_all: NULL editor");
+  return editor_flush_device(e, pick(e->ctx, stream), true);
+}
+
+extern "C" int clay_editor_tile_counts(clay_editor* eh, uint32_t* per_mip) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (!e || !per_mip) return fail(CLAY_ERR_INVALID, "clay_editor_tile_counts: NULL argument");
+  const int mips = e->vol->info.mip_levels;
+  std::vector<uint32_t> args(4 * (size_t)mips);
+  CLAY_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(args.data(), e->args, sizeof(uint32_t) * args.size(), hipMemcpyDeviceToHost));
+  for (int m = 0; m < mips; m++) per_mip[m] = args[4 * (size_t)m];
+  return CLAY_OK;
+}
+
+extern "C" int clay_editor_cells_download(clay_editor* eh, uint32_t* counts) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  if (!e || !counts) return fail(CLAY_ERR_INVALID, "clay_editor_cells_download: NULL argument");
+  CLAY_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(counts, e->counts, sizeof(uint32_t) * (size_t)e->vol->info.total_cells,
+                           hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+// ============================================================================
+// Body centroids
+// ============================================================================
+extern "C" int clay_body_centroids(clay_ctx* ctx, const float* const pos[3], const float* const rest[3],
+                                   const uint32_t* first_count, uint32_t nbodies, int32_t mode, double* out6,
+                                   clay_stream stream) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !pos || !pos[0] || !pos[1] || !pos[2] || (nbodies && (!first_count || !out6)))
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids: NULL argument");
+  if (rest && (!rest[0] || !rest[1] || !rest[2]))
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids: rest must give all three axes or be NULL");
+  if (mode != CLAY_REDUCE_F32_TREE && mode != CLAY_REDUCE_F64)
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids: unknown reduction mode");
+  if (nbodies == 0) return CLAY_OK;
+  hipStream_t s = pick(c, stream);
+  uint32_t* fc = nullptr;
+  double* out = nullptr;
+  int rc;
+  if ((rc = dalloc(&fc, 2 * (size_t)nbodies)) || (rc = dalloc(&out, 6 * (size_t)nbodies))) {
+    dfree(fc);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(fc, first_count, sizeof(uint32_t) * 2 * nbodies, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    const float* p3[3] = {pos[0], pos[1], pos[2]};
+    const float* r3[3] = {rest ? rest[0] : nullptr, rest ? rest[1] : nullptr, rest ? rest[2] : nullptr};
+    rc = launch_body_centroids(c, s, mode == CLAY_REDUCE_F64, nbodies, fc, p3, rest ? r3 : nullptr, out);
+    if (!rc) e = hipMemcpyAsync(out6, out, sizeof(double) * 6 * nbodies, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(fc);
+  dfree(out);
+  if (e != hipSuccess) return fail(CLAY_ERR_HIP, std::string("clay_body_centroids: ") + hipGetErrorString(e));
+  return rc;
+}

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -188,7 +189,31 @@ struct Fluid {
   }
 };
 
+// SdfEditor over one volume: its TileGrid (counts, ids, list, per-mip counters)
+// and the device edit list, whose count lives on the device so that edits made
+// by kernels (erosion) and by the host share one flush.
+struct Editor {
+  Ctx* ctx = nullptr;
+  Volume* vol = nullptr;         // not owned
+  int max_edits = 64;
+  int capacity = 8;
+  double safety = 0.98;
+  std::vector<clay_brush> pending;
+  clay_brush* edits_dev = nullptr;
+  uint32_t* nedit_dev = nullptr;
+  uint32_t* counts = nullptr;    // every mip, at the volume's cell offsets
+  uint32_t* ids = nullptr;       // mip-0 tiles x capacity
+  uint32_t* list = nullptr;      // every mip, at the volume's cell offsets
+  uint32_t* args = nullptr;      // per mip: live count, 1, 1, 0
+};
+
 // ---- launchers (implemented in the .hip files) ------------------------------
+// edit.hip
+int editor_flush_device(Editor* e, hipStream_t s, bool all_tiles);
+int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* contacts, double min_speed,
+                             double radius, double blend);
+int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, const uint32_t* first_count,
+                          const float* const* pos, const float* const* rest, double* out);
 // fluid.hip
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key);

@@ -304,6 +304,57 @@ int clay_fluid_surface(clay_fluid* f, clay_splat** out);
 #define CLAY_DEBUG_PERM 1
 int clay_fluid_debug_download(clay_fluid* f, int32_t what, uint32_t* host);
 
+typedef struct clay_batch clay_batch;
+
+/* ---- SdfEditor (lib/src/field/modify.ts:41-238) ---------------------------
+ * Brush edits applied in place to a volume's mip-0 tiles they reach, then the
+ * dirty-tile mask walked down the mip chain (TileGrid, tilegrid.ts:43-316) and
+ * each coarser level rebuilt over its live tiles (MipRefiner, mips.ts:36-195).
+ * The edit list and every tile count stay on the device: a flush needs no host
+ * readback.  The editor binds the volume's origin at creation, as the
+ * reference's does (modify.ts:90-97). */
+typedef struct clay_editor clay_editor;
+typedef struct {
+  int32_t max_edits;      /* 64: edits per flush (modify.ts:73) */
+  int32_t tile_capacity;  /* 8: edits recorded per tile; later ones are dropped (modify.ts:78) */
+  double safety;          /* 0.98: chamfer safety factor of the band re-expansion (mips.ts:46) */
+} clay_editor_desc;
+int clay_editor_create(clay_volume* vol, const clay_editor_desc* desc, clay_editor** out);
+int clay_editor_destroy(clay_editor* e);
+/* SdfEditor.push (modify.ts:200-212): queues edits; *accepted says how many fit
+ * before max_edits (the rest are dropped, as the reference drops them). */
+int clay_editor_push(clay_editor* e, const clay_brush* edits, int32_t n, int32_t* accepted);
+int clay_editor_pending(const clay_editor* e, int32_t* n);
+/* SdfEditor.flush (modify.ts:214-238); *flushed = 0 when nothing was queued. */
+int clay_editor_flush(clay_editor* e, clay_stream stream, int32_t* flushed);
+/* Erosion: every contact of the fluid's last substep whose speed exceeds
+ * min_speed becomes a cut sphere (BrushSet.make {kind 'sphere', size radius, op
+ * 'cut', smooth blend}) appended after the queued edits, then the editor flushes
+ * -- all on the device (the policy game/fluid.ts:86-95 leaves to the caller). */
+int clay_editor_erode(clay_editor* e, clay_fluid* fluid, double min_speed, double radius, double blend,
+                      clay_stream stream);
+/* The same from one scene of a batch. */
+int clay_editor_erode_scene(clay_editor* e, clay_batch* batch, uint32_t scene, double min_speed, double radius,
+                            double blend, clay_stream stream);
+/* Rebuilds every coarser mip from mip 0 over all tiles (MipRefiner on a full grid). */
+int clay_editor_refresh_all(clay_editor* e, clay_stream stream);
+/* TileGrid.readTileCounts (tilegrid.ts:281-297): live tiles per mip of the last flush. */
+int clay_editor_tile_counts(clay_editor* e, uint32_t* per_mip);
+/* The tile counts of every mip (the volume's total_cells entries) after the last flush. */
+int clay_editor_cells_download(clay_editor* e, uint32_t* counts);
+
+/* ---- Body centroids (ClaySolver.match, lib/src/sim/pbd.ts:240-266) -------
+ * Per body [first, first + count) of a particle set: com = sum(pos) / count and
+ * restCom = sum(rest) / count, reduced by one 256-lane workgroup per body
+ * (lane-strided sums, then a pairwise tree).  F32_TREE is the reference's f32
+ * reduction; F64 is the same order in double, the form BodyTracker.sync
+ * (particles.ts:264-294) should read.  rest may be NULL (restCom = 0). */
+#define CLAY_REDUCE_F32_TREE 0
+#define CLAY_REDUCE_F64 1
+int clay_body_centroids(clay_ctx* ctx, const float* const pos_dev[3], const float* const rest_dev[3],
+                        const uint32_t* first_count, uint32_t nbodies, int32_t mode, double* out6,
+                        clay_stream stream);
+
 /* Launch shapes of the neighbour and splat kernels.  Results do not depend on
  * them; clay-engine_amd/clayhip/autotune.py picks them per device and workload. */
 typedef struct {
@@ -323,7 +374,6 @@ int clay_fluid_get_tuning(const clay_fluid* f, clay_tuning* out);
  * table, contact list and surface volume (follow_size wide, following the
  * scene's bounds).  A scene's results are bit-identical to a clay_fluid of the
  * same options holding only that scene's particles. */
-typedef struct clay_batch clay_batch;
 typedef struct {
   uint32_t scenes;          /* 1..256 */
   uint32_t pad;

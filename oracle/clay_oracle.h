@@ -22,6 +22,7 @@
  *   lib/src/field/tilegrid.ts packTile, compact, dilate
  *   lib/src/field/mips.ts     MipRefiner (eikonal band re-expansion)
  *   lib/src/field/modify.ts   SdfEditor bin/apply
+ *   lib/src/sim/pbd.ts        ClaySolver.match centroids (body centre of mass)
  *   lib/src/math/gpu.ts       hash1/hash3, smin/smax
  *
  * Parity status of each function is recorded in the comment above it in
@@ -156,6 +157,15 @@ float orc_dec_f32(uint32_t e);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 void orc_stable_sort_keys(int n, const uint32_t* keys, uint32_t* perm_out);
+/* TileGrid.dilate (tilegrid.ts:207-242): cells of mip m whose children in
+ * [2c-1, 2c+2]^3 at mip m-1 have a non-zero count get count 1.  `counts` holds
+ * every mip at the volume's cell offsets. */
+void orc_tile_dilate(int res, int mips, int m, uint32_t* counts);
+/* Drops contacts into erosion edits: every contact whose speed exceeds
+ * min_speed becomes a cut sphere of `radius` at the contact (the policy the
+ * reference leaves to the caller, game/fluid.ts:86-95).  Returns the count. */
+int orc_contacts_to_edits(uint32_t raw_count, const float* points4, double min_speed, double radius,
+                          double blend, int max_edits, orc_brush* out);
 
 #define ORC_DECLARE(T, S)                                                              \
   void orc_keys_##S(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz, \
@@ -179,7 +189,22 @@ void orc_stable_sort_keys(int n, const uint32_t* keys, uint32_t* perm_out);
   void orc_splat_resolve_##S(const orc_splat_opts* o, const uint32_t* grid,            \
                              uint16_t* texels);                                        \
   void orc_bake_brushes_##S(int res, int mips, double world_size, const double* origin,\
-                            double band, const orc_brush* b, int nb, uint16_t* texels);
+                            double band, const orc_brush* b, int nb, uint16_t* texels);   \
+  void orc_edit_bin_##S(int res, double world_size, const double* origin, double band, \
+                        int capacity, const orc_brush* e, int ne, uint32_t* counts0,   \
+                        uint32_t* ids0);                                               \
+  void orc_edit_apply_##S(int res, double world_size, const double* origin,            \
+                          double band, int capacity, const orc_brush* e,               \
+                          const uint32_t* counts0, const uint32_t* ids0,               \
+                          uint16_t* dist, uint16_t* mat);                              \
+  void orc_mip_refresh_##S(int res, int mips, int m, double band, double safety,       \
+                           const uint32_t* counts, uint16_t* dist, uint16_t* mat);     \
+  void orc_edit_flush_##S(int res, int mips, double world_size, const double* origin,  \
+                          double band, double safety, int capacity, const orc_brush* e,\
+                          int ne, uint16_t* dist, uint16_t* mat, uint32_t* counts);    \
+  void orc_body_centroids_##S(int nb, const uint32_t* first_count, const float* px,    \
+                              const float* py, const float* pz, const float* rx,       \
+                              const float* ry, const float* rz, double* out6);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

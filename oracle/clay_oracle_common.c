@@ -229,3 +229,60 @@ void orc_stable_sort_keys(int n, const uint32_t* keys, uint32_t* perm_out) {
   free(a);
   free(b);
 }
+
+/* TileGrid.dilate (lib/src/field/tilegrid.ts:207-242).  Parity: pinned against an
+ * independent numpy formulation (upsampled mask, shifted-window maximum) in
+ * tests/test_oracle_edit.py. */
+void orc_tile_dilate(int res, int mips, int m, uint32_t* counts) {
+  int tile_res[32], off[32], total = 0, nm = 0;
+  char err[8];
+  if (m <= 0 || orc_volume_derive(res, mips, &nm, tile_res, off, &total, err, (int)sizeof(err)) != 0 || m >= nm)
+    return;
+  const int tr = tile_res[m], ptr = tile_res[m - 1];
+  for (int i = 0; i < tr * tr * tr; i++) {
+    const int bx = (i % tr) * 2, by = ((i / tr) % tr) * 2, bz = (i / (tr * tr)) * 2;
+    int live = 0;
+    for (int dz = -1; dz <= 2 && !live; dz++)
+      for (int dy = -1; dy <= 2 && !live; dy++)
+        for (int dx = -1; dx <= 2 && !live; dx++) {
+          const int qx = bx + dx, qy = by + dy, qz = bz + dz;
+          if (qx < 0 || qy < 0 || qz < 0 || qx >= ptr || qy >= ptr || qz >= ptr) continue;
+          if (counts[off[m - 1] + qx + ptr * (qy + ptr * qz)] != 0) live = 1;
+        }
+    if (live) counts[off[m] + i] = 1;
+  }
+}
+
+/* Contacts -> erosion edits: the record BrushSet.make (brush.ts:539-564) builds
+ * for {kind: 'sphere', size: radius, pos, op: 'cut', smooth: blend} -- scale 1,
+ * identity rotation, bound = radius + blend, mask -1 -- for the first
+ * min(raw_count, 32) contacts whose speed exceeds min_speed, in list order.
+ * Pinned against clayhip.make_brush of the same description in
+ * tests/test_oracle_edit.py. */
+int orc_contacts_to_edits(uint32_t raw_count, const float* points4, double min_speed, double radius,
+                          double blend, int max_edits, orc_brush* out) {
+  const uint32_t n = raw_count < ORC_CONTACTS ? raw_count : ORC_CONTACTS;
+  int k = 0;
+  for (uint32_t i = 0; i < n && k < max_edits; i++) {
+    if (!(points4[4 * i + 3] > (float)min_speed)) continue;
+    orc_brush b;
+    memset(&b, 0, sizeof(b));
+    b.pos[0] = points4[4 * i];
+    b.pos[1] = points4[4 * i + 1];
+    b.pos[2] = points4[4 * i + 2];
+    b.scale = 1.0f;
+    b.rot[3] = 1.0f;
+    b.size[0] = (float)radius;
+    b.size[1] = (float)radius;
+    b.size[2] = (float)radius;
+    b.radius = 0.0f;
+    b.material = 0.0f;
+    b.blend = (float)blend;
+    b.bound = (float)(radius + blend);
+    b.mask = -1.0f;
+    b.kind = 0;
+    b.op = 1;
+    out[k++] = b;
+  }
+  return k;
+}

@@ -767,3 +767,215 @@ void FN(orc_bake_brushes)(int res, int mips, double world_size, const double* or
     off += (size_t)r * r * r;
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * SdfEditor (lib/src/field/modify.ts:98-238) over the TileGrid
+ * (lib/src/field/tilegrid.ts) and MipRefiner (lib/src/field/mips.ts).
+ * Volume planes hold f16 bits, every mip concatenated, x fastest.  Parity:
+ * bin, dilate and refresh are pinned against independent numpy formulations
+ * and a closed form in tests/test_oracle_edit.py; apply reuses the brush fold
+ * pinned by the reference's brush tests.  Choices the reference leaves to the
+ * WebGPU implementation: pack2x16float rounds to nearest even, and the mip
+ * downsample is the trilinear filter at the children's shared corner with
+ * weights of exactly one half.
+ * ------------------------------------------------------------------------- */
+static size_t plane_off(int res, int m) {
+  size_t off = 0;
+  for (int i = 0; i < m; i++) {
+    const size_t r = (size_t)(res >> i);
+    off += r * r * r;
+  }
+  return off;
+}
+
+/* bin (modify.ts:98-128): one pass per mip-0 tile over the edits in order;
+ * an edit reaches a tile when the distance from its origin to the tile box is
+ * below bound + bandWorld.  At most `capacity` ids are kept (writeCell drops the
+ * rest, setCellCount clamps). */
+void FN(orc_edit_bin)(int res, double world_size, const double* origin, double band, int capacity,
+                      const orc_brush* e, int ne, uint32_t* counts0, uint32_t* ids0) {
+  const int tr = res / 8;
+  const double voxel = world_size / res;
+  const T tsize = rnd(voxel * 8), thalf = rnd(voxel * 8 / 2), bw = rnd(band * voxel);
+  const v3 org = V(rnd(origin[0]), rnd(origin[1]), rnd(origin[2]));
+  for (int i = 0; i < tr * tr * tr; i++) {
+    const int tx = i % tr, ty = (i / tr) % tr, tz = i / (tr * tr);
+    const v3 tc = vadd(org, V((T)tx * tsize + thalf, (T)ty * tsize + thalf, (T)tz * tsize + thalf));
+    uint32_t n = 0;
+    for (int b = 0; b < ne; b++) {
+      const v3 bp = V(e[b].pos[0], e[b].pos[1], e[b].pos[2]);
+      const v3 nr = vmax0(V(t_fabs(bp.x - tc.x) - thalf, t_fabs(bp.y - tc.y) - thalf, t_fabs(bp.z - tc.z) - thalf));
+      if (vlen(nr) < (T)e[b].bound + bw) {
+        if ((int)n < capacity) ids0[(size_t)i * capacity + n] = (uint32_t)b;
+        n++;
+      }
+    }
+    counts0[i] = n < (uint32_t)capacity ? n : (uint32_t)capacity;
+  }
+}
+
+/* apply (modify.ts:130-168) fused with copy (:170-198): every voxel of a binned
+ * mip-0 tile folds the tile's edits into (raw.x * bandWorld, raw.y) at its centre
+ * origin + (coord + 0.5) * voxel and stores clamp(d / bandWorld, -1, 1), material. */
+void FN(orc_edit_apply)(int res, double world_size, const double* origin, double band, int capacity,
+                        const orc_brush* e, const uint32_t* counts0, const uint32_t* ids0, uint16_t* dist,
+                        uint16_t* mat) {
+  const int tr = res / 8;
+  const double voxel = world_size / res;
+  const T tv = rnd(voxel), bw = rnd(band * voxel);
+  const v3 org = V(rnd(origin[0]), rnd(origin[1]), rnd(origin[2]));
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < tr * tr * tr; i++) {
+    const uint32_t n = counts0[i];
+    if (n == 0) continue;
+    const int tx = i % tr, ty = (i / tr) % tr, tz = i / (tr * tr);
+    for (int lz = 0; lz < 8; lz++)
+      for (int ly = 0; ly < 8; ly++)
+        for (int lx = 0; lx < 8; lx++) {
+          const int x = tx * 8 + lx, y = ty * 8 + ly, z = tz * 8 + lz;
+          const v3 p = vadd(org, vmuls(V((T)x + (T)0.5, (T)y + (T)0.5, (T)z + (T)0.5), tv));
+          const size_t idx = (size_t)x + (size_t)res * ((size_t)y + (size_t)res * (size_t)z);
+          T acc[2] = {(T)orc_f16_to_f32(dist[idx]) * bw, (T)orc_f16_to_f32(mat[idx])};
+          for (uint32_t k = 0; k < n; k++) apply_brush(acc, &e[ids0[(size_t)i * capacity + k]], p);
+          dist[idx] = orc_f32_to_f16((float)t_clamp(acc[0] / bw, -1, 1));
+          mat[idx] = orc_f32_to_f16((float)acc[1]);
+        }
+  }
+}
+
+/* Trilinear sample of a mip plane at the corner shared by the 8 children of
+ * coarse voxel (cx, cy, cz): every weight is one half. */
+static T child_average(const uint16_t* plane, int fr, int cx, int cy, int cz) {
+  const T h = (T)0.5;
+  const int x0 = 2 * cx, y0 = 2 * cy, z0 = 2 * cz;
+#define TX(x, y, z) ((T)orc_f16_to_f32(plane[(size_t)(x) + (size_t)fr * ((size_t)(y) + (size_t)fr * (size_t)(z))]))
+  const T c00 = t_mix(TX(x0, y0, z0), TX(x0 + 1, y0, z0), h);
+  const T c10 = t_mix(TX(x0, y0 + 1, z0), TX(x0 + 1, y0 + 1, z0), h);
+  const T c01 = t_mix(TX(x0, y0, z0 + 1), TX(x0 + 1, y0, z0 + 1), h);
+  const T c11 = t_mix(TX(x0, y0 + 1, z0 + 1), TX(x0 + 1, y0 + 1, z0 + 1), h);
+#undef TX
+  return t_mix(t_mix(c00, c10, h), t_mix(c01, c11, h), h);
+}
+
+/* MipRefiner.refresh (mips.ts:78-177) over the tiles of mip m with a non-zero
+ * count: 12^3 gather of the 2x2x2 downsample (times one half; coordinates
+ * clamped to the level), two eikonal sweeps on the 26-neighbour chamfer stencil
+ * with weights |d| * (1 / band) * safety, shrinking by one shell per sweep, and
+ * the tile core stored clamped to +-1 with the downsampled material. */
+void FN(orc_mip_refresh)(int res, int mips, int m, double band, double safety, const uint32_t* counts,
+                         uint16_t* dist, uint16_t* mat) {
+  if (m <= 0 || m >= mips) return;
+  int tile_res[32], coff[32], total = 0, nm = 0;
+  char err[8];
+  if (orc_volume_derive(res, mips, &nm, tile_res, coff, &total, err, (int)sizeof(err)) != 0) return;
+  const int mr = res >> m, fr = res >> (m - 1), tr = tile_res[m];
+  const size_t fo = plane_off(res, m - 1), co = plane_off(res, m);
+  const T step = rnd((1.0 / band) * safety);
+  T wts[27];
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++)
+        wts[(dx + 1) + 3 * ((dy + 1) + 3 * (dz + 1))] = vlen(V((T)dx, (T)dy, (T)dz)) * step;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < tr * tr * tr; i++) {
+    if (counts[coff[m] + i] == 0) continue;
+    const int E = 12, B = 2;
+    T g[2][12 * 12 * 12];
+    const int tx = (i % tr) * 8, ty = ((i / tr) % tr) * 8, tz = (i / (tr * tr)) * 8;
+    for (int c = 0; c < E * E * E; c++) {
+      int gx = tx + c % E - B, gy = ty + (c / E) % E - B, gz = tz + c / (E * E) - B;
+      gx = gx < 0 ? 0 : (gx > mr - 1 ? mr - 1 : gx);
+      gy = gy < 0 ? 0 : (gy > mr - 1 ? mr - 1 : gy);
+      gz = gz < 0 ? 0 : (gz > mr - 1 ? mr - 1 : gz);
+      g[0][c] = child_average(dist + fo, fr, gx, gy, gz) * (T)0.5;
+    }
+    for (int k = 0; k < B; k++) {
+      const T* src = g[k % 2];
+      T* dst = g[(k + 1) % 2];
+      const int margin = k + 1;
+      for (int c = 0; c < E * E * E; c++) {
+        const int x = c % E, y = (c / E) % E, z = c / (E * E);
+        T v = src[c];
+        if (x >= margin && y >= margin && z >= margin && x < E - margin && y < E - margin && z < E - margin) {
+          T lo = (T)-1e9, hi = (T)1e9;
+          for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+              for (int dx = -1; dx <= 1; dx++) {
+                if (dx == 0 && dy == 0 && dz == 0) continue;
+                const T w = wts[(dx + 1) + 3 * ((dy + 1) + 3 * (dz + 1))];
+                const T nv = src[(x + dx) + E * ((y + dy) + E * (z + dz))];
+                hi = t_min(hi, nv + w);
+                lo = t_max(lo, nv - w);
+              }
+          v = t_clamp(v, lo, hi);
+        }
+        dst[c] = v;
+      }
+    }
+    const T* out = g[B % 2];
+    for (int lz = 0; lz < 8; lz++)
+      for (int ly = 0; ly < 8; ly++)
+        for (int lx = 0; lx < 8; lx++) {
+          const int x = tx + lx, y = ty + ly, z = tz + lz;
+          const size_t idx = co + (size_t)x + (size_t)mr * ((size_t)y + (size_t)mr * (size_t)z);
+          dist[idx] = orc_f32_to_f16((float)t_clamp(out[(lx + B) + E * ((ly + B) + E * (lz + B))], -1, 1));
+          mat[idx] = orc_f32_to_f16((float)child_average(mat + fo, fr, x, y, z));
+        }
+  }
+}
+
+/* SdfEditor.flush (modify.ts:214-238): reset, bin, apply, then per coarser mip
+ * dilate and refresh.  `counts` (the volume's total cell count) returns the final
+ * tile counts of every mip. */
+void FN(orc_edit_flush)(int res, int mips, double world_size, const double* origin, double band, double safety,
+                        int capacity, const orc_brush* e, int ne, uint16_t* dist, uint16_t* mat,
+                        uint32_t* counts) {
+  int tile_res[32], coff[32], total = 0, nm = 0;
+  char err[8];
+  if (ne <= 0 || orc_volume_derive(res, mips, &nm, tile_res, coff, &total, err, (int)sizeof(err)) != 0) return;
+  memset(counts, 0, sizeof(uint32_t) * (size_t)total);
+  const int t0 = tile_res[0] * tile_res[0] * tile_res[0];
+  uint32_t* ids = (uint32_t*)calloc((size_t)t0 * (size_t)capacity, sizeof(uint32_t));
+  FN(orc_edit_bin)(res, world_size, origin, band, capacity, e, ne, counts, ids);
+  FN(orc_edit_apply)(res, world_size, origin, band, capacity, e, counts, ids, dist, mat);
+  free(ids);
+  for (int m = 1; m < mips; m++) {
+    orc_tile_dilate(res, mips, m, counts);
+    FN(orc_mip_refresh)(res, mips, m, band, safety, counts, dist, mat);
+  }
+}
+
+/* ClaySolver.match phase 1 (pbd.ts:240-266): per body, 256 lanes each sum the
+ * body's particles t = lane, lane + 256, ... in order, then a pairwise tree
+ * (s = 128 .. 1) adds lane + s into lane; centroid = sum * (1 / max(count, 1)).
+ * In f32 this is the reference's reduction; in f64 it is the FP64 form the
+ * library reports.  out6 per body: com xyz, restCom xyz (rest may be NULL). */
+void FN(orc_body_centroids)(int nb, const uint32_t* first_count, const float* px, const float* py,
+                            const float* pz, const float* rx, const float* ry, const float* rz, double* out6) {
+  enum { R = 256 };
+  for (int b = 0; b < nb; b++) {
+    const uint32_t first = first_count[2 * b], count = first_count[2 * b + 1];
+    T s[R][6];
+    for (int li = 0; li < R; li++) {
+      T a[6] = {0, 0, 0, 0, 0, 0};
+      for (uint32_t t = (uint32_t)li; t < count; t += R) {
+        const uint32_t j = first + t;
+        a[0] = a[0] + (T)px[j];
+        a[1] = a[1] + (T)py[j];
+        a[2] = a[2] + (T)pz[j];
+        if (rx) {
+          a[3] = a[3] + (T)rx[j];
+          a[4] = a[4] + (T)ry[j];
+          a[5] = a[5] + (T)rz[j];
+        }
+      }
+      for (int q = 0; q < 6; q++) s[li][q] = a[q];
+    }
+    for (int st = R / 2; st > 0; st >>= 1)
+      for (int li = 0; li < st; li++)
+        for (int q = 0; q < 6; q++) s[li][q] = s[li][q] + s[li + st][q];
+    const T fc = (T)count;
+    const T inv = (T)1 / t_max(fc, (T)1);
+    for (int q = 0; q < 6; q++) out6[6 * b + q] = (double)(s[0][q] * inv);
+  }
+}

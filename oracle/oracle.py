@@ -128,6 +128,15 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_splat_bake_{s}").argtypes = [SO, I, P, P, P, P, P, P, I, I, P]
             getattr(L, f"orc_splat_resolve_{s}").argtypes = [SO, P, P]
             getattr(L, f"orc_bake_brushes_{s}").argtypes = [I, I, C.c_double, P, C.c_double, P, I, P]
+            D = C.c_double
+            getattr(L, f"orc_edit_bin_{s}").argtypes = [I, D, P, D, I, P, I, P, P]
+            getattr(L, f"orc_edit_apply_{s}").argtypes = [I, D, P, D, I, P, P, P, P, P]
+            getattr(L, f"orc_mip_refresh_{s}").argtypes = [I, I, I, D, D, P, P, P]
+            getattr(L, f"orc_edit_flush_{s}").argtypes = [I, I, D, P, D, D, I, P, I, P, P, P]
+            getattr(L, f"orc_body_centroids_{s}").argtypes = [I, P, P, P, P, P, P, P, P]
+        L.orc_tile_dilate.argtypes = [I, I, I, P]
+        L.orc_contacts_to_edits.argtypes = [C.c_uint32, P, C.c_double, C.c_double, C.c_double, I, P]
+        L.orc_contacts_to_edits.restype = C.c_int
         _lib = L
     return _lib
 
@@ -440,4 +449,103 @@ def keys(options: dict, pos: np.ndarray) -> np.ndarray:
     out = np.empty(p.shape[1], dtype=np.uint32)
     lib().orc_keys_f32(C.byref(k), p.shape[1], p[0].ctypes.data, p[1].ctypes.data, p[2].ctypes.data,
                        out.ctypes.data)
+    return out
+
+
+# ---- SdfEditor / TileGrid / MipRefiner / body centroids ----------------------
+def _brush_array(records):
+    arr = (OrcBrush * max(1, len(records)))()
+    for i, b in enumerate(records):
+        C.memmove(C.byref(arr[i]), C.byref(b), C.sizeof(OrcBrush))
+    return arr
+
+
+class Planes:
+    """A volume's distance and material planes (f16 bits, mips concatenated, x
+    fastest), as the HIP library stores them."""
+
+    def __init__(self, res: int, mips: int, texels: list[np.ndarray] | None = None):
+        self.res, self.mips = res, mips
+        total = sum(grid_sizes(res, mips))
+        self.dist = np.empty(total, dtype=np.uint16)
+        self.mat = np.empty(total, dtype=np.uint16)
+        if texels is not None:
+            off = 0
+            for m in range(mips):
+                t = np.asarray(texels[m]).reshape(-1, 4)
+                self.dist[off:off + t.shape[0]] = t[:, 0]
+                self.mat[off:off + t.shape[0]] = t[:, 1]
+                off += t.shape[0]
+
+    def level(self, m: int, plane: str = "dist") -> np.ndarray:
+        off = sum(grid_sizes(self.res, m))
+        r = self.res >> m
+        return getattr(self, plane)[off:off + r ** 3].reshape(r, r, r)
+
+    def texels(self, m: int) -> np.ndarray:
+        """(r, r, r, 4) f16 bits as clay_volume_download returns them."""
+        r = self.res >> m
+        out = np.zeros((r, r, r, 4), dtype=np.uint16)
+        out[..., 0] = self.level(m, "dist")
+        out[..., 1] = self.level(m, "mat")
+        out[..., 3] = f32_to_f16(1.0)
+        return out
+
+
+def total_cells(res: int, mips: int) -> int:
+    return volume_derive(res, mips)[3]
+
+
+def edit_bin(res: int, world_size: float, origin, band: float, capacity: int, edits, precision: str = "f32"):
+    tr = res // 8
+    counts = np.zeros(tr ** 3, dtype=np.uint32)
+    ids = np.zeros(tr ** 3 * capacity, dtype=np.uint32)
+    getattr(lib(), f"orc_edit_bin_{precision}")(res, C.c_double(world_size), (C.c_double * 3)(*origin),
+                                                C.c_double(band), capacity, _brush_array(edits), len(edits),
+                                                counts.ctypes.data, ids.ctypes.data)
+    return counts, ids.reshape(tr ** 3, capacity)
+
+
+def tile_dilate(res: int, mips: int, m: int, counts: np.ndarray) -> None:
+    assert counts.dtype == np.uint32 and counts.flags.c_contiguous
+    lib().orc_tile_dilate(res, mips, m, counts.ctypes.data)
+
+
+def mip_refresh(planes: Planes, m: int, band: float, safety: float, counts: np.ndarray,
+                precision: str = "f32") -> None:
+    getattr(lib(), f"orc_mip_refresh_{precision}")(planes.res, planes.mips, m, C.c_double(band),
+                                                   C.c_double(safety), counts.ctypes.data,
+                                                   planes.dist.ctypes.data, planes.mat.ctypes.data)
+
+
+def edit_flush(planes: Planes, world_size: float, origin, band: float, edits, safety: float = 0.98,
+               capacity: int = 8, precision: str = "f32") -> np.ndarray:
+    counts = np.zeros(total_cells(planes.res, planes.mips), dtype=np.uint32)
+    getattr(lib(), f"orc_edit_flush_{precision}")(planes.res, planes.mips, C.c_double(world_size),
+                                                  (C.c_double * 3)(*origin), C.c_double(band), C.c_double(safety),
+                                                  capacity, _brush_array(edits), len(edits),
+                                                  planes.dist.ctypes.data, planes.mat.ctypes.data,
+                                                  counts.ctypes.data)
+    return counts
+
+
+def contacts_to_edits(raw_count: int, points: np.ndarray, min_speed: float, radius: float, blend: float,
+                      max_edits: int = 64) -> list:
+    pts = np.zeros((CONTACTS, 4), dtype=np.float32)
+    pts[:min(len(points), CONTACTS)] = np.asarray(points, dtype=np.float32)[:CONTACTS]
+    arr = (OrcBrush * max_edits)()
+    k = lib().orc_contacts_to_edits(raw_count, pts.ctypes.data, C.c_double(min_speed), C.c_double(radius),
+                                    C.c_double(blend), max_edits, arr)
+    return [arr[i] for i in range(k)]
+
+
+def body_centroids(first_count, pos: np.ndarray, rest: np.ndarray | None = None,
+                   precision: str = "f32") -> np.ndarray:
+    fc = np.ascontiguousarray(np.asarray(first_count, dtype=np.uint32).reshape(-1, 2))
+    p = np.ascontiguousarray(np.asarray(pos, dtype=np.float32).T)
+    r = None if rest is None else np.ascontiguousarray(np.asarray(rest, dtype=np.float32).T)
+    out = np.zeros((fc.shape[0], 6), dtype=np.float64)
+    rp = [None, None, None] if r is None else [r[0].ctypes.data, r[1].ctypes.data, r[2].ctypes.data]
+    getattr(lib(), f"orc_body_centroids_{precision}")(fc.shape[0], fc.ctypes.data, p[0].ctypes.data,
+                                                      p[1].ctypes.data, p[2].ctypes.data, *rp, out.ctypes.data)
     return out

@@ -1,0 +1,142 @@
+"""SdfEditor / TileGrid / MipRefiner, erosion from contacts, and body centroids
+on the GPU against the oracle, bit for bit."""
+import numpy as np
+import pytest
+
+import clayhip
+import oracle as O
+from clayhip import make_brush, scenes
+
+pytestmark = pytest.mark.gpu
+
+RES, MIPS, WORLD, ORIGIN, BAND = 64, 3, 8.0, (-4.0, -2.0, -4.0), 4.0
+
+
+def world_brushes():
+    return [{"kind": "roundBox", "size": (3, 0.5, 3), "radius": 0.25, "pos": (0, -1.0, 0), "material": 0.3},
+            {"kind": "sphere", "size": 1.1, "pos": (0.7, 0.2, -0.4), "material": 0.8}]
+
+
+def baked(root):
+    vol = clayhip.SdfVolume(root, {"resolution": RES, "mipLevels": MIPS, "worldSize": WORLD, "origin": ORIGIN,
+                                   "band": BAND})
+    vol.bakeBrushes(world_brushes())
+    planes = O.Planes(RES, MIPS, O.bake_brushes(RES, MIPS, WORLD, ORIGIN, BAND,
+                                                [make_brush(b) for b in world_brushes()]))
+    for m in range(MIPS):
+        assert np.array_equal(vol.download(m), planes.texels(m)), f"baked mip {m}"
+    return vol, planes
+
+
+def assert_volume_equal(vol, planes, what):
+    for m in range(MIPS):
+        got, want = vol.download(m), planes.texels(m)
+        bad = np.argwhere(got != want)
+        assert bad.size == 0, f"{what}: mip {m}, {len(bad)} texel words differ, first {bad[:3].tolist()}"
+
+
+def edits_mix(rng):
+    out = []
+    for i in range(20):
+        op = ("add", "cut", "paint")[i % 3]
+        kind = ("sphere", "box", "capsule")[i % 3]
+        out.append({"kind": kind, "size": float(rng.uniform(0.1, 0.5)) if kind == "sphere" else
+                    tuple(float(v) for v in rng.uniform(0.1, 0.4, 3)), "radius": 0.05,
+                    "pos": tuple(float(v) for v in rng.uniform(-2.5, 2.5, 3)), "op": op,
+                    "material": float(rng.uniform()), "smooth": 0.05})
+    # a cluster that overflows the 8-edit tile capacity
+    out += [{"kind": "sphere", "size": 0.2, "pos": (0.5 + 0.02 * i, -0.6, 0.4), "op": "cut"} for i in range(12)]
+    return out
+
+
+def test_flush_matches_oracle(root):
+    vol, planes = baked(root)
+    ed = clayhip.SdfEditor(vol)
+    rng = np.random.default_rng(1)
+    edits = edits_mix(rng)
+    for e in edits:
+        assert ed.push(e)
+    assert ed.pendingCount == len(edits)
+    assert ed.flush()
+    assert ed.pendingCount == 0
+    counts = O.edit_flush(planes, WORLD, ORIGIN, BAND, [make_brush(e) for e in edits])
+    assert np.array_equal(ed.cellCounts(), counts)
+    _, tile_res, offs, _ = O.volume_derive(RES, MIPS)
+    live = [int(np.count_nonzero(counts[offs[m]:offs[m] + tile_res[m] ** 3])) for m in range(MIPS)]
+    assert ed.readTileCounts() == live and live[0] > 0
+    assert_volume_equal(vol, planes, "after flush")
+    assert not ed.flush()  # nothing queued
+    ed.destroy()
+    vol.destroy()
+
+
+def test_push_drops_beyond_max_edits(root):
+    vol, _ = baked(root)
+    ed = clayhip.SdfEditor(vol, {"maxEdits": 4})
+    accepted = [ed.push({"kind": "sphere", "size": 0.2, "pos": (0, 0, 0), "op": "cut"}) for _ in range(6)]
+    assert accepted == [True] * 4 + [False] * 2 and ed.pendingCount == 4
+    with pytest.raises(clayhip.ClayError):
+        ed.push(clayhip._lib.Brush(kind=9))
+    ed.destroy()
+    vol.destroy()
+
+
+def test_refresh_all_matches_oracle(root):
+    vol, planes = baked(root)
+    ed = clayhip.SdfEditor(vol)
+    ed.refreshAll()
+    counts = np.ones(O.total_cells(RES, MIPS), dtype=np.uint32)
+    for m in range(1, MIPS):
+        O.mip_refresh(planes, m, BAND, 0.98, counts)
+    assert_volume_equal(vol, planes, "refresh all")
+    ed.destroy()
+    vol.destroy()
+
+
+def test_erosion_from_fluid_contacts_matches_oracle(root):
+    sc = scenes.dam_break_small()
+    floor = clayhip.analyticField(root, sc["collider"]["brushes"])
+    sim = clayhip.FluidSim(root, floor, sc["options"])
+    rng = np.random.default_rng(4)
+    vel = np.zeros_like(sc["pos"])
+    vel[:, 1] = -6.0 + rng.normal(scale=0.5, size=len(vel)).astype(np.float32)
+    # contacts report the speed left after the normal component is removed: slide sideways
+    vel[:, 0] = rng.uniform(0.0, 2.0, size=len(vel)).astype(np.float32)
+    sim.set_state(sc["pos"] - np.float32([0, 0.3, 0]), vel)
+    sim.step()
+    pts = sim.readContacts()
+    raw = sim.lastContactCount
+    assert raw > 0 and any(p[3] > 0.5 for p in pts) and any(p[3] <= 0.5 for p in pts)
+    vol, planes = baked(root)
+    ed = clayhip.SdfEditor(vol)
+    ed.push({"kind": "sphere", "size": 0.3, "pos": (1.0, -0.6, 1.0), "op": "cut"})
+    ed.erode(sim, min_speed=0.5, radius=0.15, blend=0.03)
+    host_edit = [make_brush({"kind": "sphere", "size": 0.3, "pos": (1.0, -0.6, 1.0), "op": "cut"})]
+    dev_edits = O.contacts_to_edits(raw, np.array(pts, dtype=np.float32).reshape(-1, 4), 0.5, 0.15, 0.03,
+                                    max_edits=63)
+    assert len(dev_edits) > 0
+    counts = O.edit_flush(planes, WORLD, ORIGIN, BAND, host_edit + dev_edits)
+    assert np.array_equal(ed.cellCounts(), counts)
+    assert_volume_equal(vol, planes, "after erosion")
+    ed.destroy()
+    vol.destroy()
+    sim.destroy()
+    floor.destroy()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_body_centroids_match_oracle(root, precision):
+    import torch
+
+    rng = np.random.default_rng(6)
+    n = 12000
+    pos = (rng.normal(size=(n, 3)) * 0.95 + [1.0, 2.0, -0.5]).astype(np.float32)
+    rest = rng.normal(size=(n, 3)).astype(np.float32)
+    bodies = [(0, 4096), (4096, 4096), (8192, 1000), (9192, 2808), (12000, 0)]
+    dev_p = [torch.from_numpy(np.ascontiguousarray(pos[:, a])).cuda() for a in range(3)]
+    dev_r = [torch.from_numpy(np.ascontiguousarray(rest[:, a])).cuda() for a in range(3)]
+    got = clayhip.body_centroids(root, dev_p, bodies, rest=dev_r, precision=precision)
+    want = O.body_centroids(bodies, pos, rest, precision=precision)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    got_np = clayhip.body_centroids(root, dev_p, bodies[:2], precision=precision)
+    assert np.array_equal(got_np[:, :3], want[:2, :3]) and np.all(got_np[:, 3:] == 0)
