@@ -149,9 +149,24 @@ def build_single(root, name: str, splat: str, precision: str = "f32"):
     return scene, sim, world, keep
 
 
-def measure_single(root, name: str, splat: str, precision: str, steps: int, warmup: int, stream) -> dict:
+def measure_single(root, name: str, splat: str, precision: str, steps: int, warmup: int, stream,
+                   autotune_now: bool = False) -> dict:
+    from clayhip import autotune
+
     scene, sim, world, keep = build_single(root, name, splat, precision)
     n = scene["pos"].shape[0]
+    tune_src = "default"
+    if splat == "tiled":
+        key = autotune.cache_key(name, precision)
+        if autotune_now:
+            autotune.autotune(sim, key, frames=3, state=lambda: sim.set_state(scene["pos"]))
+            tune_src = "autotuned in this run"
+        else:
+            cached = autotune.load(autotune.DEFAULT_CACHE, key)
+            if cached:
+                sim.tuning = cached
+                tune_src = "autotune cache"
+        sim.set_state(scene["pos"])
     mips = scene["options"]["bakeMipLevels"]
     ms = timed(sim.step, steps, warmup, stream)
     ks = min(steps, 5)
@@ -162,6 +177,7 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
         "substeps": scene["options"]["substeps"], "mips": mips, "live_voxels": live,
         "kernels_ms_per_step": per_step(stats, ks),
         "roofline": roofline_of(stats, n, mips, live, name) if splat == "tiled" else None,
+        "tuning": dict(sim.tuning, source=tune_src),
         "scene": scene,
     }
     sim.destroy()
@@ -314,6 +330,7 @@ def measure_batched(root, args, stream) -> dict:
     if args.precision != "f32":
         pos, vel = b.get_state("pos"), b.get_state("vel")
         ref = clayhip.FluidBatch(root, world, dict(opts, precision="f32"), scene["counts"])
+        ref.tuning = b.tuning
         ref.set_graph(False)
         for x in (b, ref):
             x.set_state(pos, vel)
@@ -348,7 +365,7 @@ def main() -> int:
     ap.add_argument("--precision", default=None, choices=["f32", "fp16lds"],
                     help="fluid kernels (default: fp16lds for the batch, f32 for a single scene)")
     ap.add_argument("--no-graph", action="store_true", help="batch: eager frames instead of a captured graph")
-    ap.add_argument("--autotune", action="store_true", help="batch: re-tune launch shapes (else the cache)")
+    ap.add_argument("--autotune", action="store_true", help="re-tune launch shapes (else the cache)")
     ap.add_argument("--no-compare", action="store_true", help="batch: skip the plain comparisons")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle work for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -394,7 +411,8 @@ def main() -> int:
                 "batched_over_plain_single_1m": r["value"] / single["value"],
             }
     else:
-        r = measure_single(root, args.workload, args.splat, args.precision, args.steps, args.warmup, stream)
+        r = measure_single(root, args.workload, args.splat, args.precision, args.steps, args.warmup, stream,
+                           autotune_now=args.autotune)
     wall = time.perf_counter() - t_wall
     scene = r["scene"]
     line = {
@@ -429,6 +447,7 @@ def main() -> int:
         line["comparisons"] = comparisons
     else:
         line["config"]["splat"] = args.splat
+        line["config"]["tuning"] = r["tuning"]
     line["wall_s"] = round(wall, 3)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

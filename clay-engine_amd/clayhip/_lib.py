@@ -148,6 +148,8 @@ class Tuning(C.Structure):
         ("stage", C.c_int32),
         ("chunk", C.c_int32),
         ("groups_per_cu", C.c_int32),
+        ("fluid_xcd", C.c_int32),
+        ("splat_xcd", C.c_int32),
     ]
 
 

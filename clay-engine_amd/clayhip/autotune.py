@@ -4,7 +4,7 @@ Tries each candidate clay_tuning on the live workload -- a few captured frames
 per candidate, timed with HIP events on one stream -- keeps the fastest, and
 caches the choice in a JSON file keyed by device and workload.  The knobs only
 change launch shapes (workgroup size, LDS staging slice, splat work-item size,
-persistent grid), never results; the cell size is not a knob because it would
+persistent grid, XCD-contiguous workgroup numbering), never results; the cell size is not a knob because it would
 change every neighbour's summation order.
 """
 from __future__ import annotations
@@ -21,6 +21,8 @@ SPACE = {
     "stage": (64, 128, 256),
     "chunk": (1024, 2048, 4096),
     "groups_per_cu": (4, 8, 16),
+    "fluid_xcd": (0, 1),
+    "splat_xcd": (0, 1),
 }
 
 
@@ -30,9 +32,13 @@ def candidates(space: dict = SPACE) -> list[dict]:
 
 
 def _device_name() -> str:
+    """Architecture and compute-unit count: stable across drivers and tools,
+    unlike the marketing name."""
     import torch
 
-    return torch.cuda.get_device_name(0)
+    props = torch.cuda.get_device_properties(0)
+    arch = getattr(props, "gcnArchName", "") or torch.cuda.get_device_name(0)
+    return f"{arch.split(':')[0]}x{props.multi_processor_count}"
 
 
 def time_frames(obj, frames: int, stream) -> float:

@@ -1244,7 +1244,11 @@ int apply_tuning(Fluid* f, const clay_tuning* t) {
   if (t->chunk < 256 || t->chunk > 65536) return fail(CLAY_ERR_INVALID, "clay_tuning: chunk must be in [256, 65536]");
   if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
     return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
+  if ((t->fluid_xcd != 0 && t->fluid_xcd != 1) || (t->splat_xcd != 0 && t->splat_xcd != 1))
+    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd and splat_xcd must be 0 or 1");
+  ft.xcd = t->fluid_xcd != 0;
   f->tune = ft;
+  f->surface->xcd = t->splat_xcd != 0;
   f->surface->chunk = (uint32_t)t->chunk;
   f->surface->persistent_blocks = f->ctx->cus * t->groups_per_cu;
   if (f->graph) {
@@ -1259,6 +1263,8 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
   out->stage = f->tune.stage;
   out->chunk = (int32_t)f->surface->chunk;
   out->groups_per_cu = f->surface->persistent_blocks / std::max(1, f->ctx->cus);
+  out->fluid_xcd = f->tune.xcd ? 1 : 0;
+  out->splat_xcd = f->surface->xcd ? 1 : 0;
 }
 
 }  // namespace

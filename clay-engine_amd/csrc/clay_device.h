@@ -69,8 +69,19 @@ struct SceneK {
   uint32_t batched;
   uint32_t cells_per_scene;
   uint32_t contact_stride;
-  uint32_t pad;
+  uint32_t xcd;  // renumber workgroups XCD-contiguously (a launch-shape choice)
 };
+
+// Workgroups are dispatched round-robin over the chip's 8 XCDs (workgroup b runs
+// on XCD b % 8), each with its own L2.  Renumbering them so that every XCD gets a
+// contiguous run of logical workgroups keeps neighbouring particles (and a splat
+// brick's chunks) in one L2.  A bijection on [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+  const uint32_t per = n / kXcds, rem = n % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
 
 // FluidSim constants as f32 shader literals, derived in f64 on the host.
 struct FluidK {

@@ -127,12 +127,14 @@ struct Splat {
   // tuning
   uint32_t chunk = 2048;                 // particles per splat work item
   int persistent_blocks = 2048;          // grid of the splat work-item kernel
+  bool xcd = false;                      // XCD-contiguous work-item numbering
 };
 
 // Tunable launch shapes of the neighbour kernels (results do not depend on them).
 struct FluidTune {
   int block = 256;   // threads per workgroup: 128 or 256
   int stage = 128;   // FP16-LDS staging slice per wave: 64, 128 or 256 neighbours
+  bool xcd = false;  // XCD-contiguous workgroup numbering in the neighbour kernels
 };
 bool fluid_tune_valid(const FluidTune& t);
 
@@ -181,6 +183,7 @@ struct Fluid {
   hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
   SceneK scene_k() const {
     SceneK sk{};
+    sk.xcd = tune.xcd ? 1u : 0u;
     sk.epoch = epoch_dev;
     sk.batched = scene_of != nullptr;
     sk.cells_per_scene = (uint32_t)grid_cells;

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(kB)
     k_density(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
               const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
               const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
-  const uint32_t s = blockIdx.x * kB + threadIdx.x;
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   if (s >= n) return;
   const uint32_t epoch = *sk.epoch;
   const uint4* cs = scene_cells(sk, cells, scene_at(sk, skeys, s));
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(kB)
     k_forces_integrate(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
                        const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
                        const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
-  const uint32_t s = blockIdx.x * kB + threadIdx.x;
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   if (s >= n) return;
   const float* __restrict__ spx = in.p[0];
   const float* __restrict__ spy = in.p[1];
@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(kB)
                   const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
 #pragma clang fp contract(fast)
   __shared__ uint2 stage[kB / 64][kS];
-  const uint32_t s = blockIdx.x * kB + threadIdx.x;
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   const bool active = s < n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t epoch = *sk.epoch;
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kB)
                            const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
 #pragma clang fp contract(fast)
   __shared__ StagedNbr stage[kB / 64][kS];
-  const uint32_t s = blockIdx.x * kB + threadIdx.x;
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   const bool active = s < n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float* __restrict__ spx = in.p[0];

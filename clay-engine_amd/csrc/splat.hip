@@ -80,6 +80,7 @@ struct MipsK {
   uint32_t gps;           // code-grid cells per scene
   uint32_t vps;           // texels per scene (all mips)
   uint32_t tps;           // 8^3 tiles per scene (all mips)
+  uint32_t xcd;           // XCD-contiguous work-item numbering
   SplatMipK m[CLAY_MAX_MIPS];
 };
 
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(kThreads)
   // ds_min_u32 rarely meet on one address.
   const uint32_t rev = __brev((uint32_t)t) >> 24;
   const uint32_t items = scan->items;
-  for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+  for (uint32_t item = mk_.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x; item < items; item += gridDim.x) {
     if (t == 0) {
       // Last brick whose first item is <= item (a brick with no items shares its
       // first item with the next brick, so the last one always has items).
@@ -580,6 +581,7 @@ static void fill_mips(Splat* sp, MipsK* mk_) {
   mk_->gps = (uint32_t)sp->grid_total;
   mk_->vps = (uint32_t)v->total_voxels;
   mk_->tps = (uint32_t)sp->tile_total;
+  mk_->xcd = sp->xcd ? 1u : 0u;
   for (int m = 0; m < sp->mips; m++) {
     SplatMipK& k = mk_->m[m];
     k.r = v->mip_res(m);

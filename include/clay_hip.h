@@ -362,6 +362,8 @@ typedef struct {
   int32_t stage;          /* FP16-LDS staging slice per wave: 64, 128 or 256 neighbours */
   int32_t chunk;          /* splat work item: 256..65536 particles of one brick */
   int32_t groups_per_cu;  /* persistent splat grid: 1..32 workgroups per compute unit */
+  int32_t fluid_xcd;      /* 1: neighbour kernels number workgroups XCD-contiguously (one L2 per run) */
+  int32_t splat_xcd;      /* 1: the same for splat work items */
 } clay_tuning;
 int clay_fluid_set_tuning(clay_fluid* f, const clay_tuning* t);
 int clay_fluid_get_tuning(const clay_fluid* f, clay_tuning* out);
