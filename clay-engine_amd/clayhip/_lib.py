@@ -150,6 +150,7 @@ class Tuning(C.Structure):
         ("groups_per_cu", C.c_int32),
         ("fluid_xcd", C.c_int32),
         ("splat_xcd", C.c_int32),
+        ("splat_owner", C.c_int32),
     ]
 
 
@@ -275,6 +276,7 @@ SIGNATURES = {
     "clay_fluid_surface": ([P, PP], C.c_int),
     "clay_fluid_debug_download": ([P, I32, P], C.c_int),
     "clay_fluid_set_tuning": ([P, C.POINTER(Tuning)], C.c_int),
+    "clay_splat_set_tuning": ([P, C.POINTER(Tuning)], C.c_int),
     "clay_fluid_get_tuning": ([P, C.POINTER(Tuning)], C.c_int),
     "clay_batch_create": ([P, P, C.POINTER(BatchDesc), PP], C.c_int),
     "clay_batch_destroy": ([P], C.c_int),

@@ -23,6 +23,7 @@ SPACE = {
     "groups_per_cu": (4, 8, 16),
     "fluid_xcd": (0, 1),
     "splat_xcd": (0, 1),
+    "splat_owner": (0, 1),
 }
 
 

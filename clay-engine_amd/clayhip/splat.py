@@ -108,6 +108,12 @@ class SplatField:
         L.check(L.lib().clay_splat_set_center(self._h, L.dvec(center), L.stream_handle(stream)))
         self.volume._load_info()
 
+    def setSchedule(self, chunk: int = 1024, groups_per_cu: int = 8, xcd: bool = False, owner: bool = True) -> None:
+        """Launch shape of the tiled bake (results do not depend on it)."""
+        t = L.Tuning(block=256, stage=128, chunk=int(chunk), groups_per_cu=int(groups_per_cu), fluid_xcd=0,
+                     splat_xcd=1 if xcd else 0, splat_owner=1 if owner else 0)
+        L.check(L.lib().clay_splat_set_tuning(self._h, C.byref(t)))
+
     def codes(self, mip: int = 0) -> np.ndarray:
         """The u32 code grid of one mip, shape (r, r, r) indexed [z, y, x]."""
         r = self.volume.resolution >> mip

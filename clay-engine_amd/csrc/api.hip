@@ -718,6 +718,21 @@ extern "C" int clay_splat_set_center(clay_splat* s, const double c[3], clay_stre
   return volume_set_origin_impl(sp->vol, o, pick(sp->ctx, stream));
 }
 
+extern "C" int clay_splat_set_tuning(clay_splat* s, const clay_tuning* t) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !t) return fail(CLAY_ERR_INVALID, "clay_splat_set_tuning: NULL argument");
+  if (t->chunk < 256 || t->chunk > 65536) return fail(CLAY_ERR_INVALID, "clay_tuning: chunk must be in [256, 65536]");
+  if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
+    return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
+  if ((t->splat_xcd != 0 && t->splat_xcd != 1) || (t->splat_owner != 0 && t->splat_owner != 1))
+    return fail(CLAY_ERR_INVALID, "clay_tuning: splat_xcd and splat_owner must be 0 or 1");
+  sp->chunk = (uint32_t)t->chunk;
+  sp->persistent_blocks = sp->ctx->cus * t->groups_per_cu;
+  sp->xcd = t->splat_xcd != 0;
+  sp->owner = t->splat_owner != 0;
+  return CLAY_OK;
+}
+
 extern "C" int clay_splat_volume(clay_splat* s, clay_volume** out) {
   Splat* sp = reinterpret_cast<Splat*>(s);
   if (!sp || !out) return fail(CLAY_ERR_INVALID, "clay_splat_volume: NULL argument");
@@ -1244,11 +1259,13 @@ int apply_tuning(Fluid* f, const clay_tuning* t) {
   if (t->chunk < 256 || t->chunk > 65536) return fail(CLAY_ERR_INVALID, "clay_tuning: chunk must be in [256, 65536]");
   if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
     return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
-  if ((t->fluid_xcd != 0 && t->fluid_xcd != 1) || (t->splat_xcd != 0 && t->splat_xcd != 1))
-    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd and splat_xcd must be 0 or 1");
+  if ((t->fluid_xcd != 0 && t->fluid_xcd != 1) || (t->splat_xcd != 0 && t->splat_xcd != 1) ||
+      (t->splat_owner != 0 && t->splat_owner != 1))
+    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd, splat_xcd and splat_owner must be 0 or 1");
   ft.xcd = t->fluid_xcd != 0;
   f->tune = ft;
   f->surface->xcd = t->splat_xcd != 0;
+  f->surface->owner = t->splat_owner != 0;
   f->surface->chunk = (uint32_t)t->chunk;
   f->surface->persistent_blocks = f->ctx->cus * t->groups_per_cu;
   if (f->graph) {
@@ -1265,6 +1282,7 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
   out->groups_per_cu = f->surface->persistent_blocks / std::max(1, f->ctx->cus);
   out->fluid_xcd = f->tune.xcd ? 1 : 0;
   out->splat_xcd = f->surface->xcd ? 1 : 0;
+  out->splat_owner = f->surface->owner ? 1 : 0;
 }
 
 }  // namespace

@@ -125,9 +125,10 @@ struct Splat {
   const uint32_t* scene_base = nullptr;  // device, scenes + 1 prefix offsets (not owned)
   bool follow = false;                   // centre each volume on its scene's AABB before a bake
   // tuning
-  uint32_t chunk = 2048;                 // particles per splat work item
+  uint32_t chunk = 1024;                 // particles per splat work item
   int persistent_blocks = 2048;          // grid of the splat work-item kernel
   bool xcd = false;                      // XCD-contiguous work-item numbering
+  bool owner = true;                     // owner schedule (brick + halo staged in LDS) where it fits
 };
 
 // Tunable launch shapes of the neighbour kernels (results do not depend on them).

@@ -50,6 +50,9 @@ constexpr int kBrickVox = kBrick * kBrick * kBrick;
 constexpr int kThreads = 256;
 constexpr int kSlices = 32;                 // binning counters per brick
 constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared distance
+constexpr int kHalo = 5;                    // owner schedule: voxels staged around a brick
+constexpr int kExt = kBrick + 2 * kHalo;    // staged edge (26)
+constexpr int kExtVox = kExt * kExt * kExt;
 
 struct SplatMipK {
   int32_t r;          // res >> m
@@ -81,6 +84,7 @@ struct MipsK {
   uint32_t vps;           // texels per scene (all mips)
   uint32_t tps;           // 8^3 tiles per scene (all mips)
   uint32_t xcd;           // XCD-contiguous work-item numbering
+  uint32_t owner;         // owner schedule: one list entry per particle and mip
   SplatMipK m[CLAY_MAX_MIPS];
 };
 
@@ -238,7 +242,19 @@ __global__ void __launch_bounds__(kThreads)
   for (int mi = 0; mi < mk_.mips; mi++) {
     const SplatMipK& m = mk_.m[mi];
     int32_t blo[3] = {0, 0, 0}, bext[3] = {1, 1, 1};
-    const int nb3 = valid ? brick_range(m, p, blo, bext) : 0;
+    int nb3 = valid ? brick_range(m, p, blo, bext) : 0;
+    if (mk_.owner && nb3 > 0) {
+      // The brick holding the particle's own voxel (clamped to the level) owns it.
+      const float pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        int32_t v = sat_i32(floorf(pc[a] * m.inv));
+        v = v < 0 ? 0 : (v > m.r - 1 ? m.r - 1 : v);
+        blo[a] = v / kBrick;
+        bext[a] = 1;
+      }
+      nb3 = 1;
+    }
     for (int q = 0;; q++) {
       const bool act = q < nb3;
       if (!__ballot(act)) break;
@@ -334,12 +350,32 @@ __device__ __forceinline__ BrickPos brick_pos(const MipsK& mk_, uint32_t b) {
   return q;
 }
 
+// Whether any particle can have written brick b: under the owner schedule a
+// brick also receives the halos of its 26 neighbours' particles.
+__device__ __forceinline__ bool brick_touched(const MipsK& mk_, const uint32_t* __restrict__ count, uint32_t b) {
+  if (count[b] != 0) return true;
+  if (!mk_.owner) return false;
+  const uint32_t scene = b / mk_.bps;
+  const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
+  const SplatMipK& m = mk_.m[q.mi];
+  const uint32_t base = scene * mk_.bps + m.boff;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int32_t x = q.bx + dx, y = q.by + dy, z = q.bz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= m.nb || y >= m.nb || z >= m.nb) continue;
+        if (count[base + (uint32_t)x + (uint32_t)m.nb * ((uint32_t)y + (uint32_t)m.nb * (uint32_t)z)] != 0)
+          return true;
+      }
+  return false;
+}
+
 // Step 2: reset the grid of bricks that received particles.
 template <bool kNormals>
 __global__ void __launch_bounds__(kThreads)
     k_brick_clear(MipsK mk_, const uint32_t* __restrict__ count, uint32_t* __restrict__ grid) {
   const uint32_t b = blockIdx.x;
-  if (count[b] == 0) return;
+  if (!brick_touched(mk_, count, b)) return;
   const uint32_t scene = b / mk_.bps;
   const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
   const SplatMipK& m = mk_.m[q.mi];
@@ -422,17 +458,21 @@ __global__ void __launch_bounds__(kThreads)
           xa = xa > x0 ? xa : x0;
           xb = xb < x1 ? xb : x1;
           uint32_t* row = lds + (y - oy) * kBrick + (z - oz) * kBrick * kBrick - ox;
-          for (int32_t x = xa; x <= xb; x++) {
-            const float dx = ((float)x + 0.5f) * m.voxel - p.x;
+          // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
+          // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
+          float fx = (float)xa + 0.5f;
+          uint32_t* rp = row + xa;
+          for (int32_t x = xa; x <= xb; x++, fx += 1.0f, rp++) {
+            const float dx = fx * m.voxel - p.x;
             if (kNormals) {
               const float len = sqrtf(dx * dx + dy2 + dz2);
               float sd = len - m.radius;
               sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
               const uint32_t code = encode_sd(m, sd);
-              if (code != kEncSat) atomicMin(&row[x], code);
+              if (code != kEncSat) atomicMin(rp, code);
             } else {
               const float s2 = dx * dx + dy2 + dz2;
-              atomicMin(&row[x], __float_as_uint(s2));
+              atomicMin(rp, __float_as_uint(s2));
             }
           }
         }
@@ -451,6 +491,113 @@ __global__ void __launch_bounds__(kThreads)
       } else if (val != init) {
         atomicMin(g, val);
       }
+    }
+    __syncthreads();
+  }
+}
+
+// Step 3, owner schedule: a work item is (brick, chunk of the particles it owns).
+// The workgroup stages the brick plus a kHalo-voxel shell (clipped to the level)
+// in LDS, so every particle's whole voxel box lies inside it and no lane's box is
+// clipped by the brick: all lanes walk boxes of the same size, which keeps the
+// wave's lanes busy.  The staged region is merged into the code grid with
+// atomicMin, because neighbouring bricks' shells overlap it.
+template <bool kNormals>
+__global__ void __launch_bounds__(kThreads)
+    k_owner_splat(MipsK mk_, CloudK c, const VolDev* __restrict__ vol, const uint32_t* __restrict__ start,
+                  const uint32_t* __restrict__ count, const uint32_t* __restrict__ item_start,
+                  const ScanOut* __restrict__ scan, uint32_t nbricks, const uint32_t* __restrict__ list,
+                  uint32_t* __restrict__ grid) {
+  __shared__ uint32_t lds[kExtVox];
+  __shared__ uint32_t sh_brick;
+  const int t = threadIdx.x;
+  const uint32_t init = kNormals ? kEncSat : kSInit;
+  const uint32_t rev = __brev((uint32_t)t) >> 24;
+  const uint32_t items = scan->items;
+  for (uint32_t item = mk_.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x; item < items; item += gridDim.x) {
+    if (t == 0) {
+      uint32_t lo = 0, hi = nbricks;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (item_start[mid] <= item) lo = mid; else hi = mid;
+      }
+      sh_brick = lo;
+    }
+    __syncthreads();
+    const uint32_t b = sh_brick;
+    const uint32_t scene = b / mk_.bps;
+    const BrickPos q = brick_pos(mk_, b - scene * mk_.bps);
+    const SplatMipK& m = mk_.m[q.mi];
+    const VolDev* sv = vol + scene;
+    uint32_t* sgrid = grid + (size_t)scene * mk_.gps + m.off;
+    const int32_t ex0 = q.ox - kHalo > 0 ? q.ox - kHalo : 0, ey0 = q.oy - kHalo > 0 ? q.oy - kHalo : 0,
+                  ez0 = q.oz - kHalo > 0 ? q.oz - kHalo : 0;
+    const int32_t ex1 = q.ox + kBrick + kHalo < m.r ? q.ox + kBrick + kHalo : m.r;
+    const int32_t ey1 = q.oy + kBrick + kHalo < m.r ? q.oy + kBrick + kHalo : m.r;
+    const int32_t ez1 = q.oz + kBrick + kHalo < m.r ? q.oz + kBrick + kHalo : m.r;
+    const int32_t nx = ex1 - ex0, ny = ey1 - ey0, nz = ez1 - ez0;
+    const int32_t nvox = nx * ny * nz;
+    for (int v = t; v < nvox; v += kThreads) lds[v] = init;
+    __syncthreads();
+    const uint32_t n_b = count[b];
+    const uint32_t chunk = item - item_start[b];
+    const uint32_t e_lo = chunk * mk_.chunk;
+    const uint32_t n_e = (n_b - e_lo) < mk_.chunk ? (n_b - e_lo) : mk_.chunk;
+    const uint32_t base = start[b] + e_lo;
+    for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
+      const uint32_t e = pass * kThreads + rev;
+      if (e >= n_e) continue;
+      const uint32_t i = list[base + e];
+      const f3 p = load_rel(c, sv, i);
+      const f3 nr = kNormals ? load_nrm(c, i) : mk(0.0f, 0.0f, 0.0f);
+      int32_t lo[3], hi[3];
+      voxel_box(m, p, lo, hi);  // non-empty: the particle was binned
+      // Inside the staged region by construction (reach <= kHalo voxels at every
+      // level); clipping to it keeps every LDS access in bounds regardless.
+      const int32_t x0 = lo[0] > ex0 ? lo[0] : ex0, x1 = hi[0] < ex1 - 1 ? hi[0] : ex1 - 1;
+      const int32_t y0 = lo[1] > ey0 ? lo[1] : ey0, y1 = hi[1] < ey1 - 1 ? hi[1] : ey1 - 1;
+      const int32_t z0 = lo[2] > ez0 ? lo[2] : ez0, z1 = hi[2] < ez1 - 1 ? hi[2] : ez1 - 1;
+      for (int32_t z = z0; z <= z1; z++) {
+        const float dz = ((float)z + 0.5f) * m.voxel - p.z;
+        const float dz2 = dz * dz;
+        if (dz2 > m.cull2) continue;
+        for (int32_t y = y0; y <= y1; y++) {
+          const float dy = ((float)y + 0.5f) * m.voxel - p.y;
+          const float dy2 = dy * dy;
+          const float rem = m.cull2 - dz2 - dy2;
+          if (rem < 0.0f) continue;
+          const float sp = __builtin_amdgcn_sqrtf(rem);
+          int32_t xa = sat_i32(floorf((p.x - sp) * m.inv - 0.5f)) - 1;
+          int32_t xb = sat_i32(floorf((p.x + sp) * m.inv - 0.5f)) + 1;
+          xa = xa > x0 ? xa : x0;
+          xb = xb < x1 ? xb : x1;
+          uint32_t* row = lds + (y - ey0) * nx + (z - ez0) * nx * ny - ex0;
+          // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
+          // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
+          float fx = (float)xa + 0.5f;
+          uint32_t* rp = row + xa;
+          for (int32_t x = xa; x <= xb; x++, fx += 1.0f, rp++) {
+            const float dx = fx * m.voxel - p.x;
+            if (kNormals) {
+              const float len = sqrtf(dx * dx + dy2 + dz2);
+              float sd = len - m.radius;
+              sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
+              const uint32_t code = encode_sd(m, sd);
+              if (code != kEncSat) atomicMin(rp, code);
+            } else {
+              const float s2 = dx * dx + dy2 + dz2;
+              atomicMin(rp, __float_as_uint(s2));
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int v = t; v < nvox; v += kThreads) {
+      const uint32_t val = lds[v];
+      if (val == init) continue;
+      const int32_t x = ex0 + v % nx, y = ey0 + (v / nx) % ny, z = ez0 + v / (nx * ny);
+      atomicMin(sgrid + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z), val);
     }
     __syncthreads();
   }
@@ -482,7 +629,7 @@ __global__ void __launch_bounds__(kThreads)
     const int32_t ti = tile_index(k & 1, (k >> 1) & 1, k >> 2);
     if (ti >= 0 && tile_live[ti]) stale |= 1u << k;
   }
-  const bool has = count[b] != 0;
+  const bool has = brick_touched(mk_, count, b);
   if (!has && !stale) return;
   if (t == 0) tflag = 0;
   __syncthreads();
@@ -570,6 +717,18 @@ __global__ void __launch_bounds__(kThreads)
 
 }  // namespace
 
+// The owner schedule needs every particle's voxel box within kHalo voxels of its
+// own voxel at every level: reach / voxel = radius / voxel + band < kHalo.
+static bool splat_owner_fits(const Splat* sp) {
+  if (!sp->owner || sp->brick_res.empty()) return false;
+  const double voxel0 = sp->vol->info.world_size / sp->vol->info.resolution;
+  for (int m = 0; m < sp->mips; m++) {
+    const double voxel = voxel0 * (double)(1 << m);
+    if (sp->desc.radius / voxel + sp->desc.band > (double)kHalo - 0.01) return false;
+  }
+  return true;
+}
+
 static void fill_mips(Splat* sp, MipsK* mk_) {
   const Volume* v = sp->vol;
   const double voxel0 = v->info.world_size / v->info.resolution;
@@ -582,6 +741,7 @@ static void fill_mips(Splat* sp, MipsK* mk_) {
   mk_->vps = (uint32_t)v->total_voxels;
   mk_->tps = (uint32_t)sp->tile_total;
   mk_->xcd = sp->xcd ? 1u : 0u;
+  mk_->owner = splat_owner_fits(sp) ? 1u : 0u;
   for (int m = 0; m < sp->mips; m++) {
     SplatMipK& k = mk_->m[m];
     k.r = v->mip_res(m);
@@ -659,9 +819,14 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   }
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_brick");
-    hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
-                       sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
-                       (uint32_t)nbricks, sp->brick_list, sp->grid);
+    if (mk_.owner)
+      hipLaunchKernelGGL(k_owner_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
+                         sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
+                         (uint32_t)nbricks, sp->brick_list, sp->grid);
+    else
+      hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
+                         sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
+                         (uint32_t)nbricks, sp->brick_list, sp->grid);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   {

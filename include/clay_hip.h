@@ -364,8 +364,13 @@ typedef struct {
   int32_t groups_per_cu;  /* persistent splat grid: 1..32 workgroups per compute unit */
   int32_t fluid_xcd;      /* 1: neighbour kernels number workgroups XCD-contiguously (one L2 per run) */
   int32_t splat_xcd;      /* 1: the same for splat work items */
+  int32_t splat_owner;    /* 1: each particle binned once per mip to the brick holding it, which
+                             stages itself plus a 5-voxel shell in LDS (when reach <= 5 voxels) */
 } clay_tuning;
 int clay_fluid_set_tuning(clay_fluid* f, const clay_tuning* t);
+/* The splat fields of clay_tuning (chunk, groups_per_cu, splat_xcd, splat_owner) for a
+ * SplatField; the others are ignored. */
+int clay_splat_set_tuning(clay_splat* s, const clay_tuning* t);
 int clay_fluid_get_tuning(const clay_fluid* f, clay_tuning* out);
 
 /* ---- Batched scenes ------------------------------------------------------

@@ -106,13 +106,16 @@ def test_erosion_from_fluid_contacts_matches_oracle(root):
     sim.step()
     pts = sim.readContacts()
     raw = sim.lastContactCount
-    assert raw > 0 and any(p[3] > 0.5 for p in pts) and any(p[3] <= 0.5 for p in pts)
+    # Which contacts fill the 32 slots is the atomic order's choice: threshold at the
+    # median so that some are kept and some filtered whatever the list holds.
+    thr = float(np.median([p[3] for p in pts]))
+    assert raw > 0 and any(p[3] > thr for p in pts) and any(p[3] <= thr for p in pts)
     vol, planes = baked(root)
     ed = clayhip.SdfEditor(vol)
     ed.push({"kind": "sphere", "size": 0.3, "pos": (1.0, -0.6, 1.0), "op": "cut"})
-    ed.erode(sim, min_speed=0.5, radius=0.15, blend=0.03)
+    ed.erode(sim, min_speed=thr, radius=0.15, blend=0.03)
     host_edit = [make_brush({"kind": "sphere", "size": 0.3, "pos": (1.0, -0.6, 1.0), "op": "cut"})]
-    dev_edits = O.contacts_to_edits(raw, np.array(pts, dtype=np.float32).reshape(-1, 4), 0.5, 0.15, 0.03,
+    dev_edits = O.contacts_to_edits(raw, np.array(pts, dtype=np.float32).reshape(-1, 4), thr, 0.15, 0.03,
                                     max_edits=63)
     assert len(dev_edits) > 0
     counts = O.edit_flush(planes, WORLD, ORIGIN, BAND, host_edit + dev_edits)

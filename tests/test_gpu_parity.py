@@ -171,8 +171,9 @@ def _cloud(pos: np.ndarray):
     return t, clayhip.ParticleCloud(pos.shape[0], (t[0], t[1], t[2]), live=("prefix", pos.shape[0]))
 
 
+@pytest.mark.parametrize("owner", [False, True], ids=["brick", "owner"])
 @pytest.mark.parametrize("res,mips,band,radius", [(64, 4, 4, 0.05), (32, 3, 2, 0.3), (80, 2, 4, 0.12)])
-def test_tiled_splat_equals_plain_and_oracle(root, res, mips, band, radius):
+def test_tiled_splat_equals_plain_and_oracle(root, res, mips, band, radius, owner):
     rng = np.random.default_rng(res)
     pos = rng.uniform(-2, 3, size=(6000, 3)).astype(np.float32)
     pos[:2000] = rng.uniform(-12.5, 20.5, size=(2000, 3))  # some outside the volume
@@ -180,6 +181,7 @@ def test_tiled_splat_equals_plain_and_oracle(root, res, mips, band, radius):
     common = dict(radius=radius, resolution=res, mipLevels=mips, band=band, worldSize=24, origin=(-12, -4, -12))
     plain = clayhip.SplatField(root, cloud, common, mode="plain")
     tiled = clayhip.SplatField(root, cloud, common, mode="tiled", keepCodes=True)
+    tiled.setSchedule(chunk=512 if owner else 2048, groups_per_cu=4, owner=owner)
     o = O.splat_opts(res, mips, 24.0, (-12, -4, -12), band, radius)
     want = O.splat_bake(o, pos)
     for rep in range(2):  # the second bake must clear the first bake's tiles
@@ -197,7 +199,10 @@ def test_tiled_splat_equals_plain_and_oracle(root, res, mips, band, radius):
     tiled.destroy()
 
 
-def test_splat_with_normals_and_range_rule(root):
+@pytest.mark.parametrize("radius", [0.2, 0.05])
+def test_splat_with_normals_and_range_rule(root, radius):
+    """radius 0.2 spans more than 5 voxels here, so the owner schedule falls back to
+    bricks; at 0.05 it runs."""
     import torch
 
     rng = np.random.default_rng(11)
@@ -208,11 +213,13 @@ def test_splat_with_normals_and_range_rule(root):
     tn = torch.from_numpy(np.ascontiguousarray(nrm.T)).cuda()
     cloud = clayhip.ParticleCloud(3000, (tp[0], tp[1], tp[2]), live=("range", 500, 1500),
                                   normals=(tn[0], tn[1], tn[2]))
-    opts = dict(radius=0.2, resolution=32, mipLevels=2, band=4, worldSize=2.7, origin=(-1.35, -1.35, -1.35))
-    o = O.splat_opts(32, 2, 2.7, (-1.35, -1.35, -1.35), 4, 0.2)
+    opts = dict(radius=radius, resolution=32, mipLevels=2, band=4, worldSize=2.7, origin=(-1.35, -1.35, -1.35))
+    o = O.splat_opts(32, 2, 2.7, (-1.35, -1.35, -1.35), 4, radius)
     want = O.splat_bake(o, pos, 500, 1500, normals=nrm)
-    for mode in ("plain", "tiled"):
-        s = clayhip.SplatField(root, cloud, opts, mode=mode, keepCodes=True)
+    for mode in ("plain", "tiled", "owner"):
+        s = clayhip.SplatField(root, cloud, opts, mode="plain" if mode == "plain" else "tiled", keepCodes=True)
+        if mode == "owner":
+            s.setSchedule(chunk=256, owner=True)
         s.bake()
         for m in range(2):
             assert np.array_equal(s.codes(m), want[m]), f"{mode} mip {m}"
@@ -223,8 +230,10 @@ def test_set_center_follows_and_far_origin_saturates(root):
     pos = scenes.lattice(8, 8, 8, 0.1, (0, 0, 0), seed=2)
     t, cloud = _cloud(pos)
     opts = dict(radius=0.15, resolution=32, mipLevels=3, band=4, worldSize=2.7, origin=(-1, -1, -1))
-    for mode in ("plain", "tiled"):
-        s = clayhip.SplatField(root, cloud, opts, mode=mode, keepCodes=True)
+    for mode in ("plain", "tiled", "owner"):
+        s = clayhip.SplatField(root, cloud, opts, mode="plain" if mode == "plain" else "tiled", keepCodes=True)
+        if mode == "owner":
+            s.setSchedule(owner=True)
         s.setCenter((0.4, 0.4, 0.4))
         s.bake()
         o = O.splat_opts(32, 3, 2.7, s.volume.origin, 4, 0.15)
