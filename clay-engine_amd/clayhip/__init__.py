@@ -8,7 +8,7 @@ include/clay_hip.h.  There is no CPU fallback: every call goes to the GPU.
 from ._lib import ClayError, LIB_PATH, lib
 from .batch import FluidBatch
 from .brush import compile_brushes, make_brush
-from .edit import SdfEditor, body_centroids
+from .edit import SdfEditor, body_centroids, body_centroids_device
 from .field import TracedField, analyticField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
 from .splat import ParticleCloud, SplatField
@@ -27,6 +27,7 @@ __all__ = [
     "TracedField",
     "analyticField",
     "body_centroids",
+    "body_centroids_device",
     "compile_brushes",
     "derive",
     "fluid_desc",

@@ -306,7 +306,19 @@ SIGNATURES = {
     "clay_editor_tile_counts": ([P, P], C.c_int),
     "clay_editor_cells_download": ([P, P], C.c_int),
     "clay_body_centroids": ([P, P, P, P, U32, I32, P, P], C.c_int),
+    "clay_body_centroids_device": ([P, P, P, P, U32, I32, P, P], C.c_int),
+    "clay_editor_push_scene": ([P, U32, C.POINTER(Brush), I32, C.POINTER(I32)], C.c_int),
+    "clay_editor_erode_batch": ([P, P, C.c_double, C.c_double, C.c_double, P], C.c_int),
+    "clay_editor_tile_counts_scene": ([P, U32, P], C.c_int),
+    "clay_editor_cells_download_scene": ([P, U32, P], C.c_int),
+    "clay_volume_create_instances": ([P, C.POINTER(VolumeDesc), U32, PP], C.c_int),
+    "clay_volume_replicate": ([P, P], C.c_int),
+    "clay_volume_instance_download": ([P, U32, I32, P], C.c_int),
+    "clay_splat_create_scenes": ([P, C.POINTER(CloudDesc), C.POINTER(SplatDesc), U32, P, I32, P, PP], C.c_int),
+    "clay_splat_instance_download": ([P, U32, I32, P], C.c_int),
 }
+
+FOLLOW_NONE, FOLLOW_BOUNDS, FOLLOW_CENTERS = 0, 1, 2
 
 _lib: C.CDLL | None = None
 

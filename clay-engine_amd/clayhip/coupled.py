@@ -1,0 +1,149 @@
+"""A coupled frame over batched scenes: fluid, a kinematic clay body and an eroding world.
+
+Per scene, every frame:
+  1. the body's particles take their pose for the frame (a synthetic kinematic path:
+     rest shape rotated about y and carried around the scene; the reference moves
+     bodies with its PBD solver, which is not part of this path),
+  2. the body centroids are reduced in FP64 on the device (ClaySolver.match's
+     centroid, pbd.ts:240-266) and the body's 32^3 volume re-centres on them
+     (SoftBody.simulate: surface.setCenter(position), softbody.ts:424-428) and bakes,
+  3. the fluid steps against union(world instance, body volume) (field.ts:246-315),
+  4. its contacts become cut edits of the scene's world instance, whose touched
+     tiles and their coarser mips are rebuilt (SdfEditor.flush, modify.ts:214-238).
+All of it runs on one stream with no host readback; scenes share every launch.
+The body state lives in torch CUDA tensors (the kinematic path is workload
+generation, not library functionality).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import scenes as S
+from .batch import FluidBatch
+from .brush import make_brush
+from .edit import SdfEditor, body_centroids_device
+from .field import unionField, volumeField
+from .splat import ParticleCloud, SplatField
+from .volume import Root, SdfVolume
+
+
+def body_rest(count: int, radius: float, seed: int) -> np.ndarray:
+    """`count` points of a jittered sphere: the cubic lattice points inside a ball of
+    `radius` (spacing chosen so that at least `count` fit, nearest to the centre
+    first), each jittered by up to a fifth of the spacing with the reference hash."""
+    spacing = radius * (4.0 / 3.0 * math.pi / count) ** (1.0 / 3.0)
+    while True:
+        n = int(math.ceil(2 * radius / spacing)) + 1
+        g = (np.arange(n) - (n - 1) / 2) * spacing
+        zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+        p = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=-1)
+        r = np.linalg.norm(p, axis=1)
+        inside = p[r <= radius]
+        if inside.shape[0] >= count:
+            break
+        spacing *= 0.98
+    order = np.argsort(np.linalg.norm(inside, axis=1), kind="stable")[:count]
+    pts = inside[order]
+    ids = np.arange(count, dtype=np.uint64)
+    seeds = ((np.uint64(seed) * np.uint64(9781) + ids) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    u = S.hash3(seeds).astype(np.float64)
+    return (pts + (u * 2 - 1) * spacing * 0.2).astype(np.float32)
+
+
+class CoupledBatch:
+    def __init__(self, root: Root, fluid_counts, fluid_pos: np.ndarray, fluid_options: dict, world: dict,
+                 world_brushes, body_count: int = 4096, body_radius: float = 0.95, body_box: float = 2.7,
+                 body_res: int = 32, body_mips: int = 2, body_splat_radius: float = 0.08,
+                 erosion=(2.0, 0.12, 0.03), seed: int = 11, orbit: float = 1.5, height: float = 1.0,
+                 phases=None, body_seeds=None):
+        import torch
+
+        self.root = root
+        counts = np.asarray(fluid_counts, dtype=np.uint32)
+        self.scenes = n = len(counts)
+        self.body_count = body_count
+        self.erosion = erosion
+        self.t = 0.0
+        self.dt = 1.0 / 60.0
+        # world: one instance per scene, baked once and replicated
+        self.world = SdfVolume(root, dict(world, instances=n))
+        self.world.bakeBrushes(world_brushes)
+        self.world.replicate()
+        self.world_field = volumeField(self.world)
+        # bodies
+        seeds = body_seeds if body_seeds is not None else [seed + s for s in range(n)]
+        rest = np.concatenate([body_rest(body_count, body_radius, int(seeds[s])) for s in range(n)], axis=0)
+        self.rest = torch.from_numpy(np.ascontiguousarray(rest.T)).cuda()
+        self.pos = torch.empty_like(self.rest)
+        ph = phases if phases is not None else np.arange(n) * (2 * math.pi / max(n, 1))
+        self.phase = torch.tensor(np.repeat(np.asarray(ph, dtype=np.float32), body_count), device="cuda")
+        self.orbit, self.height = orbit, height
+        fc = np.stack([np.arange(n) * body_count, np.full(n, body_count)], axis=-1).astype(np.int32)
+        self.body_fc = torch.from_numpy(fc.reshape(-1)).cuda()
+        self.body_com = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
+        self._pose(torch.cuda.current_stream())
+        self.body_cloud = ParticleCloud(n * body_count, (self.pos[0], self.pos[1], self.pos[2]),
+                                        live=("prefix", n * body_count))
+        self.body = SplatField(root, self.body_cloud,
+                               dict(radius=body_splat_radius, resolution=body_res, mipLevels=body_mips, band=4,
+                                    worldSize=body_box, origin=(0, 0, 0), mode="tiled"),
+                               scenes=[body_count] * n, follow=self.body_com)
+        self.collider = unionField(self.world_field, self.body.field)
+        self.fluid = FluidBatch(root, self.collider, fluid_options, counts)
+        self.fluid.set_state(fluid_pos)
+        self.editor = SdfEditor(self.world)
+
+    def _pose(self, stream) -> None:
+        """Kinematic pose at time t: rest rotated by w t about y, centred on a circle."""
+        import torch
+
+        with torch.cuda.stream(stream):
+            ang = self.phase + 0.8 * self.t
+            c, s = torch.cos(ang), torch.sin(ang)
+            rx, ry, rz = self.rest[0], self.rest[1], self.rest[2]
+            self.pos[0].copy_(c * rx + s * rz + self.orbit * torch.cos(self.phase + 0.5 * self.t))
+            self.pos[1].copy_(ry + self.height)
+            self.pos[2].copy_(c * rz - s * rx + self.orbit * torch.sin(self.phase + 0.5 * self.t))
+
+    def step(self, stream) -> None:
+        self.t += self.dt
+        self._pose(stream)
+        body_centroids_device(self.root, (self.pos[0], self.pos[1], self.pos[2]), self.body_fc, self.body_com,
+                              precision="f64", stream=stream)
+        self.body.bake(stream)
+        self.fluid.step(stream)
+        v, r, b = self.erosion
+        self.editor.erodeBatch(self.fluid, v, r, b, stream=stream)
+
+    def destroy(self) -> None:
+        self.editor.destroy()
+        self.fluid.destroy()
+        self.collider.destroy()
+        self.body.destroy()
+        self.world_field.destroy()
+        self.world.destroy()
+
+
+def coupled_scene(scenes: int = 8, fluid_count: int = 524288, world_res: int = 288, seed: int = 5):
+    """The coupled workload: per scene a 128 x 32 x 128 dam-break block (524 288
+    particles at 0.05) over the ridge world (288^3 x 3 mips over 24 units), one
+    4096-particle body of radius 0.95 circling above it, erosion of the world
+    by contacts faster than 2 units/s.  Smaller `fluid_count` keeps the block's
+    footprint and thins its height."""
+    nx = nz = 128
+    ny = max(1, fluid_count // (nx * nz))
+    block = S.lattice(nx, ny, nz, 0.05, (-3.2, 0.3, -3.2), seed=seed)[:fluid_count]
+    pos = np.concatenate([block] * scenes, axis=0)
+    opts = S.dam_break_options(1, 64, 3)
+    opts.update({"followSize": 6.4, "splatMode": "tiled"})
+    world = {"resolution": world_res, "mipLevels": 3, "worldSize": S.WORLD, "origin": S.ORIGIN, "band": 4}
+    return {
+        "name": f"coupled {scenes} scenes x ({block.shape[0]} fluid + 4096 body), {world_res}^3 world per scene",
+        "counts": np.full(scenes, block.shape[0], dtype=np.uint32),
+        "pos": pos,
+        "options": opts,
+        "world": world,
+        "world_brushes": S.ridge_world_brushes(),
+    }

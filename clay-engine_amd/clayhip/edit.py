@@ -27,11 +27,12 @@ class SdfEditor:
         self.volume = volume
         self.maxEdits = d.max_edits
 
-    def push(self, edit) -> bool:
-        """SdfEditor.push (modify.ts:200-212): a BrushDesc dict or a Brush record."""
+    def push(self, edit, scene: int = 0) -> bool:
+        """SdfEditor.push (modify.ts:200-212): a BrushDesc dict or a Brush record,
+        queued for one scene of a multi-instance volume (0 otherwise)."""
         b = edit if isinstance(edit, L.Brush) else make_brush(edit)
         accepted = C.c_int32()
-        L.check(L.lib().clay_editor_push(self._h, C.byref(b), 1, C.byref(accepted)))
+        L.check(L.lib().clay_editor_push_scene(self._h, int(scene), C.byref(b), 1, C.byref(accepted)))
         return accepted.value == 1
 
     @property
@@ -56,18 +57,23 @@ class SdfEditor:
             L.check(L.lib().clay_editor_erode_scene(self._h, fluid._h, int(scene), float(min_speed), float(radius),
                                                     float(blend), L.stream_handle(stream)))
 
+    def erodeBatch(self, batch, min_speed: float, radius: float, blend: float = 0.02, stream=None) -> None:
+        """Every scene's contacts -> that scene's world instance (one instance per scene)."""
+        L.check(L.lib().clay_editor_erode_batch(self._h, batch._h, float(min_speed), float(radius), float(blend),
+                                                L.stream_handle(stream)))
+
     def refreshAll(self, stream=None) -> None:
         L.check(L.lib().clay_editor_refresh_all(self._h, L.stream_handle(stream)))
 
-    def readTileCounts(self) -> list[int]:
+    def readTileCounts(self, scene: int = 0) -> list[int]:
         """TileGrid.readTileCounts (tilegrid.ts:281-297)."""
         out = (C.c_uint32 * 16)()
-        L.check(L.lib().clay_editor_tile_counts(self._h, out))
+        L.check(L.lib().clay_editor_tile_counts_scene(self._h, int(scene), out))
         return [out[m] for m in range(self.volume.mipLevels)]
 
-    def cellCounts(self) -> np.ndarray:
+    def cellCounts(self, scene: int = 0) -> np.ndarray:
         out = np.empty(self.volume.totalCells, dtype=np.uint32)
-        L.check(L.lib().clay_editor_cells_download(self._h, out.ctypes.data))
+        L.check(L.lib().clay_editor_cells_download_scene(self._h, int(scene), out.ctypes.data))
         return out
 
     def destroy(self) -> None:
@@ -89,3 +95,16 @@ def body_centroids(root, pos, first_count, rest=None, precision: str = "f64", st
     L.check(L.lib().clay_body_centroids(root.handle, ptrs(pos), None if rest is None else ptrs(rest),
                                         fc.ctypes.data, fc.shape[0], mode, out.ctypes.data, L.stream_handle(stream)))
     return out
+
+
+def body_centroids_device(root, pos, first_count_dev, out_dev, rest=None, precision: str = "f64", stream=None) -> None:
+    """Enqueues the centroids of bodies into `out_dev` (CUDA float64, 6 per body)
+    from a CUDA int32/uint32 table of (first, count) pairs; nothing is read back."""
+    def ptrs(arrs):
+        return (C.c_void_p * 3)(*[a if isinstance(a, int) else a.data_ptr() for a in arrs])
+
+    mode = {"f32": L.REDUCE_F32_TREE, "f64": L.REDUCE_F64}[precision]
+    n = first_count_dev.numel() // 2
+    L.check(L.lib().clay_body_centroids_device(root.handle, ptrs(pos), None if rest is None else ptrs(rest),
+                                               C.c_void_p(first_count_dev.data_ptr()), n, mode,
+                                               C.c_void_p(out_dev.data_ptr()), L.stream_handle(stream)))

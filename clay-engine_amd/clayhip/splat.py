@@ -58,9 +58,13 @@ class SplatField:
     Both give the same u32 codes."""
 
     def __init__(self, root: Root, cloud: ParticleCloud | None, options: dict | None = None, *,
-                 _handle=None, **kw):
+                 _handle=None, scenes=None, follow=None, **kw):
+        """`scenes` (particle counts, scene-major) bakes scene s's particles into
+        volume instance s; `follow` is None, 'bounds', or a CUDA float64 tensor of 6
+        values per scene (xyz first: body centroids) each instance centres on."""
         self.root = root
         self.cloud = cloud
+        self._keep = []
         if _handle is not None:
             self._h = _handle
             self._owned = False
@@ -82,7 +86,22 @@ class SplatField:
             d.mode = {"plain": L.SPLAT_PLAIN, "tiled": L.SPLAT_TILED}[mode]
             d.keep_codes = 1 if opts.get("keepCodes", False) else 0
             h = C.c_void_p()
-            L.check(L.lib().clay_splat_create(root.handle, C.byref(cloud.desc), C.byref(d), C.byref(h)))
+            if scenes is None:
+                L.check(L.lib().clay_splat_create(root.handle, C.byref(cloud.desc), C.byref(d), C.byref(h)))
+            else:
+                counts = np.ascontiguousarray(np.asarray(scenes, dtype=np.uint32))
+                if follow is None:
+                    mode, centers = L.FOLLOW_NONE, None
+                elif isinstance(follow, str) and follow == "bounds":
+                    mode, centers = L.FOLLOW_BOUNDS, None
+                else:
+                    if str(follow.dtype) != "torch.float64" or not follow.is_cuda:
+                        raise ValueError("SplatField: follow centres are a CUDA float64 tensor")
+                    mode, centers = L.FOLLOW_CENTERS, C.c_void_p(int(follow.data_ptr()))
+                    self._keep.append(follow)
+                L.check(L.lib().clay_splat_create_scenes(root.handle, C.byref(cloud.desc), C.byref(d), counts.shape[0],
+                                                         counts.ctypes.data, mode, centers, C.byref(h)))
+            self.scenes = None if scenes is None else len(scenes)
             self._h = h
             self._owned = True
         vh = C.c_void_p()
@@ -119,6 +138,13 @@ class SplatField:
         r = self.volume.resolution >> mip
         out = np.empty((r, r, r), dtype=np.uint32)
         L.check(L.lib().clay_splat_grid_download(self._h, mip, out.ctypes.data))
+        return out
+
+    def sceneCodes(self, scene: int, mip: int = 0) -> np.ndarray:
+        """One scene's u32 codes of a scene-partitioned splat, shape (r, r, r)."""
+        r = self.volume.resolution >> mip
+        out = np.empty((r, r, r), dtype=np.uint32)
+        L.check(L.lib().clay_splat_instance_download(self._h, int(scene), mip, out.ctypes.data))
         return out
 
     def liveTiles(self, mip: int = 0) -> np.ndarray:

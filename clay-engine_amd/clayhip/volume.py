@@ -82,7 +82,12 @@ class SdfVolume:
                 d.origin[i] = float(origin[i])
             d.gradient_voxels = float(opts.get("gradientVoxels", 1))
             h = C.c_void_p()
-            L.check(L.lib().clay_volume_create(root.handle, C.byref(d), C.byref(h)))
+            instances = int(opts.get("instances", 1))
+            if instances == 1:
+                L.check(L.lib().clay_volume_create(root.handle, C.byref(d), C.byref(h)))
+            else:
+                L.check(L.lib().clay_volume_create_instances(root.handle, C.byref(d), instances, C.byref(h)))
+            self.instances = instances
             self._h = h
             self._owned = True
         self._load_info()
@@ -119,12 +124,20 @@ class SdfVolume:
         L.check(L.lib().clay_volume_voxel_size_at(self._h, mip, C.byref(out)))
         return out.value
 
-    def download(self, mip: int = 0) -> np.ndarray:
-        """Texels of one mip as uint16 (f16 bits), shape (r, r, r, 4) indexed [z, y, x, c]."""
+    def download(self, mip: int = 0, instance: int | None = None) -> np.ndarray:
+        """Texels of one mip as uint16 (f16 bits), shape (r, r, r, 4) indexed [z, y, x, c];
+        `instance` picks one scene's copy of a multi-instance volume."""
         r = self.resolution >> mip
         out = np.empty((r, r, r, 4), dtype=np.uint16)
-        L.check(L.lib().clay_volume_download(self._h, mip, out.ctypes.data))
+        if instance is None:
+            L.check(L.lib().clay_volume_download(self._h, mip, out.ctypes.data))
+        else:
+            L.check(L.lib().clay_volume_instance_download(self._h, int(instance), mip, out.ctypes.data))
         return out
+
+    def replicate(self, stream=None) -> None:
+        """Copies instance 0 (what bakeBrushes / upload wrote) to every instance."""
+        L.check(L.lib().clay_volume_replicate(self._h, L.stream_handle(stream)))
 
     def upload(self, mip: int, rgba16: np.ndarray) -> None:
         r = self.resolution >> mip

@@ -235,6 +235,8 @@ Field* make_volume_field(Ctx* ctx, Volume* v) {
   l.kind = 1;
   l.vol = v->dev;
   l.dist0 = v->dist;
+  // One instance per scene: the collider reads the scene's own record and planes.
+  l.scene_stride = v->instances > 1 ? (uint32_t)v->total_voxels : 0u;
   return f;
 }
 
@@ -573,6 +575,8 @@ void splat_free(Splat* sp) {
   dfree(sp->brick_total_dev);
   dfree(sp->item_start);
   dfree(sp->tile_live);
+  dfree(sp->own_scene_of);
+  dfree(sp->own_scene_base);
   delete sp;
 }
 
@@ -827,6 +831,8 @@ void fluid_free(Fluid* f) {
   dfree(f->hist);
   dfree(f->cells);
   dfree(f->contacts);
+  dfree(f->cmask);
+  dfree(f->cdata);
   dfree(f->reduce);
   dfree(f->epoch_dev);
   dfree(f->scene_of);
@@ -842,8 +848,10 @@ int fluid_substep_impl(Fluid* f, hipStream_t s) {
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
   int rc;
-  // clearGrid's contacts reset (fluid.ts:302-304), for every scene.
-  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u))) return rc;
+  // clearGrid's contacts reset (fluid.ts:302-304), for every scene, and the per-id flags.
+  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u)) ||
+      (rc = launch_fill_u32(c, s, f->cmask, ((size_t)f->capacity + 31) / 32, 0u)))
+    return rc;
   if (n == 0) return CLAY_OK;
   // A new stamp per substep makes every older cell record stale at once.  After
   // 2^32 - 1 substeps the stamps restart, and the table is wiped once so that no
@@ -870,8 +878,11 @@ int fluid_substep_impl(Fluid* f, hipStream_t s) {
   float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
   ColliderDev none{};
   const ColliderDev& col = f->world ? f->world->col : none;
-  return launch_forces_integrate(c, s, f->k, sk, f->tune, lds, col, n, sorted, f->vals, f->keys, f->cells, out,
-                                 f->contacts);
+  if ((rc = launch_forces_integrate(c, s, f->k, sk, f->tune, lds, col, n, sorted, f->vals, f->keys, f->cells, out,
+                                    f->contacts)))
+    return rc;
+  return launch_contacts_collect(c, s, f->contacts, 4 + 4 * CLAY_CONTACTS, f->cmask, f->cdata,
+                                 f->scene_of ? f->scene_base_dev : nullptr, n, f->scenes);
 }
 
 // FluidSim.step (fluid.ts:536-558): emit, `substeps` substeps, bake every mip.
@@ -981,7 +992,8 @@ int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t s
   if ((rc = dalloc(&f->st, (size_t)P * 11)) || (rc = dalloc(&f->ss, (size_t)P * 8)) ||
       (rc = dalloc(&f->keys, P)) || (rc = dalloc(&f->vals, P)) || (rc = dalloc(&f->keys2, P)) ||
       (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, contact_words(f))) ||
-      (rc = dalloc(&f->epoch_dev, 1))) {
+      (rc = dalloc(&f->epoch_dev, 1)) || (rc = dalloc(&f->cmask, ((size_t)P + 31) / 32)) ||
+      (rc = dalloc(&f->cdata, P))) {
     fluid_free(f);
     return rc;
   }
@@ -1575,32 +1587,48 @@ void editor_free(Editor* e) {
   delete e;
 }
 
-// Uploads the host queue as the device edit list (count included) and clears it.
-// With nothing queued it only resets the device count (a kernel), so an erosion
-// flush can be captured in a graph.
+uint32_t editor_scenes(const Editor* e) { return (uint32_t)e->vol->instances; }
+
+size_t editor_pending(const Editor* e) {
+  size_t n = 0;
+  for (const auto& q : e->pending) n += q.size();
+  return n;
+}
+
+// Uploads every scene's host queue as its device edit list (counts included) and
+// clears the queues.  With nothing queued it only resets the device counts (a
+// kernel), so an erosion flush can be captured in a graph.
 int editor_stage(Editor* e, hipStream_t s) {
-  const uint32_t n = (uint32_t)e->pending.size();
-  if (n) CLAY_HIP_CHECK(hipMemcpyAsync(e->edits_dev, e->pending.data(), sizeof(clay_brush) * n,
-                                       hipMemcpyHostToDevice, s));
-  int rc = launch_fill_u32(e->ctx, s, e->nedit_dev, 1, n);
-  if (rc) return rc;
-  if (n) {
-    // The queue's storage must outlive the copy: wait for it before reusing the vector.
-    CLAY_HIP_CHECK(hipStreamSynchronize(s));
-    e->pending.clear();
+  const uint32_t S = editor_scenes(e);
+  if (editor_pending(e) == 0) return launch_fill_u32(e->ctx, s, e->nedit_dev, S, 0u);
+  e->staged_counts.assign(S, 0u);
+  for (uint32_t sc = 0; sc < S; sc++) {
+    const auto& q = e->pending[sc];
+    e->staged_counts[sc] = (uint32_t)q.size();
+    if (!q.empty())
+      CLAY_HIP_CHECK(hipMemcpyAsync(e->edits_dev + (size_t)sc * e->max_edits, q.data(), sizeof(clay_brush) * q.size(),
+                                    hipMemcpyHostToDevice, s));
   }
+  CLAY_HIP_CHECK(hipMemcpyAsync(e->nedit_dev, e->staged_counts.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, s));
+  // The queues' storage must outlive the copies: wait for them before reusing it.
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  for (auto& q : e->pending) q.clear();
   return CLAY_OK;
 }
 
-int editor_erode_impl(Editor* e, Fluid* f, uint32_t scene, double min_speed, double radius, double blend,
-                      hipStream_t s) {
-  if (scene >= f->scenes) return fail(CLAY_ERR_INVALID, "clay_editor_erode: scene out of range");
+constexpr uint32_t kContactWords = 4 + 4 * CLAY_CONTACTS;
+
+// Contacts of fluid scenes [first, first + n) -> edits of editor scenes [dst, dst + n).
+int editor_erode_impl(Editor* e, Fluid* f, uint32_t first, uint32_t n, uint32_t dst, double min_speed, double radius,
+                      double blend, hipStream_t s) {
   if (!(radius > 0)) return fail(CLAY_ERR_INVALID, "clay_editor_erode: radius must be positive");
+  if (first + n > f->scenes || dst + n > editor_scenes(e))
+    return fail(CLAY_ERR_INVALID, "clay_editor_erode: scene out of range");
   if (f->graph_stream && f->graph_stream != s) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
   int rc = editor_stage(e, s);
   if (rc) return rc;
-  rc = launch_contacts_to_edits(e->ctx, s, e, f->contacts + (size_t)scene * (4 + 4 * CLAY_CONTACTS), min_speed,
-                                radius, blend);
+  rc = launch_contacts_to_edits(e->ctx, s, e, f->contacts + (size_t)first * kContactWords, kContactWords, n, dst,
+                                min_speed, radius, blend);
   if (rc) return rc;
   return editor_flush_device(e, s, false);
 }
@@ -1610,7 +1638,6 @@ int editor_erode_impl(Editor* e, Fluid* f, uint32_t scene, double min_speed, dou
 extern "C" int clay_editor_create(clay_volume* vol, const clay_editor_desc* desc, clay_editor** out) {
   Volume* v = reinterpret_cast<Volume*>(vol);
   if (!v || !out) return fail(CLAY_ERR_INVALID, "clay_editor_create: NULL argument");
-  if (v->instances != 1) return fail(CLAY_ERR_INVALID, "SdfEditor: the volume holds several instances");
   Editor* e = new (std::nothrow) Editor();
   if (!e) return fail(CLAY_ERR_INVALID, "out of host memory");
   e->ctx = v->ctx;
@@ -1620,18 +1647,21 @@ extern "C" int clay_editor_create(clay_volume* vol, const clay_editor_desc* desc
     if (desc->tile_capacity > 0) e->capacity = desc->tile_capacity;
     if (desc->safety > 0) e->safety = desc->safety;
   }
+  const size_t S = (size_t)v->instances;
+  e->pending.assign(S, {});
   const size_t total = (size_t)v->info.total_cells;
   const size_t t0 = (size_t)v->info.tile_res[0] * v->info.tile_res[0] * v->info.tile_res[0];
   int rc;
-  if ((rc = dalloc(&e->edits_dev, (size_t)e->max_edits)) || (rc = dalloc(&e->nedit_dev, 1)) ||
-      (rc = dalloc(&e->counts, total)) || (rc = dalloc(&e->ids, t0 * (size_t)e->capacity)) ||
-      (rc = dalloc(&e->list, total)) || (rc = dalloc(&e->args, 4 * (size_t)v->info.mip_levels))) {
+  if ((rc = dalloc(&e->edits_dev, S * (size_t)e->max_edits)) || (rc = dalloc(&e->nedit_dev, S)) ||
+      (rc = dalloc(&e->counts, S * total)) || (rc = dalloc(&e->ids, S * t0 * (size_t)e->capacity)) ||
+      (rc = dalloc(&e->list, S * total)) || (rc = dalloc(&e->args, S * 4 * (size_t)v->info.mip_levels))) {
     editor_free(e);
     return rc;
   }
   hipStream_t s = e->ctx->stream;
-  if ((rc = launch_fill_u32(e->ctx, s, e->counts, total, 0u)) || (rc = launch_fill_u32(e->ctx, s, e->nedit_dev, 1, 0u)) ||
-      (rc = launch_fill_u32(e->ctx, s, e->args, 4 * (size_t)v->info.mip_levels, 0u))) {
+  if ((rc = launch_fill_u32(e->ctx, s, e->counts, S * total, 0u)) ||
+      (rc = launch_fill_u32(e->ctx, s, e->nedit_dev, S, 0u)) ||
+      (rc = launch_fill_u32(e->ctx, s, e->args, S * 4 * (size_t)v->info.mip_levels, 0u))) {
     editor_free(e);
     return rc;
   }
@@ -1650,16 +1680,19 @@ extern "C" int clay_editor_destroy(clay_editor* eh) {
   return CLAY_OK;
 }
 
-extern "C" int clay_editor_push(clay_editor* eh, const clay_brush* edits, int32_t n, int32_t* accepted) {
+extern "C" int clay_editor_push_scene(clay_editor* eh, uint32_t scene, const clay_brush* edits, int32_t n,
+                                      int32_t* accepted) {
   Editor* e = reinterpret_cast<Editor*>(eh);
   if (!e || (n > 0 && !edits)) return fail(CLAY_ERR_INVALID, "clay_editor_push: NULL argument");
+  if (scene >= editor_scenes(e)) return fail(CLAY_ERR_INVALID, "clay_editor_push: scene out of range");
+  auto& q = e->pending[scene];
   int32_t k = 0;
   for (int32_t i = 0; i < n; i++) {
     if (edits[i].kind > 6)
       return fail(CLAY_ERR_INVALID, "SdfEditor: brush kind " + std::to_string(edits[i].kind) +
                                         " is outside the supported set (sphere, box, roundBox, capsule, torus, cylinder, plane)");
-    if ((int)e->pending.size() >= e->max_edits) break;  // modify.ts:203-209 drops the rest
-    e->pending.push_back(edits[i]);
+    if ((int)q.size() >= e->max_edits) break;  // modify.ts:203-209 drops the rest
+    q.push_back(edits[i]);
     k++;
   }
   if (accepted) *accepted = k;
@@ -1668,9 +1701,13 @@ extern "C" int clay_editor_push(clay_editor* eh, const clay_brush* edits, int32_
   return CLAY_OK;
 }
 
+extern "C" int clay_editor_push(clay_editor* eh, const clay_brush* edits, int32_t n, int32_t* accepted) {
+  return clay_editor_push_scene(eh, 0, edits, n, accepted);
+}
+
 extern "C" int clay_editor_pending(const clay_editor* eh, int32_t* n) {
   if (!eh || !n) return fail(CLAY_ERR_INVALID, "clay_editor_pending: NULL argument");
-  *n = (int32_t)reinterpret_cast<const Editor*>(eh)->pending.size();
+  *n = (int32_t)editor_pending(reinterpret_cast<const Editor*>(eh));
   return CLAY_OK;
 }
 
@@ -1678,7 +1715,7 @@ extern "C" int clay_editor_flush(clay_editor* eh, clay_stream stream, int32_t* f
   Editor* e = reinterpret_cast<Editor*>(eh);
   if (!e) return fail(CLAY_ERR_INVALID, "clay_editor_flush: NULL editor");
   if (flushed) *flushed = 0;
-  if (e->pending.empty()) return CLAY_OK;  // modify.ts:215-217
+  if (editor_pending(e) == 0) return CLAY_OK;  // modify.ts:215-217
   hipStream_t s = pick(e->ctx, stream);
   int rc = editor_stage(e, s);
   if (rc) return rc;
@@ -1692,7 +1729,7 @@ extern "C" int clay_editor_erode(clay_editor* eh, clay_fluid* fh, double min_spe
   Editor* e = reinterpret_cast<Editor*>(eh);
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!e || !f) return fail(CLAY_ERR_INVALID, "clay_editor_erode: NULL argument");
-  return editor_erode_impl(e, f, 0, min_speed, radius, blend, pick(e->ctx, stream));
+  return editor_erode_impl(e, f, 0, 1, 0, min_speed, radius, blend, pick(e->ctx, stream));
 }
 
 extern "C" int clay_editor_erode_scene(clay_editor* eh, clay_batch* bh, uint32_t scene, double min_speed,
@@ -1700,7 +1737,18 @@ extern "C" int clay_editor_erode_scene(clay_editor* eh, clay_batch* bh, uint32_t
   Editor* e = reinterpret_cast<Editor*>(eh);
   Fluid* f = reinterpret_cast<Fluid*>(bh);
   if (!e || !f) return fail(CLAY_ERR_INVALID, "clay_editor_erode_scene: NULL argument");
-  return editor_erode_impl(e, f, scene, min_speed, radius, blend, pick(e->ctx, stream));
+  const uint32_t dst = editor_scenes(e) == 1 ? 0u : scene;
+  return editor_erode_impl(e, f, scene, 1, dst, min_speed, radius, blend, pick(e->ctx, stream));
+}
+
+extern "C" int clay_editor_erode_batch(clay_editor* eh, clay_batch* bh, double min_speed, double radius,
+                                       double blend, clay_stream stream) {
+  Editor* e = reinterpret_cast<Editor*>(eh);
+  Fluid* f = reinterpret_cast<Fluid*>(bh);
+  if (!e || !f) return fail(CLAY_ERR_INVALID, "clay_editor_erode_batch: NULL argument");
+  if (editor_scenes(e) != f->scenes)
+    return fail(CLAY_ERR_INVALID, "clay_editor_erode_batch: the volume needs one instance per scene of the batch");
+  return editor_erode_impl(e, f, 0, f->scenes, 0, min_speed, radius, blend, pick(e->ctx, stream));
 }
 
 extern "C" int clay_editor_refresh_all(clay_editor* eh, clay_stream stream) {
@@ -1709,24 +1757,35 @@ extern "C" int clay_editor_refresh_all(clay_editor* eh, clay_stream stream) {
   return editor_flush_device(e, pick(e->ctx, stream), true);
 }
 
-extern "C" int clay_editor_tile_counts(clay_editor* eh, uint32_t* per_mip) {
+extern "C" int clay_editor_tile_counts_scene(clay_editor* eh, uint32_t scene, uint32_t* per_mip) {
   Editor* e = reinterpret_cast<Editor*>(eh);
   if (!e || !per_mip) return fail(CLAY_ERR_INVALID, "clay_editor_tile_counts: NULL argument");
+  if (scene >= editor_scenes(e)) return fail(CLAY_ERR_INVALID, "clay_editor_tile_counts: scene out of range");
   const int mips = e->vol->info.mip_levels;
   std::vector<uint32_t> args(4 * (size_t)mips);
   CLAY_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-  CLAY_HIP_CHECK(hipMemcpy(args.data(), e->args, sizeof(uint32_t) * args.size(), hipMemcpyDeviceToHost));
+  CLAY_HIP_CHECK(hipMemcpy(args.data(), e->args + (size_t)scene * 4 * mips, sizeof(uint32_t) * args.size(),
+                           hipMemcpyDeviceToHost));
   for (int m = 0; m < mips; m++) per_mip[m] = args[4 * (size_t)m];
   return CLAY_OK;
 }
 
-extern "C" int clay_editor_cells_download(clay_editor* eh, uint32_t* counts) {
+extern "C" int clay_editor_tile_counts(clay_editor* eh, uint32_t* per_mip) {
+  return clay_editor_tile_counts_scene(eh, 0, per_mip);
+}
+
+extern "C" int clay_editor_cells_download_scene(clay_editor* eh, uint32_t scene, uint32_t* counts) {
   Editor* e = reinterpret_cast<Editor*>(eh);
   if (!e || !counts) return fail(CLAY_ERR_INVALID, "clay_editor_cells_download: NULL argument");
+  if (scene >= editor_scenes(e)) return fail(CLAY_ERR_INVALID, "clay_editor_cells_download: scene out of range");
+  const size_t total = (size_t)e->vol->info.total_cells;
   CLAY_HIP_CHECK(hipStreamSynchronize(e->ctx->stream));
-  CLAY_HIP_CHECK(hipMemcpy(counts, e->counts, sizeof(uint32_t) * (size_t)e->vol->info.total_cells,
-                           hipMemcpyDeviceToHost));
+  CLAY_HIP_CHECK(hipMemcpy(counts, e->counts + (size_t)scene * total, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
   return CLAY_OK;
+}
+
+extern "C" int clay_editor_cells_download(clay_editor* eh, uint32_t* counts) {
+  return clay_editor_cells_download_scene(eh, 0, counts);
 }
 
 // ============================================================================
@@ -1763,4 +1822,132 @@ extern "C" int clay_body_centroids(clay_ctx* ctx, const float* const pos[3], con
   dfree(out);
   if (e != hipSuccess) return fail(CLAY_ERR_HIP, std::string("clay_body_centroids: ") + hipGetErrorString(e));
   return rc;
+}
+
+// ============================================================================
+// Scenes: volume instances, splats over scene-partitioned clouds, async centroids
+// ============================================================================
+extern "C" int clay_volume_create_instances(clay_ctx* ctx, const clay_volume_desc* desc, uint32_t instances,
+                                            clay_volume** out) {
+  if (!ctx || !desc || !out) return fail(CLAY_ERR_INVALID, "clay_volume_create_instances: NULL argument");
+  if (instances < 1 || instances > 256) return fail(CLAY_ERR_INVALID, "SdfVolume: instances must be in [1, 256]");
+  Volume* v = nullptr;
+  int rc = volume_create_impl(reinterpret_cast<Ctx*>(ctx), desc, &v, (int)instances);
+  if (rc) return rc;
+  *out = reinterpret_cast<clay_volume*>(v);
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_replicate(clay_volume* vol, clay_stream stream) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v) return fail(CLAY_ERR_INVALID, "clay_volume_replicate: NULL volume");
+  hipStream_t s = pick(v->ctx, stream);
+  const size_t n = v->total_voxels;
+  for (int i = 1; i < v->instances; i++) {
+    CLAY_HIP_CHECK(hipMemcpyAsync(v->dist + (size_t)i * n, v->dist, n * 2, hipMemcpyDeviceToDevice, s));
+    CLAY_HIP_CHECK(hipMemcpyAsync(v->mat + (size_t)i * n, v->mat, n * 2, hipMemcpyDeviceToDevice, s));
+    CLAY_HIP_CHECK(hipMemcpyAsync(v->dev + i, v->dev, sizeof(VolDev), hipMemcpyDeviceToDevice, s));
+  }
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  return CLAY_OK;
+}
+
+extern "C" int clay_volume_instance_download(clay_volume* vol, uint32_t instance, int32_t mip, uint16_t* rgba) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || !rgba) return fail(CLAY_ERR_INVALID, "clay_volume_instance_download: NULL argument");
+  if ((int)instance >= v->instances) return fail(CLAY_ERR_INVALID, "clay_volume_instance_download: bad instance");
+  if (mip < 0 || mip >= v->info.mip_levels) return fail(CLAY_ERR_INVALID, "clay_volume_instance_download: bad mip");
+  const size_t r = (size_t)v->mip_res(mip), n = r * r * r;
+  const size_t off = (size_t)instance * v->total_voxels + v->voxel_off[mip];
+  std::vector<uint16_t> d(n), m(n);
+  CLAY_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(d.data(), v->dist + off, n * 2, hipMemcpyDeviceToHost));
+  CLAY_HIP_CHECK(hipMemcpy(m.data(), v->mat + off, n * 2, hipMemcpyDeviceToHost));
+  const uint16_t one = f2h_host(1.0f);
+  for (size_t i = 0; i < n; i++) {
+    rgba[4 * i] = d[i];
+    rgba[4 * i + 1] = m[i];
+    rgba[4 * i + 2] = 0;
+    rgba[4 * i + 3] = one;
+  }
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_create_scenes(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                                        uint32_t scenes, const uint32_t* counts, int32_t follow,
+                                        const double* centers_dev, clay_splat** out) {
+  if (!ctx || !cloud || !desc || !out || !counts) return fail(CLAY_ERR_INVALID, "clay_splat_create_scenes: NULL argument");
+  if (scenes < 1 || scenes > 256) return fail(CLAY_ERR_INVALID, "SplatField: scenes must be in [1, 256]");
+  if (follow != CLAY_FOLLOW_NONE && follow != CLAY_FOLLOW_BOUNDS && follow != CLAY_FOLLOW_CENTERS)
+    return fail(CLAY_ERR_INVALID, "SplatField: unknown follow mode");
+  if (follow == CLAY_FOLLOW_CENTERS && !centers_dev)
+    return fail(CLAY_ERR_INVALID, "SplatField: following centres needs the device centre array");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  std::vector<uint32_t> base(scenes + 1, 0);
+  for (uint32_t i = 0; i < scenes; i++) base[i + 1] = base[i] + counts[i];
+  const uint32_t total = base[scenes];
+  if (total > cloud->capacity) return fail(CLAY_ERR_INVALID, "SplatField: the scenes hold more particles than the cloud");
+  std::vector<uint32_t> so(std::max<uint32_t>(total, 1), 0);
+  for (uint32_t i = 0; i < scenes; i++)
+    for (uint32_t j = base[i]; j < base[i + 1]; j++) so[j] = i;
+  uint32_t *dso = nullptr, *dbase = nullptr;
+  int rc;
+  if ((rc = dalloc(&dso, so.size())) || (rc = dalloc(&dbase, base.size()))) {
+    dfree(dso);
+    return rc;
+  }
+  if (hipMemcpy(dso, so.data(), sizeof(uint32_t) * so.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dbase, base.data(), sizeof(uint32_t) * base.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    dfree(dso);
+    dfree(dbase);
+    return fail(CLAY_ERR_HIP, "SplatField: scene table upload failed");
+  }
+  clay_cloud_desc cl = *cloud;
+  cl.live_rule = CLAY_LIVE_PREFIX;
+  cl.live_count = total;
+  SplatScenes sc;
+  sc.scenes = (int)scenes;
+  sc.scene_of = scenes > 1 ? dso : nullptr;
+  sc.scene_base = dbase;
+  sc.follow = follow != CLAY_FOLLOW_NONE;
+  Splat* sp = nullptr;
+  rc = splat_create_impl(c, &cl, desc, &sp, sc);
+  if (rc) {
+    dfree(dso);
+    dfree(dbase);
+    return rc;
+  }
+  sp->own_scene_of = dso;
+  sp->own_scene_base = dbase;
+  sp->follow_centers = follow == CLAY_FOLLOW_CENTERS ? centers_dev : nullptr;
+  *out = reinterpret_cast<clay_splat*>(sp);
+  return CLAY_OK;
+}
+
+extern "C" int clay_splat_instance_download(clay_splat* s, uint32_t scene, int32_t mip, uint32_t* codes_host) {
+  Splat* sp = reinterpret_cast<Splat*>(s);
+  if (!sp || !codes_host) return fail(CLAY_ERR_INVALID, "clay_splat_instance_download: NULL argument");
+  if ((int)scene >= sp->scenes) return fail(CLAY_ERR_INVALID, "clay_splat_instance_download: bad scene");
+  if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_splat_instance_download: bad mip");
+  const size_t r = (size_t)sp->vol->mip_res(mip), n = r * r * r;
+  CLAY_HIP_CHECK(hipStreamSynchronize(sp->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(codes_host, sp->grid + (size_t)scene * sp->grid_total + sp->grid_off[mip], n * 4,
+                           hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_body_centroids_device(clay_ctx* ctx, const float* const pos[3], const float* const rest[3],
+                                          const uint32_t* first_count_dev, uint32_t nbodies, int32_t mode,
+                                          double* out6_dev, clay_stream stream) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !pos || !pos[0] || !pos[1] || !pos[2] || (nbodies && (!first_count_dev || !out6_dev)))
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids_device: NULL argument");
+  if (rest && (!rest[0] || !rest[1] || !rest[2]))
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids_device: rest must give all three axes or be NULL");
+  if (mode != CLAY_REDUCE_F32_TREE && mode != CLAY_REDUCE_F64)
+    return fail(CLAY_ERR_INVALID, "clay_body_centroids_device: unknown reduction mode");
+  const float* p3[3] = {pos[0], pos[1], pos[2]};
+  const float* r3[3] = {rest ? rest[0] : nullptr, rest ? rest[1] : nullptr, rest ? rest[2] : nullptr};
+  return launch_body_centroids(c, pick(c, stream), mode == CLAY_REDUCE_F64, nbodies, first_count_dev, p3,
+                               rest ? r3 : nullptr, out6_dev);
 }

@@ -69,7 +69,9 @@ struct SceneK {
   uint32_t batched;
   uint32_t cells_per_scene;
   uint32_t contact_stride;
-  uint32_t xcd;  // renumber workgroups XCD-contiguously (a launch-shape choice)
+  uint32_t xcd;     // renumber workgroups XCD-contiguously (a launch-shape choice)
+  uint32_t* cmask;  // one bit per particle id: in contact this substep
+  float4* cdata;    // per particle id: the contact point and speed
 };
 
 // Workgroups are dispatched round-robin over the chip's 8 XCDs (workgroup b runs

@@ -123,7 +123,10 @@ struct Splat {
   int scenes = 1;
   const uint32_t* scene_of = nullptr;    // device, per particle id (not owned)
   const uint32_t* scene_base = nullptr;  // device, scenes + 1 prefix offsets (not owned)
-  bool follow = false;                   // centre each volume on its scene's AABB before a bake
+  uint32_t* own_scene_of = nullptr;      // the two tables above when the splat owns them
+  uint32_t* own_scene_base = nullptr;
+  bool follow = false;                   // centre each volume before a bake:
+  const double* follow_centers = nullptr;  // on these (6 doubles per scene, xyz first), else on the AABB
   // tuning
   uint32_t chunk = 1024;                 // particles per splat work item
   int persistent_blocks = 2048;          // grid of the splat work-item kernel
@@ -168,6 +171,8 @@ struct Fluid {
   size_t grid_cells = 0;
   int key_bits = 0;
   uint32_t* contacts = nullptr;  // per scene: [0] = count, then 32 x float4 (as bits)
+  uint32_t* cmask = nullptr;     // one bit per particle id: in contact this substep
+  float4* cdata = nullptr;       // per particle id: contact point and speed
   double* reduce = nullptr;      // FP64 reduction partials (+ 8 results at the end)
   uint32_t reduce_blocks = 0;
   Splat* surface = nullptr;      // owned
@@ -189,33 +194,37 @@ struct Fluid {
     sk.batched = scene_of != nullptr;
     sk.cells_per_scene = (uint32_t)grid_cells;
     sk.contact_stride = 4 + 4 * CLAY_CONTACTS;
+    sk.cmask = cmask;
+    sk.cdata = cdata;
     return sk;
   }
 };
 
-// SdfEditor over one volume: its TileGrid (counts, ids, list, per-mip counters)
+// SdfEditor over a volume: its TileGrid (counts, ids, list, per-mip counters)
 // and the device edit list, whose count lives on the device so that edits made
-// by kernels (erosion) and by the host share one flush.
+// by kernels (erosion) and by the host share one flush.  A volume with one
+// instance per scene gets one of each per scene, scene-major.
 struct Editor {
   Ctx* ctx = nullptr;
   Volume* vol = nullptr;         // not owned
   int max_edits = 64;
   int capacity = 8;
   double safety = 0.98;
-  std::vector<clay_brush> pending;
-  clay_brush* edits_dev = nullptr;
-  uint32_t* nedit_dev = nullptr;
-  uint32_t* counts = nullptr;    // every mip, at the volume's cell offsets
-  uint32_t* ids = nullptr;       // mip-0 tiles x capacity
-  uint32_t* list = nullptr;      // every mip, at the volume's cell offsets
-  uint32_t* args = nullptr;      // per mip: live count, 1, 1, 0
+  std::vector<std::vector<clay_brush>> pending;  // per scene
+  std::vector<uint32_t> staged_counts;           // host copy source of the per-scene counts
+  clay_brush* edits_dev = nullptr;   // scenes x max_edits
+  uint32_t* nedit_dev = nullptr;     // scenes
+  uint32_t* counts = nullptr;    // per scene: every mip, at the volume's cell offsets
+  uint32_t* ids = nullptr;       // per scene: mip-0 tiles x capacity
+  uint32_t* list = nullptr;      // per scene: every mip, at the volume's cell offsets
+  uint32_t* args = nullptr;      // per scene and mip: live count, 1, 1, 0
 };
 
 // ---- launchers (implemented in the .hip files) ------------------------------
 // edit.hip
 int editor_flush_device(Editor* e, hipStream_t s, bool all_tiles);
-int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* contacts, double min_speed,
-                             double radius, double blend);
+int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* contacts, uint32_t contact_stride,
+                             uint32_t scenes, uint32_t first_scene, double min_speed, double radius, double blend);
 int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, const uint32_t* first_count,
                           const float* const* pos, const float* const* rest, double* out);
 // fluid.hip
@@ -232,6 +241,8 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK
                             const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
                             float* const* out, uint32_t* contacts);
 int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch);
+int launch_contacts_collect(Ctx* c, hipStream_t s, uint32_t* contacts, uint32_t stride, const uint32_t* cmask,
+                            const float4* cdata, const uint32_t* base, uint32_t n, uint32_t scenes);
 int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, double mass,
                      double* partial, uint32_t partial_blocks, double* out);

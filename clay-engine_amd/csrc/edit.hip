@@ -25,6 +25,10 @@
 // Body centroids (pbd.ts:240-266): one 256-lane workgroup per body sums its particles
 // lane-strided, then a pairwise tree; in f32 this is the reference's reduction, in f64
 // the same order at double precision.
+//
+// A volume with one instance per scene (a batch's worlds) gets one editor state per
+// scene: every kernel takes its scene from blockIdx.y and offsets the edit list, tile
+// counts, list, counters and planes by it, so all scenes flush in the same launches.
 #include "clay_internal.h"
 
 namespace clay {
@@ -48,8 +52,13 @@ struct EditK {
   float voxel, tsize, thalf, bw;
   int32_t capacity;
   int32_t max_edits;
+  uint32_t total_cells;   // tile counts / list entries per scene
+  uint32_t tiles0;        // mip-0 tiles per scene
+  uint32_t vps;           // plane entries per scene
+  int32_t mips;
   LevelK lv[CLAY_MAX_MIPS];
 };
+
 
 __device__ __forceinline__ uint32_t pack_tile(uint32_t x, uint32_t y, uint32_t z) {
   return x | (y << 10) | (z << 20);
@@ -62,6 +71,9 @@ __device__ __forceinline__ void unpack_tile(uint32_t v, int32_t t[3]) {
 
 __global__ void k_grid_reset(uint32_t* __restrict__ counts, uint32_t total, uint32_t* __restrict__ args,
                              int mips) {
+  const uint32_t sc = blockIdx.y;
+  counts += (size_t)sc * total;
+  args += (size_t)sc * mips * 4u;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) counts[i] = 0;
   if (i < (uint32_t)mips * 4u) {
@@ -72,6 +84,11 @@ __global__ void k_grid_reset(uint32_t* __restrict__ counts, uint32_t total, uint
 
 __global__ void k_edit_bin(EditK k, const clay_brush* __restrict__ edits, const uint32_t* __restrict__ nedit,
                            uint32_t* __restrict__ counts, uint32_t* __restrict__ ids) {
+  const uint32_t sc = blockIdx.y;
+  edits += (size_t)sc * k.max_edits;
+  nedit += sc;
+  counts += (size_t)sc * k.total_cells;
+  ids += (size_t)sc * k.tiles0 * k.capacity;
   const LevelK& l = k.lv[0];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint32_t)(l.tr * l.tr * l.tr)) return;
@@ -95,6 +112,10 @@ __global__ void k_edit_bin(EditK k, const clay_brush* __restrict__ edits, const 
 
 __global__ void k_compact(EditK k, int m, const uint32_t* __restrict__ counts, uint32_t* __restrict__ list,
                           uint32_t* __restrict__ args) {
+  const uint32_t sc = blockIdx.y;
+  counts += (size_t)sc * k.total_cells;
+  list += (size_t)sc * k.total_cells;
+  args += (size_t)sc * k.mips * 4u;
   const LevelK& l = k.lv[m];
   const uint32_t cells = (uint32_t)(l.tr * l.tr * l.tr);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,6 +126,7 @@ __global__ void k_compact(EditK k, int m, const uint32_t* __restrict__ counts, u
 }
 
 __global__ void k_dilate(EditK k, int m, uint32_t* __restrict__ counts) {
+  counts += (size_t)blockIdx.y * k.total_cells;
   const LevelK& l = k.lv[m];
   const LevelK& p = k.lv[m - 1];
   const uint32_t cells = (uint32_t)(l.tr * l.tr * l.tr);
@@ -131,6 +153,14 @@ __global__ void __launch_bounds__(kThreads)
     k_edit_apply(EditK k, const clay_brush* __restrict__ edits, const uint32_t* __restrict__ counts,
                  const uint32_t* __restrict__ ids, const uint32_t* __restrict__ list,
                  const uint32_t* __restrict__ args, uint16_t* __restrict__ dist, uint16_t* __restrict__ mat) {
+  const uint32_t sc = blockIdx.y;
+  edits += (size_t)sc * k.max_edits;
+  counts += (size_t)sc * k.total_cells;
+  ids += (size_t)sc * k.tiles0 * k.capacity;
+  list += (size_t)sc * k.total_cells;
+  args += (size_t)sc * k.mips * 4u;
+  dist += (size_t)sc * k.vps;
+  mat += (size_t)sc * k.vps;
   const LevelK& l = k.lv[0];
   const uint32_t live = args[0];
   const int t = threadIdx.x;
@@ -173,6 +203,11 @@ __global__ void __launch_bounds__(kThreads)
     k_mip_refresh(EditK k, int m, float step, const uint32_t* __restrict__ list, const uint32_t* __restrict__ args,
                   uint16_t* __restrict__ dist, uint16_t* __restrict__ mat) {
   __shared__ float g[2][kCells];
+  const uint32_t sc = blockIdx.y;
+  list += (size_t)sc * k.total_cells;
+  args += (size_t)sc * k.mips * 4u;
+  dist += (size_t)sc * k.vps;
+  mat += (size_t)sc * k.vps;
   const LevelK& l = k.lv[m];
   const LevelK& f = k.lv[m - 1];
   const uint32_t live = args[m * 4];
@@ -232,10 +267,14 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Contacts -> cut spheres appended to the device edit list (at most max_edits).
-__global__ void k_contacts_to_edits(const uint32_t* __restrict__ contacts, float min_speed, float radius,
-                                    float blend, float bound, uint32_t max_edits, clay_brush* __restrict__ edits,
-                                    uint32_t* __restrict__ nedit) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_contacts_to_edits(const uint32_t* __restrict__ contacts, uint32_t contact_stride,
+                                    float min_speed, float radius, float blend, float bound, uint32_t max_edits,
+                                    clay_brush* __restrict__ edits, uint32_t* __restrict__ nedit) {
+  if (threadIdx.x != 0) return;
+  const uint32_t sc = blockIdx.x;
+  contacts += (size_t)sc * contact_stride;
+  edits += (size_t)sc * max_edits;
+  nedit += sc;
   const uint32_t raw = contacts[0];
   const uint32_t n = raw < (uint32_t)CLAY_CONTACTS ? raw : (uint32_t)CLAY_CONTACTS;
   const float4* pts = reinterpret_cast<const float4*>(contacts + 4);
@@ -310,6 +349,10 @@ EditK edit_k(const Editor* e) {
   k.bw = (float)(v->info.band * voxel);
   k.capacity = e->capacity;
   k.max_edits = e->max_edits;
+  k.total_cells = (uint32_t)v->info.total_cells;
+  k.tiles0 = (uint32_t)v->info.tile_res[0] * v->info.tile_res[0] * v->info.tile_res[0];
+  k.vps = (uint32_t)v->total_voxels;
+  k.mips = v->info.mip_levels;
   for (int m = 0; m < v->info.mip_levels; m++) {
     k.lv[m].r = v->mip_res(m);
     k.lv[m].tr = v->info.tile_res[m];
@@ -327,31 +370,33 @@ int editor_flush_device(Editor* e, hipStream_t s, bool all_tiles) {
   const EditK k = edit_k(e);
   const int mips = v->info.mip_levels;
   const uint32_t total = (uint32_t)v->info.total_cells;
+  const unsigned S = (unsigned)v->instances;
   {
     LaunchTimer t(c, s, "edit_grid_reset");
-    hipLaunchKernelGGL(k_grid_reset, dim3(groups(std::max<uint32_t>(total, (uint32_t)mips * 4), 64)), dim3(64), 0,
+    hipLaunchKernelGGL(k_grid_reset, dim3(groups(std::max<uint32_t>(total, (uint32_t)mips * 4), 64), S), dim3(64), 0,
                        s, e->counts, total, e->args, mips);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   const uint32_t tiles0 = (uint32_t)(k.lv[0].tr * k.lv[0].tr * k.lv[0].tr);
   if (all_tiles) {
-    int rc = launch_fill_u32(c, s, e->counts, total, 1u);
+    int rc = launch_fill_u32(c, s, e->counts, (size_t)total * S, 1u);
     if (rc) return rc;
   } else {
     LaunchTimer t(c, s, "edit_bin");
-    hipLaunchKernelGGL(k_edit_bin, dim3(groups(tiles0, 64)), dim3(64), 0, s, k, e->edits_dev, e->nedit_dev,
+    hipLaunchKernelGGL(k_edit_bin, dim3(groups(tiles0, 64), S), dim3(64), 0, s, k, e->edits_dev, e->nedit_dev,
                        e->counts, e->ids);
     CLAY_HIP_CHECK(hipGetLastError());
   }
-  const unsigned tile_groups = (unsigned)std::min<uint32_t>(tiles0, (uint32_t)c->persistent_blocks);
+  const unsigned tile_groups =
+      (unsigned)std::max<uint32_t>(1, std::min<uint32_t>(tiles0, (uint32_t)c->persistent_blocks / S));
   {
     LaunchTimer t(c, s, "edit_compact");
-    hipLaunchKernelGGL(k_compact, dim3(groups(tiles0, 64)), dim3(64), 0, s, k, 0, e->counts, e->list, e->args);
+    hipLaunchKernelGGL(k_compact, dim3(groups(tiles0, 64), S), dim3(64), 0, s, k, 0, e->counts, e->list, e->args);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   if (!all_tiles) {
     LaunchTimer t(c, s, "edit_apply");
-    hipLaunchKernelGGL(k_edit_apply, dim3(tile_groups), dim3(kThreads), 0, s, k, e->edits_dev, e->counts, e->ids,
+    hipLaunchKernelGGL(k_edit_apply, dim3(tile_groups, S), dim3(kThreads), 0, s, k, e->edits_dev, e->counts, e->ids,
                        e->list, e->args, v->dist, v->mat);
     CLAY_HIP_CHECK(hipGetLastError());
   }
@@ -360,18 +405,18 @@ int editor_flush_device(Editor* e, hipStream_t s, bool all_tiles) {
     const uint32_t cells = (uint32_t)(k.lv[m].tr * k.lv[m].tr * k.lv[m].tr);
     if (!all_tiles) {
       LaunchTimer t(c, s, "edit_dilate");
-      hipLaunchKernelGGL(k_dilate, dim3(groups(cells, 64)), dim3(64), 0, s, k, m, e->counts);
+      hipLaunchKernelGGL(k_dilate, dim3(groups(cells, 64), S), dim3(64), 0, s, k, m, e->counts);
       CLAY_HIP_CHECK(hipGetLastError());
     }
     {
       LaunchTimer t(c, s, "edit_compact");
-      hipLaunchKernelGGL(k_compact, dim3(groups(cells, 64)), dim3(64), 0, s, k, m, e->counts, e->list, e->args);
+      hipLaunchKernelGGL(k_compact, dim3(groups(cells, 64), S), dim3(64), 0, s, k, m, e->counts, e->list, e->args);
       CLAY_HIP_CHECK(hipGetLastError());
     }
     {
       LaunchTimer t(c, s, "mip_refresh");
-      const unsigned g = (unsigned)std::min<uint32_t>(cells, (uint32_t)c->persistent_blocks);
-      hipLaunchKernelGGL(k_mip_refresh, dim3(g), dim3(kThreads), 0, s, k, m, step, e->list, e->args, v->dist,
+      const unsigned g = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>(cells, (uint32_t)c->persistent_blocks / S));
+      hipLaunchKernelGGL(k_mip_refresh, dim3(g, S), dim3(kThreads), 0, s, k, m, step, e->list, e->args, v->dist,
                          v->mat);
       CLAY_HIP_CHECK(hipGetLastError());
     }
@@ -379,11 +424,12 @@ int editor_flush_device(Editor* e, hipStream_t s, bool all_tiles) {
   return CLAY_OK;
 }
 
-int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* contacts, double min_speed,
-                             double radius, double blend) {
+int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* contacts, uint32_t contact_stride,
+                             uint32_t scenes, uint32_t first_scene, double min_speed, double radius, double blend) {
   LaunchTimer t(c, s, "edit_from_contacts");
-  hipLaunchKernelGGL(k_contacts_to_edits, dim3(1), dim3(64), 0, s, contacts, (float)min_speed, (float)radius,
-                     (float)blend, (float)(radius + blend), (uint32_t)e->max_edits, e->edits_dev, e->nedit_dev);
+  hipLaunchKernelGGL(k_contacts_to_edits, dim3(scenes), dim3(64), 0, s, contacts, contact_stride, (float)min_speed,
+                     (float)radius, (float)blend, (float)(radius + blend), (uint32_t)e->max_edits,
+                     e->edits_dev + (size_t)first_scene * e->max_edits, e->nedit_dev + first_scene);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

@@ -190,10 +190,11 @@ struct StateOut {
 
 // The end of the force pass (fluid.ts:430-437) and the integrate pass
 // (fluid.ts:449-491) for one particle, writing its id-ordered state.
-__device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderDev& col, uint32_t scene,
-                                                uint32_t s, f3 pi, f3 vi, float rhoi, float pri, f3 fp,
-                                                f3 vsum, float wsum, const uint32_t* __restrict__ svals,
+__device__ __forceinline__ void finish_particle(const FluidK& k, const SceneK& sk, const ColliderDev& col,
+                                                uint32_t scene, uint32_t s, f3 pi, f3 vi, float rhoi, float pri,
+                                                f3 fp, f3 vsum, float wsum, const uint32_t* __restrict__ svals,
                                                 const StateOut& out, uint32_t* __restrict__ contacts) {
+  const uint32_t j = svals[s];
   const float irho = 1.0f / rhoi;
   const f3 acc = mk(fp.x * irho + k.g[0], fp.y * irho + k.g[1], fp.z * irho + k.g[2]);
   const float tv = fclamp_s(k.visc, 0.0f, 1.0f);
@@ -217,18 +218,17 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const ColliderD
       const float vn = fmin_s(dot3(v, nr), 0.0f);
       v = mk(v.x - nr.x * vn, v.y - nr.y * vn, v.z - nr.z * vn);
       if (dw < k.cdepth) {
-        const uint32_t slot = atomicAdd(contacts, 1u);
-        if (slot < CLAY_CONTACTS) {
-          float4* pts = reinterpret_cast<float4*>(contacts + 4);
-          pts[slot] = make_float4(p.x, p.y, p.z, len3(v));
-        }
+        // The count is the reference's atomic counter; which 32 contacts the list
+        // keeps is decided afterwards, by particle id (k_contacts_collect).
+        atomicAdd(contacts, 1u);
+        sk.cdata[j] = make_float4(p.x, p.y, p.z, len3(v));
+        atomicOr(&sk.cmask[j >> 5], 1u << (j & 31u));
       }
     }
   }
   const float clx = fmin_s(fmax_s(p.x, k.lo[0]), k.hi[0]);
   const float cly = fmin_s(fmax_s(p.y, k.lo[1]), k.hi[1]);
   const float clz = fmin_s(fmax_s(p.z, k.lo[2]), k.hi[2]);
-  const uint32_t j = svals[s];
   out.p[0][j] = clx;
   out.p[1][j] = cly;
   out.p[2][j] = clz;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kB)
       fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
     }
   });
-  finish_particle(k, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
+  finish_particle(k, sk, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
                   contacts + (size_t)scene * sk.contact_stride);
 }
 
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(kB)
     }
   }
   if (!active) return;
-  finish_particle(k, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
+  finish_particle(k, sk, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
                   contacts + (size_t)scene * sk.contact_stride);
 }
 
@@ -669,6 +669,55 @@ __global__ void k_bump(uint32_t* epoch) {
   if (threadIdx.x == 0 && blockIdx.x == 0) epoch[0] = epoch[0] + 1u;
 }
 
+// The contact list (fluid.ts:54-58, 470-477) of each scene: its first 32 contacts
+// by particle id, gathered from the per-id flags the integrate pass set.  The
+// reference keeps whichever 32 its atomic counter admitted first; ascending id is
+// one admissible order, it does not depend on scheduling, and it is the order the
+// oracle appends in.  One workgroup per scene scans the scene's flag words.
+__global__ void __launch_bounds__(kBlock)
+    k_contacts_collect(uint32_t* __restrict__ contacts, uint32_t stride, const uint32_t* __restrict__ cmask,
+                       const float4* __restrict__ cdata, const uint32_t* __restrict__ base, uint32_t n) {
+  __shared__ uint32_t sh[kBlock];
+  __shared__ uint32_t found;
+  const uint32_t sc = blockIdx.x;
+  const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
+  float4* pts = reinterpret_cast<float4*>(contacts + (size_t)sc * stride + 4);
+  const int t = threadIdx.x;
+  if (t == 0) found = 0;
+  __syncthreads();
+  if (lo >= hi) return;
+  const uint32_t w0 = lo >> 5, w1 = (hi + 31u) >> 5;
+  for (uint32_t wb = w0; wb < w1; wb += kBlock) {
+    const uint32_t w = wb + (uint32_t)t;
+    uint32_t bits = 0;
+    if (w < w1) {
+      bits = cmask[w];
+      if (w == w0) bits &= ~0u << (lo & 31u);
+      const uint32_t last = hi - (w << 5);  // ids of this word below hi
+      if (last < 32u) bits &= (1u << last) - 1u;
+    }
+    const uint32_t cnt = (uint32_t)__popc(bits);
+    sh[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      const uint32_t add = t >= off ? sh[t - off] : 0u;
+      __syncthreads();
+      sh[t] += add;
+      __syncthreads();
+    }
+    uint32_t r = found + sh[t] - cnt;
+    while (bits && r < (uint32_t)CLAY_CONTACTS) {
+      const uint32_t b = (uint32_t)__ffs(bits) - 1u;
+      pts[r++] = cdata[(w << 5) + b];
+      bits &= bits - 1u;
+    }
+    __syncthreads();
+    if (t == kBlock - 1) found += sh[t];
+    __syncthreads();
+    if (found >= (uint32_t)CLAY_CONTACTS) break;
+  }
+}
+
 inline unsigned grid_for(uint32_t n, int block = kBlock) { return (n + block - 1) / block; }
 
 template <int kB>
@@ -772,6 +821,14 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK
     launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
   else
     launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int launch_contacts_collect(Ctx* c, hipStream_t s, uint32_t* contacts, uint32_t stride, const uint32_t* cmask,
+                            const float4* cdata, const uint32_t* base, uint32_t n, uint32_t scenes) {
+  LaunchTimer t(c, s, "fluid_contacts");
+  hipLaunchKernelGGL(k_contacts_collect, dim3(scenes), dim3(kBlock), 0, s, contacts, stride, cmask, cdata, base, n);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

@@ -671,17 +671,29 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Follows each scene's particles: origin = centre of their AABB - worldSize / 2,
-// as SplatField.setCenter followed by SdfVolume.setOrigin computes it
-// (splat.ts:284-287, volume.ts:327-334: f64 arithmetic, stored as f32).  The
-// centre is the f32 midpoint of the f32 bounds.  A scene with no particles keeps
-// its origin.  One workgroup per scene; `base` holds scenes + 1 prefix offsets,
-// or is null for one scene of `n` particles.
+// Follows each scene: origin = centre - worldSize / 2, as SplatField.setCenter
+// followed by SdfVolume.setOrigin computes it (splat.ts:284-287, volume.ts:327-334:
+// f64 arithmetic, stored as f32).  The centre is `centers` (6 doubles per scene,
+// xyz first: a body centroid) when given, else the f32 midpoint of the f32 bounds
+// of the scene's particles.  A scene with no particles keeps its origin.  One
+// workgroup per scene; `base` holds scenes + 1 prefix offsets, or is null for one
+// scene of `n` particles.
 __global__ void __launch_bounds__(kThreads)
     k_follow(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-             const uint32_t* __restrict__ base, uint32_t n, VolDev* __restrict__ vols, double half) {
+             const uint32_t* __restrict__ base, uint32_t n, const double* __restrict__ centers,
+             VolDev* __restrict__ vols, double half) {
   __shared__ float sh[6][kThreads];
   const uint32_t sc = blockIdx.x;
+  if (centers) {
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < 3; q++) {
+        const double o = centers[6 * sc + q] - half;
+        vols[sc].origin[q] = (float)o;
+        vols[sc].center[q] = (float)(o + half);
+      }
+    }
+    return;
+  }
   const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
   if (lo >= hi) return;
   const int t = threadIdx.x;
@@ -847,7 +859,7 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
   if (sp->follow) {
     LaunchTimer t(ctx, s, "splat_follow");
     hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
-                       sp->scene_base, c.end, sp->vol->dev, sp->vol->info.world_size / 2);
+                       sp->scene_base, c.end, sp->follow_centers, sp->vol->dev, sp->vol->info.world_size / 2);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   if (sp->desc.mode == CLAY_SPLAT_PLAIN) {

@@ -336,12 +336,41 @@ int clay_editor_erode(clay_editor* e, clay_fluid* fluid, double min_speed, doubl
 /* The same from one scene of a batch. */
 int clay_editor_erode_scene(clay_editor* e, clay_batch* batch, uint32_t scene, double min_speed, double radius,
                             double blend, clay_stream stream);
+/* Several scenes: a volume with one instance per scene (clay_volume_create_instances)
+ * gets one queue, tile grid and edit list per scene; every scene flushes in the same
+ * launches. */
+int clay_editor_push_scene(clay_editor* e, uint32_t scene, const clay_brush* edits, int32_t n, int32_t* accepted);
+/* Every scene's contacts -> that scene's edits -> flush, for an editor with one
+ * instance per scene of `batch`. */
+int clay_editor_erode_batch(clay_editor* e, clay_batch* batch, double min_speed, double radius, double blend,
+                            clay_stream stream);
+int clay_editor_tile_counts_scene(clay_editor* e, uint32_t scene, uint32_t* per_mip);
+int clay_editor_cells_download_scene(clay_editor* e, uint32_t scene, uint32_t* counts);
 /* Rebuilds every coarser mip from mip 0 over all tiles (MipRefiner on a full grid). */
 int clay_editor_refresh_all(clay_editor* e, clay_stream stream);
 /* TileGrid.readTileCounts (tilegrid.ts:281-297): live tiles per mip of the last flush. */
 int clay_editor_tile_counts(clay_editor* e, uint32_t* per_mip);
 /* The tile counts of every mip (the volume's total_cells entries) after the last flush. */
 int clay_editor_cells_download(clay_editor* e, uint32_t* counts);
+
+/* ---- Scenes: per-scene volume instances and splats ------------------------
+ * A volume can hold one instance per scene (the same geometry, its own planes and
+ * origin record, scene-major).  clay_volume_bake_brushes and the download/upload
+ * calls address instance 0; clay_volume_replicate copies it to every instance.  A
+ * volumeField over it reads, for a particle of batch scene s, instance s.  A splat
+ * over a cloud partitioned into scenes bakes scene s's particles into instance s;
+ * each instance can follow its particles' bounds, or device-side centres (6 doubles
+ * per scene, xyz first, as clay_body_centroids_device writes them), before a bake. */
+int clay_volume_create_instances(clay_ctx* ctx, const clay_volume_desc* desc, uint32_t instances, clay_volume** out);
+int clay_volume_replicate(clay_volume* vol, clay_stream stream);
+int clay_volume_instance_download(clay_volume* vol, uint32_t instance, int32_t mip, uint16_t* rgba);
+#define CLAY_FOLLOW_NONE 0
+#define CLAY_FOLLOW_BOUNDS 1
+#define CLAY_FOLLOW_CENTERS 2
+int clay_splat_create_scenes(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                             uint32_t scenes, const uint32_t* counts, int32_t follow, const double* centers_dev,
+                             clay_splat** out);
+int clay_splat_instance_download(clay_splat* s, uint32_t scene, int32_t mip, uint32_t* codes_host);
 
 /* ---- Body centroids (ClaySolver.match, lib/src/sim/pbd.ts:240-266) -------
  * Per body [first, first + count) of a particle set: com = sum(pos) / count and
@@ -354,6 +383,11 @@ int clay_editor_cells_download(clay_editor* e, uint32_t* counts);
 int clay_body_centroids(clay_ctx* ctx, const float* const pos_dev[3], const float* const rest_dev[3],
                         const uint32_t* first_count, uint32_t nbodies, int32_t mode, double* out6,
                         clay_stream stream);
+/* The same, all on the device (table and result in device memory), enqueued only:
+ * what a captured frame calls before a centre-following bake. */
+int clay_body_centroids_device(clay_ctx* ctx, const float* const pos_dev[3], const float* const rest_dev[3],
+                               const uint32_t* first_count_dev, uint32_t nbodies, int32_t mode, double* out6_dev,
+                               clay_stream stream);
 
 /* Launch shapes of the neighbour and splat kernels.  Results do not depend on
  * them; clay-engine_amd/clayhip/autotune.py picks them per device and workload. */

@@ -78,9 +78,14 @@ def test_free_run_ten_steps_bit_exact(root, small):
     sim.destroy()
 
 
-def test_contacts_match_oracle_as_a_set(root):
-    # A slab resting just above the floor so that contacts occur.
-    pos = scenes.lattice(12, 3, 12, 0.05, (-0.3, -0.42, -0.3), seed=3)
+@pytest.mark.parametrize("layers", [1, 3])
+def test_contacts_match_oracle_in_id_order(root, layers):
+    """The list keeps the first 32 contacts by particle id (the order the oracle
+    appends in); the raw count is the reference's counter.  One layer gives fewer
+    than 32 contacts, three layers more."""
+    pos = scenes.lattice(12, layers, 12, 0.05, (-0.3, -0.42, -0.3), seed=3)
+    if layers == 1:
+        pos = pos[:20]
     scene = scenes.dam_break_small()
     scene = dict(scene, pos=pos, options=dict(scene["options"], capacity=pos.shape[0]))
     sim, orc, _ = make_pair(root, scene)
@@ -89,10 +94,8 @@ def test_contacts_match_oracle_as_a_set(root):
     got = sim.readContacts()
     raw, want = orc.read_contacts()
     assert sim.lastContactCount == raw and raw > 0
-    if raw <= 32:
-        assert sorted(got) == sorted(tuple(np.float32(v) for v in p) for p in want)
-    else:
-        assert len(got) == 32
+    assert (raw <= 32) == (layers == 1)
+    assert got == [tuple(float(np.float32(v)) for v in p) for p in want]
     sim.destroy()
 
 
