@@ -355,12 +355,51 @@ def measure_batched(root, args, stream) -> dict:
     return out
 
 
+def measure_coupled(root, args, stream) -> dict:
+    """The coupled frame (fluid + kinematic body + eroding world per scene), timed
+    with FP16-LDS staging and with the plain f32 path on the same scenes."""
+    import numpy as np
+
+    from clayhip.coupled import CoupledBatch, coupled_scene
+
+    sc = coupled_scene(scenes=args.scenes)
+    out = {"workload": sc["name"], "particles": int(sc["counts"].sum()), "scenes": args.scenes,
+           "substeps": sc["options"]["substeps"], "mips": sc["options"]["bakeMipLevels"], "scene": sc}
+    ends = {}
+    for precision in (args.precision, "f32") if args.precision != "f32" else ("f32",):
+        cb = CoupledBatch(root, sc["counts"], sc["pos"], dict(sc["options"], precision=precision), sc["world"],
+                          sc["world_brushes"])
+        ms = timed(cb.step, args.steps, args.warmup, stream)
+        cb.fluid.set_graph(False)
+        stats = kernel_stats(root, cb.step, 2, stream)
+        ends[precision] = cb.fluid.get_state("pos")
+        entry = {"ms_per_step": ms, "value": out["particles"] / (ms / 1e3),
+                 "kernels_ms_per_step": per_step(stats, 2),
+                 "eroded_tiles_mip0_last_frame": int(sum(cb.editor.readTileCounts(s)[0] for s in range(args.scenes)))}
+        if precision == args.precision:
+            out.update(entry)
+            out["live_voxels"] = int(sum(int(cb.fluid.live_tiles(s, m).sum()) * 512 for s in range(args.scenes)
+                                         for m in range(out["mips"])))
+            out["roofline"] = roofline_of(stats, out["particles"], out["mips"], out["live_voxels"], "coupled")
+        else:
+            out["plain_f32"] = entry
+        cb.destroy()
+    if "plain_f32" in out:
+        d = np.linalg.norm(ends[args.precision].astype(np.float64) - ends["f32"], axis=1) / 0.1
+        out["error"] = {"vs": f"f32 coupled frames, same start, after {args.warmup + args.steps + 2} frames",
+                        "position_over_h_rms": float(np.sqrt(np.mean(d * d))), "position_over_h_max": float(d.max())}
+        out["fp16lds_over_f32"] = out["plain_f32"]["ms_per_step"] / out["ms_per_step"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="batched-256", choices=["batched-256", "dam-break-1m", "dam-break-16k"])
+    ap.add_argument("--workload", default="batched-256",
+                    choices=["batched-256", "dam-break-1m", "dam-break-16k", "coupled"])
+    ap.add_argument("--scenes", type=int, default=8, help="coupled workload: scenes of 524 288 + 4096 particles")
     ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"], help="single-scene workloads only")
     ap.add_argument("--precision", default=None, choices=["f32", "fp16lds"],
                     help="fluid kernels (default: fp16lds for the batch, f32 for a single scene)")
@@ -388,8 +427,9 @@ def main() -> int:
               file=sys.stderr)
         return 2
     batched = args.workload == "batched-256"
+    coupled = args.workload == "coupled"
     if args.precision is None:
-        args.precision = "fp16lds" if batched else "f32"
+        args.precision = "fp16lds" if (batched or coupled) else "f32"
 
     import torch
 
@@ -410,6 +450,8 @@ def main() -> int:
                 "batched_over_plain_per_scene": r["value"] / r["plain_per_scene"]["value"],
                 "batched_over_plain_single_1m": r["value"] / single["value"],
             }
+    elif coupled:
+        r = measure_coupled(root, args, stream)
     else:
         r = measure_single(root, args.workload, args.splat, args.precision, args.steps, args.warmup, stream,
                            autotune_now=args.autotune)
@@ -445,11 +487,16 @@ def main() -> int:
         line["config"].update({"scenes": r["scenes"], "graph": r["graph"], "tuning": r["tuning"]})
         line["error"] = r["error"]
         line["comparisons"] = comparisons
+    elif coupled:
+        line["config"].update({"scenes": r["scenes"], "bake": "64^3 x 3 mips following + 32^3 x 2 body + 288^3 x 3 world edits"})
+        line["error"] = r.get("error")
+        line["comparisons"] = {"plain_f32": r.get("plain_f32"), "fp16lds_over_f32": r.get("fp16lds_over_f32"),
+                               "eroded_tiles_mip0_last_frame": r.get("eroded_tiles_mip0_last_frame")}
     else:
         line["config"]["splat"] = args.splat
         line["config"]["tuning"] = r["tuning"]
     line["wall_s"] = round(wall, 3)
-    if not args.no_cpu:
+    if not args.no_cpu and not coupled:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = (cpu_baseline_batched(scene, args.cpu_budget, threads) if batched
                                 else cpu_baseline_single(scene, args.cpu_budget, threads))

@@ -128,13 +128,14 @@ class CoupledBatch:
 
 def coupled_scene(scenes: int = 8, fluid_count: int = 524288, world_res: int = 288, seed: int = 5):
     """The coupled workload: per scene a 128 x 32 x 128 dam-break block (524 288
-    particles at 0.05) over the ridge world (288^3 x 3 mips over 24 units), one
-    4096-particle body of radius 0.95 circling above it, erosion of the world
-    by contacts faster than 2 units/s.  Smaller `fluid_count` keeps the block's
-    footprint and thins its height."""
+    particles at 0.05, beside the ridge: x in [-3.2, 3.2], z in [1, 7.4]) over the
+    ridge world (288^3 x 3 mips over 24 units), one 4096-particle body of radius
+    0.95 circling the ridge at height 1, erosion of the world by contacts faster
+    than 2 units/s.  Smaller `fluid_count` keeps the block's footprint and thins
+    its height."""
     nx = nz = 128
     ny = max(1, fluid_count // (nx * nz))
-    block = S.lattice(nx, ny, nz, 0.05, (-3.2, 0.3, -3.2), seed=seed)[:fluid_count]
+    block = S.lattice(nx, ny, nz, 0.05, (-3.2, 0.3, 1.0), seed=seed)[:fluid_count]
     pos = np.concatenate([block] * scenes, axis=0)
     opts = S.dam_break_options(1, 64, 3)
     opts.update({"followSize": 6.4, "splatMode": "tiled"})

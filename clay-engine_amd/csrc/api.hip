@@ -577,6 +577,7 @@ void splat_free(Splat* sp) {
   dfree(sp->tile_live);
   dfree(sp->own_scene_of);
   dfree(sp->own_scene_base);
+  dfree(sp->follow_slots);
   delete sp;
 }
 
@@ -667,6 +668,10 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
       splat_free(sp);
       return fail(CLAY_ERR_HIP, "SplatField: tile mask initialisation failed");
     }
+  }
+  if (sc.follow && (rc = dalloc(&sp->follow_slots, 6 * S))) {
+    splat_free(sp);
+    return rc;
   }
   // The material channel is constant for a splat (splat.ts:273): written once.
   if ((rc = fill_u16(ctx, s, sp->vol->mat, sp->vol->total_voxels * S, f2h_host((float)desc->material)))) {

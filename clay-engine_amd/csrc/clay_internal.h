@@ -127,6 +127,7 @@ struct Splat {
   uint32_t* own_scene_base = nullptr;
   bool follow = false;                   // centre each volume before a bake:
   const double* follow_centers = nullptr;  // on these (6 doubles per scene, xyz first), else on the AABB
+  uint32_t* follow_slots = nullptr;      // per scene: the AABB's six ordered-float slots
   // tuning
   uint32_t chunk = 1024;                 // particles per splat work item
   int persistent_blocks = 2048;          // grid of the splat work-item kernel

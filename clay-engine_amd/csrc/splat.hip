@@ -727,6 +727,69 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// Bounds of each scene's particles for the bounds-following bake, over the whole
+// grid: every workgroup reduces a slice of one scene (blockIdx.y) in LDS, then
+// folds it into the scene's six slots with integer atomics on order-preserving
+// encodings of the floats (exact, so the result does not depend on timing).
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_bounds_partial(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+                     const uint32_t* __restrict__ base, uint32_t n, uint32_t* __restrict__ slots) {
+  __shared__ float sh[6][kThreads];
+  const uint32_t sc = blockIdx.y;
+  const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
+  const int t = threadIdx.x;
+  const float inf = __uint_as_float(0x7f800000u);
+  float a[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (uint32_t i = lo + blockIdx.x * kThreads + t; i < hi; i += gridDim.x * kThreads) {
+    const float x = px[i], y = py[i], z = pz[i];
+    a[0] = fmin_s(a[0], x);
+    a[1] = fmin_s(a[1], y);
+    a[2] = fmin_s(a[2], z);
+    a[3] = fmax_s(a[3], x);
+    a[4] = fmax_s(a[4], y);
+    a[5] = fmax_s(a[5], z);
+  }
+  for (int q = 0; q < 6; q++) sh[q][t] = a[q];
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      for (int q = 0; q < 3; q++) sh[q][t] = fmin_s(sh[q][t], sh[q][t + off]);
+      for (int q = 3; q < 6; q++) sh[q][t] = fmax_s(sh[q][t], sh[q][t + off]);
+    }
+    __syncthreads();
+  }
+  if (t < 3) atomicMin(&slots[6 * sc + t], ordered_bits(sh[t][0]));
+  else if (t < 6) atomicMax(&slots[6 * sc + t], ordered_bits(sh[t][0]));
+}
+
+// Slots start at (max, max, max, 0, 0, 0) encoded: an empty scene keeps its origin.
+__global__ void k_bounds_init(uint32_t* __restrict__ slots, uint32_t scenes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * scenes) slots[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+__global__ void k_bounds_follow(const uint32_t* __restrict__ slots, const uint32_t* __restrict__ base, uint32_t n,
+                                VolDev* __restrict__ vols, double half, uint32_t scenes) {
+  const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sc >= scenes) return;
+  const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
+  if (lo >= hi) return;
+  for (int q = 0; q < 3; q++) {
+    const float c = (from_ordered(slots[6 * sc + q]) + from_ordered(slots[6 * sc + 3 + q])) * 0.5f;
+    const double o = (double)c - half;
+    vols[sc].origin[q] = (float)o;
+    vols[sc].center[q] = (float)(o + half);
+  }
+}
+
 }  // namespace
 
 // The owner schedule needs every particle's voxel box within kHalo voxels of its
@@ -858,8 +921,20 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
   if (sp->follow) {
     LaunchTimer t(ctx, s, "splat_follow");
-    hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
-                       sp->scene_base, c.end, sp->follow_centers, sp->vol->dev, sp->vol->info.world_size / 2);
+    const double half = sp->vol->info.world_size / 2;
+    if (sp->follow_centers) {
+      hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
+                         sp->scene_base, c.end, sp->follow_centers, sp->vol->dev, half);
+    } else {
+      const unsigned S = (unsigned)sp->scenes;
+      // Enough workgroups to fill the chip once over all scenes.
+      const unsigned per = (unsigned)std::max<int>(1, std::min<int>(64, (sp->persistent_blocks + S - 1) / S));
+      hipLaunchKernelGGL(k_bounds_init, dim3((6 * S + 255) / 256), dim3(256), 0, s, sp->follow_slots, S);
+      hipLaunchKernelGGL(k_bounds_partial, dim3(per, S), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
+                         sp->scene_base, c.end, sp->follow_slots);
+      hipLaunchKernelGGL(k_bounds_follow, dim3((S + 63) / 64), dim3(64), 0, s, sp->follow_slots, sp->scene_base,
+                         c.end, sp->vol->dev, half, S);
+    }
     CLAY_HIP_CHECK(hipGetLastError());
   }
   if (sp->desc.mode == CLAY_SPLAT_PLAIN) {
