@@ -122,6 +122,19 @@ __device__ __forceinline__ bool voxel_box(const SplatMipK& m, f3 p, int32_t lo[3
   return ok;
 }
 
+// Voxels [a, b] of one axis whose centres (i + 0.5) * voxel lie within s of c,
+// padded by 1/64 voxel each side and clipped to [lo, hi].  s is the chord of the
+// cull ball, whose radius is 1.001 reach: a voxel left out lies beyond the reach by
+// more than the pad, far more than the f32 error of this test (~1e-4 voxel at
+// 2048 voxels), so its code is saturated and leaving it out changes nothing.
+__device__ __forceinline__ void axis_span(float c, float s, float inv, int32_t lo, int32_t hi, int32_t& a,
+                                          int32_t& b) {
+  a = sat_i32(ceilf((c - s) * inv - 0.515625f));
+  b = sat_i32(floorf((c + s) * inv - 0.484375f));
+  a = a > lo ? a : lo;
+  b = b < hi ? b : hi;
+}
+
 // enc of a signed distance (splat.ts:246-247), or 65535 outside the band.
 __device__ __forceinline__ uint32_t encode_sd(const SplatMipK& m, float sd) {
   if (sd < m.bw) {
@@ -440,23 +453,24 @@ __global__ void __launch_bounds__(kThreads)
       const int32_t x0 = lo[0] > ox ? lo[0] : ox, x1 = hi[0] < ox + kBrick - 1 ? hi[0] : ox + kBrick - 1;
       const int32_t y0 = lo[1] > oy ? lo[1] : oy, y1 = hi[1] < oy + kBrick - 1 ? hi[1] : oy + kBrick - 1;
       const int32_t z0 = lo[2] > oz ? lo[2] : oz, z1 = hi[2] < oz + kBrick - 1 ? hi[2] : oz + kBrick - 1;
-      for (int32_t z = z0; z <= z1; z++) {
+      // Planes, rows and row spans of the cull ball (axis_span).  The hardware
+      // square root (within 1 ulp) is enough: the spans carry a margin.
+      int32_t za, zb;
+      axis_span(p.z, __builtin_amdgcn_sqrtf(m.cull2), m.inv, z0, z1, za, zb);
+      for (int32_t z = za; z <= zb; z++) {
         const float dz = ((float)z + 0.5f) * m.voxel - p.z;
         const float dz2 = dz * dz;
-        if (dz2 > m.cull2) continue;
-        for (int32_t y = y0; y <= y1; y++) {
+        const float rz = m.cull2 - dz2;
+        if (rz < 0.0f) continue;
+        int32_t ya, yb;
+        axis_span(p.y, __builtin_amdgcn_sqrtf(rz), m.inv, y0, y1, ya, yb);
+        for (int32_t y = ya; y <= yb; y++) {
           const float dy = ((float)y + 0.5f) * m.voxel - p.y;
           const float dy2 = dy * dy;
-          const float rem = m.cull2 - dz2 - dy2;
+          const float rem = rz - dy2;
           if (rem < 0.0f) continue;
-          // x-span of the reach ball on this row, widened by one voxel each side.
-          // The hardware square root (within 1 ulp) is enough here: the span is
-          // widened by a whole voxel on each side.
-          const float s = __builtin_amdgcn_sqrtf(rem);
-          int32_t xa = sat_i32(floorf((p.x - s) * m.inv - 0.5f)) - 1;
-          int32_t xb = sat_i32(floorf((p.x + s) * m.inv - 0.5f)) + 1;
-          xa = xa > x0 ? xa : x0;
-          xb = xb < x1 ? xb : x1;
+          int32_t xa, xb;
+          axis_span(p.x, __builtin_amdgcn_sqrtf(rem), m.inv, x0, x1, xa, xb);
           uint32_t* row = lds + (y - oy) * kBrick + (z - oz) * kBrick * kBrick - ox;
           // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
           // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
@@ -557,20 +571,22 @@ __global__ void __launch_bounds__(kThreads)
       const int32_t x0 = lo[0] > ex0 ? lo[0] : ex0, x1 = hi[0] < ex1 - 1 ? hi[0] : ex1 - 1;
       const int32_t y0 = lo[1] > ey0 ? lo[1] : ey0, y1 = hi[1] < ey1 - 1 ? hi[1] : ey1 - 1;
       const int32_t z0 = lo[2] > ez0 ? lo[2] : ez0, z1 = hi[2] < ez1 - 1 ? hi[2] : ez1 - 1;
-      for (int32_t z = z0; z <= z1; z++) {
+      int32_t za, zb;
+      axis_span(p.z, __builtin_amdgcn_sqrtf(m.cull2), m.inv, z0, z1, za, zb);
+      for (int32_t z = za; z <= zb; z++) {
         const float dz = ((float)z + 0.5f) * m.voxel - p.z;
         const float dz2 = dz * dz;
-        if (dz2 > m.cull2) continue;
-        for (int32_t y = y0; y <= y1; y++) {
+        const float rz = m.cull2 - dz2;
+        if (rz < 0.0f) continue;
+        int32_t ya, yb;
+        axis_span(p.y, __builtin_amdgcn_sqrtf(rz), m.inv, y0, y1, ya, yb);
+        for (int32_t y = ya; y <= yb; y++) {
           const float dy = ((float)y + 0.5f) * m.voxel - p.y;
           const float dy2 = dy * dy;
-          const float rem = m.cull2 - dz2 - dy2;
+          const float rem = rz - dy2;
           if (rem < 0.0f) continue;
-          const float sp = __builtin_amdgcn_sqrtf(rem);
-          int32_t xa = sat_i32(floorf((p.x - sp) * m.inv - 0.5f)) - 1;
-          int32_t xb = sat_i32(floorf((p.x + sp) * m.inv - 0.5f)) + 1;
-          xa = xa > x0 ? xa : x0;
-          xb = xb < x1 ? xb : x1;
+          int32_t xa, xb;
+          axis_span(p.x, __builtin_amdgcn_sqrtf(rem), m.inv, x0, x1, xa, xb);
           uint32_t* row = lds + (y - ey0) * nx + (z - ez0) * nx * ny - ex0;
           // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
           // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
