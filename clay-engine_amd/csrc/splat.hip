@@ -127,12 +127,14 @@ __device__ __forceinline__ bool voxel_box(const SplatMipK& m, f3 p, int32_t lo[3
 // cull ball, whose radius is 1.001 reach: a voxel left out lies beyond the reach by
 // more than the pad, far more than the f32 error of this test (~1e-4 voxel at
 // 2048 voxels), so its code is saturated and leaving it out changes nothing.
+// The clamps run in f32 (lo and hi are exact there) and keep a in [lo, hi + 1] and
+// b in [lo - 1, hi], so the conversions are exact and branch-free; a NaN bound
+// gives the whole clipped span (maxnum/minnum return the number).
 __device__ __forceinline__ void axis_span(float c, float s, float inv, int32_t lo, int32_t hi, int32_t& a,
                                           int32_t& b) {
-  a = sat_i32(ceilf((c - s) * inv - 0.515625f));
-  b = sat_i32(floorf((c + s) * inv - 0.484375f));
-  a = a > lo ? a : lo;
-  b = b < hi ? b : hi;
+  const float flo = (float)lo, fhi = (float)hi;
+  a = (int32_t)fminf(fmaxf(ceilf((c - s) * inv - 0.515625f), flo), fhi + 1.0f);
+  b = (int32_t)fmaxf(fminf(floorf((c + s) * inv - 0.484375f), fhi), flo - 1.0f);
 }
 
 // enc of a signed distance (splat.ts:246-247), or 65535 outside the band.
