@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(kB)
 }
 
 struct StagedNbr {
-  float4 vr;   // velocity and density
+  float4 vr;   // velocity and 1 / density
   uint2 off;   // f16 position offset from the centre of the particle's cell
   float pres;
   float pad;
@@ -512,20 +512,21 @@ __global__ void __launch_bounds__(kB)
   f3 fp = mk(0.0f, 0.0f, 0.0f);
   f3 vsum = vi * k.h6;
   float wsum = k.h6;
-  auto term = [&](uint32_t t, f3 dv, f3 vj, float rhoj, float prj) {
+  // The neighbour's 1 / density comes in precomputed (once per staged particle,
+  // not once per pair), and r, 1 / r come from the hardware reciprocal square
+  // root (1 ulp): r > 1e-6 is tested as r^2 > 1e-12.
+  auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float prj) {
     if (t == s) return;
     const float r2 = dot3(dv, dv);
     if (r2 >= k.hh) return;
-    const float inv = 1.0f / rhoj;
     const float w0 = k.hh - r2;
     const float w = w0 * w0 * w0;
     vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
     wsum = wsum + w;
-    const float r = sqrtf(r2);
-    if (r > 1e-6f) {
-      const float hr = k.h - r;
+    if (r2 > 1e-12f) {
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float hr = k.h - r2 * ir;
       const float s2 = k.mass * (pri + prj) * 0.5f * inv * k.spiky * (hr * hr);
-      const float ir = 1.0f / r;
       fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
     }
   };
@@ -539,7 +540,7 @@ __global__ void __launch_bounds__(kB)
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
           StagedNbr e;
-          e.vr = make_float4(svx[t], svy[t], svz[t], srho[t]);
+          e.vr = make_float4(svx[t], svy[t], svz[t], 1.0f / srho[t]);
           e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
           e.pres = spres[t];
           e.pad = 0.0f;
@@ -556,7 +557,7 @@ __global__ void __launch_bounds__(kB)
         });
       } else {
         walk_row(k, r, [&](uint32_t t) {
-          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), srho[t], spres[t]);
+          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t], spres[t]);
         });
       }
       __syncthreads();

@@ -172,6 +172,34 @@ __device__ __forceinline__ f3 load_nrm(const CloudK& c, uint32_t i) {
   return c.nrm[0] ? mk(c.nrm[0][i], c.nrm[1][i], c.nrm[2][i]) : mk(0.0f, 0.0f, 0.0f);
 }
 
+// One row of a particle's ball in LDS (the voxel loop of splat.ts:243-250): voxels
+// rp..re (inclusive; none when re < rp) whose centres have x-coordinate fx, fx + 1,
+// ... in voxels.  fx = x + 0.5 and its steps are exact (integers and halves below
+// 2^23), so each dx is the value ((float)x + 0.5f) * voxel - p.x gives.  Voxels go
+// in pairs, with one more when the count is odd.  Without normals the squared
+// distance is min-merged (its f32 bits order as the value); with normals the code.
+template <bool kNormals>
+__device__ __forceinline__ void splat_row(const SplatMipK& m, f3 p, f3 nr, float dy, float dz, float dy2,
+                                          float dz2, uint32_t* rp, uint32_t* re, float fx) {
+  auto voxel = [&](uint32_t* q, float f) {
+    const float dx = f * m.voxel - p.x;
+    if (kNormals) {
+      const float len = sqrtf(dx * dx + dy2 + dz2);
+      float sd = len - m.radius;
+      sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
+      const uint32_t code = encode_sd(m, sd);
+      if (code != kEncSat) atomicMin(q, code);
+    } else {
+      atomicMin(q, __float_as_uint(dx * dx + dy2 + dz2));
+    }
+  };
+  for (; rp < re; rp += 2, fx += 2.0f) {
+    voxel(rp, fx);
+    voxel(rp + 1, fx + 1.0f);
+  }
+  if (rp == re) voxel(rp, fx);
+}
+
 // ---------------------------------------------------------------------------
 // PLAIN
 // ---------------------------------------------------------------------------
@@ -474,23 +502,7 @@ __global__ void __launch_bounds__(kThreads)
           int32_t xa, xb;
           axis_span(p.x, __builtin_amdgcn_sqrtf(rem), m.inv, x0, x1, xa, xb);
           uint32_t* row = lds + (y - oy) * kBrick + (z - oz) * kBrick * kBrick - ox;
-          // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
-          // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
-          float fx = (float)xa + 0.5f;
-          uint32_t* rp = row + xa;
-          for (int32_t x = xa; x <= xb; x++, fx += 1.0f, rp++) {
-            const float dx = fx * m.voxel - p.x;
-            if (kNormals) {
-              const float len = sqrtf(dx * dx + dy2 + dz2);
-              float sd = len - m.radius;
-              sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
-              const uint32_t code = encode_sd(m, sd);
-              if (code != kEncSat) atomicMin(rp, code);
-            } else {
-              const float s2 = dx * dx + dy2 + dz2;
-              atomicMin(rp, __float_as_uint(s2));
-            }
-          }
+          splat_row<kNormals>(m, p, nr, dy, dz, dy2, dz2, row + xa, row + xb, (float)xa + 0.5f);
         }
       }
     }
@@ -590,23 +602,7 @@ __global__ void __launch_bounds__(kThreads)
           int32_t xa, xb;
           axis_span(p.x, __builtin_amdgcn_sqrtf(rem), m.inv, x0, x1, xa, xb);
           uint32_t* row = lds + (y - ey0) * nx + (z - ez0) * nx * ny - ex0;
-          // fx runs through x + 0.5 exactly (integers and halves below 2^23 are exact),
-          // so dx is the same value ((float)x + 0.5f) * voxel - p.x gives.
-          float fx = (float)xa + 0.5f;
-          uint32_t* rp = row + xa;
-          for (int32_t x = xa; x <= xb; x++, fx += 1.0f, rp++) {
-            const float dx = fx * m.voxel - p.x;
-            if (kNormals) {
-              const float len = sqrtf(dx * dx + dy2 + dz2);
-              float sd = len - m.radius;
-              sd = fmax_s(dx * nr.x + dy * nr.y + dz * nr.z, sd);
-              const uint32_t code = encode_sd(m, sd);
-              if (code != kEncSat) atomicMin(rp, code);
-            } else {
-              const float s2 = dx * dx + dy2 + dz2;
-              atomicMin(rp, __float_as_uint(s2));
-            }
-          }
+          splat_row<kNormals>(m, p, nr, dy, dz, dy2, dz2, row + xa, row + xb, (float)xa + 0.5f);
         }
       }
     }

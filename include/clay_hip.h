@@ -228,7 +228,8 @@ typedef struct {
 /* Arithmetic of the density and force passes.  F32: every operation rounds once
  * in the reference's order (bit-exact against an IEEE-f32 evaluation).  FP16LDS:
  * neighbour positions staged in LDS as f16 offsets from their cell's centre,
- * compensated f32 density sums, fused multiply-add allowed. */
+ * compensated f32 density sums, fused multiply-add allowed, pair distances from
+ * the hardware reciprocal square root (1 ulp), 1/density once per neighbour. */
 #define CLAY_PRECISION_F32 0
 #define CLAY_PRECISION_FP16LDS 1
 
