@@ -324,6 +324,15 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
+// Orders a wave's LDS accesses around this point (writes before it are seen by
+// every lane after it), without waiting for the workgroup's other waves: each
+// wave stages into a slice of its own.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Neumaier's compensated sum.
 struct KahanSum {
   float s = 0.0f, c = 0.0f;
@@ -451,7 +460,7 @@ __global__ void __launch_bounds__(kB)
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64)
           stage[wave][t - ulo] = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
-      __syncthreads();
+      wave_sync();
       if (staged) {
         const float ey = (float)dy * k.h, ez = (float)dz * k.h;
         walk_cells(k, r, [&](uint32_t t, int j) {
@@ -461,7 +470,7 @@ __global__ void __launch_bounds__(kB)
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
-      __syncthreads();
+      wave_sync();
     }
   }
   if (!active) return;
@@ -546,7 +555,7 @@ __global__ void __launch_bounds__(kB)
           e.pad = 0.0f;
           stage[wave][t - ulo] = e;
         }
-      __syncthreads();
+      wave_sync();
       if (staged) {
         const float ey = (float)dy * k.h, ez = (float)dz * k.h;
         walk_cells(k, r, [&](uint32_t t, int j) {
@@ -560,7 +569,7 @@ __global__ void __launch_bounds__(kB)
           term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t], spres[t]);
         });
       }
-      __syncthreads();
+      wave_sync();
     }
   }
   if (!active) return;
