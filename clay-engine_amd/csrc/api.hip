@@ -838,6 +838,7 @@ void fluid_free(Fluid* f) {
   dfree(f->contacts);
   dfree(f->cmask);
   dfree(f->cdata);
+  dfree(f->cover);
   dfree(f->reduce);
   dfree(f->epoch_dev);
   dfree(f->scene_of);
@@ -848,15 +849,14 @@ void fluid_free(Fluid* f) {
 
 size_t contact_words(const Fluid* f) { return (size_t)f->scenes * (4 + 4 * CLAY_CONTACTS); }
 
-int fluid_substep_impl(Fluid* f, hipStream_t s) {
+// First half of a substep (fluid.ts:293-368): cell stamp, keys, sort, cell
+// table and reordered state, then densities and pressures.  With `cover`, the
+// density pass also writes each particle's interior radius for the surface bake.
+int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
   int rc;
-  // clearGrid's contacts reset (fluid.ts:302-304), for every scene, and the per-id flags.
-  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u)) ||
-      (rc = launch_fill_u32(c, s, f->cmask, ((size_t)f->capacity + 31) / 32, 0u)))
-    return rc;
   if (n == 0) return CLAY_OK;
   // A new stamp per substep makes every older cell record stale at once.  After
   // 2^32 - 1 substeps the stamps restart, and the table is wiped once so that no
@@ -876,9 +876,23 @@ int fluid_substep_impl(Fluid* f, hipStream_t s) {
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
   if ((rc = launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6))) return rc;
   const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
-  if ((rc = launch_density(c, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells, f->srho,
-                           f->spres)))
+  return launch_density(c, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells, f->srho,
+                        f->spres, f->vals, cover);
+}
+
+// Second half of a substep (fluid.ts:370-491): clearGrid's contacts reset
+// (fluid.ts:302-304) for every scene and the per-id flags, then forces and
+// integrate over the sorted state and cell table of fluid_sort_density.
+int fluid_forces(Fluid* f, hipStream_t s) {
+  Ctx* c = f->ctx;
+  const uint32_t n = f->live;
+  int rc;
+  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u)) ||
+      (rc = launch_fill_u32(c, s, f->cmask, ((size_t)f->capacity + 31) / 32, 0u)))
     return rc;
+  if (n == 0) return CLAY_OK;
+  const SceneK sk = f->scene_k();
+  const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
   const float* sorted[8] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz, f->srho, f->spres};
   float* out[11] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz, f->rho, f->pres, f->nvx, f->nvy, f->nvz};
   ColliderDev none{};
@@ -901,16 +915,28 @@ int fluid_frame_impl(Fluid* f, hipStream_t s) {
     int rc = launch_emit(f->ctx, s, f, emit_base, emit_count, f->emit_seed);
     if (rc) return rc;
   }
+  if (emit_count > 0) f->presorted = false;
   bool sorted = false;
-  for (int i = 0; i < f->consts.substeps && f->live > 0; i++) {
-    int rc = fluid_substep_impl(f, s);
-    if (rc) return rc;
+  if (f->live > 0 && f->consts.substeps > 0) {
+    // Substep i sorts and computes densities from the positions substep i - 1 left;
+    // the first one's were computed at the end of the previous frame unless the
+    // state has changed since.  The frame ends with the next frame's first sort
+    // and densities, from the positions the bake sees (the same values the next
+    // frame would compute first), so the bake can use its cell table.
+    int rc;
+    if (!f->presorted && (rc = fluid_sort_density(f, s, nullptr))) return rc;
+    for (int i = 0; i < f->consts.substeps; i++) {
+      if (i > 0 && (rc = fluid_sort_density(f, s, nullptr))) return rc;
+      if ((rc = fluid_forces(f, s))) return rc;
+    }
+    if ((rc = fluid_sort_density(f, s, f->cover))) return rc;
+    f->presorted = true;
     sorted = true;
   }
   f->surface->cloud.live_count = f->live;
-  // The last substep's permutation is a spatially coherent visiting order for the
+  // The permutation of that sort is a spatially coherent visiting order for the
   // splat; the codes do not depend on it.
-  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr);
+  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr, sorted ? f->cover : nullptr);
 }
 
 // Shared by FluidSim (one scene, `counts` null) and a batch of `scenes` scenes
@@ -998,7 +1024,7 @@ int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t s
       (rc = dalloc(&f->keys, P)) || (rc = dalloc(&f->vals, P)) || (rc = dalloc(&f->keys2, P)) ||
       (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, contact_words(f))) ||
       (rc = dalloc(&f->epoch_dev, 1)) || (rc = dalloc(&f->cmask, ((size_t)P + 31) / 32)) ||
-      (rc = dalloc(&f->cdata, P))) {
+      (rc = dalloc(&f->cdata, P)) || (rc = dalloc(&f->cover, P))) {
     fluid_free(f);
     return rc;
   }
@@ -1147,6 +1173,7 @@ extern "C" int clay_fluid_set_state(clay_fluid* fh, const float* pos, const floa
   f->live = n;
   f->cursor = n % f->capacity;
   f->surface->cloud.live_count = n;
+  f->presorted = false;
   return CLAY_OK;
 }
 
@@ -1188,6 +1215,7 @@ extern "C" int clay_fluid_configure(clay_fluid* fh, const clay_fluid_params* p) 
   if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_fluid_configure: NULL argument");
   f->params = *p;
   fluid_build_k(f);
+  f->presorted = false;  // densities of the next substep depend on the parameters
   return CLAY_OK;
 }
 
@@ -1201,7 +1229,11 @@ extern "C" int clay_fluid_substep(clay_fluid* fh, clay_stream stream) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_substep: NULL fluid");
   fluid_build_k(f);
-  return fluid_substep_impl(f, pick(f->ctx, stream));
+  hipStream_t s = pick(f->ctx, stream);
+  int rc;
+  if (f->live > 0 && !f->presorted && (rc = fluid_sort_density(f, s, nullptr))) return rc;
+  f->presorted = false;  // the forces pass moves the particles
+  return fluid_forces(f, s);
 }
 
 extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
@@ -1338,7 +1370,14 @@ int batch_frame(Fluid* f, hipStream_t s) {
   fluid_build_k(f);
   const uint32_t subs = (uint32_t)f->consts.substeps;
   // A stamp wrap wipes the cell table, which only an eager frame does.
-  if (!f->use_graph || f->epoch > 0xFFFFFFFFu - subs - 1u) return fluid_frame_impl(f, s);
+  if (!f->use_graph || f->epoch > 0xFFFFFFFFu - subs - 2u) return fluid_frame_impl(f, s);
+  // The graph holds a frame that starts from presorted state; after a state
+  // upload the first sort and densities run eagerly here.
+  if (!f->presorted && f->live > 0 && subs > 0) {
+    const int rc = fluid_sort_density(f, s, nullptr);
+    if (rc) return rc;
+    f->presorted = true;
+  }
   if (!f->graph || f->graph_stream != s) {
     drop_graph(f);
     const uint32_t epoch0 = f->epoch;
@@ -1415,6 +1454,7 @@ extern "C" int clay_batch_set_state(clay_batch* b, const float* pos, const float
     else
       CLAY_HIP_CHECK(hipMemset(dv[a], 0, sizeof(float) * n));
   }
+  f->presorted = false;
   return CLAY_OK;
 }
 
@@ -1446,6 +1486,7 @@ extern "C" int clay_batch_configure(clay_batch* b, const clay_fluid_params* p) {
   drop_graph(f);
   f->params = *p;
   fluid_build_k(f);
+  f->presorted = false;
   return CLAY_OK;
 }
 

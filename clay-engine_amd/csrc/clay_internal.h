@@ -177,6 +177,11 @@ struct Fluid {
   double* reduce = nullptr;      // FP64 reduction partials (+ 8 results at the end)
   uint32_t reduce_blocks = 0;
   Splat* surface = nullptr;      // owned
+  float* cover = nullptr;        // per particle id: squared interior radius for the surface bake
+  // The sort, cell table and densities of the next substep are computed at the end
+  // of a frame, from the positions the bake sees; `presorted` says they are still
+  // valid (no state upload, emission or configure since).
+  bool presorted = false;
   FluidK k{};
   FluidTune tune{};
   // batch of independent scenes (one scene for a FluidSim)
@@ -234,9 +239,11 @@ int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float*
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
                    float* const* dst, int planes);
+// With `cover` (by particle id, via svals), also each particle's squared interior
+// radius for the splat (+inf when its neighbours do not surround it).
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
                    uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cells, float* srho, float* spres);
+                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover);
 int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune,
                             bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
                             const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
@@ -253,7 +260,10 @@ int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*&
                      uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist);
 // splat.hip
 int splat_bake(Splat* sp, hipStream_t s);
-int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order);
+// `order`: visiting order (codes do not depend on it).  `cover`: per particle id,
+// a squared radius beyond which the particle holds no voxel's minimum (a fluid
+// surface's interior radius), or null.
+int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
 int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
                          std::vector<size_t>* brick_off, size_t* brick_total,
                          std::vector<size_t>* tile_off, size_t* tile_total);

@@ -156,11 +156,58 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
     if (p < planes) pl.dst[p][s] = pl.src[p][j];
 }
 
-template <int kB>
+// ---------------------------------------------------------------------------
+// Interior radius of a particle, for the splat (a voxel's code only needs the
+// particle nearest its centre).  The offset d between the particle and each
+// neighbour found by the density pass is classed by the signs of its components
+// that exceed 0.35 |d| into one of the 26 lattice directions: every direction
+// falls in exactly one class, within 30.5 degrees of its axis (measured over 4M
+// random directions).  When every class holds a neighbour q at distance in
+// (0.1875 h, dmax), a voxel centre c at distance r > 1.118 dmax from the
+// particle p has, in the class of c - p, a neighbour within 61 degrees of that
+// direction, so |c - q|^2 <= r^2 - |q - p| (2 r cos 62deg - |q - p|) <
+// r^2 - 0.049 |q - p| dmax: q is strictly nearer c, by far more than the rounding
+// of the f32 squared distances and of the f16 offsets (<= 5e-5 in d, which also
+// bends the class angle by < 0.2 degree).  Such a pair cannot hold c's minimum,
+// so the splat skips voxels beyond that radius.  Levels dmax = h, 1.25h, 1.5h
+// over every candidate of the 27-cell walk (a fluid at rest spreads past h on
+// the diagonals within a few frames); the smallest level whose 26 classes are
+// all filled gives the radius.
+// ---------------------------------------------------------------------------
+struct Cover {
+  uint32_t m0 = 0, m1 = 0, m2 = 0;
+  __device__ __forceinline__ void add(f3 d, float r2, float hh) {
+    if (!(r2 > 0.03515625f * hh) || !(r2 < 2.25f * hh)) return;  // 0.1875 h < |d| < 1.5 h
+    const float t2 = 0.1225f * r2;         // (0.35 |d|)^2
+    const int sx = d.x * d.x > t2 ? (d.x > 0.0f ? 1 : -1) : 0;
+    const int sy = d.y * d.y > t2 ? (d.y > 0.0f ? 1 : -1) : 0;
+    const int sz = d.z * d.z > t2 ? (d.z > 0.0f ? 1 : -1) : 0;
+    const uint32_t bit = 1u << (13 + sx + 3 * sy + 9 * sz);
+    if (r2 < hh) m0 |= bit;
+    if (r2 < 1.5625f * hh) m1 |= bit;
+    m2 |= bit;
+  }
+  // Squared radius (1.118 dmax)^2 = 1.25 dmax^2, or +inf when no level is full.
+  __device__ __forceinline__ float radius2(float hh) const {
+    constexpr uint32_t kAll = 0x7FFFFFFu & ~(1u << 13);
+    if (m0 == kAll) return 1.25f * hh;
+    if (m1 == kAll) return 1.953125f * hh;
+    if (m2 == kAll) return 2.8125f * hh;
+    return __uint_as_float(0x7f800000u);
+  }
+};
+
+// Where the density pass writes the interior radius: by particle id.
+struct CoverOut {
+  const uint32_t* svals;
+  float* r2;
+};
+
+template <int kB, bool kCover>
 __global__ void __launch_bounds__(kB)
     k_density(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
               const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
-              const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
+              const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co) {
   const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   if (s >= n) return;
   const uint32_t epoch = *sk.epoch;
@@ -169,6 +216,7 @@ __global__ void __launch_bounds__(kB)
   int32_t b[3];
   cell_of(k, pi, b);
   float rho = 0.0f;
+  Cover cov;
   for_neighbours(k, cs, epoch, b, [&](uint32_t t) {
     const f3 d = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
     const float r2 = dot3(d, d);
@@ -176,9 +224,11 @@ __global__ void __launch_bounds__(kB)
       const float w = k.hh - r2;
       rho = rho + k.mass_poly6 * (w * w * w);
     }
+    if (kCover) cov.add(d, r2, k.hh);
   });
   srho[s] = fmax_s(rho, k.rest_floor);
   spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
+  if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
 }
 
 struct SortedIn {
@@ -425,11 +475,11 @@ __device__ __forceinline__ f3 unpack_off(uint2 u) {
   return mk(h2f((uint16_t)(u.x & 0xffffu)), h2f((uint16_t)(u.x >> 16)), h2f((uint16_t)(u.y & 0xffffu)));
 }
 
-template <int kB, int kS>
+template <int kB, int kS, bool kCover>
 __global__ void __launch_bounds__(kB)
     k_density_lds(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
                   const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
-                  const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres) {
+                  const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co) {
 #pragma clang fp contract(fast)
   __shared__ uint2 stage[kB / 64][kS];
   const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
@@ -443,12 +493,14 @@ __global__ void __launch_bounds__(kB)
   if (active) cell_of(k, pi, b);
   const f3 oi = cell_rel(k, pi, b);
   KahanSum rho;
+  Cover cov;
   auto term = [&](f3 d) {
     const float r2 = dot3(d, d);
     if (r2 < k.hh) {
       const float w = k.hh - r2;
       rho.add(k.mass_poly6 * (w * w * w));
     }
+    if (kCover) cov.add(d, r2, k.hh);
   };
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
@@ -477,6 +529,7 @@ __global__ void __launch_bounds__(kB)
   const float rv = rho.value();
   srho[s] = fmax_s(rv, k.rest_floor);
   spres[s] = fmax_s(k.stiff * (rv - k.rest), 0.0f);
+  if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
 }
 
 struct StagedNbr {
@@ -730,22 +783,22 @@ __global__ void __launch_bounds__(kBlock)
 
 inline unsigned grid_for(uint32_t n, int block = kBlock) { return (n + block - 1) / block; }
 
-template <int kB>
+template <int kB, bool kCover>
 void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t n, const float* spx,
                       const float* spy, const float* spz, const uint32_t* skeys, const uint4* cells,
-                      float* srho, float* spres, int stage) {
+                      float* srho, float* spres, int stage, CoverOut co) {
   if (stage == 0) {
-    hipLaunchKernelGGL(k_density<kB>, dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz, skeys,
-                       cells, srho, spres);
+    hipLaunchKernelGGL((k_density<kB, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz,
+                       skeys, cells, srho, spres, co);
   } else if (stage == 64) {
-    hipLaunchKernelGGL((k_density_lds<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz,
-                       skeys, cells, srho, spres);
+    hipLaunchKernelGGL((k_density_lds<kB, 64, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
+                       spy, spz, skeys, cells, srho, spres, co);
   } else if (stage == 256) {
-    hipLaunchKernelGGL((k_density_lds<kB, 256>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy,
-                       spz, skeys, cells, srho, spres);
+    hipLaunchKernelGGL((k_density_lds<kB, 256, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
+                       spy, spz, skeys, cells, srho, spres, co);
   } else {
-    hipLaunchKernelGGL((k_density_lds<kB, 128>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy,
-                       spz, skeys, cells, srho, spres);
+    hipLaunchKernelGGL((k_density_lds<kB, 128, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
+                       spy, spz, skeys, cells, srho, spres, co);
   }
 }
 
@@ -802,15 +855,22 @@ int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, con
 
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
                    uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cells, float* srho, float* spres) {
+                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover) {
   if (n == 0) return CLAY_OK;
   const uint4* cl = reinterpret_cast<const uint4*>(cells);
   const int stage = lds ? tune.stage : 0;
+  const CoverOut co{svals, cover};
   LaunchTimer t(c, s, lds ? "fluid_density_lds" : "fluid_density");
-  if (tune.block == 128)
-    launch_density_t<128>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage);
-  else
-    launch_density_t<256>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage);
+  if (cover) {
+    if (tune.block == 128)
+      launch_density_t<128, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+    else
+      launch_density_t<256, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+  } else if (tune.block == 128) {
+    launch_density_t<128, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+  } else {
+    launch_density_t<256, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+  }
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

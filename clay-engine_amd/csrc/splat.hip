@@ -35,6 +35,13 @@
 // Voxels whose centre is farther than `reach` from the particle are skipped row
 // by row; they fail sd < bandWorld, so skipping them cannot change a code.
 //
+// A fluid's surface bake also gets each particle's interior radius (from the
+// density pass over the same positions): when a particle's neighbours surround it
+// in all 26 lattice directions, every voxel centre beyond that radius has another
+// particle strictly nearer, so the particle cannot hold that voxel's minimum and
+// the walk stops at the radius.  Only particles at the fluid's surface walk their
+// whole reach.  The codes are those of the full walk.
+//
 // A batch bakes one volume instance per scene in the same launches: brick ids run
 // scene-major (scene * bricks-per-scene + brick), a particle's scene comes from a
 // per-particle table, and each scene has its own origin record, code grid block,
@@ -53,6 +60,7 @@ constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared dis
 constexpr int kHalo = 5;                    // owner schedule: voxels staged around a brick
 constexpr int kExt = kBrick + 2 * kHalo;    // staged edge (26)
 constexpr int kExtVox = kExt * kExt * kExt;
+constexpr int kPart = 2048;                 // owner schedule: particles partitioned per LDS pass
 
 struct SplatMipK {
   int32_t r;          // res >> m
@@ -70,6 +78,8 @@ struct CloudK {
   const float* nrm[3];
   const uint32_t* order;     // optional visiting order (any permutation gives the same codes)
   const uint32_t* scene_of;  // batch: scene of each particle id (else every particle is scene 0)
+  const float* cover;        // optional, per id: squared radius beyond which the particle holds
+                             // no voxel's minimum (no normals); the walk stops there
   uint32_t first, end;       // live particles: ids in [first, end)
   uint32_t count;            // end - first
 };
@@ -478,6 +488,7 @@ __global__ void __launch_bounds__(kThreads)
       const uint32_t i = list[base + e];
       const f3 p = load_rel(c, sv, i);
       const f3 nr = kNormals ? load_nrm(c, i) : mk(0.0f, 0.0f, 0.0f);
+      const float cull2 = (!kNormals && c.cover) ? fminf(m.cull2, c.cover[i]) : m.cull2;
       int32_t lo[3], hi[3];
       voxel_box(m, p, lo, hi);  // non-empty: the particle was binned here
       const int32_t x0 = lo[0] > ox ? lo[0] : ox, x1 = hi[0] < ox + kBrick - 1 ? hi[0] : ox + kBrick - 1;
@@ -486,11 +497,11 @@ __global__ void __launch_bounds__(kThreads)
       // Planes, rows and row spans of the cull ball (axis_span).  The hardware
       // square root (within 1 ulp) is enough: the spans carry a margin.
       int32_t za, zb;
-      axis_span(p.z, __builtin_amdgcn_sqrtf(m.cull2), m.inv, z0, z1, za, zb);
+      axis_span(p.z, __builtin_amdgcn_sqrtf(cull2), m.inv, z0, z1, za, zb);
       for (int32_t z = za; z <= zb; z++) {
         const float dz = ((float)z + 0.5f) * m.voxel - p.z;
         const float dz2 = dz * dz;
-        const float rz = m.cull2 - dz2;
+        const float rz = cull2 - dz2;
         if (rz < 0.0f) continue;
         int32_t ya, yb;
         axis_span(p.y, __builtin_amdgcn_sqrtf(rz), m.inv, y0, y1, ya, yb);
@@ -537,6 +548,8 @@ __global__ void __launch_bounds__(kThreads)
                   const ScanOut* __restrict__ scan, uint32_t nbricks, const uint32_t* __restrict__ list,
                   uint32_t* __restrict__ grid) {
   __shared__ uint32_t lds[kExtVox];
+  __shared__ uint32_t part[kPart];    // interior-radius schedule: the sub-chunk, surface particles first
+  __shared__ uint32_t part_n[2];      // surface and interior counts
   __shared__ uint32_t sh_brick;
   const int t = threadIdx.x;
   const uint32_t init = kNormals ? kEncSat : kSInit;
@@ -572,12 +585,10 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t e_lo = chunk * mk_.chunk;
     const uint32_t n_e = (n_b - e_lo) < mk_.chunk ? (n_b - e_lo) : mk_.chunk;
     const uint32_t base = start[b] + e_lo;
-    for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
-      const uint32_t e = pass * kThreads + rev;
-      if (e >= n_e) continue;
-      const uint32_t i = list[base + e];
+    auto walk = [&](uint32_t i) {
       const f3 p = load_rel(c, sv, i);
       const f3 nr = kNormals ? load_nrm(c, i) : mk(0.0f, 0.0f, 0.0f);
+      const float cull2 = (!kNormals && c.cover) ? fminf(m.cull2, c.cover[i]) : m.cull2;
       int32_t lo[3], hi[3];
       voxel_box(m, p, lo, hi);  // non-empty: the particle was binned
       // Inside the staged region by construction (reach <= kHalo voxels at every
@@ -586,11 +597,11 @@ __global__ void __launch_bounds__(kThreads)
       const int32_t y0 = lo[1] > ey0 ? lo[1] : ey0, y1 = hi[1] < ey1 - 1 ? hi[1] : ey1 - 1;
       const int32_t z0 = lo[2] > ez0 ? lo[2] : ez0, z1 = hi[2] < ez1 - 1 ? hi[2] : ez1 - 1;
       int32_t za, zb;
-      axis_span(p.z, __builtin_amdgcn_sqrtf(m.cull2), m.inv, z0, z1, za, zb);
+      axis_span(p.z, __builtin_amdgcn_sqrtf(cull2), m.inv, z0, z1, za, zb);
       for (int32_t z = za; z <= zb; z++) {
         const float dz = ((float)z + 0.5f) * m.voxel - p.z;
         const float dz2 = dz * dz;
-        const float rz = m.cull2 - dz2;
+        const float rz = cull2 - dz2;
         if (rz < 0.0f) continue;
         int32_t ya, yb;
         axis_span(p.y, __builtin_amdgcn_sqrtf(rz), m.inv, y0, y1, ya, yb);
@@ -604,6 +615,43 @@ __global__ void __launch_bounds__(kThreads)
           uint32_t* row = lds + (y - ey0) * nx + (z - ez0) * nx * ny - ex0;
           splat_row<kNormals>(m, p, nr, dy, dz, dy2, dz2, row + xa, row + xb, (float)xa + 0.5f);
         }
+      }
+    };
+    if (!kNormals && c.cover) {
+      // A wave takes as long as its longest walk, so particles that walk their whole
+      // reach (the fluid's surface) and interior ones (a voxel or so) go to different
+      // waves: each sub-chunk is partitioned in LDS, surface particles first, and
+      // each part is cut into one contiguous run per thread; step k of every thread
+      // takes entry k of its run, so a wave's lanes hold particles a run apart
+      // (distinct voxels for their ds_min_u32).
+      for (uint32_t sub = 0; sub < n_e; sub += kPart) {
+        const uint32_t mm = n_e - sub < (uint32_t)kPart ? n_e - sub : (uint32_t)kPart;
+        if (t < 2) part_n[t] = 0;
+        __syncthreads();
+        for (uint32_t e = t; e < mm; e += kThreads) {
+          const uint32_t i = list[base + sub + e];
+          if (c.cover[i] < m.cull2)
+            part[mm - 1u - atomicAdd(&part_n[1], 1u)] = i;
+          else
+            part[atomicAdd(&part_n[0], 1u)] = i;
+        }
+        __syncthreads();
+        const uint32_t ns = part_n[0], ni = mm - ns;
+        const uint32_t rs = (ns + kThreads - 1) / kThreads, ri = (ni + kThreads - 1) / kThreads;
+        for (uint32_t k = 0; k < rs; k++) {
+          const uint32_t e = (uint32_t)t * rs + k;
+          if (e < ns) walk(part[e]);
+        }
+        for (uint32_t k = 0; k < ri; k++) {
+          const uint32_t e = (uint32_t)t * ri + k;
+          if (e < ni) walk(part[ns + e]);
+        }
+        __syncthreads();
+      }
+    } else {
+      for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
+        const uint32_t e = pass * kThreads + rev;
+        if (e < n_e) walk(list[base + e]);
       }
     }
     __syncthreads();
@@ -851,7 +899,7 @@ static void fill_mips(Splat* sp, MipsK* mk_) {
   }
 }
 
-static CloudK cloud_k(const Splat* sp, const uint32_t* order) {
+static CloudK cloud_k(const Splat* sp, const uint32_t* order, const float* cover) {
   CloudK c{};
   for (int a = 0; a < 3; a++) {
     c.pos[a] = sp->cloud.pos[a];
@@ -871,6 +919,7 @@ static CloudK cloud_k(const Splat* sp, const uint32_t* order) {
   c.count = count;
   c.order = order;
   c.scene_of = sp->scene_of;
+  c.cover = c.nrm[0] ? nullptr : cover;
   return c;
 }
 
@@ -927,11 +976,11 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   return CLAY_OK;
 }
 
-int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
+int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover) {
   Ctx* ctx = sp->ctx;
   MipsK mk_{};
   fill_mips(sp, &mk_);
-  const CloudK c = cloud_k(sp, order);
+  const CloudK c = cloud_k(sp, order, cover);
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
   if (sp->follow) {
     LaunchTimer t(ctx, s, "splat_follow");
@@ -978,7 +1027,7 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order) {
   return CLAY_OK;
 }
 
-int splat_bake(Splat* sp, hipStream_t s) { return splat_bake_ordered(sp, s, nullptr); }
+int splat_bake(Splat* sp, hipStream_t s) { return splat_bake_ordered(sp, s, nullptr, nullptr); }
 
 int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res, std::vector<size_t>* brick_off,
                          size_t* brick_total, std::vector<size_t>* tile_off, size_t* tile_total) {

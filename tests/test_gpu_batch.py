@@ -115,6 +115,31 @@ def test_following_origin_and_codes_match_the_oracle(root, floor):
     b.destroy()
 
 
+def test_codes_after_impact_match_the_oracle(root, floor):
+    """The surface bake skips, for each particle surrounded by neighbours in all 26
+    lattice directions, the voxels beyond its interior radius.  After 30 frames the
+    low blocks have hit the floor and spread (irregular neighbourhoods, many
+    particles with and without full cover); every scene's codes must still be the
+    oracle's full splat of the same positions, for the FP16-LDS frame as for f32."""
+    counts, pos = small_batch(seed=5)
+    for precision in ("fp16lds", "f32"):
+        opts = dict(scenes.batch_options(), precision=precision)
+        b = make_batch(root, floor, opts, counts, pos)
+        for _ in range(30):
+            b.step()
+        p = b.get_state("pos")
+        for s in range(len(counts)):
+            first, n = b.scene(s)
+            o = O.splat_opts(opts["bakeResolution"], opts["bakeMipLevels"], opts["followSize"],
+                             [float(v) for v in b.origin(s)], opts["bakeBand"], opts["surfaceRadius"])
+            grids = O.splat_bake(o, p[first:first + n])
+            for m in range(b.mips):
+                got = b.codes(s, m)
+                assert np.array_equal(got, grids[m].reshape(-1)), \
+                    f"{precision} scene {s} mip {m}: {(got != grids[m].reshape(-1)).sum()} codes differ"
+        b.destroy()
+
+
 def test_tuning_does_not_change_results(root, floor):
     counts, pos = small_batch(seed=11)
     opts = scenes.batch_options()
