@@ -621,9 +621,9 @@ __global__ void __launch_bounds__(kThreads)
       // A wave takes as long as its longest walk, so particles that walk their whole
       // reach (the fluid's surface) and interior ones (a voxel or so) go to different
       // waves: each sub-chunk is partitioned in LDS, surface particles first, and
-      // each part is cut into one contiguous run per thread; step k of every thread
-      // takes entry k of its run, so a wave's lanes hold particles a run apart
-      // (distinct voxels for their ds_min_u32).
+      // each part is walked in the spread lane order, so a wave's lanes hold
+      // particles of one part and far apart in it (distinct voxels for their
+      // ds_min_u32).
       for (uint32_t sub = 0; sub < n_e; sub += kPart) {
         const uint32_t mm = n_e - sub < (uint32_t)kPart ? n_e - sub : (uint32_t)kPart;
         if (t < 2) part_n[t] = 0;
@@ -637,13 +637,12 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
         const uint32_t ns = part_n[0], ni = mm - ns;
-        const uint32_t rs = (ns + kThreads - 1) / kThreads, ri = (ni + kThreads - 1) / kThreads;
-        for (uint32_t k = 0; k < rs; k++) {
-          const uint32_t e = (uint32_t)t * rs + k;
+        for (uint32_t e0 = 0; e0 < ns; e0 += kThreads) {
+          const uint32_t e = e0 + rev;
           if (e < ns) walk(part[e]);
         }
-        for (uint32_t k = 0; k < ri; k++) {
-          const uint32_t e = (uint32_t)t * ri + k;
+        for (uint32_t e0 = 0; e0 < ni; e0 += kThreads) {
+          const uint32_t e = e0 + rev;
           if (e < ni) walk(part[ns + e]);
         }
         __syncthreads();
