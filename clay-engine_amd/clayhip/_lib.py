@@ -151,6 +151,7 @@ class Tuning(C.Structure):
         ("fluid_xcd", C.c_int32),
         ("splat_xcd", C.c_int32),
         ("splat_owner", C.c_int32),
+        ("exact_lds", C.c_int32),
     ]
 
 

@@ -130,7 +130,7 @@ class SplatField:
     def setSchedule(self, chunk: int = 1024, groups_per_cu: int = 8, xcd: bool = False, owner: bool = True) -> None:
         """Launch shape of the tiled bake (results do not depend on it)."""
         t = L.Tuning(block=256, stage=128, chunk=int(chunk), groups_per_cu=int(groups_per_cu), fluid_xcd=0,
-                     splat_xcd=1 if xcd else 0, splat_owner=1 if owner else 0)
+                     splat_xcd=1 if xcd else 0, splat_owner=1 if owner else 0, exact_lds=1)
         L.check(L.lib().clay_splat_set_tuning(self._h, C.byref(t)))
 
     def codes(self, mip: int = 0) -> np.ndarray:

@@ -1309,9 +1309,10 @@ int apply_tuning(Fluid* f, const clay_tuning* t) {
   if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
     return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
   if ((t->fluid_xcd != 0 && t->fluid_xcd != 1) || (t->splat_xcd != 0 && t->splat_xcd != 1) ||
-      (t->splat_owner != 0 && t->splat_owner != 1))
-    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd, splat_xcd and splat_owner must be 0 or 1");
+      (t->splat_owner != 0 && t->splat_owner != 1) || (t->exact_lds != 0 && t->exact_lds != 1))
+    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd, splat_xcd, splat_owner and exact_lds must be 0 or 1");
   ft.xcd = t->fluid_xcd != 0;
+  ft.exact_lds = t->exact_lds != 0;
   f->tune = ft;
   f->surface->xcd = t->splat_xcd != 0;
   f->surface->owner = t->splat_owner != 0;
@@ -1332,6 +1333,7 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
   out->fluid_xcd = f->tune.xcd ? 1 : 0;
   out->splat_xcd = f->surface->xcd ? 1 : 0;
   out->splat_owner = f->surface->owner ? 1 : 0;
+  out->exact_lds = f->tune.exact_lds ? 1 : 0;
 }
 
 }  // namespace

@@ -137,9 +137,10 @@ struct Splat {
 
 // Tunable launch shapes of the neighbour kernels (results do not depend on them).
 struct FluidTune {
-  int block = 256;   // threads per workgroup: 128 or 256
-  int stage = 128;   // FP16-LDS staging slice per wave: 64, 128 or 256 neighbours
-  bool xcd = false;  // XCD-contiguous workgroup numbering in the neighbour kernels
+  int block = 256;         // threads per workgroup: 128 or 256
+  int stage = 128;         // LDS staging slice per wave: 64, 128 or 256 neighbours
+  bool xcd = false;        // XCD-contiguous workgroup numbering in the neighbour kernels
+  bool exact_lds = true;   // f32 precision: LDS-staged exact kernels (else global-memory walks)
 };
 bool fluid_tune_valid(const FluidTune& t);
 

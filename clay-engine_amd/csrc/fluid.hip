@@ -532,6 +532,156 @@ __global__ void __launch_bounds__(kB)
   if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
 }
 
+// ---------------------------------------------------------------------------
+// Exact LDS-staged variants of k_density / k_forces_integrate: the same wave row
+// staging as the FP16 kernels, but the staged copy is each neighbour's f32 data
+// and every term is the plain kernel's, evaluated in the same order with no
+// contraction, so the results are the plain path's bit for bit.  A row's three
+// cells are contiguous in sorted order, so walking them cell by cell from LDS
+// visits the candidates in the order of the plain walk.  1/rho of a neighbour is
+// taken once at staging: the same IEEE quotient the plain kernel takes per pair.
+// ---------------------------------------------------------------------------
+template <int kB, int kS, bool kCover>
+__global__ void __launch_bounds__(kB)
+    k_density_x(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
+                const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
+                const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co) {
+  __shared__ float4 stage[kB / 64][kS];
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
+  const bool active = s < n;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t epoch = *sk.epoch;
+  const uint4* cs = scene_cells(sk, cells, active ? scene_at(sk, skeys, s) : 0u);
+  f3 pi = mk(0.0f, 0.0f, 0.0f);
+  if (active) pi = mk(spx[s], spy[s], spz[s]);
+  int32_t b[3] = {0, 0, 0};
+  if (active) cell_of(k, pi, b);
+  float rho = 0.0f;
+  Cover cov;
+  auto term = [&](f3 d) {
+    const float r2 = dot3(d, d);
+    if (r2 < k.hh) {
+      const float w = k.hh - r2;
+      rho = rho + k.mass_poly6 * (w * w * w);
+    }
+    if (kCover) cov.add(d, r2, k.hh);
+  };
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const bool any = r.hi > r.lo;
+      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
+      if (staged)
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = make_float4(spx[t], spy[t], spz[t], 0.0f);
+      wave_sync();
+      if (staged) {
+        walk_cells(k, r, [&](uint32_t t, int) {
+          const float4 q = stage[wave][t - ulo];
+          term(mk(pi.x - q.x, pi.y - q.y, pi.z - q.z));
+        });
+      } else {
+        walk_row(k, r, [&](uint32_t t) { term(mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t])); });
+      }
+      wave_sync();
+    }
+  }
+  if (!active) return;
+  srho[s] = fmax_s(rho, k.rest_floor);
+  spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
+  if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
+}
+
+struct StagedX {
+  float4 pr;  // position and 1 / density
+  float4 vp;  // velocity and pressure
+};
+
+template <int kB, int kS>
+__global__ void __launch_bounds__(kB)
+    k_forces_integrate_x(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
+                         const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
+                         const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
+  __shared__ StagedX stage[kB / 64][kS];
+  const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
+  const bool active = s < n;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float* __restrict__ spx = in.p[0];
+  const float* __restrict__ spy = in.p[1];
+  const float* __restrict__ spz = in.p[2];
+  const float* __restrict__ svx = in.p[3];
+  const float* __restrict__ svy = in.p[4];
+  const float* __restrict__ svz = in.p[5];
+  const float* __restrict__ srho = in.p[6];
+  const float* __restrict__ spres = in.p[7];
+  const uint32_t epoch = *sk.epoch;
+  const uint32_t scene = active ? scene_at(sk, skeys, s) : 0u;
+  const uint4* cs = scene_cells(sk, cells, scene);
+  f3 pi = mk(0.0f, 0.0f, 0.0f), vi = mk(0.0f, 0.0f, 0.0f);
+  float rhoi = 1.0f, pri = 0.0f;
+  int32_t b[3] = {0, 0, 0};
+  if (active) {
+    pi = mk(spx[s], spy[s], spz[s]);
+    vi = mk(svx[s], svy[s], svz[s]);
+    rhoi = srho[s];
+    pri = spres[s];
+    cell_of(k, pi, b);
+  }
+  f3 fp = mk(0.0f, 0.0f, 0.0f);
+  f3 vsum = vi * k.h6;
+  float wsum = k.h6;
+  // k_forces_integrate's term, operation for operation.
+  auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float prj) {
+    if (t == s) return;
+    const float r2 = dot3(dv, dv);
+    if (r2 >= k.hh) return;
+    const float w0 = k.hh - r2;
+    const float w = w0 * w0 * w0;
+    vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
+    wsum = wsum + w;
+    const float r = sqrtf(r2);
+    if (r > 1e-6f) {
+      const float hr = k.h - r;
+      const float s2 = k.mass * (pri + prj) * 0.5f * inv * k.spiky * (hr * hr);
+      const float ir = 1.0f / r;
+      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+    }
+  };
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
+      const bool any = r.hi > r.lo;
+      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
+      if (staged)
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) {
+          StagedX e;
+          e.pr = make_float4(spx[t], spy[t], spz[t], 1.0f / srho[t]);
+          e.vp = make_float4(svx[t], svy[t], svz[t], spres[t]);
+          stage[wave][t - ulo] = e;
+        }
+      wave_sync();
+      if (staged) {
+        walk_cells(k, r, [&](uint32_t t, int) {
+          const StagedX& e = stage[wave][t - ulo];
+          term(t, mk(pi.x - e.pr.x, pi.y - e.pr.y, pi.z - e.pr.z), mk(e.vp.x, e.vp.y, e.vp.z), e.pr.w, e.vp.w);
+        });
+      } else {
+        walk_row(k, r, [&](uint32_t t) {
+          term(t, mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t],
+               spres[t]);
+        });
+      }
+      wave_sync();
+    }
+  }
+  if (!active) return;
+  finish_particle(k, sk, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
+                  contacts + (size_t)scene * sk.contact_stride);
+}
+
 struct StagedNbr {
   float4 vr;   // velocity and 1 / density
   uint2 off;   // f16 position offset from the centre of the particle's cell
@@ -783,13 +933,26 @@ __global__ void __launch_bounds__(kBlock)
 
 inline unsigned grid_for(uint32_t n, int block = kBlock) { return (n + block - 1) / block; }
 
+// Kernel family of a neighbour pass.
+enum NbrMode { kGlobal = 0, kHalfLds = 1, kExactLds = 2 };
+
 template <int kB, bool kCover>
 void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t n, const float* spx,
                       const float* spy, const float* spz, const uint32_t* skeys, const uint4* cells,
-                      float* srho, float* spres, int stage, CoverOut co) {
-  if (stage == 0) {
+                      float* srho, float* spres, int mode, int stage, CoverOut co) {
+  if (mode == kGlobal) {
     hipLaunchKernelGGL((k_density<kB, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz,
                        skeys, cells, srho, spres, co);
+  } else if (mode == kExactLds) {
+    if (stage == 64)
+      hipLaunchKernelGGL((k_density_x<kB, 64, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy,
+                         spz, skeys, cells, srho, spres, co);
+    else if (stage == 256)
+      hipLaunchKernelGGL((k_density_x<kB, 256, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
+                         spy, spz, skeys, cells, srho, spres, co);
+    else
+      hipLaunchKernelGGL((k_density_x<kB, 128, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
+                         spy, spz, skeys, cells, srho, spres, co);
   } else if (stage == 64) {
     hipLaunchKernelGGL((k_density_lds<kB, 64, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
                        spy, spz, skeys, cells, srho, spres, co);
@@ -805,10 +968,20 @@ void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t
 template <int kB>
 void launch_forces_t(hipStream_t s, const FluidK& k, const SceneK& sk, const ColliderDev& col, uint32_t n,
                      const SortedIn& in, const uint32_t* svals, const uint32_t* skeys, const uint4* cells,
-                     const StateOut& o, uint32_t* contacts, int stage) {
-  if (stage == 0) {
+                     const StateOut& o, uint32_t* contacts, int mode, int stage) {
+  if (mode == kGlobal) {
     hipLaunchKernelGGL(k_forces_integrate<kB>, dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n, in, svals,
                        skeys, cells, o, contacts);
+  } else if (mode == kExactLds) {
+    if (stage == 64)
+      hipLaunchKernelGGL((k_forces_integrate_x<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n, in,
+                         svals, skeys, cells, o, contacts);
+    else if (stage == 256)
+      hipLaunchKernelGGL((k_forces_integrate_x<kB, 256>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n,
+                         in, svals, skeys, cells, o, contacts);
+    else
+      hipLaunchKernelGGL((k_forces_integrate_x<kB, 128>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n,
+                         in, svals, skeys, cells, o, contacts);
   } else if (stage == 64) {
     hipLaunchKernelGGL((k_forces_integrate_lds<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n,
                        in, svals, skeys, cells, o, contacts);
@@ -858,18 +1031,18 @@ int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, con
                    const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover) {
   if (n == 0) return CLAY_OK;
   const uint4* cl = reinterpret_cast<const uint4*>(cells);
-  const int stage = lds ? tune.stage : 0;
+  const int mode = lds ? kHalfLds : (tune.exact_lds ? kExactLds : kGlobal);
   const CoverOut co{svals, cover};
   LaunchTimer t(c, s, lds ? "fluid_density_lds" : "fluid_density");
   if (cover) {
     if (tune.block == 128)
-      launch_density_t<128, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+      launch_density_t<128, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
     else
-      launch_density_t<256, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+      launch_density_t<256, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
   } else if (tune.block == 128) {
-    launch_density_t<128, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+    launch_density_t<128, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
   } else {
-    launch_density_t<256, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, stage, co);
+    launch_density_t<256, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
   }
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
@@ -885,12 +1058,12 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK
   StateOut o{};
   for (int p = 0; p < 11; p++) o.p[p] = out[p];
   const uint4* cl = reinterpret_cast<const uint4*>(cells);
-  const int stage = lds ? tune.stage : 0;
+  const int mode = lds ? kHalfLds : (tune.exact_lds ? kExactLds : kGlobal);
   LaunchTimer t(c, s, lds ? "fluid_forces_integrate_lds" : "fluid_forces_integrate");
   if (tune.block == 128)
-    launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
+    launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage);
   else
-    launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, stage);
+    launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

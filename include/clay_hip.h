@@ -401,6 +401,8 @@ typedef struct {
   int32_t splat_xcd;      /* 1: the same for splat work items */
   int32_t splat_owner;    /* 1: each particle binned once per mip to the brick holding it, which
                              stages itself plus a 5-voxel shell in LDS (when reach <= 5 voxels) */
+  int32_t exact_lds;      /* F32 precision: 1 stages neighbour rows in LDS as f32 (results are the
+                             global-memory kernels' bit for bit), 0 walks them in global memory */
 } clay_tuning;
 int clay_fluid_set_tuning(clay_fluid* f, const clay_tuning* t);
 /* The splat fields of clay_tuning (chunk, groups_per_cu, splat_xcd, splat_owner) for a
