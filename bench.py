@@ -356,7 +356,7 @@ def measure_batched(root, args, stream) -> dict:
 
 
 def measure_coupled(root, args, stream) -> dict:
-    """The coupled frame (fluid + kinematic body + eroding world per scene), timed
+    """The coupled frame (fluid + ClaySolver body + eroding world per scene), timed
     with FP16-LDS staging and with the plain f32 path on the same scenes."""
     import numpy as np
 

@@ -11,11 +11,13 @@ from .brush import compile_brushes, make_brush
 from .edit import SdfEditor, body_centroids, body_centroids_device
 from .field import TracedField, analyticField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
+from .pbd import ClaySolver
 from .splat import ParticleCloud, SplatField
 from .volume import Root, SdfVolume, derive
 
 __all__ = [
     "ClayError",
+    "ClaySolver",
     "FluidBatch",
     "FluidSim",
     "LIB_PATH",

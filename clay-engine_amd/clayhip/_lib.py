@@ -205,6 +205,33 @@ class EditorDesc(C.Structure):
 
 REDUCE_F32_TREE, REDUCE_F64 = 0, 1
 
+class SolverDesc(C.Structure):
+    _fields_ = [
+        ("gravity", C.c_double * 3),
+        ("dt", C.c_double),
+        ("substeps", C.c_int32),
+        ("rotation_iterations", C.c_int32),
+        ("radius", C.c_double),
+        ("friction", C.c_double),
+        ("damping", C.c_double),
+        ("max_speed", C.c_double),
+    ]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [
+        ("first", C.c_uint32),
+        ("count", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("stiffness", C.c_float),
+        ("plasticity", C.c_float),
+        ("pad", C.c_float),
+    ]
+
+
+BODY_KINEMATIC, BODY_WEIGHTLESS = 1, 2
+FREE_PARTICLE = 0xFFFFFFFF
+
 STRUCTS = {
     "clay_brush": Brush,
     "clay_volume_desc": VolumeDesc,
@@ -218,6 +245,8 @@ STRUCTS = {
     "clay_tuning": Tuning,
     "clay_batch_desc": BatchDesc,
     "clay_editor_desc": EditorDesc,
+    "clay_solver_desc": SolverDesc,
+    "clay_body_desc": BodyDesc,
 }
 
 P = C.c_void_p
@@ -317,6 +346,16 @@ SIGNATURES = {
     "clay_volume_instance_download": ([P, U32, I32, P], C.c_int),
     "clay_splat_create_scenes": ([P, C.POINTER(CloudDesc), C.POINTER(SplatDesc), U32, P, I32, P, PP], C.c_int),
     "clay_splat_instance_download": ([P, U32, I32, P], C.c_int),
+    "clay_solver_desc_default": ([C.POINTER(SolverDesc)], None),
+    "clay_solver_create": ([P, P, C.POINTER(SolverDesc), U32, U32, PP], C.c_int),
+    "clay_solver_destroy": ([P], C.c_int),
+    "clay_solver_set_bodies": ([P, C.POINTER(BodyDesc), U32], C.c_int),
+    "clay_solver_set_particles": ([P, P, P, P, U32], C.c_int),
+    "clay_solver_set_force": ([P, U32, DP, DP], C.c_int),
+    "clay_solver_step": ([P, P], C.c_int),
+    "clay_solver_get_particles": ([P, I32, P], C.c_int),
+    "clay_solver_get_bodies": ([P, P], C.c_int),
+    "clay_solver_positions": ([P, P], C.c_int),
 }
 
 FOLLOW_NONE, FOLLOW_BOUNDS, FOLLOW_CENTERS = 0, 1, 2

@@ -1,18 +1,20 @@
-"""A coupled frame over batched scenes: fluid, a kinematic clay body and an eroding world.
+"""A coupled frame over batched scenes: fluid, a clay body and an eroding world.
 
-Per scene, every frame:
-  1. the body's particles take their pose for the frame (a synthetic kinematic path:
-     rest shape rotated about y and carried around the scene; the reference moves
-     bodies with its PBD solver, which is not part of this path),
+Per scene, every frame (SoftBody.simulate, softbody.ts:405-429, then the fluid and
+the erosion it drives):
+  1. the body steps in a ClaySolver (pbd.ts:392-407) that holds every scene's body;
+     body s collides with its scene's world instance,
   2. the body centroids are reduced in FP64 on the device (ClaySolver.match's
      centroid, pbd.ts:240-266) and the body's 32^3 volume re-centres on them
-     (SoftBody.simulate: surface.setCenter(position), softbody.ts:424-428) and bakes,
+     (surface.setCenter(position), softbody.ts:424-428) and bakes,
   3. the fluid steps against union(world instance, body volume) (field.ts:246-315),
   4. its contacts become cut edits of the scene's world instance, whose touched
      tiles and their coarser mips are rebuilt (SdfEditor.flush, modify.ts:214-238).
 All of it runs on one stream with no host readback; scenes share every launch.
-The body state lives in torch CUDA tensors (the kinematic path is workload
-generation, not library functionality).
+The solver takes SoftBody's settings (softbody.ts:228-233, 285-297): stiffness
+0.55, plasticity 0.35 per second as a per-substep rate, gravity -18, friction
+0.45, 3 substeps, damping 0.997, speed cap 26, radius 0.6 particle spacings.
+The body's particles fill a jittered ball instead of the surface extractor's shell.
 """
 from __future__ import annotations
 
@@ -22,9 +24,9 @@ import numpy as np
 
 from . import scenes as S
 from .batch import FluidBatch
-from .brush import make_brush
 from .edit import SdfEditor, body_centroids_device
 from .field import unionField, volumeField
+from .pbd import ClaySolver
 from .splat import ParticleCloud, SplatField
 from .volume import Root, SdfVolume
 
@@ -52,12 +54,17 @@ def body_rest(count: int, radius: float, seed: int) -> np.ndarray:
     return (pts + (u * 2 - 1) * spacing * 0.2).astype(np.float32)
 
 
+FRAME = 1.0 / 60.0
+SUBSTEPS = 3
+
+
 class CoupledBatch:
     def __init__(self, root: Root, fluid_counts, fluid_pos: np.ndarray, fluid_options: dict, world: dict,
                  world_brushes, body_count: int = 4096, body_radius: float = 0.95, body_box: float = 2.7,
                  body_res: int = 32, body_mips: int = 2, body_splat_radius: float = 0.08,
                  erosion=(2.0, 0.12, 0.03), seed: int = 11, orbit: float = 1.5, height: float = 1.0,
                  phases=None, body_seeds=None):
+        """Body s starts at (orbit cos phase_s, height, orbit sin phase_s) and falls."""
         import torch
 
         self.root = root
@@ -65,27 +72,32 @@ class CoupledBatch:
         self.scenes = n = len(counts)
         self.body_count = body_count
         self.erosion = erosion
-        self.t = 0.0
-        self.dt = 1.0 / 60.0
         # world: one instance per scene, baked once and replicated
         self.world = SdfVolume(root, dict(world, instances=n))
         self.world.bakeBrushes(world_brushes)
         self.world.replicate()
         self.world_field = volumeField(self.world)
-        # bodies
+        # bodies: one solver, body s owns particles [s * body_count, (s + 1) * body_count)
         seeds = body_seeds if body_seeds is not None else [seed + s for s in range(n)]
         rest = np.concatenate([body_rest(body_count, body_radius, int(seeds[s])) for s in range(n)], axis=0)
-        self.rest = torch.from_numpy(np.ascontiguousarray(rest.T)).cuda()
-        self.pos = torch.empty_like(self.rest)
-        ph = phases if phases is not None else np.arange(n) * (2 * math.pi / max(n, 1))
-        self.phase = torch.tensor(np.repeat(np.asarray(ph, dtype=np.float32), body_count), device="cuda")
-        self.orbit, self.height = orbit, height
+        ph = np.asarray(phases if phases is not None else np.arange(n) * (2 * math.pi / max(n, 1)), np.float64)
+        centre = np.stack([orbit * np.cos(ph), np.full(n, height), orbit * np.sin(ph)], axis=-1)
+        start = (rest + np.repeat(centre, body_count, axis=0)).astype(np.float32)
+        spacing = body_radius * (4.0 / 3.0 * math.pi / body_count) ** (1.0 / 3.0)
+        self.solver = ClaySolver(root, self.world_field,
+                                 dict(gravity=(0.0, -18.0, 0.0), dt=FRAME, substeps=SUBSTEPS, radius=0.6 * spacing,
+                                      friction=0.45, damping=0.997, maxSpeed=26.0),
+                                 capacity=n * body_count, maxBodies=n)
+        plasticity = 1.0 - (1.0 - 0.35) ** (FRAME / SUBSTEPS)
+        self.solver.setBodies([dict(first=s * body_count, count=body_count, stiffness=0.55, plasticity=plasticity)
+                               for s in range(n)])
+        self.solver.set_particles(start, rest, np.repeat(np.arange(n, dtype=np.uint32), body_count))
+        self.pos = self.solver.positions()
         fc = np.stack([np.arange(n) * body_count, np.full(n, body_count)], axis=-1).astype(np.int32)
         self.body_fc = torch.from_numpy(fc.reshape(-1)).cuda()
         self.body_com = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
-        self._pose(torch.cuda.current_stream())
-        self.body_cloud = ParticleCloud(n * body_count, (self.pos[0], self.pos[1], self.pos[2]),
-                                        live=("prefix", n * body_count))
+        self._centroids(torch.cuda.current_stream())
+        self.body_cloud = ParticleCloud(n * body_count, self.pos, live=("prefix", n * body_count))
         self.body = SplatField(root, self.body_cloud,
                                dict(radius=body_splat_radius, resolution=body_res, mipLevels=body_mips, band=4,
                                     worldSize=body_box, origin=(0, 0, 0), mode="tiled"),
@@ -95,23 +107,16 @@ class CoupledBatch:
         self.fluid.set_state(fluid_pos)
         self.editor = SdfEditor(self.world)
 
-    def _pose(self, stream) -> None:
-        """Kinematic pose at time t: rest rotated by w t about y, centred on a circle."""
-        import torch
+    def _centroids(self, stream) -> None:
+        body_centroids_device(self.root, self.pos, self.body_fc, self.body_com, precision="f64", stream=stream)
 
-        with torch.cuda.stream(stream):
-            ang = self.phase + 0.8 * self.t
-            c, s = torch.cos(ang), torch.sin(ang)
-            rx, ry, rz = self.rest[0], self.rest[1], self.rest[2]
-            self.pos[0].copy_(c * rx + s * rz + self.orbit * torch.cos(self.phase + 0.5 * self.t))
-            self.pos[1].copy_(ry + self.height)
-            self.pos[2].copy_(c * rz - s * rx + self.orbit * torch.sin(self.phase + 0.5 * self.t))
+    def body_positions(self) -> np.ndarray:
+        """(scenes * body_count, 3) float32 body particle positions (synchronises)."""
+        return self.solver.get("pos")
 
     def step(self, stream) -> None:
-        self.t += self.dt
-        self._pose(stream)
-        body_centroids_device(self.root, (self.pos[0], self.pos[1], self.pos[2]), self.body_fc, self.body_com,
-                              precision="f64", stream=stream)
+        self.solver.step(stream)
+        self._centroids(stream)
         self.body.bake(stream)
         self.fluid.step(stream)
         v, r, b = self.erosion
@@ -122,6 +127,7 @@ class CoupledBatch:
         self.fluid.destroy()
         self.collider.destroy()
         self.body.destroy()
+        self.solver.destroy()
         self.world_field.destroy()
         self.world.destroy()
 

@@ -72,23 +72,6 @@ LaunchTimer::~LaunchTimer() {
 
 namespace {
 
-template <typename T>
-int dalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
-  if (e != hipSuccess)
-    return fail(CLAY_ERR_HIP, std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
-                                   " bytes): " + hipGetErrorString(e));
-  return CLAY_OK;
-}
-
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
 int volume_geometry(int32_t resolution, int32_t mips_req, clay_volume_info* out) {
   // volume.ts:139-179
   memset(out, 0, sizeof(*out));

@@ -69,6 +69,24 @@ inline hipStream_t pick(Ctx* c, clay_stream s) {
   return s ? reinterpret_cast<hipStream_t>(s) : c->stream;
 }
 
+// Device allocation of `count` elements (at least one), failing with the message.
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count);
+  if (e != hipSuccess)
+    return fail(CLAY_ERR_HIP, std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
+                                   " bytes): " + hipGetErrorString(e));
+  return CLAY_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
 // ---- objects ---------------------------------------------------------------
 struct Volume {
   Ctx* ctx = nullptr;

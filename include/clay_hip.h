@@ -452,6 +452,61 @@ int clay_batch_origin(clay_batch* b, uint32_t scene, float origin[3]);
 /* clay_fluid_conserved over one scene's particles. */
 int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]);
 
+/* ---- ClaySolver: shape-matched particle bodies (lib/src/sim/pbd.ts:75-424) ----
+ * A ParticleSet (lib/src/sim/particles.ts:18-226) of `capacity` particles and up
+ * to `max_bodies` bodies, stepped against a TracedField.  Per substep: predict
+ * (BDF2 + field collision with positional friction), match (centroids and the
+ * covariance, reduced per body in the reference's 256-lane order, then the
+ * warm-started polar rotation of Mueller et al. 2016), apply (shape matching,
+ * collision projection, plasticity), finalize (velocity from positions).  Body b
+ * collides with instance b of a per-scene world field (or the one world). */
+typedef struct clay_solver clay_solver;
+
+typedef struct {          /* ClaySolverOptions, pbd.ts:48-59 (defaults pbd.ts:102-115) */
+  double gravity[3];      /* (0, -9.81, 0) */
+  double dt;              /* 1/60 */
+  int32_t substeps;       /* 2 */
+  int32_t rotation_iterations; /* 2 */
+  double radius;          /* 0.1 */
+  double friction;        /* 0.35 */
+  double damping;         /* 0.995 */
+  double max_speed;       /* 40 */
+} clay_solver_desc;
+void clay_solver_desc_default(clay_solver_desc* d);
+
+#define CLAY_BODY_KINEMATIC 1u  /* BodyFlags.kinematic (particles.ts:38-43) */
+#define CLAY_BODY_WEIGHTLESS 2u /* BodyFlags.weightless */
+
+typedef struct {          /* Body (particles.ts:52-68), the host-owned fields */
+  uint32_t first, count;  /* the body's slice of particles */
+  uint32_t flags;
+  float stiffness;        /* ParticleSet.addBody default 0.5 */
+  float plasticity;       /* per substep; default 0 */
+  float pad;
+} clay_body_desc;
+
+/* new ParticleSet({capacity, maxBodies}) + new ClaySolver(set, world, options). */
+int clay_solver_create(clay_ctx* ctx, clay_field* world, const clay_solver_desc* desc, uint32_t capacity,
+                       uint32_t max_bodies, clay_solver** out);
+int clay_solver_destroy(clay_solver* s);
+/* ParticleSet.addBody / setBody for bodies [0, n): com = restCom = 0 and rot =
+ * identity, as addBody writes them; count of declared bodies = n. */
+int clay_solver_set_bodies(clay_solver* s, const clay_body_desc* bodies, uint32_t n);
+/* Particle slots [0, n): pos and rest (SoA host, 3*n floats each), owning body
+ * (0xFFFFFFFF = free); prev = pos, vel = velPrev = 0 (what an extractor writes). */
+int clay_solver_set_particles(clay_solver* s, const float* pos, const float* rest, const uint32_t* body,
+                              uint32_t n);
+/* ClaySolver.setForce (pbd.ts:372-386): acceleration for this frame, impulse once. */
+int clay_solver_set_force(clay_solver* s, uint32_t body, const double accel[3], const double impulse[3]);
+/* ClaySolver.step (pbd.ts:392-407), enqueued on the stream. */
+int clay_solver_step(clay_solver* s, clay_stream stream);
+/* Particle planes (SoA host, 3 * capacity floats): 0 pos, 1 prev, 2 vel, 3 velPrev, 4 rest. */
+int clay_solver_get_particles(clay_solver* s, int32_t what, float* host);
+/* Per body: com xyz, restCom xyz, rot xyzw (10 floats). */
+int clay_solver_get_bodies(clay_solver* s, float* host);
+/* Device pointers of the position planes (a ParticleCloud over the set). */
+int clay_solver_positions(clay_solver* s, const float* out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,6 +167,19 @@ void orc_tile_dilate(int res, int mips, int m, uint32_t* counts);
 int orc_contacts_to_edits(uint32_t raw_count, const float* points4, double min_speed, double radius,
                           double blend, int max_edits, orc_brush* out);
 
+/* ClaySolver (lib/src/sim/pbd.ts:12-40, 75-424): SolveParams in f64, rounded to the
+ * working precision once; `h` is dt / substeps.  Bodies: first, count, flags per
+ * body (3 u32) and stiffness, plasticity (2 f32, as the Body record holds them). */
+typedef struct {
+  double gravity[3];
+  double h;
+  double radius, friction, damping, max_speed;
+  double push[3];
+  double kick[3];
+  uint32_t push_body;
+  int32_t rot_iters;
+} orc_pbd_params;
+
 #define ORC_DECLARE(T, S)                                                              \
   void orc_keys_##S(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz, \
                     uint32_t* keys);                                                   \
@@ -204,7 +217,15 @@ int orc_contacts_to_edits(uint32_t raw_count, const float* points4, double min_s
                           int ne, uint16_t* dist, uint16_t* mat, uint32_t* counts);    \
   void orc_body_centroids_##S(int nb, const uint32_t* first_count, const float* px,    \
                               const float* py, const float* pz, const float* rx,       \
-                              const float* ry, const float* rz, double* out6);
+                              const float* ry, const float* rz, double* out6);         \
+  /* One ClaySolver.step: the kick when `kicking`, then `substeps` x (predict, match, \
+   * apply, finalize).  planes: 15 * capacity values, SoA pos, prev, vel, velPrev,    \
+   * rest (x, y, z each); state: 10 per body (com, restCom, rot xyzw). */             \
+  void orc_pbd_step_##S(const orc_pbd_params* k, const orc_collider* col, int capacity,\
+                        int max_bodies, int nbodies, const uint32_t* body3,             \
+                        const float* body2, const uint32_t* owner, T* planes, T* state, \
+                        int kicking, int substeps);                                     \
+  void orc_sin_cos_##S(T x, T* s, T* c);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

@@ -979,3 +979,228 @@ void FN(orc_body_centroids)(int nb, const uint32_t* first_count, const float* px
     for (int q = 0; q < 6; q++) out6[6 * b + q] = (double)(s[0][q] * inv);
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * ClaySolver (lib/src/sim/pbd.ts:141-364) over a ParticleSet
+ * (lib/src/sim/particles.ts:18-68).  WGSL leaves sin/cos and normalize to the
+ * implementation: sin/cos here are a Cody-Waite reduction to |r| <= pi/4 (the
+ * first constant has 8 significant bits, so k * 1.5703125 is exact) and the
+ * Taylor series of sin to r^9 and of cos to r^10 in Horner form (the choice the
+ * HIP kernels make; parity unpinned against any WGSL implementation, pinned
+ * against libm's sin/cos by tests/test_oracle_pbd.py); normalize(v) is v / length(v).
+ * ------------------------------------------------------------------------- */
+void FN(orc_sin_cos)(T x, T* s, T* c) {
+  const T k = t_floor(x * KF(0.636619772) + KF(0.5));
+  const T r = (x - k * KF(1.5703125)) - k * KF(4.83826794897e-4);
+  const T r2 = r * r;
+  const T sp = r + r * r2 * (KF(-0.166666667) + r2 * (KF(8.33333333e-3) + r2 * (KF(-1.98412698e-4) + r2 * KF(2.75573192e-6))));
+  const T cp = KF(1.0) + r2 * (KF(-0.5) + r2 * (KF(4.16666667e-2) + r2 * (KF(-1.38888889e-3) +
+                                                                          r2 * (KF(2.48015873e-5) - r2 * KF(2.75573192e-7)))));
+  const int q = ((int)k) & 3;
+  const T ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
+  *s = (q == 2 || q == 3) ? -ss : ss;
+  *c = (q == 1 || q == 2) ? -cc : cc;
+}
+
+typedef struct { T x, y, z, w; } q4;
+
+static q4 quat_mul(q4 a, q4 b) { /* math/gpu.ts:5-14 */
+  q4 r = {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+          a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+  return r;
+}
+
+static v3 quat_rotate(q4 q, v3 v) { /* math/gpu.ts:15-25 */
+  v3 qv = V(q.x, q.y, q.z);
+  v3 t = vmuls(vcross(qv, v), 2);
+  return vadd(vadd(v, vmuls(t, q.w)), vcross(qv, t));
+}
+
+/* extractRotation, math/gpu.ts:55-78 (quatToMat3 :32-45, quatFromAxisAngle :27-30) */
+static q4 extract_rotation(v3 a0, v3 a1, v3 a2, q4 q, int iterations) {
+  for (int i = 0; i < iterations; i++) {
+    T x = q.x, y = q.y, z = q.z, w = q.w;
+    v3 c0 = V(1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w));
+    v3 c1 = V(2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w));
+    v3 c2 = V(2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y));
+    T denom = t_fabs(vdot(c0, a0) + vdot(c1, a1) + vdot(c2, a2)) + KF(1e-9);
+    v3 omega = vmuls(vadd(vadd(vcross(c0, a0), vcross(c1, a1)), vcross(c2, a2)), (T)1 / denom);
+    T wl = vlen(omega);
+    if (wl < KF(1e-9)) break;
+    v3 axis = vmuls(omega, (T)1 / wl);
+    T half = wl * KF(0.5), sh, ch;
+    FN(orc_sin_cos)(half, &sh, &ch);
+    q4 a = {axis.x * sh, axis.y * sh, axis.z * sh, ch};
+    q4 m = quat_mul(a, q);
+    T l = t_sqrt(m.x * m.x + m.y * m.y + m.z * m.z + m.w * m.w);
+    q4 n = {m.x / l, m.y / l, m.z / l, m.w / l};
+    q = n;
+  }
+  return q;
+}
+
+typedef struct {
+  T *p[3], *q[3], *v[3], *u[3], *r[3];
+} pbd_planes;
+
+static v3 pl_ld(T* const a[3], int i) { return V(a[0][i], a[1][i], a[2][i]); }
+static void pl_st(T* const a[3], int i, v3 v) {
+  a[0][i] = v.x;
+  a[1][i] = v.y;
+  a[2][i] = v.z;
+}
+
+/* ownerOf, pbd.ts:125-139 */
+static uint32_t pbd_owner(int max_bodies, int nbodies, const uint32_t* body3, const uint32_t* owner, uint32_t i) {
+  const uint32_t b = owner[i];
+  if (b >= (uint32_t)max_bodies) return (uint32_t)max_bodies;
+  if (b >= (uint32_t)nbodies) return (uint32_t)max_bodies; /* undeclared: a zero record, count 0 */
+  const uint32_t first = body3[3 * b], count = body3[3 * b + 1];
+  if (i < first || i >= first + count) return (uint32_t)max_bodies;
+  return b;
+}
+
+static v3 v3_mix(v3 a, v3 b, T t) { return V(t_mix(a.x, b.x, t), t_mix(a.y, b.y, t), t_mix(a.z, b.z, t)); }
+
+void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capacity, int max_bodies, int nbodies,
+                      const uint32_t* body3, const float* body2, const uint32_t* owner, T* planes, T* state,
+                      int kicking, int substeps) {
+  pbd_planes P;
+  T** dst[5] = {P.p, P.q, P.v, P.u, P.r};
+  for (int f = 0; f < 5; f++)
+    for (int a = 0; a < 3; a++) dst[f][a] = planes + (size_t)(3 * f + a) * capacity;
+  const T h = rnd(k->h), radius = rnd(k->radius), friction = rnd(k->friction), damping = rnd(k->damping);
+  const T max_speed = rnd(k->max_speed);
+  const v3 g = V(rnd(k->gravity[0]), rnd(k->gravity[1]), rnd(k->gravity[2]));
+  const v3 push = V(rnd(k->push[0]), rnd(k->push[1]), rnd(k->push[2]));
+  const v3 kick = V(rnd(k->kick[0]), rnd(k->kick[1]), rnd(k->kick[2]));
+  const int use_col = col && col->nleaf > 0;
+  if (kicking) { /* pbd.ts:141-162 */
+    for (int i = 0; i < capacity; i++) {
+      uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
+      if (b != k->push_body || b >= (uint32_t)max_bodies) continue;
+      if (body3[3 * b + 2] & 1u) continue;
+      pl_st(P.v, i, vadd(pl_ld(P.v, i), kick));
+      pl_st(P.u, i, vadd(pl_ld(P.u, i), kick));
+      pl_st(P.q, i, vsub(pl_ld(P.q, i), vmuls(kick, h)));
+    }
+  }
+  for (int sub = 0; sub < substeps; sub++) {
+    /* predict, pbd.ts:164-213 */
+    for (int i = 0; i < capacity; i++) {
+      uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
+      if (b >= (uint32_t)max_bodies) continue;
+      uint32_t flags = body3[3 * b + 2];
+      if (flags & 1u) continue;
+      v3 acc = (flags & 2u) ? V(0, 0, 0) : g;
+      if (b == k->push_body) acc = vadd(acc, push);
+      v3 pos = pl_ld(P.p, i), prev = pl_ld(P.q, i), vel = pl_ld(P.v, i), velp = pl_ld(P.u, i);
+      const T third = (T)1 / (T)3;
+      const T h23 = (T)2 * h / (T)3;
+      v3 vpred = vadd(vmuls(vsub(vmuls(vel, 4), velp), third), vmuls(acc, h23));
+      v3 q = vadd(vmuls(vsub(vmuls(pos, 4), prev), third), vmuls(vpred, h23));
+      v3 prev_out = pos;
+      if (use_col) {
+        T dw;
+        int leaf = collider_pick(col, q, &dw);
+        if (dw < radius) {
+          v3 n = leaf_normal(&col->leaf[leaf], q);
+          T ps = radius - dw;
+          q = vadd(q, vmuls(n, ps));
+          prev_out = vadd(pos, vmuls(n, ps));
+          v3 dp = vsub(q, pos);
+          q = vsub(q, vmuls(vsub(dp, vmuls(n, vdot(dp, n))), friction));
+        }
+      }
+      pl_st(P.q, i, prev_out);
+      pl_st(P.p, i, q);
+    }
+    /* match, pbd.ts:215-297: 256 lanes per body, lane-strided sums, pairwise tree */
+    enum { R = 256 };
+    for (int b = 0; b < nbodies; b++) {
+      const uint32_t first = body3[3 * b], count = body3[3 * b + 1];
+      static T sh[R][9];
+      for (int li = 0; li < R; li++) {
+        v3 sp = V(0, 0, 0), sr = V(0, 0, 0);
+        for (uint32_t t = (uint32_t)li; t < count; t += R) {
+          sp = vadd(sp, pl_ld(P.p, (int)(first + t)));
+          sr = vadd(sr, pl_ld(P.r, (int)(first + t)));
+        }
+        sh[li][0] = sp.x; sh[li][1] = sp.y; sh[li][2] = sp.z;
+        sh[li][3] = sr.x; sh[li][4] = sr.y; sh[li][5] = sr.z;
+      }
+      for (int s = R / 2; s > 0; s >>= 1)
+        for (int li = 0; li < s; li++)
+          for (int c = 0; c < 6; c++) sh[li][c] = sh[li][c] + sh[li + s][c];
+      const T fc = (T)count;
+      const T inv = (T)1 / (fc < (T)1 ? (T)1 : fc);
+      const v3 com = vmuls(V(sh[0][0], sh[0][1], sh[0][2]), inv);
+      const v3 rcom = vmuls(V(sh[0][3], sh[0][4], sh[0][5]), inv);
+      for (int li = 0; li < R; li++) {
+        v3 a0 = V(0, 0, 0), a1 = V(0, 0, 0), a2 = V(0, 0, 0);
+        for (uint32_t t = (uint32_t)li; t < count; t += R) {
+          v3 pr = vsub(pl_ld(P.p, (int)(first + t)), com);
+          v3 rr = vsub(pl_ld(P.r, (int)(first + t)), rcom);
+          a0 = vadd(a0, vmuls(pr, rr.x));
+          a1 = vadd(a1, vmuls(pr, rr.y));
+          a2 = vadd(a2, vmuls(pr, rr.z));
+        }
+        sh[li][0] = a0.x; sh[li][1] = a0.y; sh[li][2] = a0.z;
+        sh[li][3] = a1.x; sh[li][4] = a1.y; sh[li][5] = a1.z;
+        sh[li][6] = a2.x; sh[li][7] = a2.y; sh[li][8] = a2.z;
+      }
+      for (int s = R / 2; s > 0; s >>= 1)
+        for (int li = 0; li < s; li++)
+          for (int c = 0; c < 9; c++) sh[li][c] = sh[li][c] + sh[li + s][c];
+      if (count > 0) {
+        T* st = state + 10 * b;
+        st[0] = com.x; st[1] = com.y; st[2] = com.z;
+        st[3] = rcom.x; st[4] = rcom.y; st[5] = rcom.z;
+        q4 q0 = {st[6], st[7], st[8], st[9]};
+        q4 q = extract_rotation(V(sh[0][0], sh[0][1], sh[0][2]), V(sh[0][3], sh[0][4], sh[0][5]),
+                                V(sh[0][6], sh[0][7], sh[0][8]), q0, k->rot_iters);
+        st[6] = q.x; st[7] = q.y; st[8] = q.z; st[9] = q.w;
+      }
+    }
+    /* apply, pbd.ts:299-335 */
+    for (int i = 0; i < capacity; i++) {
+      uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
+      if (b >= (uint32_t)max_bodies) continue;
+      if (body3[3 * b + 2] & 1u) continue;
+      const T* st = state + 10 * b;
+      q4 rot = {st[6], st[7], st[8], st[9]};
+      v3 com = V(st[0], st[1], st[2]), rcom = V(st[3], st[4], st[5]);
+      v3 pos = pl_ld(P.p, i), rest = pl_ld(P.r, i);
+      v3 target = vadd(com, quat_rotate(rot, vsub(rest, rcom)));
+      v3 q = v3_mix(pos, target, t_clamp((T)body2[2 * b], 0, 1));
+      if (use_col) {
+        T dw;
+        int leaf = collider_pick(col, q, &dw);
+        if (dw < radius) {
+          v3 n = leaf_normal(&col->leaf[leaf], q);
+          T ps = radius - dw;
+          q = vadd(q, vmuls(n, ps));
+          pl_st(P.q, i, vadd(pl_ld(P.q, i), vmuls(n, ps)));
+        }
+      }
+      pl_st(P.p, i, q);
+      const T plast = (T)body2[2 * b + 1];
+      if (plast > 0) {
+        q4 cj = {-rot.x, -rot.y, -rot.z, rot.w};
+        v3 rnew = vadd(quat_rotate(cj, vsub(q, com)), rcom);
+        pl_st(P.r, i, v3_mix(rest, rnew, t_clamp(plast, 0, 1)));
+      }
+    }
+    /* finalize, pbd.ts:337-364 */
+    for (int i = 0; i < capacity; i++) {
+      uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
+      if (b >= (uint32_t)max_bodies) continue;
+      if (body3[3 * b + 2] & 1u) continue;
+      v3 v = vmuls(vmuls(vsub(pl_ld(P.p, i), pl_ld(P.q, i)), (T)1 / t_max(h, KF(1e-6))), damping);
+      T speed = vlen(v);
+      if (speed > max_speed) v = vmuls(v, max_speed / speed);
+      pl_st(P.u, i, pl_ld(P.v, i));
+      pl_st(P.v, i, v);
+    }
+  }
+}

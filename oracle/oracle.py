@@ -134,6 +134,8 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_mip_refresh_{s}").argtypes = [I, I, I, D, D, P, P, P]
             getattr(L, f"orc_edit_flush_{s}").argtypes = [I, I, D, P, D, D, I, P, I, P, P, P]
             getattr(L, f"orc_body_centroids_{s}").argtypes = [I, P, P, P, P, P, P, P, P]
+            getattr(L, f"orc_pbd_step_{s}").argtypes = [C.POINTER(OrcPbdParams), COL, I, I, I, P, P, P, P, P, I, I]
+            getattr(L, f"orc_sin_cos_{s}").argtypes = [C.c_float if s == "f32" else C.c_double, P, P]
         L.orc_tile_dilate.argtypes = [I, I, I, P]
         L.orc_contacts_to_edits.argtypes = [C.c_uint32, P, C.c_double, C.c_double, C.c_double, I, P]
         L.orc_contacts_to_edits.restype = C.c_int
@@ -537,6 +539,90 @@ def contacts_to_edits(raw_count: int, points: np.ndarray, min_speed: float, radi
     k = lib().orc_contacts_to_edits(raw_count, pts.ctypes.data, C.c_double(min_speed), C.c_double(radius),
                                     C.c_double(blend), max_edits, arr)
     return [arr[i] for i in range(k)]
+
+
+class OrcPbdParams(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("h", C.c_double), ("radius", C.c_double), ("friction", C.c_double),
+                ("damping", C.c_double), ("max_speed", C.c_double), ("push", C.c_double * 3),
+                ("kick", C.c_double * 3), ("push_body", C.c_uint32), ("rot_iters", C.c_int32)]
+
+
+def sin_cos(x: float, precision: str = "f32") -> tuple[float, float]:
+    """The oracle's sin and cos (the ClaySolver's quatFromAxisAngle)."""
+    dt = C.c_float if precision == "f32" else C.c_double
+    s, c = dt(), dt()
+    getattr(lib(), f"orc_sin_cos_{precision}")(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+class Solver:
+    """ClaySolver (lib/src/sim/pbd.ts:75-424) over a ParticleSet, in the oracle.
+
+    State is held as numpy arrays: `planes` (15, capacity) = pos, prev, vel,
+    velPrev, rest (x, y, z each) and `state` (max_bodies, 10) = com, restCom, rot."""
+
+    def __init__(self, capacity: int, max_bodies: int, gravity=(0.0, -9.81, 0.0), dt: float = 1 / 60,
+                 substeps: int = 2, radius: float = 0.1, friction: float = 0.35, damping: float = 0.995,
+                 max_speed: float = 40.0, rotation_iterations: int = 2, collider: "Collider | None" = None,
+                 precision: str = "f32"):
+        self.capacity, self.max_bodies, self.substeps = capacity, max_bodies, substeps
+        self.precision = precision
+        self.dtype = np.float32 if precision == "f32" else np.float64
+        self.k = OrcPbdParams()
+        for i in range(3):
+            self.k.gravity[i] = gravity[i]
+        self.k.h = dt / substeps
+        self.k.radius, self.k.friction, self.k.damping, self.k.max_speed = radius, friction, damping, max_speed
+        self.k.push_body = 0xFFFFFFFF
+        self.k.rot_iters = rotation_iterations
+        self.collider = collider
+        self.planes = np.zeros((15, capacity), dtype=self.dtype)
+        self.owner = np.full(capacity, 0xFFFFFFFF, dtype=np.uint32)
+        self.body3 = np.zeros((max_bodies, 3), dtype=np.uint32)
+        self.body2 = np.zeros((max_bodies, 2), dtype=np.float32)
+        self.state = np.zeros((max_bodies, 10), dtype=self.dtype)
+        self.nbodies = 0
+        self.kicking = False
+
+    def set_bodies(self, bodies) -> None:
+        """bodies: dicts with first, count, flags, stiffness, plasticity."""
+        self.nbodies = len(bodies)
+        self.state[:] = 0
+        for b, d in enumerate(bodies):
+            self.body3[b] = (d["first"], d["count"], d.get("flags", 0))
+            self.body2[b] = (d.get("stiffness", 0.5), d.get("plasticity", 0.0))
+            self.state[b, 9] = 1.0
+
+    def set_particles(self, pos: np.ndarray, rest: np.ndarray, body: np.ndarray) -> None:
+        n = pos.shape[0]
+        p = np.asarray(pos, dtype=self.dtype).T
+        self.planes[:, :] = 0
+        self.planes[0:3, :n] = p
+        self.planes[3:6, :n] = p
+        self.planes[12:15, :n] = np.asarray(rest, dtype=self.dtype).T
+        self.owner[:] = 0xFFFFFFFF
+        self.owner[:n] = body
+
+    def set_force(self, body: int, accel, impulse=(0.0, 0.0, 0.0)) -> None:
+        for i in range(3):
+            self.k.push[i] = accel[i]
+            self.k.kick[i] = impulse[i]
+        self.k.push_body = body
+        self.kicking = any(v != 0 for v in impulse)
+
+    def step(self) -> None:
+        if self.nbodies == 0:
+            return
+        col = C.byref(self.collider.c) if self.collider is not None else None
+        getattr(lib(), f"orc_pbd_step_{self.precision}")(
+            C.byref(self.k), col, self.capacity, self.max_bodies, self.nbodies, self.body3.ctypes.data,
+            self.body2.ctypes.data, self.owner.ctypes.data, self.planes.ctypes.data, self.state.ctypes.data,
+            1 if self.kicking else 0, self.substeps)
+        self.kicking = False
+
+    def get(self, what: int) -> np.ndarray:
+        """(capacity, 3) of pos / prev / vel / velPrev / rest (0..4)."""
+        return self.planes[3 * what:3 * what + 3].T.copy()
 
 
 def body_centroids(first_count, pos: np.ndarray, rest: np.ndarray | None = None,

@@ -1,9 +1,9 @@
-"""The coupled frame (fluid + kinematic body + eroding world) over batched scenes.
+"""The coupled frame (fluid + ClaySolver body + eroding world) over batched scenes.
 
 Each object on its own is checked against the oracle elsewhere.  Here: scene s of a
 3-scene coupled batch must equal, bit for bit, the same scene run as a 1-scene
-coupled batch (fluid state, the world instance after erosion at every mip, the
-body volume), which checks every per-scene offset of the coupling; the body volume
+coupled batch (fluid state, body particles, the world instance after erosion at
+every mip, the body volume), which checks every per-scene offset of the coupling; the body volume
 must centre on the FP64 centroid; erosion must touch only binned tiles.
 """
 import numpy as np
@@ -48,6 +48,7 @@ def test_batched_coupling_equals_single_scene_coupling(root):
         batch.step(stream)
     torch.cuda.synchronize()
     bp = batch.fluid.get_state("pos")
+    bb = batch.body_positions()
     eroded = [batch.editor.readTileCounts(s) for s in range(3)]
     assert sum(e[0] for e in eroded) > 0, "no erosion happened"
     for s in range(3):
@@ -58,6 +59,8 @@ def test_batched_coupling_equals_single_scene_coupling(root):
         first, n = batch.fluid.scene(s)
         sp = single.fluid.get_state("pos")
         assert np.array_equal(bp[first:first + n].view(np.uint32), sp.view(np.uint32)), f"scene {s} fluid"
+        assert np.array_equal(bb[s * 512:(s + 1) * 512].view(np.uint32), single.body_positions().view(np.uint32)), \
+            f"scene {s} body particles"
         for m in range(3):
             assert np.array_equal(batch.world.download(m, instance=s), single.world.download(m)), f"scene {s} world mip {m}"
         for m in range(2):
@@ -74,7 +77,7 @@ def test_body_volume_follows_the_fp64_centroid(root):
     stream = torch.cuda.current_stream()
     c.step(stream)
     torch.cuda.synchronize()
-    pos = c.pos.cpu().numpy().T
+    pos = c.body_positions()
     fc = [(s * 512, 512) for s in range(2)]
     com = O.body_centroids(fc, pos, precision="f64")[:, :3]
     assert np.array_equal(c.body_com.cpu().numpy().reshape(2, 6)[:, :3].view(np.uint64), com.view(np.uint64))
