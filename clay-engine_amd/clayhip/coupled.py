@@ -2,7 +2,7 @@
 
 Per scene, every frame (SoftBody.simulate, softbody.ts:405-429, then the fluid and
 the erosion it drives):
-  1. the body steps in a ClaySolver (pbd.ts:392-407) that holds every scene's body;
+  1. the body steps in a ClaySolver (pbd.ts:400-415) that holds every scene's body;
      body s collides with its scene's world instance,
   2. the body centroids are reduced in FP64 on the device (ClaySolver.match's
      centroid, pbd.ts:240-266) and the body's 32^3 volume re-centres on them

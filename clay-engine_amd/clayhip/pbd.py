@@ -1,4 +1,4 @@
-"""ClaySolver and ParticleSet (lib/src/sim/pbd.ts:75-424, particles.ts:105-226) over the HIP library.
+"""ClaySolver and ParticleSet (lib/src/sim/pbd.ts:78-424, particles.ts:105-206) over the HIP library.
 
 A ParticleSet's particle and body records live on the GPU.  Bodies are declared
 with `addBody` (the reference's defaults: stiffness 0.5, plasticity 0, flags 0)
@@ -57,7 +57,7 @@ class ClaySolver:
         return len(self._bodies)
 
     def addBody(self, first: int, count: int, stiffness: float = 0.5, plasticity: float = 0.0, flags: int = 0) -> int:
-        """ParticleSet.addBody (particles.ts:141-168); the slice is given here instead
+        """ParticleSet.addBody (particles.ts:146-168); the slice is given here instead
         of being filled in by an extractor."""
         if len(self._bodies) >= self.maxBodies:
             raise L.ClayError(L.ERR_INVALID, f"ParticleSet: body limit {self.maxBodies} reached")
@@ -88,13 +88,13 @@ class ClaySolver:
         L.check(L.lib().clay_solver_set_particles(self._h, p.ctypes.data, r.ctypes.data, b.ctypes.data, p.shape[1]))
 
     def setForce(self, body: int, accel, impulse=(0.0, 0.0, 0.0)) -> None:
-        """ClaySolver.setForce (pbd.ts:372-386)."""
+        """ClaySolver.setForce (pbd.ts:383-394)."""
         a = (C.c_double * 3)(*accel)
         i = (C.c_double * 3)(*impulse)
         L.check(L.lib().clay_solver_set_force(self._h, int(body), a, i))
 
     def step(self, stream=None) -> None:
-        """ClaySolver.step (pbd.ts:392-407)."""
+        """ClaySolver.step (pbd.ts:400-415)."""
         L.check(L.lib().clay_solver_step(self._h, L.stream_handle(stream)))
 
     def get(self, what: str) -> np.ndarray:

@@ -1,7 +1,7 @@
 // pbd.hip -- ClaySolver: shape-matched particle bodies against a TracedField
-// (lib/src/sim/pbd.ts:75-424) over a ParticleSet (lib/src/sim/particles.ts:18-226).
+// (lib/src/sim/pbd.ts:78-424) over a ParticleSet (lib/src/sim/particles.ts:18-206).
 //
-// Per substep four launches, as the reference dispatches them (pbd.ts:392-407):
+// Per substep four launches, as the reference dispatches them (pbd.ts:400-415):
 //   k_pbd_predict   per particle: BDF2 predictor, field collision, positional friction
 //   k_pbd_match     per body (256 lanes): centroids, covariance, polar rotation
 //   k_pbd_apply     per particle: shape matching, collision projection, plasticity
@@ -118,7 +118,7 @@ __device__ f4 extract_rotation(f3 a0, f3 a1, f3 a2, f4 q, int iterations) {
   return q;
 }
 
-// ownerOf (pbd.ts:125-139): the particle's body, or max_bodies when the slot is
+// ownerOf (pbd.ts:131-140): the particle's body, or max_bodies when the slot is
 // not inside that body's [first, first + count).
 __device__ __forceinline__ uint32_t owner_of(const PbdK& k, const Parts& pt, const BodyDev* bodies, uint32_t i) {
   const uint32_t b = pt.body[i];
@@ -132,7 +132,7 @@ __device__ __forceinline__ f3 mix3(f3 a, f3 b, float t) {
   return mk(a.x * (1.0f - t) + b.x * t, a.y * (1.0f - t) + b.y * t, a.z * (1.0f - t) + b.z * t);
 }
 
-// kick (pbd.ts:141-162)
+// kick (pbd.ts:144-170)
 __global__ void __launch_bounds__(kGroup) k_pbd_kick(PbdK k, Parts pt, const BodyDev* __restrict__ bodies) {
   const uint32_t i = blockIdx.x * kGroup + threadIdx.x;
   if (i >= k.capacity) return;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(kGroup) k_pbd_kick(PbdK k, Parts pt, const Bod
   st(pt.q, i, ld(pt.q, i) - dv * k.h);
 }
 
-// predict (pbd.ts:164-213)
+// predict (pbd.ts:172-223)
 __global__ void __launch_bounds__(kGroup)
     k_pbd_predict(PbdK k, ColliderDev col, Parts pt, const BodyDev* __restrict__ bodies) {
   const uint32_t i = blockIdx.x * kGroup + threadIdx.x;
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kGroup)
   st(pt.p, i, q);
 }
 
-// match (pbd.ts:215-297): one workgroup of kReduce lanes per body.
+// match (pbd.ts:227-297): one workgroup of kReduce lanes per body.
 __global__ void __launch_bounds__(kReduce) k_pbd_match(PbdK k, Parts pt, BodyDev* __restrict__ bodies) {
   __shared__ float sh[9][kReduce];
   const uint32_t b = blockIdx.x;
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kReduce) k_pbd_match(PbdK k, Parts pt, BodyDev
   }
 }
 
-// apply (pbd.ts:299-335)
+// apply (pbd.ts:299-340)
 __global__ void __launch_bounds__(kGroup)
     k_pbd_apply(PbdK k, ColliderDev col, Parts pt, const BodyDev* __restrict__ bodies) {
   const uint32_t i = blockIdx.x * kGroup + threadIdx.x;
@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(kGroup)
   }
 }
 
-// finalize (pbd.ts:337-364)
+// finalize (pbd.ts:342-373)
 __global__ void __launch_bounds__(kGroup) k_pbd_finalize(PbdK k, Parts pt, const BodyDev* __restrict__ bodies) {
   const uint32_t i = blockIdx.x * kGroup + threadIdx.x;
   if (i >= k.capacity) return;

@@ -452,8 +452,8 @@ int clay_batch_origin(clay_batch* b, uint32_t scene, float origin[3]);
 /* clay_fluid_conserved over one scene's particles. */
 int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]);
 
-/* ---- ClaySolver: shape-matched particle bodies (lib/src/sim/pbd.ts:75-424) ----
- * A ParticleSet (lib/src/sim/particles.ts:18-226) of `capacity` particles and up
+/* ---- ClaySolver: shape-matched particle bodies (lib/src/sim/pbd.ts:78-424) ----
+ * A ParticleSet (lib/src/sim/particles.ts:18-206) of `capacity` particles and up
  * to `max_bodies` bodies, stepped against a TracedField.  Per substep: predict
  * (BDF2 + field collision with positional friction), match (centroids and the
  * covariance, reduced per body in the reference's 256-lane order, then the
@@ -462,7 +462,7 @@ int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]);
  * collides with instance b of a per-scene world field (or the one world). */
 typedef struct clay_solver clay_solver;
 
-typedef struct {          /* ClaySolverOptions, pbd.ts:48-59 (defaults pbd.ts:102-115) */
+typedef struct {          /* ClaySolverOptions, pbd.ts:44-55 (defaults pbd.ts:102-115) */
   double gravity[3];      /* (0, -9.81, 0) */
   double dt;              /* 1/60 */
   int32_t substeps;       /* 2 */
@@ -496,9 +496,9 @@ int clay_solver_set_bodies(clay_solver* s, const clay_body_desc* bodies, uint32_
  * (0xFFFFFFFF = free); prev = pos, vel = velPrev = 0 (what an extractor writes). */
 int clay_solver_set_particles(clay_solver* s, const float* pos, const float* rest, const uint32_t* body,
                               uint32_t n);
-/* ClaySolver.setForce (pbd.ts:372-386): acceleration for this frame, impulse once. */
+/* ClaySolver.setForce (pbd.ts:383-394): acceleration for this frame, impulse once. */
 int clay_solver_set_force(clay_solver* s, uint32_t body, const double accel[3], const double impulse[3]);
-/* ClaySolver.step (pbd.ts:392-407), enqueued on the stream. */
+/* ClaySolver.step (pbd.ts:400-415), enqueued on the stream. */
 int clay_solver_step(clay_solver* s, clay_stream stream);
 /* Particle planes (SoA host, 3 * capacity floats): 0 pos, 1 prev, 2 vel, 3 velPrev, 4 rest. */
 int clay_solver_get_particles(clay_solver* s, int32_t what, float* host);

@@ -167,7 +167,7 @@ void orc_tile_dilate(int res, int mips, int m, uint32_t* counts);
 int orc_contacts_to_edits(uint32_t raw_count, const float* points4, double min_speed, double radius,
                           double blend, int max_edits, orc_brush* out);
 
-/* ClaySolver (lib/src/sim/pbd.ts:12-40, 75-424): SolveParams in f64, rounded to the
+/* ClaySolver (lib/src/sim/pbd.ts:12-40, 78-424): SolveParams in f64, rounded to the
  * working precision once; `h` is dt / substeps.  Bodies: first, count, flags per
  * body (3 u32) and stiffness, plasticity (2 f32, as the Body record holds them). */
 typedef struct {

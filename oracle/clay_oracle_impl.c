@@ -981,7 +981,7 @@ void FN(orc_body_centroids)(int nb, const uint32_t* first_count, const float* px
 }
 
 /* ---------------------------------------------------------------------------
- * ClaySolver (lib/src/sim/pbd.ts:141-364) over a ParticleSet
+ * ClaySolver (lib/src/sim/pbd.ts:144-373) over a ParticleSet
  * (lib/src/sim/particles.ts:18-68).  WGSL leaves sin/cos and normalize to the
  * implementation: sin/cos here are a Cody-Waite reduction to |r| <= pi/4 (the
  * first constant has 8 significant bits, so k * 1.5703125 is exact) and the
@@ -1050,7 +1050,7 @@ static void pl_st(T* const a[3], int i, v3 v) {
   a[2][i] = v.z;
 }
 
-/* ownerOf, pbd.ts:125-139 */
+/* ownerOf, pbd.ts:131-140 */
 static uint32_t pbd_owner(int max_bodies, int nbodies, const uint32_t* body3, const uint32_t* owner, uint32_t i) {
   const uint32_t b = owner[i];
   if (b >= (uint32_t)max_bodies) return (uint32_t)max_bodies;
@@ -1075,7 +1075,7 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
   const v3 push = V(rnd(k->push[0]), rnd(k->push[1]), rnd(k->push[2]));
   const v3 kick = V(rnd(k->kick[0]), rnd(k->kick[1]), rnd(k->kick[2]));
   const int use_col = col && col->nleaf > 0;
-  if (kicking) { /* pbd.ts:141-162 */
+  if (kicking) { /* pbd.ts:144-170 */
     for (int i = 0; i < capacity; i++) {
       uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
       if (b != k->push_body || b >= (uint32_t)max_bodies) continue;
@@ -1086,7 +1086,7 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
     }
   }
   for (int sub = 0; sub < substeps; sub++) {
-    /* predict, pbd.ts:164-213 */
+    /* predict, pbd.ts:172-223 */
     for (int i = 0; i < capacity; i++) {
       uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
       if (b >= (uint32_t)max_bodies) continue;
@@ -1115,7 +1115,7 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
       pl_st(P.q, i, prev_out);
       pl_st(P.p, i, q);
     }
-    /* match, pbd.ts:215-297: 256 lanes per body, lane-strided sums, pairwise tree */
+    /* match, pbd.ts:227-297: 256 lanes per body, lane-strided sums, pairwise tree */
     enum { R = 256 };
     for (int b = 0; b < nbodies; b++) {
       const uint32_t first = body3[3 * b], count = body3[3 * b + 1];
@@ -1162,7 +1162,7 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
         st[6] = q.x; st[7] = q.y; st[8] = q.z; st[9] = q.w;
       }
     }
-    /* apply, pbd.ts:299-335 */
+    /* apply, pbd.ts:299-340 */
     for (int i = 0; i < capacity; i++) {
       uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
       if (b >= (uint32_t)max_bodies) continue;
@@ -1191,7 +1191,7 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
         pl_st(P.r, i, v3_mix(rest, rnew, t_clamp(plast, 0, 1)));
       }
     }
-    /* finalize, pbd.ts:337-364 */
+    /* finalize, pbd.ts:342-373 */
     for (int i = 0; i < capacity; i++) {
       uint32_t b = pbd_owner(max_bodies, nbodies, body3, owner, (uint32_t)i);
       if (b >= (uint32_t)max_bodies) continue;

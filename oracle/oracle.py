@@ -556,7 +556,7 @@ def sin_cos(x: float, precision: str = "f32") -> tuple[float, float]:
 
 
 class Solver:
-    """ClaySolver (lib/src/sim/pbd.ts:75-424) over a ParticleSet, in the oracle.
+    """ClaySolver (lib/src/sim/pbd.ts:78-424) over a ParticleSet, in the oracle.
 
     State is held as numpy arrays: `planes` (15, capacity) = pos, prev, vel,
     velPrev, rest (x, y, z each) and `state` (max_bodies, 10) = com, restCom, rot."""

@@ -1,4 +1,4 @@
-"""The HIP ClaySolver (lib/src/sim/pbd.ts:75-424) against the oracle, bit for bit.
+"""The HIP ClaySolver (lib/src/sim/pbd.ts:78-424) against the oracle, bit for bit.
 
 Every pass follows the reference's operation order with no contraction, the match
 reductions keep its lane-strided sums and pairwise tree, and sin/cos come from one

@@ -1,4 +1,4 @@
-"""The oracle's ClaySolver (lib/src/sim/pbd.ts:75-424), pinned on the CPU.
+"""The oracle's ClaySolver (lib/src/sim/pbd.ts:78-424), pinned on the CPU.
 
 * sin/cos (quatFromAxisAngle) against libm;
 * extractRotation (math/gpu.ts:55-78) recovering a known rotation;
@@ -91,7 +91,7 @@ def test_body_comes_to_rest_on_a_floor():
     """Dropped from 0.3 above a roundBox floor (top at y = -0.4), the body settles: its
     centroid stops moving and no particle ends closer to the surface than the solver
     radius.  (The contact particles keep a downward velocity: the pushout moves `prev`
-    with `pos`, pbd.ts:196-206, so the predicted motion survives into the velocity and
+    with `pos`, pbd.ts:201-219, so the predicted motion survives into the velocity and
     only the speed cap bounds it; the positions are projected out every substep.)"""
     floor = make_brush(scenes.floor_brush())
     col = O.Collider().add_analytic([floor])
