@@ -11,13 +11,14 @@ from .brush import compile_brushes, make_brush
 from .edit import SdfEditor, body_centroids, body_centroids_device
 from .field import TracedField, analyticField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
-from .pbd import ClaySolver
+from .pbd import ClaySolver, SurfaceExtractor
 from .splat import ParticleCloud, SplatField
 from .volume import Root, SdfVolume, derive
 
 __all__ = [
     "ClayError",
     "ClaySolver",
+    "SurfaceExtractor",
     "FluidBatch",
     "FluidSim",
     "LIB_PATH",

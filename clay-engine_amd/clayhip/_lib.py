@@ -229,6 +229,17 @@ class BodyDesc(C.Structure):
     ]
 
 
+class ExtractorDesc(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_uint32),
+        ("body", C.c_uint32),
+        ("base", C.c_uint32),
+        ("capacity", C.c_uint32),
+        ("motion", C.c_int32),
+        ("instance", C.c_uint32),
+    ]
+
+
 BODY_KINEMATIC, BODY_WEIGHTLESS = 1, 2
 FREE_PARTICLE = 0xFFFFFFFF
 
@@ -247,6 +258,7 @@ STRUCTS = {
     "clay_editor_desc": EditorDesc,
     "clay_solver_desc": SolverDesc,
     "clay_body_desc": BodyDesc,
+    "clay_extractor_desc": ExtractorDesc,
 }
 
 P = C.c_void_p
@@ -350,12 +362,22 @@ SIGNATURES = {
     "clay_solver_create": ([P, P, C.POINTER(SolverDesc), U32, U32, PP], C.c_int),
     "clay_solver_destroy": ([P], C.c_int),
     "clay_solver_set_bodies": ([P, C.POINTER(BodyDesc), U32], C.c_int),
+    "clay_solver_add_body": ([P, C.POINTER(BodyDesc), P], C.c_int),
+    "clay_solver_get_owners": ([P, P], C.c_int),
+    "clay_solver_get_slices": ([P, P], C.c_int),
     "clay_solver_set_particles": ([P, P, P, P, U32], C.c_int),
     "clay_solver_set_force": ([P, U32, DP, DP], C.c_int),
     "clay_solver_step": ([P, P], C.c_int),
     "clay_solver_get_particles": ([P, I32, P], C.c_int),
     "clay_solver_get_bodies": ([P, P], C.c_int),
     "clay_solver_positions": ([P, P], C.c_int),
+    "clay_extractor_create": ([P, P, C.POINTER(ExtractorDesc), PP], C.c_int),
+    "clay_extractor_destroy": ([P], C.c_int),
+    "clay_extractor_set_region": ([P, DP, C.c_double, DP, I32], C.c_int),
+    "clay_extractor_extract": ([P, P], C.c_int),
+    "clay_extractor_read": ([P, P, P, P], C.c_int),
+    "clay_extractor_quad_capacity": ([P, P], C.c_int),
+    "clay_extractor_motion": ([P, P, P], C.c_int),
 }
 
 FOLLOW_NONE, FOLLOW_BOUNDS, FOLLOW_CENTERS = 0, 1, 2

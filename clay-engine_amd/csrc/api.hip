@@ -218,6 +218,7 @@ Field* make_volume_field(Ctx* ctx, Volume* v) {
   l.kind = 1;
   l.vol = v->dev;
   l.dist0 = v->dist;
+  l.mat0 = v->mat;
   // One instance per scene: the collider reads the scene's own record and planes.
   l.scene_stride = v->instances > 1 ? (uint32_t)v->total_voxels : 0u;
   return f;
@@ -471,6 +472,7 @@ extern "C" int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int
   l.nbrush = n;
   l.brushes = f->brushes_dev;
   l.eps = (float)(opts && opts->epsilon > 0 ? opts->epsilon : 1e-3);
+  l.material = (float)(opts ? opts->material : 0.0);
   *out = reinterpret_cast<clay_field*>(f);
   return CLAY_OK;
 }

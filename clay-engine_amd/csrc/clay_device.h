@@ -48,6 +48,8 @@ struct LeafDev {
   uint32_t scene_stride;  // volume leaf of a batch: one instance per scene, planes this far apart
   const VolDev* vol;      // scene_stride != 0: an array of one record per scene
   const uint16_t* dist0;  // mip-0 distance plane (f16 bits), x fastest
+  const uint16_t* mat0;   // mip-0 material plane (f16 bits), same layout
+  float material;         // analytic: the material `field` reports (field.ts:213-216)
 };
 
 // unionField folded left over up to CLAY_MAX_LEAVES leaves (field.ts:246-315).
@@ -317,6 +319,16 @@ __device__ inline f3 leaf_normal(const LeafDev& l, uint32_t scene, f3 p) {
   return len > 1e-8f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
 }
 
+// field(p, 0).y of one leaf: analyticField's constant material, or volumeField's
+// trilinear material at mip 0 and 0 outside the volume box (volume.ts:288-296).
+__device__ inline float leaf_material(const LeafDev& l, uint32_t scene, f3 p) {
+  if (l.kind == 0) return l.material;
+  const VolDev& v = leaf_vol(l, scene);
+  f3 rel = mk(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
+  if (sd_box(rel, mk(v.half, v.half, v.half)) > 0.0f) return 0.0f;
+  return sample_raw0(v, l.mat0 + (size_t)l.scene_stride * scene, p);
+}
+
 // Returns the leaf the union selects (right child wins unless left is strictly nearer).
 __device__ inline int collider_pick(const ColliderDev& c, uint32_t scene, f3 p, float* dist) {
   int best = 0;
@@ -337,6 +349,13 @@ __device__ inline f3 collider_normal(const ColliderDev& c, uint32_t scene, int l
   for (int i = 0; i < CLAY_MAX_LEAVES; i++)
     if (i == leaf) return leaf_normal(c.leaf[i], scene, p);
   return mk(0.0f, 1.0f, 0.0f);
+}
+
+// unionField.field(p, 0).y: the material of the leaf the union selects at p.
+__device__ inline float collider_material(const ColliderDev& c, uint32_t scene, int leaf, f3 p) {
+  for (int i = 0; i < CLAY_MAX_LEAVES; i++)
+    if (i == leaf) return leaf_material(c.leaf[i], scene, p);
+  return 0.0f;
 }
 
 }  // namespace clay

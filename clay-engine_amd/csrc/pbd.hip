@@ -299,6 +299,7 @@ struct Solver {
   clay_solver_desc desc{};
   uint32_t capacity = 0, max_bodies = 0, declared = 0;
   float* planes = nullptr;  // 15 * capacity: pos, prev, vel, velPrev, rest (x, y, z each)
+  float* aux = nullptr;     // 4 * capacity: restNormal (x, y, z), material
   uint32_t* body = nullptr;
   BodyDev* bodies = nullptr;
   float push[3] = {0, 0, 0}, kick[3] = {0, 0, 0};
@@ -339,6 +340,7 @@ PbdK pbd_k(const Solver* s) {
 void solver_free(Solver* s) {
   if (!s) return;
   dfree(s->planes);
+  dfree(s->aux);
   dfree(s->body);
   dfree(s->bodies);
   delete s;
@@ -379,13 +381,15 @@ extern "C" int clay_solver_create(clay_ctx* ctx, clay_field* world, const clay_s
   s->capacity = capacity;
   s->max_bodies = max_bodies;
   int rc;
-  if ((rc = dalloc(&s->planes, (size_t)15 * capacity)) || (rc = dalloc(&s->body, capacity)) ||
+  if ((rc = dalloc(&s->planes, (size_t)15 * capacity)) || (rc = dalloc(&s->aux, (size_t)4 * capacity)) ||
+      (rc = dalloc(&s->body, capacity)) ||
       (rc = dalloc(&s->bodies, max_bodies))) {
     solver_free(s);
     return rc;
   }
   hipStream_t st = c->stream;
   if ((rc = launch_fill_u32(c, st, reinterpret_cast<uint32_t*>(s->planes), (size_t)15 * capacity, 0u)) ||
+      (rc = launch_fill_u32(c, st, reinterpret_cast<uint32_t*>(s->aux), (size_t)4 * capacity, 0u)) ||
       (rc = launch_fill_u32(c, st, s->body, capacity, 0xFFFFFFFFu)) ||
       (rc = launch_fill_u32(c, st, reinterpret_cast<uint32_t*>(s->bodies), sizeof(BodyDev) / 4 * max_bodies, 0u))) {
     solver_free(s);
@@ -427,6 +431,26 @@ extern "C" int clay_solver_set_bodies(clay_solver* sv, const clay_body_desc* b, 
   return CLAY_OK;
 }
 
+extern "C" int clay_solver_add_body(clay_solver* sv, const clay_body_desc* b, uint32_t* index) {
+  Solver* s = reinterpret_cast<Solver*>(sv);
+  if (!s || !b) return fail(CLAY_ERR_INVALID, "clay_solver_add_body: NULL argument");
+  if (s->declared >= s->max_bodies)
+    return fail(CLAY_ERR_INVALID, "ParticleSet: body limit " + std::to_string(s->max_bodies) + " reached");
+  BodyDev d{};
+  d.first = b->first;
+  d.count = b->count;
+  d.flags = b->flags;
+  d.stiffness = b->stiffness;
+  d.plasticity = b->plasticity;
+  d.rot[3] = 1.0f;
+  // Only the new record is written: the others keep the state the GPU maintains.
+  CLAY_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(s->bodies + s->declared, &d, sizeof(BodyDev), hipMemcpyHostToDevice));
+  if (index) *index = s->declared;
+  s->declared++;
+  return CLAY_OK;
+}
+
 extern "C" int clay_solver_set_particles(clay_solver* sv, const float* pos, const float* rest, const uint32_t* body,
                                          uint32_t n) {
   Solver* s = reinterpret_cast<Solver*>(sv);
@@ -444,6 +468,7 @@ extern "C" int clay_solver_set_particles(clay_solver* sv, const float* pos, cons
     CLAY_HIP_CHECK(hipMemcpy(s->planes + (12 + a) * cap, rest + (size_t)a * n, sizeof(float) * n,
                              hipMemcpyHostToDevice));
   }
+  for (int a = 0; a < 4 && n; a++) CLAY_HIP_CHECK(hipMemset(s->aux + a * cap, 0, sizeof(float) * n));
   if (n) CLAY_HIP_CHECK(hipMemcpy(s->body, body, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
   return CLAY_OK;
 }
@@ -502,10 +527,19 @@ extern "C" int clay_solver_step(clay_solver* sv, clay_stream stream) {
 extern "C" int clay_solver_get_particles(clay_solver* sv, int32_t what, float* host) {
   Solver* s = reinterpret_cast<Solver*>(sv);
   if (!s || !host) return fail(CLAY_ERR_INVALID, "clay_solver_get_particles: NULL argument");
-  if (what < 0 || what > 4) return fail(CLAY_ERR_INVALID, "clay_solver_get_particles: unknown field");
+  if (what < 0 || what > 6) return fail(CLAY_ERR_INVALID, "clay_solver_get_particles: unknown field");
   CLAY_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
-  CLAY_HIP_CHECK(hipMemcpy(host, s->planes + (size_t)3 * what * s->capacity, sizeof(float) * 3 * s->capacity,
-                           hipMemcpyDeviceToHost));
+  const size_t cap = s->capacity;
+  const float* src = what < 5 ? s->planes + 3 * what * cap : s->aux + 3 * (what - 5) * cap;
+  CLAY_HIP_CHECK(hipMemcpy(host, src, sizeof(float) * (what < 6 ? 3 : 1) * cap, hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_solver_get_owners(clay_solver* sv, uint32_t* host) {
+  Solver* s = reinterpret_cast<Solver*>(sv);
+  if (!s || !host) return fail(CLAY_ERR_INVALID, "clay_solver_get_owners: NULL argument");
+  CLAY_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+  CLAY_HIP_CHECK(hipMemcpy(host, s->body, sizeof(uint32_t) * s->capacity, hipMemcpyDeviceToHost));
   return CLAY_OK;
 }
 
@@ -526,9 +560,548 @@ extern "C" int clay_solver_get_bodies(clay_solver* sv, float* host) {
   return CLAY_OK;
 }
 
+extern "C" int clay_solver_get_slices(clay_solver* sv, uint32_t* host) {
+  Solver* s = reinterpret_cast<Solver*>(sv);
+  if (!s || !host) return fail(CLAY_ERR_INVALID, "clay_solver_get_slices: NULL argument");
+  CLAY_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+  std::vector<BodyDev> h(s->max_bodies);
+  CLAY_HIP_CHECK(hipMemcpy(h.data(), s->bodies, sizeof(BodyDev) * s->max_bodies, hipMemcpyDeviceToHost));
+  for (uint32_t b = 0; b < s->max_bodies; b++) {
+    host[3 * (size_t)b] = h[b].first;
+    host[3 * (size_t)b + 1] = h[b].count;
+    host[3 * (size_t)b + 2] = h[b].flags;
+  }
+  return CLAY_OK;
+}
+
 extern "C" int clay_solver_positions(clay_solver* sv, const float* out[3]) {
   Solver* s = reinterpret_cast<Solver*>(sv);
   if (!s || !out) return fail(CLAY_ERR_INVALID, "clay_solver_positions: NULL argument");
   for (int a = 0; a < 3; a++) out[a] = s->planes + (size_t)a * s->capacity;
+  return CLAY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SurfaceExtractor (lib/src/sim/extract.ts:85-484): naive surface nets over a
+// field's zero level set on a res^3 lattice, writing particles into a slice of the
+// set and a quad index list.  The reference takes particle and quad slots with
+// atomic counters, so their order is whatever the atomics give; here slots go in
+// ascending flat cell index (quads: lattice point, then edge axis x, y, z), which is
+// one of those orders and makes the extraction reproducible.  Each of the two grid
+// passes is three launches: mark (per-cell work plus the rank within the
+// workgroup), a scan of the workgroup counts, then the write.
+// ---------------------------------------------------------------------------
+namespace clay {
+
+namespace {
+
+constexpr int kExtGroup = 256;
+
+struct ExtK {
+  float origin[3];
+  float cell;
+  float vel[3];
+  uint32_t preserve;
+  uint32_t res, cells, body, base, capacity, quad_capacity, scene, set_capacity;
+  const float* motion;  // ExtractMotion {center, pad, velocity, pad}, or null
+};
+
+// The 12 lattice-edge accumulation of one cell (extract.ts:156-164, 285-297).
+struct Acc4 {
+  float x, y, z, w;
+};
+__device__ __forceinline__ Acc4 crossing(f3 pa, float da, f3 pb, float db) {
+  if (da * db >= 0.0f) return Acc4{0.0f, 0.0f, 0.0f, 0.0f};
+  const float t = fclamp_s(da / (da - db), 0.0f, 1.0f);
+  return Acc4{fmix(pa.x, pb.x, t), fmix(pa.y, pb.y, t), fmix(pa.z, pb.z, t), 1.0f};
+}
+__device__ __forceinline__ Acc4 operator+(Acc4 a, Acc4 b) { return Acc4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+
+__device__ __forceinline__ f3 scan_origin(const ExtK& k) {
+  f3 o = mk(k.origin[0], k.origin[1], k.origin[2]);
+  if (k.motion) o = o + mk(k.motion[0], k.motion[1], k.motion[2]);
+  return o;
+}
+
+__device__ __forceinline__ float field_dist(const ColliderDev& col, uint32_t scene, f3 p) {
+  float d;
+  (void)collider_pick(col, scene, p, &d);
+  return d;
+}
+
+// Rank of this lane's `count` (0..3) among the workgroup's lanes in order, and the
+// workgroup total, from two wave64 ballots per bit and a per-wave LDS table.
+__device__ __forceinline__ uint32_t group_rank(uint32_t count, uint32_t* total) {
+  __shared__ uint32_t wave_sum[kExtGroup / 64];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t m0 = __ballot(count & 1u), m1 = __ballot((count >> 1) & 1u);
+  const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+  const uint32_t in_wave = (uint32_t)__popcll(m0 & below) + 2u * (uint32_t)__popcll(m1 & below);
+  if (lane == 0) wave_sum[wave] = (uint32_t)__popcll(m0) + 2u * (uint32_t)__popcll(m1);
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < kExtGroup / 64; w++) {
+    if (w < wave) before += wave_sum[w];
+    all += wave_sum[w];
+  }
+  *total = all;
+  return before + in_wave;
+}
+
+__device__ __forceinline__ void lattice_of(const ExtK& k, uint32_t idx, uint32_t* i, uint32_t* j, uint32_t* l) {
+  *i = idx % k.res;
+  *j = (idx / k.res) % k.res;
+  *l = idx / (k.res * k.res);
+}
+
+// reset (extract.ts:193-254): snapshot the body's centre and mean velocity into
+// the motion record when asked to, before the cloud is replaced.
+__global__ void __launch_bounds__(kReduce) k_ext_reset(ExtK k, Parts pt, const BodyDev* __restrict__ bodies, float* motion) {
+  __shared__ float sh[6][kReduce];
+  const int li = threadIdx.x;
+  const uint32_t first = bodies[k.body].first, count = bodies[k.body].count;
+  const bool keep = k.preserve != 0u;
+  f3 ps = mk(0.0f, 0.0f, 0.0f), vs = mk(0.0f, 0.0f, 0.0f);
+  if (keep)
+    for (uint32_t t = (uint32_t)li; t < count && first + t < k.set_capacity; t += kReduce) {
+      ps = ps + ld(pt.p, first + t);
+      vs = vs + ld(pt.v, first + t);
+    }
+  sh[0][li] = ps.x; sh[1][li] = ps.y; sh[2][li] = ps.z;
+  sh[3][li] = vs.x; sh[4][li] = vs.y; sh[5][li] = vs.z;
+  __syncthreads();
+  for (int s = kReduce / 2; s > 0; s >>= 1) {
+    if (li < s)
+      for (int c = 0; c < 6; c++) sh[c][li] = sh[c][li] + sh[c][li + s];
+    __syncthreads();
+  }
+  if (li == 0 && keep && count > 0u) {
+    const float inv = 1.0f / (float)count;
+    for (int c = 0; c < 3; c++) {
+      motion[c] = sh[c][0] * inv;
+      motion[4 + c] = sh[3 + c][0] * inv;
+    }
+    motion[3] = 0.0f;
+    motion[7] = 0.0f;
+  }
+}
+
+// cell pass, mark (extract.ts:256-329): the cell's dual vertex, one Lipschitz snap,
+// and its material; rank = 1 + rank among the workgroup's vertices, 0 for none.
+__global__ void __launch_bounds__(kExtGroup)
+    k_ext_cells(ExtK k, ColliderDev col, float* __restrict__ vtx, uint32_t* __restrict__ rank,
+                uint32_t* __restrict__ group_count) {
+  const uint32_t idx = blockIdx.x * kExtGroup + threadIdx.x;
+  uint32_t has = 0;
+  if (idx < k.cells) {
+    uint32_t i, j, l;
+    lattice_of(k, idx, &i, &j, &l);
+    const float c = k.cell;
+    const f3 p0 = scan_origin(k) + mk((float)i, (float)j, (float)l) * c;
+    const f3 px = p0 + mk(c, 0.0f, 0.0f), py = p0 + mk(0.0f, c, 0.0f), pz = p0 + mk(0.0f, 0.0f, c);
+    const f3 pxy = p0 + mk(c, c, 0.0f), pxz = p0 + mk(c, 0.0f, c), pyz = p0 + mk(0.0f, c, c);
+    const f3 pxyz = p0 + mk(c, c, c);
+    const uint32_t sc = k.scene;
+    const float d0 = field_dist(col, sc, p0), dx = field_dist(col, sc, px), dy = field_dist(col, sc, py);
+    const float dz = field_dist(col, sc, pz), dxy = field_dist(col, sc, pxy), dxz = field_dist(col, sc, pxz);
+    const float dyz = field_dist(col, sc, pyz), dxyz = field_dist(col, sc, pxyz);
+    const Acc4 acc = crossing(p0, d0, px, dx) + crossing(py, dy, pxy, dxy) + crossing(pz, dz, pxz, dxz) +
+                     crossing(pyz, dyz, pxyz, dxyz) + crossing(p0, d0, py, dy) + crossing(px, dx, pxy, dxy) +
+                     crossing(pz, dz, pyz, dyz) + crossing(pxz, dxz, pxyz, dxyz) + crossing(p0, d0, pz, dz) +
+                     crossing(px, dx, pxz, dxz) + crossing(py, dy, pyz, dyz) + crossing(pxy, dxy, pxyz, dxyz);
+    if (!(acc.w < 0.5f)) {
+      has = 1;
+      const float inv = 1.0f / acc.w;
+      const f3 v = mk(acc.x * inv, acc.y * inv, acc.z * inv);
+      float dv;
+      const int lv = collider_pick(col, sc, v, &dv);
+      const f3 n = collider_normal(col, sc, lv, v);
+      const f3 p = v - n * dv;
+      float dp;
+      const int lp = collider_pick(col, sc, p, &dp);
+      const float mat = collider_material(col, sc, lp, p);
+      const size_t C = k.cells;
+      vtx[idx] = p.x;
+      vtx[C + idx] = p.y;
+      vtx[2 * C + idx] = p.z;
+      vtx[3 * C + idx] = n.x;
+      vtx[4 * C + idx] = n.y;
+      vtx[5 * C + idx] = n.z;
+      vtx[6 * C + idx] = mat;
+    }
+  }
+  uint32_t total;
+  const uint32_t r = group_rank(has, &total);
+  if (idx < k.cells) rank[idx] = has ? r + 1u : 0u;
+  if (threadIdx.x == 0) group_count[blockIdx.x] = total;
+}
+
+// Exclusive scan of the workgroup counts (one workgroup); the total goes to *total.
+__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
+                                                    uint32_t n, uint32_t* __restrict__ total) {
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += 1024) {
+    const uint32_t v = base + t < n ? cnt[base + t] : 0u;
+    buf[t] = v;
+    __syncthreads();
+    for (uint32_t s = 1; s < 1024; s <<= 1) {
+      const uint32_t add = t >= s ? buf[t - s] : 0u;
+      __syncthreads();
+      buf[t] += add;
+      __syncthreads();
+    }
+    if (base + t < n) off[base + t] = carry + buf[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+// cell pass, write: particle slot base + rank, or no vertex past the capacity.
+__global__ void __launch_bounds__(kExtGroup)
+    k_ext_emit(ExtK k, Parts pt, float* __restrict__ aux, uint32_t* __restrict__ owner, const float* __restrict__ vtx,
+               const uint32_t* __restrict__ rank, const uint32_t* __restrict__ group_off, uint32_t* __restrict__ ids) {
+  const uint32_t idx = blockIdx.x * kExtGroup + threadIdx.x;
+  if (idx >= k.cells) return;
+  const uint32_t r = rank[idx];
+  if (r == 0u) {
+    ids[idx] = 0u;
+    return;
+  }
+  const uint32_t slot = group_off[blockIdx.x] + r - 1u;
+  if (slot >= k.capacity) {
+    ids[idx] = 0u;
+    return;
+  }
+  const uint32_t id = k.base + slot;
+  const size_t C = k.cells, S = k.set_capacity;
+  const f3 p = mk(vtx[idx], vtx[C + idx], vtx[2 * C + idx]);
+  const f3 vel = k.motion ? mk(k.motion[4], k.motion[5], k.motion[6]) : mk(k.vel[0], k.vel[1], k.vel[2]);
+  st(pt.p, id, p);
+  st(pt.q, id, p);
+  st(pt.v, id, vel);
+  st(pt.u, id, vel);
+  st(pt.r, id, p);
+  aux[id] = vtx[3 * C + idx];
+  aux[S + id] = vtx[4 * C + idx];
+  aux[2 * S + id] = vtx[5 * C + idx];
+  aux[3 * S + id] = vtx[6 * C + idx];
+  owner[id] = k.body;
+  ids[idx] = id + 1u;
+}
+
+__device__ __forceinline__ uint32_t cell_id(const ExtK& k, const uint32_t* ids, uint32_t i, uint32_t j, uint32_t l) {
+  return ids[i + k.res * (j + k.res * l)];
+}
+
+// quad pass, mark (extract.ts:353-404): which of the three edges leaving lattice
+// point (i, j, l) towards +x, +y, +z straddle the surface with all four cells
+// holding a vertex (bits 0-2), whether the low end is inside (bit 3), and the rank
+// of its first quad within the workgroup (bits 4-).
+__global__ void __launch_bounds__(kExtGroup)
+    k_ext_quads(ExtK k, ColliderDev col, const uint32_t* __restrict__ ids, uint32_t* __restrict__ info,
+                uint32_t* __restrict__ group_count) {
+  const uint32_t idx = blockIdx.x * kExtGroup + threadIdx.x;
+  uint32_t mask = 0, inside = 0;
+  if (idx < k.cells) {
+    uint32_t i, j, l;
+    lattice_of(k, idx, &i, &j, &l);
+    const float c = k.cell;
+    const f3 o = scan_origin(k) + mk((float)i, (float)j, (float)l) * c;
+    const uint32_t sc = k.scene;
+    const float dl = field_dist(col, sc, o);
+    inside = dl < 0.0f ? 1u : 0u;
+    if (j >= 1 && l >= 1 && field_dist(col, sc, o + mk(c, 0.0f, 0.0f)) * dl < 0.0f &&
+        cell_id(k, ids, i, j - 1, l - 1) && cell_id(k, ids, i, j, l - 1) && cell_id(k, ids, i, j, l) &&
+        cell_id(k, ids, i, j - 1, l))
+      mask |= 1u;
+    if (i >= 1 && l >= 1 && field_dist(col, sc, o + mk(0.0f, c, 0.0f)) * dl < 0.0f &&
+        cell_id(k, ids, i - 1, j, l - 1) && cell_id(k, ids, i - 1, j, l) && cell_id(k, ids, i, j, l) &&
+        cell_id(k, ids, i, j, l - 1))
+      mask |= 2u;
+    if (i >= 1 && j >= 1 && field_dist(col, sc, o + mk(0.0f, 0.0f, c)) * dl < 0.0f &&
+        cell_id(k, ids, i - 1, j - 1, l) && cell_id(k, ids, i, j - 1, l) && cell_id(k, ids, i, j, l) &&
+        cell_id(k, ids, i - 1, j, l))
+      mask |= 4u;
+  }
+  uint32_t total;
+  const uint32_t n = (mask & 1u) + ((mask >> 1) & 1u) + ((mask >> 2) & 1u);
+  const uint32_t r = group_rank(n, &total);
+  if (idx < k.cells) info[idx] = mask | (inside << 3) | (r << 4);
+  if (threadIdx.x == 0) group_count[blockIdx.x] = total;
+}
+
+// emit (extract.ts:331-351): two triangles per quad, wound by the low end's sign.
+__device__ __forceinline__ void put_quad(const ExtK& k, uint32_t* indices, uint32_t q, uint32_t a, uint32_t b,
+                                         uint32_t cc, uint32_t dd, bool flip) {
+  if (q >= k.quad_capacity) return;
+  uint32_t* o = indices + (size_t)q * 6;
+  const uint32_t i0 = a - 1u, i1 = (flip ? dd : b) - 1u, i2 = cc - 1u, i3 = (flip ? b : dd) - 1u;
+  o[0] = i0;
+  o[1] = i1;
+  o[2] = i2;
+  o[3] = i0;
+  o[4] = i2;
+  o[5] = i3;
+}
+
+__global__ void __launch_bounds__(kExtGroup)
+    k_ext_quads_write(ExtK k, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ info,
+                      const uint32_t* __restrict__ group_off, uint32_t* __restrict__ indices) {
+  const uint32_t idx = blockIdx.x * kExtGroup + threadIdx.x;
+  if (idx >= k.cells) return;
+  const uint32_t f = info[idx];
+  if ((f & 7u) == 0u) return;
+  uint32_t i, j, l;
+  lattice_of(k, idx, &i, &j, &l);
+  const bool flip = ((f >> 3) & 1u) == 0u;  // emit(..., !inside)
+  uint32_t q = group_off[blockIdx.x] + (f >> 4);
+  if (f & 1u)
+    put_quad(k, indices, q++, cell_id(k, ids, i, j - 1, l - 1), cell_id(k, ids, i, j, l - 1), cell_id(k, ids, i, j, l),
+             cell_id(k, ids, i, j - 1, l), flip);
+  if (f & 2u)
+    put_quad(k, indices, q++, cell_id(k, ids, i - 1, j, l - 1), cell_id(k, ids, i - 1, j, l), cell_id(k, ids, i, j, l),
+             cell_id(k, ids, i, j, l - 1), flip);
+  if (f & 4u)
+    put_quad(k, indices, q, cell_id(k, ids, i - 1, j - 1, l), cell_id(k, ids, i, j - 1, l), cell_id(k, ids, i, j, l),
+             cell_id(k, ids, i - 1, j, l), flip);
+}
+
+// finish (extract.ts:406-441): the body's slice, centroid, identity rotation, and
+// the indexed draw's arguments.
+__global__ void __launch_bounds__(kReduce)
+    k_ext_finish(ExtK k, Parts pt, BodyDev* __restrict__ bodies, const uint32_t* __restrict__ counters,
+                 uint32_t* __restrict__ draw_args) {
+  __shared__ float sh[3][kReduce];
+  const int li = threadIdx.x;
+  const uint32_t n = counters[0] < k.capacity ? counters[0] : k.capacity;
+  f3 sum = mk(0.0f, 0.0f, 0.0f);
+  for (uint32_t t = (uint32_t)li; t < n; t += kReduce) sum = sum + ld(pt.r, k.base + t);
+  sh[0][li] = sum.x;
+  sh[1][li] = sum.y;
+  sh[2][li] = sum.z;
+  __syncthreads();
+  for (int s = kReduce / 2; s > 0; s >>= 1) {
+    if (li < s)
+      for (int c = 0; c < 3; c++) sh[c][li] = sh[c][li] + sh[c][li + s];
+    __syncthreads();
+  }
+  if (li == 0) {
+    const float inv = 1.0f / fmax_s((float)n, 1.0f);
+    BodyDev& b = bodies[k.body];
+    b.first = k.base;
+    b.count = n;
+    for (int c = 0; c < 3; c++) {
+      b.rest_com[c] = sh[c][0] * inv;
+      b.com[c] = sh[c][0] * inv;
+    }
+    b.rot[0] = 0.0f;
+    b.rot[1] = 0.0f;
+    b.rot[2] = 0.0f;
+    b.rot[3] = 1.0f;
+    const uint32_t q = counters[1] < k.quad_capacity ? counters[1] : k.quad_capacity;
+    draw_args[0] = q * 6u;
+    draw_args[1] = 1u;
+    draw_args[2] = 0u;
+    draw_args[3] = 0u;
+  }
+}
+
+}  // namespace
+
+struct Extractor {
+  Solver* set = nullptr;
+  Field* field = nullptr;
+  clay_extractor_desc desc{};
+  uint32_t cells = 0, groups = 0, quad_capacity = 0;
+  float origin[3] = {0, 0, 0}, cell = 0.1f, vel[3] = {0, 0, 0};
+  uint32_t preserve = 0;
+  float* motion = nullptr;     // 8 floats, or null
+  float* vtx = nullptr;        // 7 * cells: vertex xyz, normal xyz, material
+  uint32_t* rank = nullptr;    // per cell, then per lattice point (quad info)
+  uint32_t* ids = nullptr;     // per cell: particle id + 1, or 0
+  uint32_t* gcount = nullptr;  // per workgroup
+  uint32_t* goff = nullptr;    // per workgroup
+  uint32_t* counters = nullptr;  // particles, quads (as the reference's counters read), draw args (4)
+  uint32_t* indices = nullptr;   // quad_capacity * 6
+};
+
+namespace {
+
+void extractor_free(Extractor* e) {
+  if (!e) return;
+  dfree(e->motion);
+  dfree(e->vtx);
+  dfree(e->rank);
+  dfree(e->ids);
+  dfree(e->gcount);
+  dfree(e->goff);
+  dfree(e->counters);
+  dfree(e->indices);
+  delete e;
+}
+
+ExtK ext_k(const Extractor* e) {
+  ExtK k{};
+  for (int a = 0; a < 3; a++) {
+    k.origin[a] = e->origin[a];
+    k.vel[a] = e->vel[a];
+  }
+  k.cell = e->cell;
+  k.preserve = e->preserve;
+  k.res = e->desc.resolution;
+  k.cells = e->cells;
+  k.body = e->desc.body;
+  k.base = e->desc.base;
+  k.capacity = e->desc.capacity;
+  k.quad_capacity = e->quad_capacity;
+  k.scene = e->desc.instance;
+  k.set_capacity = e->set->capacity;
+  k.motion = e->motion;
+  return k;
+}
+
+}  // namespace
+
+}  // namespace clay
+
+extern "C" int clay_extractor_create(clay_solver* sv, clay_field* field, const clay_extractor_desc* d,
+                                     clay_extractor** out) {
+  Solver* s = reinterpret_cast<Solver*>(sv);
+  Field* f = reinterpret_cast<Field*>(field);
+  if (!s || !f || !d || !out) return fail(CLAY_ERR_INVALID, "clay_extractor_create: NULL argument");
+  const uint32_t res = d->resolution ? d->resolution : 32u;
+  if (res > 256) return fail(CLAY_ERR_INVALID, "SurfaceExtractor: resolution " + std::to_string(res) + " exceeds 256");
+  if ((uint64_t)d->base + d->capacity > s->capacity)
+    return fail(CLAY_ERR_INVALID, "SurfaceExtractor: slice [" + std::to_string(d->base) + ", " +
+                                      std::to_string((uint64_t)d->base + d->capacity) + ") exceeds set capacity");
+  if (d->body >= s->max_bodies)
+    return fail(CLAY_ERR_INVALID, "SurfaceExtractor: body " + std::to_string(d->body) + " is outside the set's " +
+                                      std::to_string(s->max_bodies) + " bodies");
+  Extractor* e = new (std::nothrow) Extractor();
+  if (!e) return fail(CLAY_ERR_INVALID, "out of host memory");
+  e->set = s;
+  e->field = f;
+  e->desc = *d;
+  e->desc.resolution = res;
+  e->cells = res * res * res;
+  e->groups = (e->cells + kExtGroup - 1) / kExtGroup;
+  // A closed surface-nets mesh has about as many quads as vertices (extract.ts:110-112).
+  e->quad_capacity = (uint32_t)std::ceil((double)d->capacity * 1.5);
+  Ctx* c = s->ctx;
+  int rc;
+  if ((d->motion && (rc = dalloc(&e->motion, 8))) || (rc = dalloc(&e->vtx, (size_t)7 * e->cells)) ||
+      (rc = dalloc(&e->rank, e->cells)) || (rc = dalloc(&e->ids, e->cells)) || (rc = dalloc(&e->gcount, e->groups)) ||
+      (rc = dalloc(&e->goff, e->groups)) || (rc = dalloc(&e->counters, 8)) ||
+      (rc = dalloc(&e->indices, (size_t)6 * std::max(e->quad_capacity, 1u)))) {
+    extractor_free(e);
+    return rc;
+  }
+  hipStream_t st = c->stream;
+  if ((e->motion && (rc = launch_fill_u32(c, st, reinterpret_cast<uint32_t*>(e->motion), 8, 0u))) ||
+      (rc = launch_fill_u32(c, st, e->ids, e->cells, 0u)) || (rc = launch_fill_u32(c, st, e->counters, 8, 0u)) ||
+      (rc = launch_fill_u32(c, st, e->indices, (size_t)6 * std::max(e->quad_capacity, 1u), 0u))) {
+    extractor_free(e);
+    return rc;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    extractor_free(e);
+    return fail(CLAY_ERR_HIP, "SurfaceExtractor: initialisation failed");
+  }
+  *out = reinterpret_cast<clay_extractor*>(e);
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_destroy(clay_extractor* eh) {
+  Extractor* e = reinterpret_cast<Extractor*>(eh);
+  if (e) (void)hipStreamSynchronize(e->set->ctx->stream);
+  extractor_free(e);
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_set_region(clay_extractor* eh, const double origin[3], double size, const double vel[3],
+                                         int32_t preserve_motion) {
+  Extractor* e = reinterpret_cast<Extractor*>(eh);
+  if (!e || !origin) return fail(CLAY_ERR_INVALID, "clay_extractor_set_region: NULL argument");
+  for (int a = 0; a < 3; a++) {
+    e->origin[a] = (float)origin[a];
+    e->vel[a] = vel ? (float)vel[a] : 0.0f;
+  }
+  e->cell = (float)(size / e->desc.resolution);  // size / this.resolution in f64, stored as f32
+  e->preserve = preserve_motion ? 1u : 0u;
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_extract(clay_extractor* eh, clay_stream stream) {
+  Extractor* e = reinterpret_cast<Extractor*>(eh);
+  if (!e) return fail(CLAY_ERR_INVALID, "clay_extractor_extract: NULL extractor");
+  Solver* s = e->set;
+  Ctx* c = s->ctx;
+  hipStream_t st = pick(c, stream);
+  const ExtK k = ext_k(e);
+  const Parts pt = parts_of(s);
+  const ColliderDev& col = e->field->col;
+  if (e->motion) {
+    LaunchTimer t(c, st, "extract_reset");
+    hipLaunchKernelGGL(k_ext_reset, dim3(1), dim3(kReduce), 0, st, k, pt, s->bodies, e->motion);
+  }
+  {
+    LaunchTimer t(c, st, "extract_cells");
+    hipLaunchKernelGGL(k_ext_cells, dim3(e->groups), dim3(kExtGroup), 0, st, k, col, e->vtx, e->rank, e->gcount);
+  }
+  hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, st, e->gcount, e->goff, e->groups, e->counters);
+  {
+    LaunchTimer t(c, st, "extract_emit");
+    hipLaunchKernelGGL(k_ext_emit, dim3(e->groups), dim3(kExtGroup), 0, st, k, pt, s->aux, s->body, e->vtx, e->rank,
+                       e->goff, e->ids);
+  }
+  {
+    LaunchTimer t(c, st, "extract_quads");
+    hipLaunchKernelGGL(k_ext_quads, dim3(e->groups), dim3(kExtGroup), 0, st, k, col, e->ids, e->rank, e->gcount);
+  }
+  hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, st, e->gcount, e->goff, e->groups, e->counters + 1);
+  hipLaunchKernelGGL(k_ext_quads_write, dim3(e->groups), dim3(kExtGroup), 0, st, k, e->ids, e->rank, e->goff,
+                     e->indices);
+  hipLaunchKernelGGL(k_ext_finish, dim3(1), dim3(kReduce), 0, st, k, pt, s->bodies, e->counters, e->counters + 4);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_read(clay_extractor* eh, uint32_t counts[2], uint32_t draw_args[4], uint32_t* indices) {
+  Extractor* e = reinterpret_cast<Extractor*>(eh);
+  if (!e) return fail(CLAY_ERR_INVALID, "clay_extractor_read: NULL extractor");
+  CLAY_HIP_CHECK(hipStreamSynchronize(e->set->ctx->stream));
+  uint32_t h[8];
+  CLAY_HIP_CHECK(hipMemcpy(h, e->counters, sizeof(h), hipMemcpyDeviceToHost));
+  if (counts) {
+    counts[0] = h[0];
+    counts[1] = h[1];
+  }
+  if (draw_args)
+    for (int i = 0; i < 4; i++) draw_args[i] = h[4 + i];
+  if (indices && e->quad_capacity)
+    CLAY_HIP_CHECK(hipMemcpy(indices, e->indices, sizeof(uint32_t) * 6 * e->quad_capacity, hipMemcpyDeviceToHost));
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_quad_capacity(const clay_extractor* eh, uint32_t* out) {
+  const Extractor* e = reinterpret_cast<const Extractor*>(eh);
+  if (!e || !out) return fail(CLAY_ERR_INVALID, "clay_extractor_quad_capacity: NULL argument");
+  *out = e->quad_capacity;
+  return CLAY_OK;
+}
+
+extern "C" int clay_extractor_motion(clay_extractor* eh, const float* in, float* out) {
+  Extractor* e = reinterpret_cast<Extractor*>(eh);
+  if (!e) return fail(CLAY_ERR_INVALID, "clay_extractor_motion: NULL extractor");
+  if (!e->motion) return fail(CLAY_ERR_INVALID, "SurfaceExtractor: created without a motion record");
+  CLAY_HIP_CHECK(hipStreamSynchronize(e->set->ctx->stream));
+  if (in) CLAY_HIP_CHECK(hipMemcpy(e->motion, in, sizeof(float) * 8, hipMemcpyHostToDevice));
+  if (out) CLAY_HIP_CHECK(hipMemcpy(out, e->motion, sizeof(float) * 8, hipMemcpyDeviceToHost));
   return CLAY_OK;
 }

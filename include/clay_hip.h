@@ -492,20 +492,71 @@ int clay_solver_destroy(clay_solver* s);
 /* ParticleSet.addBody / setBody for bodies [0, n): com = restCom = 0 and rot =
  * identity, as addBody writes them; count of declared bodies = n. */
 int clay_solver_set_bodies(clay_solver* s, const clay_body_desc* bodies, uint32_t n);
+/* ParticleSet.addBody (particles.ts:146-168): declares one more body and writes
+ * only its record; *index = its body index. Fails past max_bodies with the
+ * reference's message. */
+int clay_solver_add_body(clay_solver* s, const clay_body_desc* body, uint32_t* index);
 /* Particle slots [0, n): pos and rest (SoA host, 3*n floats each), owning body
- * (0xFFFFFFFF = free); prev = pos, vel = velPrev = 0 (what an extractor writes). */
+ * (0xFFFFFFFF = free); prev = pos, vel = velPrev = 0, restNormal = 0, material 0. */
 int clay_solver_set_particles(clay_solver* s, const float* pos, const float* rest, const uint32_t* body,
                               uint32_t n);
 /* ClaySolver.setForce (pbd.ts:383-394): acceleration for this frame, impulse once. */
 int clay_solver_set_force(clay_solver* s, uint32_t body, const double accel[3], const double impulse[3]);
 /* ClaySolver.step (pbd.ts:400-415), enqueued on the stream. */
 int clay_solver_step(clay_solver* s, clay_stream stream);
-/* Particle planes (SoA host, 3 * capacity floats): 0 pos, 1 prev, 2 vel, 3 velPrev, 4 rest. */
+/* Particle planes (SoA host, 3 * capacity floats): 0 pos, 1 prev, 2 vel, 3 velPrev,
+ * 4 rest, 5 restNormal; 6 material (capacity floats). */
 int clay_solver_get_particles(clay_solver* s, int32_t what, float* host);
+/* Owning body per particle (capacity u32). */
+int clay_solver_get_owners(clay_solver* s, uint32_t* host);
 /* Per body: com xyz, restCom xyz, rot xyzw (10 floats). */
 int clay_solver_get_bodies(clay_solver* s, float* host);
+/* Per body: first, count, flags (3 u32), as the GPU holds them. */
+int clay_solver_get_slices(clay_solver* s, uint32_t* host);
 /* Device pointers of the position planes (a ParticleCloud over the set). */
 int clay_solver_positions(clay_solver* s, const float* out[3]);
+
+/* ---- SurfaceExtractor (lib/src/sim/extract.ts:85-484) ----------------------
+ * Naive surface nets of a field's zero level set on a res^3 lattice: per cell the
+ * average of the 12 edge crossings, snapped once along the normal; the particles go
+ * to slots [base, base + capacity) of the set's particle planes (pos = prev = rest,
+ * vel = velPrev = the region's or the motion record's velocity, restNormal,
+ * material, owner = body) and one quad per lattice edge that straddles the surface
+ * to an index list of ceil(1.5 capacity) quads.  The reference takes slots with
+ * atomic counters; here particles are numbered in ascending cell index and quads in
+ * ascending lattice point then edge axis x, y, z, one of the orders it allows.
+ * `finish` sets the body's first/count, com = restCom = the centroid of the rest
+ * positions, rot = identity.  The set and the field must outlive the extractor. */
+typedef struct clay_extractor clay_extractor;
+
+typedef struct {
+  uint32_t resolution; /* lattice cells per axis; 0 -> 32 (extract.ts:105), at most 256 */
+  uint32_t body;       /* body index of the emitted particles */
+  uint32_t base;       /* first particle slot of the body's slice */
+  uint32_t capacity;   /* particles the extractor may emit */
+  int32_t motion;      /* nonzero: keep an ExtractMotion record (extract.ts:37-42) on the device */
+  uint32_t instance;   /* the instance an instanced field is read at (0 otherwise) */
+} clay_extractor_desc;
+
+/* new SurfaceExtractor(set, field, options) (extract.ts:103-444); fails with the
+ * reference's message when the slice exceeds the set capacity. */
+int clay_extractor_create(clay_solver* set, clay_field* field, const clay_extractor_desc* desc,
+                          clay_extractor** out);
+int clay_extractor_destroy(clay_extractor* e);
+/* SurfaceExtractor.setRegion (extract.ts:451-463): lattice origin (relative to the
+ * motion centre when there is a motion record), size = res * cell, velocity for
+ * emitted particles, and whether the next extract snapshots the body's motion. */
+int clay_extractor_set_region(clay_extractor* e, const double origin[3], double size, const double vel[3],
+                              int32_t preserve_motion);
+/* SurfaceExtractor.extract (extract.ts:469-475), enqueued on the stream. */
+int clay_extractor_extract(clay_extractor* e, clay_stream stream);
+/* Synchronises; counts = particles and quads found (before the capacities cut them),
+ * draw_args = (6 * quads kept, 1, 0, 0), indices = 6 * quad_capacity u32. Each may be NULL. */
+int clay_extractor_read(clay_extractor* e, uint32_t counts[2], uint32_t draw_args[4], uint32_t* indices);
+int clay_extractor_quad_capacity(const clay_extractor* e, uint32_t* out);
+/* The ExtractMotion record {centre xyz, 0, velocity xyz, 0}: written from `in`, then
+ * read into `out` (either may be NULL). Synchronises. */
+int clay_extractor_motion(clay_extractor* e, const float* in, float* out);
 
 #ifdef __cplusplus
 }

@@ -107,6 +107,8 @@ typedef struct {
   double world_size;
   double band;
   double grad_voxels;
+  const uint16_t* mat16;   /* volume: mip-0 `.g` channel (f16 bits), NULL reads 0 */
+  double material;         /* analytic: the material `field` reports (field.ts:213-216) */
 } orc_leaf;
 
 /* unionField over leaves, folded left: union(union(a, b), c) (field.ts:246-315). */
@@ -180,6 +182,17 @@ typedef struct {
   int32_t rot_iters;
 } orc_pbd_params;
 
+/* SurfaceExtractor (lib/src/sim/extract.ts:85-484): the region (f64, rounded
+ * once), the body and slice, and the quad capacity ceil(1.5 capacity). */
+typedef struct {
+  double origin[3];
+  double cell;             /* size / resolution */
+  double vel[3];
+  int32_t preserve;
+  int32_t res;
+  uint32_t body, base, capacity, quad_capacity;
+} orc_extract_params;
+
 #define ORC_DECLARE(T, S)                                                              \
   void orc_keys_##S(const orc_fluid_k* k, int n, const T* px, const T* py, const T* pz, \
                     uint32_t* keys);                                                   \
@@ -225,7 +238,16 @@ typedef struct {
                         int max_bodies, int nbodies, const uint32_t* body3,             \
                         const float* body2, const uint32_t* owner, T* planes, T* state, \
                         int kicking, int substeps);                                     \
-  void orc_sin_cos_##S(T x, T* s, T* c);
+  void orc_sin_cos_##S(T x, T* s, T* c);                                               \
+  /* One SurfaceExtractor.extract: reset (motion snapshot when `motion`), cells,      \
+   * quads, finish.  aux: 4 * capacity values (restNormal xyz, material); ids: res^3; \
+   * indices: 6 * quad_capacity; counts: particles and quads found; draw: 4 u32.      \
+   * Slots are taken in ascending cell index, quads in ascending lattice point then   \
+   * edge axis. */                                                                    \
+  void orc_extract_##S(const orc_extract_params* k, const orc_collider* col,          \
+                       int capacity, uint32_t* body3, T* planes, T* aux,               \
+                       uint32_t* owner, T* state, T* motion, uint32_t* ids,            \
+                       uint32_t* indices, uint32_t* counts, uint32_t* draw);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

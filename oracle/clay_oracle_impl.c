@@ -183,15 +183,17 @@ static volp vol_params(const orc_leaf* l) {
   return v;
 }
 
-static T texel(const orc_leaf* l, int x, int y, int z) {
-  const int n = l->res;
+static T texel_of(const uint16_t* plane, int n, int x, int y, int z) {
   x = x < 0 ? 0 : (x >= n ? n - 1 : x);
   y = y < 0 ? 0 : (y >= n ? n - 1 : y);
   z = z < 0 ? 0 : (z >= n ? n - 1 : z);
-  return (T)orc_f16_to_f32(l->dist16[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)]);
+  return (T)orc_f16_to_f32(plane[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * z)]);
 }
 
-static T sample_raw(const orc_leaf* l, const volp* v, v3 p) {
+#define texel(l, x, y, z) texel_of(plane, (l)->res, x, y, z)
+
+/* textureSampleLevel(linear, clamp-to-edge) of one channel at mip 0. */
+static T sample_plane(const orc_leaf* l, const uint16_t* plane, const volp* v, v3 p) {
   T u = (p.x - v->origin[0]) / v->extent;
   T w = (p.y - v->origin[1]) / v->extent;
   T s = (p.z - v->origin[2]) / v->extent;
@@ -208,6 +210,10 @@ static T sample_raw(const orc_leaf* l, const volp* v, v3 p) {
   T c1 = t_mix(c01, c11, fy);
   return t_mix(c0, c1, fz);
 }
+
+#undef texel
+
+static T sample_raw(const orc_leaf* l, const volp* v, v3 p) { return sample_plane(l, l->dist16, v, p); }
 
 /* SdfVolume.sampleDist at mip 0 (volume.ts:277-286); equals volumeField.sample(p,0).x. */
 static T vol_dist(const orc_leaf* l, const volp* v, v3 p) {
@@ -240,6 +246,17 @@ static v3 leaf_normal(const orc_leaf* l, v3 p) {
            vol_dist(l, &v, V(p.x, p.y, p.z + h)) - vol_dist(l, &v, V(p.x, p.y, p.z - h)));
   T len = vlen(g);
   return len > KF(1e-8) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
+}
+
+/* field(p, 0).y of one leaf: analyticField's constant material, or volumeField's
+ * trilinear material at mip 0 and 0 outside the volume box (volume.ts:288-296). */
+static T leaf_material(const orc_leaf* l, v3 p) {
+  if (l->kind == 0) return rnd(l->material);
+  volp v = vol_params(l);
+  v3 rel = V(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
+  if (sd_box(rel, V(v.half, v.half, v.half)) > 0) return 0;
+  if (!l->mat16) return 0;
+  return sample_plane(l, l->mat16, &v, p);
 }
 
 /* unionField fold: keep the left child unless the right one is strictly nearer
@@ -1203,4 +1220,197 @@ void FN(orc_pbd_step)(const orc_pbd_params* k, const orc_collider* col, int capa
       pl_st(P.v, i, v);
     }
   }
+}
+
+/* ---------------------------------------------------------------------------
+ * SurfaceExtractor.extract (lib/src/sim/extract.ts:193-441): reset, cells,
+ * quads, finish, with particle slots taken in ascending cell index and quads in
+ * ascending lattice point, then edge axis x, y, z.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  T x, y, z, w;
+} acc4;
+
+/* crossing (extract.ts:157-164): the edge's zero crossing, weight 1, or nothing. */
+static acc4 ext_crossing(v3 pa, T da, v3 pb, T db) {
+  acc4 r = {0, 0, 0, 0};
+  if (da * db >= 0) return r;
+  const T t = t_clamp(da / (da - db), 0, 1);
+  r.x = t_mix(pa.x, pb.x, t);
+  r.y = t_mix(pa.y, pb.y, t);
+  r.z = t_mix(pa.z, pb.z, t);
+  r.w = 1;
+  return r;
+}
+
+static acc4 acc_add(acc4 a, acc4 b) {
+  acc4 r = {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w};
+  return r;
+}
+
+static T ext_dist(const orc_collider* col, v3 p) {
+  T d;
+  (void)collider_pick(col, p, &d);
+  return d;
+}
+
+void FN(orc_extract)(const orc_extract_params* k, const orc_collider* col, int capacity, uint32_t* body3,
+                     T* planes, T* aux, uint32_t* owner, T* state, T* motion, uint32_t* ids, uint32_t* indices,
+                     uint32_t* counts, uint32_t* draw) {
+  pbd_planes P;
+  T** dst[5] = {P.p, P.q, P.v, P.u, P.r};
+  for (int f = 0; f < 5; f++)
+    for (int a = 0; a < 3; a++) dst[f][a] = planes + (size_t)(3 * f + a) * capacity;
+  const int res = k->res;
+  const size_t cells = (size_t)res * res * res;
+  const uint32_t body = k->body;
+  /* reset (extract.ts:197-247): the body's mean position and velocity, summed by
+   * 256 lanes strided then a pairwise tree, as the match pass sums. */
+  if (motion && k->preserve) {
+    const uint32_t first = body3[3 * body], count = body3[3 * body + 1];
+    v3 ps[256], vs[256];
+    for (int li = 0; li < 256; li++) {
+      ps[li] = V(0, 0, 0);
+      vs[li] = V(0, 0, 0);
+      for (uint32_t t = (uint32_t)li; t < count && first + t < (uint32_t)capacity; t += 256) {
+        ps[li] = vadd(ps[li], pl_ld(P.p, (int)(first + t)));
+        vs[li] = vadd(vs[li], pl_ld(P.v, (int)(first + t)));
+      }
+    }
+    for (int s2 = 128; s2 > 0; s2 >>= 1)
+      for (int li = 0; li < s2; li++) {
+        ps[li] = vadd(ps[li], ps[li + s2]);
+        vs[li] = vadd(vs[li], vs[li + s2]);
+      }
+    if (count > 0) {
+      const T inv = (T)1 / (T)count;
+      motion[0] = ps[0].x * inv;
+      motion[1] = ps[0].y * inv;
+      motion[2] = ps[0].z * inv;
+      motion[3] = 0;
+      motion[4] = vs[0].x * inv;
+      motion[5] = vs[0].y * inv;
+      motion[6] = vs[0].z * inv;
+      motion[7] = 0;
+    }
+  }
+  v3 org = V(rnd(k->origin[0]), rnd(k->origin[1]), rnd(k->origin[2]));
+  if (motion) org = vadd(org, V(motion[0], motion[1], motion[2]));
+  const v3 vel = motion ? V(motion[4], motion[5], motion[6]) : V(rnd(k->vel[0]), rnd(k->vel[1]), rnd(k->vel[2]));
+  const T c = rnd(k->cell);
+  /* cells (extract.ts:256-329) */
+  uint32_t found = 0;
+  for (size_t idx = 0; idx < cells; idx++) {
+    const int i = (int)(idx % res), j = (int)((idx / res) % res), l = (int)(idx / ((size_t)res * res));
+    const v3 p0 = vadd(org, vmuls(V((T)i, (T)j, (T)l), c));
+    const v3 px = vadd(p0, V(c, 0, 0)), py = vadd(p0, V(0, c, 0)), pz = vadd(p0, V(0, 0, c));
+    const v3 pxy = vadd(p0, V(c, c, 0)), pxz = vadd(p0, V(c, 0, c)), pyz = vadd(p0, V(0, c, c));
+    const v3 pxyz = vadd(p0, V(c, c, c));
+    const T d0 = ext_dist(col, p0), dx = ext_dist(col, px), dy = ext_dist(col, py), dz = ext_dist(col, pz);
+    const T dxy = ext_dist(col, pxy), dxz = ext_dist(col, pxz), dyz = ext_dist(col, pyz);
+    const T dxyz = ext_dist(col, pxyz);
+    acc4 acc = ext_crossing(p0, d0, px, dx);
+    acc = acc_add(acc, ext_crossing(py, dy, pxy, dxy));
+    acc = acc_add(acc, ext_crossing(pz, dz, pxz, dxz));
+    acc = acc_add(acc, ext_crossing(pyz, dyz, pxyz, dxyz));
+    acc = acc_add(acc, ext_crossing(p0, d0, py, dy));
+    acc = acc_add(acc, ext_crossing(px, dx, pxy, dxy));
+    acc = acc_add(acc, ext_crossing(pz, dz, pyz, dyz));
+    acc = acc_add(acc, ext_crossing(pxz, dxz, pxyz, dxyz));
+    acc = acc_add(acc, ext_crossing(p0, d0, pz, dz));
+    acc = acc_add(acc, ext_crossing(px, dx, pxz, dxz));
+    acc = acc_add(acc, ext_crossing(py, dy, pyz, dyz));
+    acc = acc_add(acc, ext_crossing(pxy, dxy, pxyz, dxyz));
+    if (acc.w < KF(0.5)) {
+      ids[idx] = 0;
+      continue;
+    }
+    const T inv = (T)1 / acc.w;
+    const v3 v = V(acc.x * inv, acc.y * inv, acc.z * inv);
+    T dv;
+    const int lv = collider_pick(col, v, &dv);
+    const v3 n = leaf_normal(&col->leaf[lv], v);
+    const v3 p = vsub(v, vmuls(n, dv));
+    const uint32_t slot = found++;
+    if (slot >= k->capacity) {
+      ids[idx] = 0;
+      continue;
+    }
+    T dp;
+    const int lp = collider_pick(col, p, &dp);
+    const uint32_t id = k->base + slot;
+    pl_st(P.p, (int)id, p);
+    pl_st(P.q, (int)id, p);
+    pl_st(P.v, (int)id, vel);
+    pl_st(P.u, (int)id, vel);
+    pl_st(P.r, (int)id, p);
+    aux[id] = n.x;
+    aux[(size_t)capacity + id] = n.y;
+    aux[2 * (size_t)capacity + id] = n.z;
+    aux[3 * (size_t)capacity + id] = leaf_material(&col->leaf[lp], p);
+    owner[id] = body;
+    ids[idx] = id + 1;
+  }
+  /* quads (extract.ts:331-404) */
+  uint32_t quads = 0;
+#define CID(a, b, cc) ids[(size_t)(a) + (size_t)res * ((size_t)(b) + (size_t)res * (size_t)(cc))]
+  for (size_t idx = 0; idx < cells; idx++) {
+    const int i = (int)(idx % res), j = (int)((idx / res) % res), l = (int)(idx / ((size_t)res * res));
+    const v3 o = vadd(org, vmuls(V((T)i, (T)j, (T)l), c));
+    const T dl = ext_dist(col, o);
+    const int flip = !(dl < 0);
+    for (int axis = 0; axis < 3; axis++) {
+      uint32_t a, b, cc, dd;
+      T de;
+      if (axis == 0) {
+        if (!(j >= 1 && l >= 1)) continue;
+        de = ext_dist(col, vadd(o, V(c, 0, 0)));
+        a = CID(i, j - 1, l - 1); b = CID(i, j, l - 1); cc = CID(i, j, l); dd = CID(i, j - 1, l);
+      } else if (axis == 1) {
+        if (!(i >= 1 && l >= 1)) continue;
+        de = ext_dist(col, vadd(o, V(0, c, 0)));
+        a = CID(i - 1, j, l - 1); b = CID(i - 1, j, l); cc = CID(i, j, l); dd = CID(i, j, l - 1);
+      } else {
+        if (!(i >= 1 && j >= 1)) continue;
+        de = ext_dist(col, vadd(o, V(0, 0, c)));
+        a = CID(i - 1, j - 1, l); b = CID(i, j - 1, l); cc = CID(i, j, l); dd = CID(i - 1, j, l);
+      }
+      if (!(de * dl < 0)) continue;
+      if (a == 0 || b == 0 || cc == 0 || dd == 0) continue;
+      const uint32_t q = quads++;
+      if (q >= k->quad_capacity) continue;
+      uint32_t* out = indices + (size_t)q * 6;
+      out[0] = a - 1;
+      out[1] = (flip ? dd : b) - 1;
+      out[2] = cc - 1;
+      out[3] = a - 1;
+      out[4] = cc - 1;
+      out[5] = (flip ? b : dd) - 1;
+    }
+  }
+#undef CID
+  /* finish (extract.ts:407-441) */
+  const uint32_t n = found < k->capacity ? found : k->capacity;
+  v3 sum[256];
+  for (int li = 0; li < 256; li++) {
+    sum[li] = V(0, 0, 0);
+    for (uint32_t t = (uint32_t)li; t < n; t += 256) sum[li] = vadd(sum[li], pl_ld(P.r, (int)(k->base + t)));
+  }
+  for (int s2 = 128; s2 > 0; s2 >>= 1)
+    for (int li = 0; li < s2; li++) sum[li] = vadd(sum[li], sum[li + s2]);
+  const T inv = (T)1 / t_max((T)n, 1);
+  const v3 com = vmuls(sum[0], inv);
+  body3[3 * body] = k->base;
+  body3[3 * body + 1] = n;
+  T* st = state + 10 * (size_t)body;
+  st[0] = com.x; st[1] = com.y; st[2] = com.z;
+  st[3] = com.x; st[4] = com.y; st[5] = com.z;
+  st[6] = 0; st[7] = 0; st[8] = 0; st[9] = 1;
+  counts[0] = found;
+  counts[1] = quads;
+  const uint32_t kept = quads < k->quad_capacity ? quads : k->quad_capacity;
+  draw[0] = kept * 6;
+  draw[1] = 1;
+  draw[2] = 0;
+  draw[3] = 0;
 }

@@ -60,6 +60,7 @@ class OrcLeaf(C.Structure):
         ("kind", C.c_int32), ("nbrush", C.c_int32), ("brushes", C.c_void_p), ("eps", C.c_double),
         ("dist16", C.c_void_p), ("res", C.c_int32), ("pad1", C.c_int32), ("origin", C.c_double * 3),
         ("world_size", C.c_double), ("band", C.c_double), ("grad_voxels", C.c_double),
+        ("mat16", C.c_void_p), ("material", C.c_double),
     ]
 
 
@@ -136,6 +137,8 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_body_centroids_{s}").argtypes = [I, P, P, P, P, P, P, P, P]
             getattr(L, f"orc_pbd_step_{s}").argtypes = [C.POINTER(OrcPbdParams), COL, I, I, I, P, P, P, P, P, I, I]
             getattr(L, f"orc_sin_cos_{s}").argtypes = [C.c_float if s == "f32" else C.c_double, P, P]
+            getattr(L, f"orc_extract_{s}").argtypes = [C.POINTER(OrcExtractParams), COL, I, P, P, P, P, P, P, P,
+                                                       P, P, P]
         L.orc_tile_dilate.argtypes = [I, I, I, P]
         L.orc_contacts_to_edits.argtypes = [C.c_uint32, P, C.c_double, C.c_double, C.c_double, I, P]
         L.orc_contacts_to_edits.restype = C.c_int
@@ -244,7 +247,7 @@ class Collider:
         self.c = OrcCollider()
         self._keep = []
 
-    def add_analytic(self, brush_records, eps: float = 1e-3) -> "Collider":
+    def add_analytic(self, brush_records, eps: float = 1e-3, material: float = 0.0) -> "Collider":
         arr = (OrcBrush * len(brush_records))()
         for i, b in enumerate(brush_records):
             C.memmove(C.byref(arr[i]), C.byref(b), C.sizeof(OrcBrush))
@@ -254,11 +257,13 @@ class Collider:
         leaf.nbrush = len(brush_records)
         leaf.brushes = C.cast(arr, C.c_void_p)
         leaf.eps = eps
+        leaf.material = material
         self.c.nleaf += 1
         return self
 
     def add_volume(self, dist16: np.ndarray, res: int, origin, world_size: float, band: float,
-                   grad_voxels: float = 1.0) -> "Collider":
+                   grad_voxels: float = 1.0, mat16: np.ndarray | None = None) -> "Collider":
+        """`mat16`: the mip-0 material channel (f16 bits); None reads material 0."""
         d = np.ascontiguousarray(dist16, dtype=np.uint16).reshape(-1)
         assert d.size == res ** 3
         self._keep.append(d)
@@ -271,6 +276,11 @@ class Collider:
         leaf.world_size = world_size
         leaf.band = band
         leaf.grad_voxels = grad_voxels
+        if mat16 is not None:
+            m = np.ascontiguousarray(mat16, dtype=np.uint16).reshape(-1)
+            assert m.size == res ** 3
+            self._keep.append(m)
+            leaf.mat16 = m.ctypes.data
         self.c.nleaf += 1
         return self
 
@@ -541,6 +551,14 @@ def contacts_to_edits(raw_count: int, points: np.ndarray, min_speed: float, radi
     return [arr[i] for i in range(k)]
 
 
+class OrcExtractParams(C.Structure):
+    _fields_ = [
+        ("origin", C.c_double * 3), ("cell", C.c_double), ("vel", C.c_double * 3), ("preserve", C.c_int32),
+        ("res", C.c_int32), ("body", C.c_uint32), ("base", C.c_uint32), ("capacity", C.c_uint32),
+        ("quad_capacity", C.c_uint32),
+    ]
+
+
 class OrcPbdParams(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("h", C.c_double), ("radius", C.c_double), ("friction", C.c_double),
                 ("damping", C.c_double), ("max_speed", C.c_double), ("push", C.c_double * 3),
@@ -577,6 +595,7 @@ class Solver:
         self.k.rot_iters = rotation_iterations
         self.collider = collider
         self.planes = np.zeros((15, capacity), dtype=self.dtype)
+        self.aux = np.zeros((4, capacity), dtype=self.dtype)  # restNormal xyz, material
         self.owner = np.full(capacity, 0xFFFFFFFF, dtype=np.uint32)
         self.body3 = np.zeros((max_bodies, 3), dtype=np.uint32)
         self.body2 = np.zeros((max_bodies, 2), dtype=np.float32)
@@ -593,10 +612,23 @@ class Solver:
             self.body2[b] = (d.get("stiffness", 0.5), d.get("plasticity", 0.0))
             self.state[b, 9] = 1.0
 
+    def add_body(self, first: int = 0, count: int = 0, stiffness: float = 0.5, plasticity: float = 0.0,
+                 flags: int = 0) -> int:
+        """ParticleSet.addBody: one more record, the others untouched."""
+        b = self.nbodies
+        assert b < self.max_bodies, f"ParticleSet: body limit {self.max_bodies} reached"
+        self.body3[b] = (first, count, flags)
+        self.body2[b] = (stiffness, plasticity)
+        self.state[b] = 0
+        self.state[b, 9] = 1.0
+        self.nbodies += 1
+        return b
+
     def set_particles(self, pos: np.ndarray, rest: np.ndarray, body: np.ndarray) -> None:
         n = pos.shape[0]
         p = np.asarray(pos, dtype=self.dtype).T
         self.planes[:, :] = 0
+        self.aux[:, :] = 0
         self.planes[0:3, :n] = p
         self.planes[3:6, :n] = p
         self.planes[12:15, :n] = np.asarray(rest, dtype=self.dtype).T
@@ -623,6 +655,41 @@ class Solver:
     def get(self, what: int) -> np.ndarray:
         """(capacity, 3) of pos / prev / vel / velPrev / rest (0..4)."""
         return self.planes[3 * what:3 * what + 3].T.copy()
+
+
+class Extractor:
+    """SurfaceExtractor (lib/src/sim/extract.ts:85-484) writing into an oracle Solver's
+    particle set; particle slots in ascending cell index, quads in ascending lattice
+    point then edge axis."""
+
+    def __init__(self, solver: Solver, collider: "Collider", resolution: int = 32, body: int = 0, base: int = 0,
+                 capacity: int = 3000, motion: bool = False):
+        assert base + capacity <= solver.capacity, "SurfaceExtractor: slice exceeds set capacity"
+        self.s, self.col = solver, collider
+        self.k = OrcExtractParams()
+        self.k.res, self.k.body, self.k.base, self.k.capacity = resolution, body, base, capacity
+        self.k.quad_capacity = int(np.ceil(capacity * 1.5))
+        self.k.cell = 0.1
+        self.motion = np.zeros(8, dtype=solver.dtype) if motion else None
+        self.ids = np.zeros(resolution ** 3, dtype=np.uint32)
+        self.indices = np.zeros(6 * max(self.k.quad_capacity, 1), dtype=np.uint32)
+        self.counts = np.zeros(2, dtype=np.uint32)
+        self.draw = np.zeros(4, dtype=np.uint32)
+
+    def set_region(self, origin, size: float, vel=(0.0, 0.0, 0.0), preserve_motion: bool = False) -> None:
+        for i in range(3):
+            self.k.origin[i] = origin[i]
+            self.k.vel[i] = vel[i]
+        self.k.cell = size / self.k.res
+        self.k.preserve = 1 if preserve_motion else 0
+
+    def extract(self) -> None:
+        s = self.s
+        getattr(lib(), f"orc_extract_{s.precision}")(
+            C.byref(self.k), C.byref(self.col.c), s.capacity, s.body3.ctypes.data, s.planes.ctypes.data,
+            s.aux.ctypes.data, s.owner.ctypes.data, s.state.ctypes.data,
+            None if self.motion is None else self.motion.ctypes.data, self.ids.ctypes.data,
+            self.indices.ctypes.data, self.counts.ctypes.data, self.draw.ctypes.data)
 
 
 def body_centroids(first_count, pos: np.ndarray, rest: np.ndarray | None = None,
