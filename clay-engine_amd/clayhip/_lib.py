@@ -67,6 +67,19 @@ class VolumeDesc(C.Structure):
     ]
 
 
+class WireHeader(C.Structure):
+    _fields_ = [
+        ("magic", C.c_uint32), ("version", C.c_uint32), ("resolution", C.c_uint32), ("mip_levels", C.c_uint32),
+        ("band", C.c_float), ("world_size", C.c_float), ("gradient_voxels", C.c_float), ("tiles", C.c_uint32),
+        ("origin", C.c_double * 3), ("stored_tiles", C.c_uint32), ("reserved", C.c_uint32),
+        ("table_offset", C.c_uint64), ("bitmap_offset", C.c_uint64), ("payload_offset", C.c_uint64),
+        ("total_bytes", C.c_uint64),
+    ]
+
+
+WIRE_MAGIC, WIRE_VERSION = 0x42564C43, 1
+
+
 class VolumeInfo(C.Structure):
     _fields_ = [
         ("resolution", C.c_int32),
@@ -258,6 +271,7 @@ STRUCTS = {
     "clay_editor_desc": EditorDesc,
     "clay_solver_desc": SolverDesc,
     "clay_body_desc": BodyDesc,
+    "clay_wire_header": WireHeader,
     "clay_extractor_desc": ExtractorDesc,
 }
 
@@ -285,6 +299,10 @@ SIGNATURES = {
     "clay_volume_voxel_size_at": ([P, I32, DP], C.c_int),
     "clay_volume_download": ([P, I32, P], C.c_int),
     "clay_volume_upload": ([P, I32, P], C.c_int),
+    "clay_volume_pack_bound": ([P, C.POINTER(C.c_uint64)], C.c_int),
+    "clay_volume_pack": ([P, U32, P, C.c_uint64, C.POINTER(C.c_uint64), P], C.c_int),
+    "clay_wire_check": ([C.POINTER(WireHeader), P], C.c_int),
+    "clay_volume_unpack": ([P, U32, P, C.c_uint64, P], C.c_int),
     "clay_volume_bake_brushes": ([P, C.POINTER(Brush), I32, P], C.c_int),
     "clay_field_analytic": ([P, C.POINTER(Brush), I32, C.POINTER(AnalyticOpts), PP], C.c_int),
     "clay_field_volume": ([P, P, PP], C.c_int),

@@ -146,6 +146,41 @@ class SdfVolume:
             raise ValueError(f"upload: expected {r}^3 x 4 halves, got {arr.size}")
         L.check(L.lib().clay_volume_upload(self._h, mip, arr.ctypes.data))
 
+    def packBound(self) -> int:
+        """Largest stream this volume's shape can produce (every tile stored)."""
+        out = C.c_uint64()
+        L.check(L.lib().clay_volume_pack_bound(self._h, C.byref(out)))
+        return out.value
+
+    def pack_device(self, out=None, instance: int = 0, stream=None, sync: bool = True):
+        """Packs one instance into a CUDA uint8 tensor (allocated at packBound when
+        `out` is None).  Returns (tensor, length); length is None when `sync` is False."""
+        import torch
+
+        cap = self.packBound()
+        if out is None:
+            out = torch.empty(cap, dtype=torch.uint8, device="cuda")
+        n = C.c_uint64()
+        L.check(L.lib().clay_volume_pack(self._h, int(instance), C.c_void_p(out.data_ptr()), out.numel(),
+                                         C.byref(n) if sync else None, L.stream_handle(stream)))
+        return out, (n.value if sync else None)
+
+    def pack(self, instance: int = 0) -> bytes:
+        """The stream of one instance, as host bytes (a file's contents)."""
+        t, n = self.pack_device(instance=instance)
+        return t[:n].cpu().numpy().tobytes()
+
+    def unpack_device(self, src, length: int, instance: int = 0, stream=None) -> None:
+        L.check(L.lib().clay_volume_unpack(self._h, int(instance), C.c_void_p(src.data_ptr()), int(length),
+                                           L.stream_handle(stream)))
+
+    def unpack(self, data: bytes, instance: int = 0) -> None:
+        """Replaces one instance's texels, every mip, with a stream from host bytes."""
+        import torch
+
+        src = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
+        self.unpack_device(src, len(data), instance)
+
     def bakeBrushes(self, brushes, stream=None) -> None:
         """SdfBuilder.setBrushes + rebuild (builder.ts:145-172)."""
         arr = brush_array(brushes)

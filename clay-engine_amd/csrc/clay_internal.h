@@ -100,6 +100,8 @@ struct Volume {
   uint16_t* mat = nullptr;       // all mips of every instance, material plane (f16 bits)
   std::vector<size_t> voxel_off; // per mip offset into the planes
   size_t total_voxels = 0;
+  uint32_t* wire_scratch = nullptr;  // pack: per tile row counts and offsets (allocated on first use)
+  size_t wire_scratch_n = 0;
   int mip_res(int m) const { return info.resolution >> m; }
 };
 

@@ -121,6 +121,53 @@ int clay_volume_voxel_size_at(const clay_volume* vol, int32_t mip, double* out);
  * x fastest, (res >> mip)^3 * 4 halves. */
 int clay_volume_download(clay_volume* vol, int32_t mip, uint16_t* rgba16_host);
 int clay_volume_upload(clay_volume* vol, int32_t mip, const uint16_t* rgba16_host);
+
+/* ---- The baked volume as a stream (a wire / on-disk format) ----------------
+ * One instance, every mip, as 8^3 tiles in mip-major, then tile z, y, x order.  A
+ * tile whose 512 texels are one (distance, material) bit pattern is stored as that
+ * texel; any other tile is stored whole as 512 rgba16float texels (distance/band,
+ * material, 0, 1; lib/src/field/volume.ts:77-90), x fastest, then y, then z.
+ *   [0, 96)                    clay_wire_header
+ *   [table_offset, +4 tiles)   u32 per tile: (material16 << 16) | dist16 for a
+ *                              constant tile, the payload index of a stored one
+ *   [bitmap_offset, ...)       1 bit per tile (u32 words, tile t: word t/32, bit t%32),
+ *                              set for stored tiles
+ *   [payload_offset, total)    4096 bytes per stored tile; payload_offset is 256-aligned
+ * All fields little-endian.  This format has no counterpart in the reference, whose
+ * bake lives only in a GPU texture; it carries exactly the reference's texels. */
+#define CLAY_WIRE_VERSION 1u
+typedef struct {
+  uint32_t magic;          /* 0x42564C43, "CLVB" */
+  uint32_t version;        /* CLAY_WIRE_VERSION */
+  uint32_t resolution;     /* mip-0 texels per axis */
+  uint32_t mip_levels;
+  float band;              /* voxels; texels hold distance / (band * voxel * 2^m) */
+  float world_size;
+  float gradient_voxels;
+  uint32_t tiles;          /* sum over mips of (resolution / 2^m / 8)^3 */
+  double origin[3];        /* the volume origin (setOrigin) */
+  uint32_t stored_tiles;
+  uint32_t reserved;
+  uint64_t table_offset, bitmap_offset, payload_offset, total_bytes;
+} clay_wire_header;
+
+/* Largest stream a volume of this shape can produce (every tile stored). */
+int clay_volume_pack_bound(const clay_volume* vol, uint64_t* bytes);
+/* Writes instance `instance` as a stream into device memory `dst` of `capacity`
+ * bytes (at least clay_volume_pack_bound), enqueued on the stream.  With `bytes`
+ * non-NULL it synchronises and returns the stream's length (also header.total_bytes). */
+int clay_volume_pack(clay_volume* vol, uint32_t instance, void* dst_device, uint64_t capacity, uint64_t* bytes,
+                     clay_stream stream);
+/* Checks a stream header against a volume: magic, version, resolution, mip count,
+ * band and section offsets must agree.  Host memory. */
+int clay_wire_check(const clay_wire_header* header, const clay_volume* vol);
+/* Replaces instance `instance`'s texels, every mip, with a stream in device memory
+ * (`bytes` long).  Synchronises: reads the header first and fails on a mismatch
+ * (clay_wire_check), a truncated stream, or a table word naming a payload tile the
+ * stream does not hold (the planes may then be partly written).  The volume's origin
+ * and geometry are left as they are. */
+int clay_volume_unpack(clay_volume* vol, uint32_t instance, const void* src_device, uint64_t bytes,
+                       clay_stream stream);
 /* SdfBuilder.setBrushes + rebuild (lib/src/field/builder.ts:87-172): dense bake of
  * a brush list into every mip. */
 int clay_volume_bake_brushes(clay_volume* vol, const clay_brush* brushes, int32_t n,

@@ -702,3 +702,98 @@ def body_centroids(first_count, pos: np.ndarray, rest: np.ndarray | None = None,
     getattr(lib(), f"orc_body_centroids_{precision}")(fc.shape[0], fc.ctypes.data, p[0].ctypes.data,
                                                       p[1].ctypes.data, p[2].ctypes.data, *rp, out.ctypes.data)
     return out
+
+
+# ---------------------------------------------------------------------------
+# The baked volume's stream (wire) format, restated in numpy from its definition:
+# 96-byte header, a u32 per tile (constant texel or payload index), a bitmap of stored
+# tiles, then 4096-byte rgba16float tiles from a 256-aligned offset.  Tiles run
+# mip-major, then tile z, y, x; texels within a tile x fastest.
+# ---------------------------------------------------------------------------
+WIRE_HEADER = np.dtype([
+    ("magic", "<u4"), ("version", "<u4"), ("resolution", "<u4"), ("mip_levels", "<u4"),
+    ("band", "<f4"), ("world_size", "<f4"), ("gradient_voxels", "<f4"), ("tiles", "<u4"),
+    ("origin", "<f8", 3), ("stored_tiles", "<u4"), ("reserved", "<u4"),
+    ("table_offset", "<u8"), ("bitmap_offset", "<u8"), ("payload_offset", "<u8"), ("total_bytes", "<u8"),
+])
+assert WIRE_HEADER.itemsize == 96
+
+
+def _tiles(a: np.ndarray) -> np.ndarray:
+    """(r, r, r) [z, y, x] -> (tiles, 512) in tile z, y, x order, texels x fastest."""
+    t = a.shape[0] // 8
+    return a.reshape(t, 8, t, 8, t, 8).transpose(0, 2, 4, 1, 3, 5).reshape(t ** 3, 512)
+
+
+def _untiles(b: np.ndarray, r: int) -> np.ndarray:
+    t = r // 8
+    return b.reshape(t, t, t, 8, 8, 8).transpose(0, 3, 1, 4, 2, 5).reshape(r, r, r)
+
+
+def wire_layout(tiles: int):
+    table = 96
+    bitmap = table + 4 * tiles
+    payload = (bitmap + 4 * ((tiles + 31) // 32) + 255) & ~255
+    return table, bitmap, payload
+
+
+def wire_pack(dist, mat, band: float, world_size: float, origin, gradient_voxels: float = 1.0) -> bytes:
+    """dist, mat: per-mip (r, r, r) uint16 planes (f16 bits) indexed [z, y, x]."""
+    d = np.concatenate([_tiles(np.asarray(x, np.uint16)) for x in dist])
+    m = np.concatenate([_tiles(np.asarray(x, np.uint16)) for x in mat])
+    tiles = d.shape[0]
+    const = (d == d[:, :1]).all(1) & (m == m[:, :1]).all(1)
+    table = np.where(const, (m[:, 0].astype(np.uint32) << 16) | d[:, 0], 0).astype(np.uint32)
+    stored = np.flatnonzero(~const)
+    table[stored] = np.arange(stored.size, dtype=np.uint32)
+    bits = np.zeros(((tiles + 31) // 32) * 32, dtype=np.uint8)
+    bits[stored] = 1
+    bitmap = np.packbits(bits.reshape(-1, 32)[:, ::-1], axis=1).view(">u4").astype("<u4").ravel()
+    t_off, b_off, p_off = wire_layout(tiles)
+    texels = np.zeros((stored.size, 512, 4), dtype=np.uint16)
+    texels[..., 0] = d[stored]
+    texels[..., 1] = m[stored]
+    texels[..., 3] = 0x3C00
+    h = np.zeros((), dtype=WIRE_HEADER)
+    h["magic"], h["version"] = 0x42564C43, 1
+    h["resolution"], h["mip_levels"] = dist[0].shape[0], len(dist)
+    h["band"], h["world_size"], h["gradient_voxels"], h["tiles"] = band, world_size, gradient_voxels, tiles
+    h["origin"] = origin
+    h["stored_tiles"] = stored.size
+    h["table_offset"], h["bitmap_offset"], h["payload_offset"] = t_off, b_off, p_off
+    h["total_bytes"] = p_off + 4096 * stored.size
+    out = bytearray(int(h["total_bytes"]))
+    out[0:96] = h.tobytes()
+    out[t_off:t_off + 4 * tiles] = table.astype("<u4").tobytes()
+    out[b_off:b_off + bitmap.nbytes] = bitmap.tobytes()
+    out[p_off:] = texels.astype("<u2").tobytes()
+    return bytes(out)
+
+
+def wire_unpack(data: bytes):
+    """Returns (header record, per-mip dist planes, per-mip mat planes)."""
+    h = np.frombuffer(data[:96], dtype=WIRE_HEADER)[0]
+    if h["magic"] != 0x42564C43 or h["version"] != 1:
+        raise ValueError("volume stream: bad magic or version")
+    if len(data) < h["total_bytes"]:
+        raise ValueError("volume stream: truncated")
+    tiles = int(h["tiles"])
+    table = np.frombuffer(data, dtype="<u4", count=tiles, offset=int(h["table_offset"]))
+    words = np.frombuffer(data, dtype="<u4", count=(tiles + 31) // 32, offset=int(h["bitmap_offset"]))
+    stored = ((words[:, None] >> np.arange(32, dtype=np.uint32)) & 1).ravel()[:tiles].astype(bool)
+    n = int(h["stored_tiles"])
+    pay = np.frombuffer(data, dtype="<u2", count=n * 2048, offset=int(h["payload_offset"])).reshape(n, 512, 4)
+    if (table[stored] >= n).any():
+        raise ValueError("volume stream: a stored tile's index lies outside the payload")
+    d = np.where(stored[:, None], 0, (table & 0xFFFF)[:, None]).astype(np.uint16) * np.ones((1, 512), np.uint16)
+    m = np.where(stored[:, None], 0, (table >> 16)[:, None]).astype(np.uint16) * np.ones((1, 512), np.uint16)
+    d[stored] = pay[table[stored], :, 0]
+    m[stored] = pay[table[stored], :, 1]
+    dist, mat, at = [], [], 0
+    for lvl in range(int(h["mip_levels"])):
+        r = int(h["resolution"]) >> lvl
+        k = (r // 8) ** 3
+        dist.append(_untiles(d[at:at + k], r))
+        mat.append(_untiles(m[at:at + k], r))
+        at += k
+    return h, dist, mat
