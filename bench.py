@@ -15,7 +15,10 @@ autotuned launch shapes.  Beside it, in the same run:
   * the position error of the FP16-LDS batch against the f32 batch over one step
     from the same state,
   * the CPU oracle on a bounded sample of the scenes.
-`--workload dam-break-1m` / `dam-break-16k` measure a single FluidSim instead.
+`--workload dam-break-1m` / `dam-break-16k` measure a single FluidSim instead,
+`--workload coupled` the coupled frame (fluid + ClaySolver body + eroding world per
+scene), and `--workload wire` the baked volume's stream format (pack + unpack of the
+288^3 x 3 world; its own metric, texels/s).
 
 Prints one JSON line.  --gpus N > 1 is refused: this build measures one GPU.
 """
@@ -392,13 +395,125 @@ def measure_coupled(root, args, stream) -> dict:
     return out
 
 
+WIRE_METRIC = "baked-volume texels packed + unpacked per second (1 MI355X)"
+
+
+def wire_bytes(kernel: str, texels: int, tiles: int, stored: int) -> int:
+    """Algorithmic HBM bytes of one launch of a stream kernel: every (distance,
+    material) pair of every texel is 4 bytes in the planes, a stored tile's texel 8
+    bytes in the stream, a tile's table word 4 bytes and its bitmap bit 1/8 byte."""
+    meta = 4 * tiles + tiles // 8
+    return {"wire_classify": 4 * texels + 4 * (tiles - stored) + tiles // 8,
+            "wire_write": 512 * stored * (4 + 8) + meta,
+            "wire_unpack": 4 * texels + 512 * stored * 8 + meta}[kernel]
+
+
+def measure_wire(root, args, stream) -> dict:
+    """pack and unpack of the 288^3 x 3 ridge world (the dam-break-1m collider)
+    through the stream format, the stream buffer resident in HBM; unpack is given
+    the header, so neither synchronises."""
+    import numpy as np
+
+    import clayhip
+    from clayhip import _lib as L
+    from clayhip import scenes
+
+    vol = clayhip.SdfVolume(root, resolution=288, mipLevels=3, worldSize=scenes.WORLD, origin=scenes.ORIGIN,
+                            band=4)
+    vol.bakeBrushes([dict(b, material=float(i + 1)) for i, b in enumerate(scenes.ridge_world_brushes())])
+    buf, length = vol.pack_device(stream=stream)
+    hdr = L.WireHeader.from_buffer_copy(buf[:96].cpu().numpy().tobytes())
+    texels = sum((288 >> m) ** 3 for m in range(3))
+    tiles, stored = int(hdr.tiles), int(hdr.stored_tiles)
+    pack = lambda s: vol.pack_device(buf, stream=s, sync=False)
+    unpack = lambda s: vol.unpack_device(buf, length, stream=s, header=hdr)
+    pack_ms = timed(pack, args.steps, args.warmup, stream)
+    unpack_ms = timed(unpack, args.steps, args.warmup, stream)
+    stats = kernel_stats(root, lambda s: (pack(s), unpack(s)), args.steps, stream)
+    kernels = {}
+    for name in ("wire_classify", "wire_write", "wire_unpack"):
+        tot_ms, n = stats[name]
+        avg_s = tot_ms / n / 1e3
+        nb = wire_bytes(name, texels, tiles, stored)
+        kernels[name] = {"avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nb,
+                         "achieved_GBs": round(nb / avg_s / 1e9, 1)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    d = kernels[dom]
+    vol.wireStatus()
+    host = buf[:length].cpu().numpy().tobytes()
+    vol.destroy()
+    return {"pack_ms": pack_ms, "unpack_ms": unpack_ms, "texels": texels, "tiles": tiles, "stored": stored,
+            "stream_bytes": length, "plane_bytes": 4 * texels, "kernels": kernels,
+            "roofline": {"bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 5),
+                         "traffic": committed_traffic(dom, "wire-288"), "kernel": dom, "avg_ms": d["avg_ms"],
+                         "bytes_per_launch": d["bytes_per_launch"]},
+            "stream": host}
+
+
+def cpu_baseline_wire(stream: bytes, budget_s: float) -> dict:
+    """The numpy restatement unpacking then packing the same stream on the host."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    t0 = time.perf_counter()
+    runs = 0
+    texels = 0
+    while True:
+        h, dist, mat = O.wire_unpack(stream)
+        O.wire_pack(dist, mat, float(h["band"]), float(h["world_size"]), tuple(h["origin"]),
+                    float(h["gradient_voxels"]))
+        texels += sum(x.size for x in dist)
+        runs += 1
+        if time.perf_counter() - t0 > budget_s or runs >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": texels / dt, "unit": "texels/s", "cores": 1, "kind": "port",
+            "sample": f"{runs} unpack + pack round trips of the 288^3 x 3 world stream with the numpy restatement"}
+
+
+def main_wire(root, args, stream) -> int:
+    import torch
+
+    t_wall = time.perf_counter()
+    r = measure_wire(root, args, stream)
+    ms = r["pack_ms"] + r["unpack_ms"]
+    line = {
+        "metric": WIRE_METRIC,
+        "value": r["texels"] / (ms / 1e3),
+        "unit": "texels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic (the ridge world baked at 288^3 x 3 mips)",
+        "config": {"workload": "wire-288: pack + unpack of one 288^3 x 3-mip baked world",
+                   "texels": r["texels"], "tiles": r["tiles"], "stored_tiles": r["stored"],
+                   "stream_bytes": r["stream_bytes"], "plane_bytes": r["plane_bytes"],
+                   "pack_ms": round(r["pack_ms"], 4), "unpack_ms": round(r["unpack_ms"], 4)},
+        "roofline": r["roofline"],
+        "kernels": r["kernels"],
+    }
+    line["wall_s"] = round(time.perf_counter() - t_wall, 3)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline_wire(r["stream"], args.cpu_budget)
+    root.destroy()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched-256",
-                    choices=["batched-256", "dam-break-1m", "dam-break-16k", "coupled"])
+                    choices=["batched-256", "dam-break-1m", "dam-break-16k", "coupled", "wire"])
     ap.add_argument("--scenes", type=int, default=8, help="coupled workload: scenes of 524 288 + 4096 particles")
     ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"], help="single-scene workloads only")
     ap.add_argument("--precision", default=None, choices=["f32", "fp16lds"],
@@ -437,6 +552,8 @@ def main() -> int:
 
     root = clayhip.Root(0)
     stream = torch.cuda.Stream()
+    if args.workload == "wire":
+        return main_wire(root, args, stream)
     t_wall = time.perf_counter()
     comparisons = None
     if batched:

@@ -302,7 +302,8 @@ SIGNATURES = {
     "clay_volume_pack_bound": ([P, C.POINTER(C.c_uint64)], C.c_int),
     "clay_volume_pack": ([P, U32, P, C.c_uint64, C.POINTER(C.c_uint64), P], C.c_int),
     "clay_wire_check": ([C.POINTER(WireHeader), P], C.c_int),
-    "clay_volume_unpack": ([P, U32, P, C.c_uint64, P], C.c_int),
+    "clay_volume_unpack": ([P, U32, P, C.c_uint64, C.POINTER(WireHeader), P], C.c_int),
+    "clay_volume_wire_status": ([P, C.POINTER(C.c_uint32)], C.c_int),
     "clay_volume_bake_brushes": ([P, C.POINTER(Brush), I32, P], C.c_int),
     "clay_field_analytic": ([P, C.POINTER(Brush), I32, C.POINTER(AnalyticOpts), PP], C.c_int),
     "clay_field_volume": ([P, P, PP], C.c_int),

@@ -170,9 +170,16 @@ class SdfVolume:
         t, n = self.pack_device(instance=instance)
         return t[:n].cpu().numpy().tobytes()
 
-    def unpack_device(self, src, length: int, instance: int = 0, stream=None) -> None:
+    def unpack_device(self, src, length: int, instance: int = 0, stream=None, header=None) -> None:
+        """From a CUDA uint8 tensor.  With `header` (a WireHeader already read) nothing
+        synchronises and a bad table word is reported later by `wireStatus`."""
         L.check(L.lib().clay_volume_unpack(self._h, int(instance), C.c_void_p(src.data_ptr()), int(length),
-                                           L.stream_handle(stream)))
+                                           None if header is None else C.byref(header), L.stream_handle(stream)))
+
+    def wireStatus(self) -> int:
+        bad = C.c_uint32()
+        L.check(L.lib().clay_volume_wire_status(self._h, C.byref(bad)))
+        return bad.value
 
     def unpack(self, data: bytes, instance: int = 0) -> None:
         """Replaces one instance's texels, every mip, with a stream from host bytes."""

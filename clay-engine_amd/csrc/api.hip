@@ -190,6 +190,7 @@ int volume_create_impl(Ctx* ctx, const clay_volume_desc* d, Volume** out, int in
 void volume_destroy_impl(Volume* v) {
   if (!v) return;
   dfree(v->wire_scratch);
+  dfree(v->wire_flag);
   dfree(v->dev);
   dfree(v->dist);
   dfree(v->mat);

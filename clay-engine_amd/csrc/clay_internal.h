@@ -102,6 +102,7 @@ struct Volume {
   size_t total_voxels = 0;
   uint32_t* wire_scratch = nullptr;  // pack: per tile row counts and offsets (allocated on first use)
   size_t wire_scratch_n = 0;
+  uint32_t* wire_flag = nullptr;     // unpack: set when a stream names a payload tile it lacks
   int mip_res(int m) const { return info.resolution >> m; }
 };
 

@@ -19,9 +19,9 @@
 //
 // Each pass takes one workgroup per row of tiles (fixed mip, tile y and z): its lanes
 // walk the row's 64 texel lines, 16 bytes of each plane (one tile line) per lane and
-// line, so a wave reads whole lines.  pack = classify (constant or not, per tile),
-// scan of the stored-tile counts per row, write (table, bitmap, payload, header);
-// unpack = one pass back into the planes.
+// line, so a wave reads whole lines.  pack = classify (constant or not, per tile,
+// which sets the bitmap), scan of the stored-tile counts per row, write (table,
+// payload through LDS, header); unpack = one pass back into the planes.
 #include "clay_internal.h"
 
 namespace clay {
@@ -150,39 +150,72 @@ __host__ __device__ inline Layout wire_layout(uint32_t tiles) {
   return l;
 }
 
+// The stored tiles of a row have consecutive payload indices, so their payload is
+// one contiguous run.  Batches of up to kBatch tiles go through LDS: lanes read
+// 16-byte tile lines of both planes (tile fastest, so a wave reads along a texel
+// line), expand them to rgba16float in LDS, then the workgroup writes the batch out
+// as contiguous 16-byte chunks.  Each staged tile is padded by 16 bytes so the lanes
+// of one line land in different LDS banks.
+constexpr int kBatch = 8;
+constexpr int kTileChunks = 256;  // 4096 bytes of payload per tile / 16
+
 __global__ void __launch_bounds__(kWireGroup)
     k_wire_write(RowMap rm, clay_wire_header hdr, const uint16_t* __restrict__ dist, const uint16_t* __restrict__ mat,
                  const uint32_t* __restrict__ row_off, const uint32_t* __restrict__ stored_total,
                  uint8_t* __restrict__ out) {
-  __shared__ uint32_t index[kMaxRowTiles];
+  __shared__ uint32_t list[kMaxRowTiles];
+  __shared__ uint32_t nstored;
+  __shared__ uint4 stage[kBatch * (kTileChunks + 1)];
   const RowPos r = row_pos(rm, blockIdx.x);
   const Layout L = wire_layout(hdr.tiles);
   uint32_t* table = reinterpret_cast<uint32_t*>(out + L.table);
-  uint32_t* bitmap = reinterpret_cast<uint32_t*>(out + L.bitmap);
+  const uint32_t* bitmap = reinterpret_cast<const uint32_t*>(out + L.bitmap);
   const uint32_t t = threadIdx.x;
+  const uint32_t first = row_off[blockIdx.x];
+  // Rank the row's stored tiles in order: one lane per tile, wave ballots, per-wave counts.
+  __shared__ uint32_t wave_count[kWireGroup / 64];
+  uint32_t bit = 0;
+  if (t < r.tr) {
+    const uint32_t g = r.first_tile + t;
+    bit = (bitmap[g / 32u] >> (g % 32u)) & 1u;
+  }
+  const uint32_t lane = t & 63u, wave = t >> 6;
+  const uint64_t ballot = __ballot(bit);
+  if (lane == 0) wave_count[wave] = (uint32_t)__popcll(ballot);
+  __syncthreads();
+  uint32_t rank = (uint32_t)__popcll(ballot & (lane ? (~0ull >> (64u - lane)) : 0ull));
+  for (uint32_t w = 0; w < wave; w++) rank += wave_count[w];
+  if (bit) {
+    list[rank] = t;
+    table[r.first_tile + t] = first + rank;
+  }
   if (t == 0) {
-    uint32_t k = row_off[blockIdx.x];
-    for (uint32_t i = 0; i < r.tr; i++) {
-      const uint32_t g = r.first_tile + i;
-      index[i] = (bitmap[g / 32u] >> (g % 32u)) & 1u ? k++ : 0xFFFFFFFFu;
-    }
+    uint32_t n = 0;
+    for (uint32_t w = 0; w < kWireGroup / 64; w++) n += wave_count[w];
+    nstored = n;
   }
   __syncthreads();
-  if (t < r.tr && index[t] != 0xFFFFFFFFu) table[r.first_tile + t] = index[t];
-  const uint32_t units = 64u * r.tr;
-  for (uint32_t u = t; u < units; u += kWireGroup) {
-    const uint32_t line = u / r.tr, tile = u % r.tr;
-    const uint32_t idx = index[tile];
-    if (idx == 0xFFFFFFFFu) continue;
-    const uint64_t o = line_off(r, tile, line);
-    const uint4 d = *reinterpret_cast<const uint4*>(dist + o);
-    const uint4 m = *reinterpret_cast<const uint4*>(mat + o);
-    uint4* dst = reinterpret_cast<uint4*>(out + L.payload + (uint64_t)idx * 4096u + (uint64_t)line * 64u);
-    const uint32_t dd[4] = {d.x, d.y, d.z, d.w}, mm[4] = {m.x, m.y, m.z, m.w};
-    // Two texels per 16-byte store: (d, m) then (0, 1.0) as f16 bits.
+  const uint32_t ns = nstored;
+  uint4* payload = reinterpret_cast<uint4*>(out + L.payload);
+  for (uint32_t b0 = 0; b0 < ns; b0 += kBatch) {
+    const uint32_t nb = ns - b0 < (uint32_t)kBatch ? ns - b0 : (uint32_t)kBatch;
+    for (uint32_t u = t; u < 64u * nb; u += kWireGroup) {
+      const uint32_t j = u % nb, line = u / nb;
+      const uint64_t o = line_off(r, list[b0 + j], line);
+      const uint4 d = *reinterpret_cast<const uint4*>(dist + o);
+      const uint4 m = *reinterpret_cast<const uint4*>(mat + o);
+      const uint32_t dd[4] = {d.x, d.y, d.z, d.w}, mm[4] = {m.x, m.y, m.z, m.w};
+      uint4* st = stage + j * (kTileChunks + 1) + line * 4u;
+      // Two texels per 16 bytes: (d, m) then (0, 1.0) as f16 bits.
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-      dst[q] = make_uint4(texel_of(dd[q], mm[q], 0), 0x3C000000u, texel_of(dd[q], mm[q], 1), 0x3C000000u);
+      for (int q = 0; q < 4; q++)
+        st[q] = make_uint4(texel_of(dd[q], mm[q], 0), 0x3C000000u, texel_of(dd[q], mm[q], 1), 0x3C000000u);
+    }
+    __syncthreads();
+    uint4* dst = payload + (uint64_t)(first + b0) * kTileChunks;
+    for (uint32_t c = t; c < nb * (uint32_t)kTileChunks; c += kWireGroup)
+      dst[c] = stage[(c / kTileChunks) * (kTileChunks + 1) + c % kTileChunks];
+    __syncthreads();
   }
   if (blockIdx.x == 0 && t == 0) {
     clay_wire_header h = hdr;
@@ -196,52 +229,81 @@ __global__ void __launch_bounds__(kWireGroup)
 }
 
 // A stored tile whose table word is not below `stored` sets *bad and is skipped.
+// Constant tiles are written line by line across the row (tile fastest); stored
+// tiles go in batches of kBatch: their 4096-byte payloads are read whole into LDS
+// (each tile contiguous, wherever its index points), then written out as 16-byte
+// tile lines of both planes.
 __global__ void __launch_bounds__(kWireGroup)
     k_wire_unpack(RowMap rm, uint32_t tiles, uint32_t stored_tiles, const uint8_t* __restrict__ in,
                   uint16_t* __restrict__ dist, uint16_t* __restrict__ mat, uint32_t* __restrict__ bad) {
   __shared__ uint32_t word[kMaxRowTiles];
-  __shared__ uint32_t stored[kMaxRowTiles];
+  __shared__ uint32_t kind[kMaxRowTiles];  // 0 constant, 1 stored, 2 stored with a bad index
+  __shared__ uint32_t list[kMaxRowTiles];
+  __shared__ uint32_t wave_count[kWireGroup / 64];
+  __shared__ uint4 stage[kBatch * (kTileChunks + 1)];
   const RowPos r = row_pos(rm, blockIdx.x);
   const Layout L = wire_layout(tiles);
   const uint32_t* table = reinterpret_cast<const uint32_t*>(in + L.table);
   const uint32_t* bitmap = reinterpret_cast<const uint32_t*>(in + L.bitmap);
+  const uint4* payload = reinterpret_cast<const uint4*>(in + L.payload);
   const uint32_t t = threadIdx.x;
+  uint32_t stored = 0;
   if (t < r.tr) {
     const uint32_t g = r.first_tile + t;
     word[t] = table[g];
-    stored[t] = (bitmap[g / 32u] >> (g % 32u)) & 1u;
-    if (stored[t] && word[t] >= stored_tiles) {
-      stored[t] = 2u;
+    uint32_t k = (bitmap[g / 32u] >> (g % 32u)) & 1u;
+    if (k && word[t] >= stored_tiles) {
+      k = 2u;
       atomicOr(bad, 1u);
     }
+    kind[t] = k;
+    stored = k == 1u;
   }
+  const uint32_t lane = t & 63u, wave = t >> 6;
+  const uint64_t ballot = __ballot(stored);
+  if (lane == 0) wave_count[wave] = (uint32_t)__popcll(ballot);
+  __syncthreads();
+  uint32_t rank = (uint32_t)__popcll(ballot & (lane ? (~0ull >> (64u - lane)) : 0ull));
+  uint32_t ns = 0;
+  for (uint32_t w = 0; w < kWireGroup / 64; w++) {
+    if (w < wave) rank += wave_count[w];
+    ns += wave_count[w];
+  }
+  if (stored) list[rank] = t;
   __syncthreads();
   const uint32_t units = 64u * r.tr;
   for (uint32_t u = t; u < units; u += kWireGroup) {
     const uint32_t line = u / r.tr, tile = u % r.tr;
+    if (kind[tile] != 0u) continue;
+    const uint32_t w = word[tile];
+    const uint32_t d2 = (w & 0xFFFFu) * 0x10001u, m2 = (w >> 16) * 0x10001u;
     const uint64_t o = line_off(r, tile, line);
-    uint4 d, m;
-    if (stored[tile] == 2u) continue;
-    if (stored[tile]) {
-      const uint4* src =
-          reinterpret_cast<const uint4*>(in + L.payload + (uint64_t)word[tile] * 4096u + (uint64_t)line * 64u);
+    *reinterpret_cast<uint4*>(dist + o) = make_uint4(d2, d2, d2, d2);
+    *reinterpret_cast<uint4*>(mat + o) = make_uint4(m2, m2, m2, m2);
+  }
+  for (uint32_t b0 = 0; b0 < ns; b0 += kBatch) {
+    const uint32_t nb = ns - b0 < (uint32_t)kBatch ? ns - b0 : (uint32_t)kBatch;
+    for (uint32_t c = t; c < nb * (uint32_t)kTileChunks; c += kWireGroup) {
+      const uint32_t j = c / kTileChunks;
+      stage[j * (kTileChunks + 1) + c % kTileChunks] =
+          payload[(uint64_t)word[list[b0 + j]] * kTileChunks + c % kTileChunks];
+    }
+    __syncthreads();
+    for (uint32_t u = t; u < 64u * nb; u += kWireGroup) {
+      const uint32_t j = u % nb, line = u / nb;
+      const uint4* src = stage + j * (kTileChunks + 1) + line * 4u;
       uint32_t dd[4], mm[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const uint4 s = src[q];  // (d0 m0) (b0 a0) (d1 m1) (b1 a1)
-        dd[q] = (s.x & 0xFFFFu) | (s.z << 16);
-        mm[q] = (s.x >> 16) | (s.z & 0xFFFF0000u);
+        const uint4 sv = src[q];  // (d0 m0) (b0 a0) (d1 m1) (b1 a1)
+        dd[q] = (sv.x & 0xFFFFu) | (sv.z << 16);
+        mm[q] = (sv.x >> 16) | (sv.z & 0xFFFF0000u);
       }
-      d = make_uint4(dd[0], dd[1], dd[2], dd[3]);
-      m = make_uint4(mm[0], mm[1], mm[2], mm[3]);
-    } else {
-      const uint32_t w = word[tile];
-      const uint32_t d2 = (w & 0xFFFFu) * 0x10001u, m2 = (w >> 16) * 0x10001u;
-      d = make_uint4(d2, d2, d2, d2);
-      m = make_uint4(m2, m2, m2, m2);
+      const uint64_t o = line_off(r, list[b0 + j], line);
+      *reinterpret_cast<uint4*>(dist + o) = make_uint4(dd[0], dd[1], dd[2], dd[3]);
+      *reinterpret_cast<uint4*>(mat + o) = make_uint4(mm[0], mm[1], mm[2], mm[3]);
     }
-    *reinterpret_cast<uint4*>(dist + o) = d;
-    *reinterpret_cast<uint4*>(mat + o) = m;
+    __syncthreads();
   }
 }
 
@@ -372,26 +434,30 @@ extern "C" int clay_wire_check(const clay_wire_header* h, const clay_volume* vol
 }
 
 extern "C" int clay_volume_unpack(clay_volume* vol, uint32_t instance, const void* src, uint64_t bytes,
-                                  clay_stream stream) {
+                                  const clay_wire_header* header, clay_stream stream) {
   Volume* v = reinterpret_cast<Volume*>(vol);
   if (!v || !src) return fail(CLAY_ERR_INVALID, "clay_volume_unpack: NULL argument");
   int rc = check_volume(v, instance, "clay_volume_unpack");
   if (rc) return rc;
   hipStream_t s = pick(v->ctx, stream);
   clay_wire_header h{};
-  if (bytes < sizeof(h)) return fail(CLAY_ERR_INVALID, "volume stream: shorter than its header");
-  CLAY_HIP_CHECK(hipMemcpyAsync(&h, src, sizeof(h), hipMemcpyDeviceToHost, s));
-  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  if (header) {
+    h = *header;
+  } else {
+    if (bytes < sizeof(h)) return fail(CLAY_ERR_INVALID, "volume stream: shorter than its header");
+    CLAY_HIP_CHECK(hipMemcpyAsync(&h, src, sizeof(h), hipMemcpyDeviceToHost, s));
+    CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  }
   if ((rc = clay_wire_check(&h, vol))) return rc;
   if (bytes < h.total_bytes) return fail(CLAY_ERR_INVALID, "volume stream: truncated");
   uint32_t rows, tiles;
   const RowMap rm = row_map(v, &rows, &tiles);
-  if (v->wire_scratch_n < 1) {
-    if ((rc = dalloc(&v->wire_scratch, 1))) return rc;
-    v->wire_scratch_n = 1;
+  if (!v->wire_flag) {
+    if ((rc = dalloc(&v->wire_flag, 1))) return rc;
+    if ((rc = launch_fill_u32(v->ctx, s, v->wire_flag, 1, 0u))) return rc;
   }
-  uint32_t* bad = v->wire_scratch;
-  if ((rc = launch_fill_u32(v->ctx, s, bad, 1, 0u))) return rc;
+  // The flag is sticky: set by a bad stream, cleared when clay_volume_wire_status reads it.
+  uint32_t* bad = v->wire_flag;
   const size_t base = (size_t)instance * v->total_voxels;
   {
     LaunchTimer lt(v->ctx, s, "wire_unpack");
@@ -399,9 +465,23 @@ extern "C" int clay_volume_unpack(clay_volume* vol, uint32_t instance, const voi
                        static_cast<const uint8_t*>(src), v->dist + base, v->mat + base, bad);
   }
   CLAY_HIP_CHECK(hipGetLastError());
+  if (header) return CLAY_OK;
   uint32_t flag = 0;
-  CLAY_HIP_CHECK(hipMemcpyAsync(&flag, bad, sizeof(flag), hipMemcpyDeviceToHost, s));
-  CLAY_HIP_CHECK(hipStreamSynchronize(s));
-  if (flag) return fail(CLAY_ERR_INVALID, "volume stream: a stored tile's index lies outside the payload");
+  return clay_volume_wire_status(vol, &flag) ? CLAY_ERR_INVALID : CLAY_OK;
+}
+
+extern "C" int clay_volume_wire_status(clay_volume* vol, uint32_t* bad) {
+  Volume* v = reinterpret_cast<Volume*>(vol);
+  if (!v || !bad) return fail(CLAY_ERR_INVALID, "clay_volume_wire_status: NULL argument");
+  *bad = 0;
+  if (!v->wire_flag) return CLAY_OK;
+  CLAY_HIP_CHECK(hipDeviceSynchronize());
+  CLAY_HIP_CHECK(hipMemcpy(bad, v->wire_flag, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (*bad) {
+    int rc = launch_fill_u32(v->ctx, v->ctx->stream, v->wire_flag, 1, 0u);
+    if (rc) return rc;
+    CLAY_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    return fail(CLAY_ERR_INVALID, "volume stream: a stored tile's index lies outside the payload");
+  }
   return CLAY_OK;
 }

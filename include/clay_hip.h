@@ -162,12 +162,17 @@ int clay_volume_pack(clay_volume* vol, uint32_t instance, void* dst_device, uint
  * band and section offsets must agree.  Host memory. */
 int clay_wire_check(const clay_wire_header* header, const clay_volume* vol);
 /* Replaces instance `instance`'s texels, every mip, with a stream in device memory
- * (`bytes` long).  Synchronises: reads the header first and fails on a mismatch
- * (clay_wire_check), a truncated stream, or a table word naming a payload tile the
- * stream does not hold (the planes may then be partly written).  The volume's origin
- * and geometry are left as they are. */
+ * (`bytes` long).  With `header` NULL it synchronises: it reads the header from the
+ * stream, and fails on a mismatch (clay_wire_check), a truncated stream, or a table
+ * word naming a payload tile the stream does not hold (such tiles are skipped; the
+ * others are written).  With the caller's copy of the header it only enqueues the
+ * kernel; a bad table word then sets a flag that clay_volume_wire_status reports.  The
+ * volume's origin and geometry are left as they are. */
 int clay_volume_unpack(clay_volume* vol, uint32_t instance, const void* src_device, uint64_t bytes,
-                       clay_stream stream);
+                       const clay_wire_header* header, clay_stream stream);
+/* Synchronises the device; *bad = 1 (and an error) when an unpack since the last call
+ * met a table word outside its payload; clears the flag. */
+int clay_volume_wire_status(clay_volume* vol, uint32_t* bad);
 /* SdfBuilder.setBrushes + rebuild (lib/src/field/builder.ts:87-172): dense bake of
  * a brush list into every mip. */
 int clay_volume_bake_brushes(clay_volume* vol, const clay_brush* brushes, int32_t n,

@@ -88,5 +88,17 @@ def test_bad_streams_are_refused(root):
     bad[at:at + 4] = np.uint32(h["stored_tiles"]).tobytes()
     with pytest.raises(L.ClayError, match="outside the payload"):
         src.unpack(bytes(bad))
+    # Asynchronous form: the caller's header, no synchronisation; the flag reports later.
+    import torch
+
+    hdr = L.WireHeader.from_buffer_copy(data[:96])
+    dev = torch.frombuffer(bytearray(bad), dtype=torch.uint8).cuda()
+    src.unpack_device(dev, len(bad), header=hdr)
+    with pytest.raises(L.ClayError, match="outside the payload"):
+        src.wireStatus()
+    assert src.wireStatus() == 0                      # cleared once reported
+    good = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
+    src.unpack_device(good, len(data), header=hdr)
+    assert src.wireStatus() == 0
     other.destroy()
     src.destroy()
