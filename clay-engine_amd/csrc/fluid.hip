@@ -342,9 +342,8 @@ __global__ void __launch_bounds__(kB)
 }
 
 // ---------------------------------------------------------------------------
-// Precision variant: FP16 cell-relative offsets staged in LDS, compensated
-// density sums.  Tolerance-graded against the f64 oracle; FMA contraction is
-// allowed in the neighbour loops.
+// Precision variant: FP16 cell-relative offsets staged in LDS.  Tolerance-graded
+// against the f64 oracle; FMA contraction is allowed in the neighbour loops.
 //
 // For each of the 9 rows of the 27-cell neighbourhood, a wave takes the union of
 // its lanes' neighbour runs (consecutive sorted particles share most of them),
@@ -382,20 +381,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-
-// Neumaier's compensated sum.
-struct KahanSum {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float x) {
-    const float t = s + x;
-    if (fabsf(s) >= fabsf(x))
-      c += (s - t) + x;
-    else
-      c += (x - t) + s;
-    s = t;
-  }
-  __device__ __forceinline__ float value() const { return s + c; }
-};
 
 // One row (dz, dy) of a particle's neighbourhood: its three cells' run [lo, hi)
 // when no cell overflows the bucket (capped = false), else the per-cell spans.
@@ -492,13 +477,13 @@ __global__ void __launch_bounds__(kB)
   int32_t b[3] = {0, 0, 0};
   if (active) cell_of(k, pi, b);
   const f3 oi = cell_rel(k, pi, b);
-  KahanSum rho;
+  float rho = 0.0f;
   Cover cov;
   auto term = [&](f3 d) {
     const float r2 = dot3(d, d);
     if (r2 < k.hh) {
       const float w = k.hh - r2;
-      rho.add(k.mass_poly6 * (w * w * w));
+      rho = rho + k.mass_poly6 * (w * w * w);
     }
     if (kCover) cov.add(d, r2, k.hh);
   };
@@ -514,11 +499,10 @@ __global__ void __launch_bounds__(kB)
           stage[wave][t - ulo] = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
       wave_sync();
       if (staged) {
-        const float ey = (float)dy * k.h, ez = (float)dz * k.h;
-        walk_cells(k, r, [&](uint32_t t, int j) {
-          const f3 d = oi - unpack_off(stage[wave][t - ulo]);
-          term(mk(d.x - (float)(j - 1) * k.h, d.y - ey, d.z - ez));
-        });
+        // p_i - p_j = (o_i - (c_j - c_i) h) - o_j: the first part once per cell.
+        const f3 base[3] = {oi - mk(-k.h, (float)dy * k.h, (float)dz * k.h), oi - mk(0.0f, (float)dy * k.h, (float)dz * k.h),
+                            oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
+        walk_cells(k, r, [&](uint32_t t, int j) { term(base[j] - unpack_off(stage[wave][t - ulo])); });
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
@@ -526,9 +510,8 @@ __global__ void __launch_bounds__(kB)
     }
   }
   if (!active) return;
-  const float rv = rho.value();
-  srho[s] = fmax_s(rv, k.rest_floor);
-  spres[s] = fmax_s(k.stiff * (rv - k.rest), 0.0f);
+  srho[s] = fmax_s(rho, k.rest_floor);
+  spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
   if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
 }
 
@@ -685,7 +668,7 @@ __global__ void __launch_bounds__(kB)
 struct StagedNbr {
   float4 vr;   // velocity and 1 / density
   uint2 off;   // f16 position offset from the centre of the particle's cell
-  float pres;
+  float pinv;  // pressure / density
   float pad;
 };
 
@@ -724,10 +707,12 @@ __global__ void __launch_bounds__(kB)
   f3 fp = mk(0.0f, 0.0f, 0.0f);
   f3 vsum = vi * k.h6;
   float wsum = k.h6;
-  // The neighbour's 1 / density comes in precomputed (once per staged particle,
-  // not once per pair), and r, 1 / r come from the hardware reciprocal square
-  // root (1 ulp): r > 1e-6 is tested as r^2 > 1e-12.
-  auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float prj) {
+  // The neighbour's 1 / density and pressure / density come in precomputed (once
+  // per staged particle, not once per pair), the constant factor m spiky / 2 is
+  // taken once, and r, 1 / r come from the hardware reciprocal square root (1 ulp):
+  // r > 1e-6 is tested as r^2 > 1e-12.
+  const float cpress = k.mass * 0.5f * k.spiky;
+  auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float pinv) {
     if (t == s) return;
     const float r2 = dot3(dv, dv);
     if (r2 >= k.hh) return;
@@ -738,8 +723,8 @@ __global__ void __launch_bounds__(kB)
     if (r2 > 1e-12f) {
       const float ir = __builtin_amdgcn_rsqf(r2);
       const float hr = k.h - r2 * ir;
-      const float s2 = k.mass * (pri + prj) * 0.5f * inv * k.spiky * (hr * hr);
-      fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
+      const float f = cpress * (pri * inv + pinv) * (hr * hr) * ir;
+      fp = mk(fp.x - dv.x * f, fp.y - dv.y * f, fp.z - dv.z * f);
     }
   };
   for (int dz = -1; dz <= 1; dz++) {
@@ -752,24 +737,25 @@ __global__ void __launch_bounds__(kB)
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
           StagedNbr e;
-          e.vr = make_float4(svx[t], svy[t], svz[t], 1.0f / srho[t]);
+          const float inv = 1.0f / srho[t];
+          e.vr = make_float4(svx[t], svy[t], svz[t], inv);
           e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
-          e.pres = spres[t];
+          e.pinv = spres[t] * inv;
           e.pad = 0.0f;
           stage[wave][t - ulo] = e;
         }
       wave_sync();
       if (staged) {
-        const float ey = (float)dy * k.h, ez = (float)dz * k.h;
+        const f3 base[3] = {oi - mk(-k.h, (float)dy * k.h, (float)dz * k.h), oi - mk(0.0f, (float)dy * k.h, (float)dz * k.h),
+                            oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
         walk_cells(k, r, [&](uint32_t t, int j) {
           const StagedNbr& e = stage[wave][t - ulo];
-          const f3 d = oi - unpack_off(e.off);
-          term(t, mk(d.x - (float)(j - 1) * k.h, d.y - ey, d.z - ez), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w,
-               e.pres);
+          term(t, base[j] - unpack_off(e.off), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
         });
       } else {
         walk_row(k, r, [&](uint32_t t) {
-          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t], spres[t]);
+          const float inv = 1.0f / srho[t];
+          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), inv, spres[t] * inv);
         });
       }
       wave_sync();
