@@ -459,6 +459,23 @@ __device__ __forceinline__ uint2 pack_off(f3 d) {
 __device__ __forceinline__ f3 unpack_off(uint2 u) {
   return mk(h2f((uint16_t)(u.x & 0xffffu)), h2f((uint16_t)(u.x >> 16)), h2f((uint16_t)(u.y & 0xffffu)));
 }
+// base - (the staged f16 offset): each component is one v_fma_mix_f32, which widens
+// the f16 half (exact) and computes base - x with one rounding -- the result of a
+// convert followed by a subtract, in one instruction instead of two.
+__device__ __forceinline__ float sub_f16_lo(float base, uint32_t h2) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(base));
+  return r;
+}
+__device__ __forceinline__ float sub_f16_hi(float base, uint32_t h2) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(base));
+  return r;
+}
+__device__ __forceinline__ f3 sub_off(f3 base, uint2 u) {
+  return mk(sub_f16_lo(base.x, u.x), sub_f16_hi(base.y, u.x), sub_f16_lo(base.z, u.y));
+}
+__device__ __forceinline__ float dot3_fma(f3 d) { return __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)); }
 
 template <int kB, int kS, bool kCover>
 __global__ void __launch_bounds__(kB)
@@ -480,7 +497,7 @@ __global__ void __launch_bounds__(kB)
   float rho = 0.0f;
   Cover cov;
   auto term = [&](f3 d) {
-    const float r2 = dot3(d, d);
+    const float r2 = dot3_fma(d);
     if (r2 < k.hh) {
       const float w = k.hh - r2;
       rho = rho + k.mass_poly6 * (w * w * w);
@@ -502,7 +519,7 @@ __global__ void __launch_bounds__(kB)
         // p_i - p_j = (o_i - (c_j - c_i) h) - o_j: the first part once per cell.
         const f3 base[3] = {oi - mk(-k.h, (float)dy * k.h, (float)dz * k.h), oi - mk(0.0f, (float)dy * k.h, (float)dz * k.h),
                             oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
-        walk_cells(k, r, [&](uint32_t t, int j) { term(base[j] - unpack_off(stage[wave][t - ulo])); });
+        walk_cells(k, r, [&](uint32_t t, int j) { term(sub_off(base[j], stage[wave][t - ulo])); });
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
@@ -712,10 +729,10 @@ __global__ void __launch_bounds__(kB)
   // taken once, and r, 1 / r come from the hardware reciprocal square root (1 ulp):
   // r > 1e-6 is tested as r^2 > 1e-12.
   const float cpress = k.mass * 0.5f * k.spiky;
+  // The self pair (t == s) is within h: it is left out inside the interacting branch.
   auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float pinv) {
-    if (t == s) return;
-    const float r2 = dot3(dv, dv);
-    if (r2 >= k.hh) return;
+    const float r2 = dot3_fma(dv);
+    if (r2 >= k.hh || t == s) return;
     const float w0 = k.hh - r2;
     const float w = w0 * w0 * w0;
     vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
@@ -750,7 +767,7 @@ __global__ void __launch_bounds__(kB)
                             oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
         walk_cells(k, r, [&](uint32_t t, int j) {
           const StagedNbr& e = stage[wave][t - ulo];
-          term(t, base[j] - unpack_off(e.off), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+          term(t, sub_off(base[j], e.off), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
         });
       } else {
         walk_row(k, r, [&](uint32_t t) {
