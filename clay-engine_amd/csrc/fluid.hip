@@ -345,16 +345,18 @@ __global__ void __launch_bounds__(kB)
 // Precision variant: FP16 cell-relative offsets staged in LDS.  Tolerance-graded
 // against the f64 oracle; FMA contraction is allowed in the neighbour loops.
 //
-// For each of the 9 rows of the 27-cell neighbourhood, a wave takes the union of
-// its lanes' neighbour runs (consecutive sorted particles share most of them),
-// stages that run in its LDS slice -- positions as f16 offsets from the centre of
-// each particle's own cell (|offset| <= h/2), the other fields as f32 -- and every
-// lane walks its three cells out of LDS.  A lane knows which of its cells a staged
-// index falls in, so it rebuilds p_i - p_j = (o_i - o_j) - (c_j - c_i) h with o_i
-// in full f32.  A row whose union is wider than the slice, or whose cells
-// overflow the bucket, is walked from global memory instead.  In a batch a wave
-// may straddle two scenes: each lane reads its own scene's cell records, and the
-// union of runs is still one span of the sorted order.
+// For each of the 9 rows of the 27-cell neighbourhood, every lane first drops the
+// neighbour cells of the row whose box lies beyond the kernel's reach from its
+// particle (row_span), leaving one sorted range.  The wave takes the union of its
+// lanes' ranges (consecutive sorted particles share most of them) and stages it in
+// its LDS slice -- x as f32, y and z as f16 offsets from the centre of each
+// particle's own cell (|offset| <= h/2), the other fields as f32 -- and every lane
+// walks its range out of LDS.  The row fixes the cells' y and z, so a lane rebuilds
+// p_i - p_j = (x_i - x_j, (o_iy - dy h) - o_jy, (o_iz - dz h) - o_jz).  A row
+// whose union is wider than the slice, or whose cells overflow the bucket, is
+// walked from global memory instead.  In a batch a wave may straddle two scenes:
+// each lane reads its own scene's cell records, and the union of ranges is still
+// one span of the sorted order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
@@ -447,18 +449,44 @@ __device__ __forceinline__ f3 cell_rel(const FluidK& k, f3 p, const int32_t b[3]
   return mk(p.x - (k.go[0] + ((float)b[0] + 0.5f) * k.h), p.y - (k.go[1] + ((float)b[1] + 0.5f) * k.h),
             p.z - (k.go[2] + ((float)b[2] + 0.5f) * k.h));
 }
-__device__ __forceinline__ f3 own_cell_rel(const FluidK& k, f3 p) {
+
+// A staged neighbour position: x in f32, y and z as f16 offsets from the centre of
+// the neighbour's own cell (|offset| <= h/2).  All candidates of one lane's row lie
+// in cells of the same y and z, so for that lane y_i - y_j = (o_iy - dy h) - o_jy, and
+// x_i - x_j is a plain f32 difference.
+__device__ __forceinline__ uint2 pack_stage(const FluidK& k, f3 p) {
   int32_t c[3];
   cell_of(k, p, c);
-  return cell_rel(k, p, c);
+  const float oy = p.y - (k.go[1] + ((float)c[1] + 0.5f) * k.h);
+  const float oz = p.z - (k.go[2] + ((float)c[2] + 0.5f) * k.h);
+  return make_uint2(__float_as_uint(p.x), (uint32_t)f2h(oy) | ((uint32_t)f2h(oz) << 16));
 }
 
-__device__ __forceinline__ uint2 pack_off(f3 d) {
-  return make_uint2((uint32_t)f2h(d.x) | ((uint32_t)f2h(d.y) << 16), (uint32_t)f2h(d.z));
+// One lane's share of a row (dy, dz): the sorted range of the row's cells that can
+// hold a particle within `reach` of p_i.  A neighbour cell is dropped when the gap
+// between p_i and the cell's box is at least reach (with 1e-4 of margin); the row's
+// three cells are consecutive keys, so what remains is one range.
+struct RowSpan {
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ RowSpan row_span(const RowRun& r, f3 oi, int dy, int dz, float h, float reach2) {
+  const float hh = 0.5f * h;
+  const float gy = dy == 0 ? 0.0f : fmax_s(dy < 0 ? oi.y + hh : hh - oi.y, 0.0f);
+  const float gz = dz == 0 ? 0.0f : fmax_s(dz < 0 ? oi.z + hh : hh - oi.z, 0.0f);
+  const float gyz = gy * gy + gz * gz;
+  const float gl = fmax_s(oi.x + hh, 0.0f), gr = fmax_s(hh - oi.x, 0.0f);
+  const float lim = reach2 * 1.0001f;
+  const bool keep[3] = {gl * gl + gyz < lim, gyz < lim, gr * gr + gyz < lim};
+  RowSpan sp{0xFFFFFFFFu, 0u};
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+    if (r.ok[j] && keep[j]) {
+      sp.lo = r.st[j] < sp.lo ? r.st[j] : sp.lo;
+      sp.hi = r.en[j] > sp.hi ? r.en[j] : sp.hi;
+    }
+  return sp;
 }
-__device__ __forceinline__ f3 unpack_off(uint2 u) {
-  return mk(h2f((uint16_t)(u.x & 0xffffu)), h2f((uint16_t)(u.x >> 16)), h2f((uint16_t)(u.y & 0xffffu)));
-}
+
 // base - (the staged f16 offset): each component is one v_fma_mix_f32, which widens
 // the f16 half (exact) and computes base - x with one rounding -- the result of a
 // convert followed by a subtract, in one instruction instead of two.
@@ -471,9 +499,6 @@ __device__ __forceinline__ float sub_f16_hi(float base, uint32_t h2) {
   float r;
   asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h2), "v"(base));
   return r;
-}
-__device__ __forceinline__ f3 sub_off(f3 base, uint2 u) {
-  return mk(sub_f16_lo(base.x, u.x), sub_f16_hi(base.y, u.x), sub_f16_lo(base.z, u.y));
 }
 __device__ __forceinline__ float dot3_fma(f3 d) { return __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)); }
 
@@ -504,22 +529,25 @@ __global__ void __launch_bounds__(kB)
     }
     if (kCover) cov.add(d, r2, k.hh);
   };
+  // The frame-end pass classes neighbours out to 1.5 h for the interior radius.
+  const float reach2 = kCover ? 2.25f * k.hh : k.hh;
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
-      const bool any = r.hi > r.lo;
-      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const RowSpan sp = row_span(r, oi, dy, dz, k.h, reach2);
+      const bool any = sp.hi > sp.lo;
+      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
-        for (uint32_t t = ulo + lane; t < uhi; t += 64)
-          stage[wave][t - ulo] = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = pack_stage(k, mk(spx[t], spy[t], spz[t]));
       wave_sync();
       if (staged) {
-        // p_i - p_j = (o_i - (c_j - c_i) h) - o_j: the first part once per cell.
-        const f3 base[3] = {oi - mk(-k.h, (float)dy * k.h, (float)dz * k.h), oi - mk(0.0f, (float)dy * k.h, (float)dz * k.h),
-                            oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
-        walk_cells(k, r, [&](uint32_t t, int j) { term(sub_off(base[j], stage[wave][t - ulo])); });
+        const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
+        for (uint32_t t = sp.lo; t < sp.hi; t++) {
+          const uint2 u = stage[wave][t - ulo];
+          term(mk(pi.x - __uint_as_float(u.x), sub_f16_lo(by, u.y), sub_f16_hi(bz, u.y)));
+        }
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
@@ -684,7 +712,7 @@ __global__ void __launch_bounds__(kB)
 
 struct StagedNbr {
   float4 vr;   // velocity and 1 / density
-  uint2 off;   // f16 position offset from the centre of the particle's cell
+  uint2 pos;   // x (f32) and the y, z offsets from its cell centre (f16), as pack_stage
   float pinv;  // pressure / density
   float pad;
 };
@@ -747,28 +775,29 @@ __global__ void __launch_bounds__(kB)
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
-      const bool any = r.hi > r.lo;
-      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      const RowSpan sp = row_span(r, oi, dy, dz, k.h, k.hh);
+      const bool any = sp.hi > sp.lo;
+      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
           StagedNbr e;
           const float inv = 1.0f / srho[t];
           e.vr = make_float4(svx[t], svy[t], svz[t], inv);
-          e.off = pack_off(own_cell_rel(k, mk(spx[t], spy[t], spz[t])));
+          e.pos = pack_stage(k, mk(spx[t], spy[t], spz[t]));
           e.pinv = spres[t] * inv;
           e.pad = 0.0f;
           stage[wave][t - ulo] = e;
         }
       wave_sync();
       if (staged) {
-        const f3 base[3] = {oi - mk(-k.h, (float)dy * k.h, (float)dz * k.h), oi - mk(0.0f, (float)dy * k.h, (float)dz * k.h),
-                            oi - mk(k.h, (float)dy * k.h, (float)dz * k.h)};
-        walk_cells(k, r, [&](uint32_t t, int j) {
+        const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
+        for (uint32_t t = sp.lo; t < sp.hi; t++) {
           const StagedNbr& e = stage[wave][t - ulo];
-          term(t, sub_off(base[j], e.off), mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
-        });
+          term(t, mk(pi.x - __uint_as_float(e.pos.x), sub_f16_lo(by, e.pos.y), sub_f16_hi(bz, e.pos.y)),
+               mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+        }
       } else {
         walk_row(k, r, [&](uint32_t t) {
           const float inv = 1.0f / srho[t];
