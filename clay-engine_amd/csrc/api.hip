@@ -826,6 +826,8 @@ void fluid_free(Fluid* f) {
   dfree(f->cmask);
   dfree(f->cdata);
   dfree(f->cover);
+  dfree(f->spos);
+  dfree(f->frec);
   dfree(f->reduce);
   dfree(f->epoch_dev);
   dfree(f->scene_of);
@@ -861,10 +863,18 @@ int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
-  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6))) return rc;
   const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
+  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6, f->k,
+                           lds ? f->spos : nullptr)))
+    return rc;
+  HalfState half;
+  half.spos = f->spos;
+  half.svx = f->svx;
+  half.svy = f->svy;
+  half.svz = f->svz;
+  half.frec = f->frec;
   return launch_density(c, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells, f->srho,
-                        f->spres, f->vals, cover);
+                        f->spres, f->vals, cover, half);
 }
 
 // Second half of a substep (fluid.ts:370-491): clearGrid's contacts reset
@@ -885,7 +895,7 @@ int fluid_forces(Fluid* f, hipStream_t s) {
   ColliderDev none{};
   const ColliderDev& col = f->world ? f->world->col : none;
   if ((rc = launch_forces_integrate(c, s, f->k, sk, f->tune, lds, col, n, sorted, f->vals, f->keys, f->cells, out,
-                                    f->contacts)))
+                                    f->contacts, f->frec)))
     return rc;
   return launch_contacts_collect(c, s, f->contacts, 4 + 4 * CLAY_CONTACTS, f->cmask, f->cdata,
                                  f->scene_of ? f->scene_base_dev : nullptr, n, f->scenes);
@@ -1012,6 +1022,11 @@ int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t s
       (rc = dalloc(&f->vals2, P)) || (rc = dalloc(&f->contacts, contact_words(f))) ||
       (rc = dalloc(&f->epoch_dev, 1)) || (rc = dalloc(&f->cmask, ((size_t)P + 31) / 32)) ||
       (rc = dalloc(&f->cdata, P)) || (rc = dalloc(&f->cover, P))) {
+    fluid_free(f);
+    return rc;
+  }
+  if (d->precision == CLAY_PRECISION_FP16LDS &&
+      ((rc = dalloc(&f->spos, P)) || (rc = dalloc(&f->frec, (size_t)P * 2)))) {
     fluid_free(f);
     return rc;
   }

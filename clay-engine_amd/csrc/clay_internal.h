@@ -200,6 +200,8 @@ struct Fluid {
   uint32_t reduce_blocks = 0;
   Splat* surface = nullptr;      // owned
   float* cover = nullptr;        // per particle id: squared interior radius for the surface bake
+  uint2* spos = nullptr;         // FP16-LDS: per sorted particle, x and the f16 offsets of y, z in its cell
+  float4* frec = nullptr;        // FP16-LDS: per sorted particle, 2 float4: the forces pass's staged record
   // The sort, cell table and densities of the next substep are computed at the end
   // of a frame, from the positions the bake sees; `presorted` says they are still
   // valid (no state upload, emission or configure since).
@@ -258,18 +260,27 @@ int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, con
 // fluid.hip
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key);
+// With `spos` (FP16-LDS), also each sorted particle's staged position.
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
-                   float* const* dst, int planes);
+                   float* const* dst, int planes, const FluidK& k, uint2* spos);
+// The FP16-LDS passes' per-particle records: staged positions (from the reorder),
+// the sorted velocities, and the records the density pass writes for the forces pass.
+struct HalfState {
+  const uint2* spos = nullptr;
+  const float *svx = nullptr, *svy = nullptr, *svz = nullptr;
+  float4* frec = nullptr;
+};
 // With `cover` (by particle id, via svals), also each particle's squared interior
 // radius for the splat (+inf when its neighbours do not surround it).
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
                    uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover);
+                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover,
+                   const HalfState& half);
 int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune,
                             bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
                             const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
-                            float* const* out, uint32_t* contacts);
+                            float* const* out, uint32_t* contacts, const float4* frec);
 int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch);
 int launch_contacts_collect(Ctx* c, hipStream_t s, uint32_t* contacts, uint32_t stride, const uint32_t* cmask,
                             const float4* cdata, const uint32_t* base, uint32_t n, uint32_t scenes);

@@ -133,10 +133,32 @@ struct Planes8 {
 
 // Cell records of this substep (run boundaries of the sorted keys) and the sorted
 // snapshot of the state planes.
+// A staged neighbour position: x in f32, y and z as f16 offsets from the centre of
+// the neighbour's own cell (|offset| <= h/2).  All candidates of one lane's row lie
+// in cells of the same y and z, so for that lane y_i - y_j = (o_iy - dy h) - o_jy, and
+// x_i - x_j is a plain f32 difference.
+__device__ __forceinline__ uint2 pack_stage(const FluidK& k, f3 p) {
+  int32_t c[3];
+  cell_of(k, p, c);
+  const float oy = p.y - (k.go[1] + ((float)c[1] + 0.5f) * k.h);
+  const float oz = p.z - (k.go[2] + ((float)c[2] + 0.5f) * k.h);
+  return make_uint2(__float_as_uint(p.x), (uint32_t)f2h(oy) | ((uint32_t)f2h(oz) << 16));
+}
+
+// The FP16-LDS forces pass stages each neighbour as this record, which the density
+// pass writes for its own particle once its density is known.
+struct StagedNbr {
+  float4 vr;   // velocity and 1 / density
+  uint2 pos;   // x (f32) and the y, z offsets from its cell centre (f16), as pack_stage
+  float pinv;  // pressure / density
+  float pad;
+};
+
+// With `spos` (FP16-LDS), also each sorted particle's staged position (pack_stage).
 __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ svals, SceneK sk,
                                                     uint32_t invalid, uint32_t* __restrict__ cells,
-                                                    Planes8 pl, int planes) {
+                                                    Planes8 pl, int planes, FluidK k, uint2* __restrict__ spos) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t key = skeys[s];
@@ -154,6 +176,7 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
 #pragma unroll
   for (int p = 0; p < 8; p++)
     if (p < planes) pl.dst[p][s] = pl.src[p][j];
+  if (spos) spos[s] = pack_stage(k, mk(pl.src[0][j], pl.src[1][j], pl.src[2][j]));
 }
 
 // ---------------------------------------------------------------------------
@@ -450,18 +473,6 @@ __device__ __forceinline__ f3 cell_rel(const FluidK& k, f3 p, const int32_t b[3]
             p.z - (k.go[2] + ((float)b[2] + 0.5f) * k.h));
 }
 
-// A staged neighbour position: x in f32, y and z as f16 offsets from the centre of
-// the neighbour's own cell (|offset| <= h/2).  All candidates of one lane's row lie
-// in cells of the same y and z, so for that lane y_i - y_j = (o_iy - dy h) - o_jy, and
-// x_i - x_j is a plain f32 difference.
-__device__ __forceinline__ uint2 pack_stage(const FluidK& k, f3 p) {
-  int32_t c[3];
-  cell_of(k, p, c);
-  const float oy = p.y - (k.go[1] + ((float)c[1] + 0.5f) * k.h);
-  const float oz = p.z - (k.go[2] + ((float)c[2] + 0.5f) * k.h);
-  return make_uint2(__float_as_uint(p.x), (uint32_t)f2h(oy) | ((uint32_t)f2h(oz) << 16));
-}
-
 // One lane's share of a row (dy, dz): the sorted range of the row's cells that can
 // hold a particle within `reach` of p_i.  A neighbour cell is dropped when the gap
 // between p_i and the cell's box is at least reach (with 1e-4 of margin); the row's
@@ -506,7 +517,9 @@ template <int kB, int kS, bool kCover>
 __global__ void __launch_bounds__(kB)
     k_density_lds(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
                   const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
-                  const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co) {
+                  const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co,
+                  const uint2* __restrict__ spos, const float* __restrict__ svx, const float* __restrict__ svy,
+                  const float* __restrict__ svz, StagedNbr* __restrict__ frec) {
 #pragma clang fp contract(fast)
   __shared__ uint2 stage[kB / 64][kS];
   const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
@@ -540,7 +553,7 @@ __global__ void __launch_bounds__(kB)
       const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
-        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = pack_stage(k, mk(spx[t], spy[t], spz[t]));
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = spos[t];
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
@@ -555,9 +568,18 @@ __global__ void __launch_bounds__(kB)
     }
   }
   if (!active) return;
-  srho[s] = fmax_s(rho, k.rest_floor);
-  spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
+  const float rr = fmax_s(rho, k.rest_floor), pr = fmax_s(k.stiff * (rho - k.rest), 0.0f);
+  srho[s] = rr;
+  spres[s] = pr;
   if (kCover) co.r2[co.svals[s]] = cov.radius2(k.hh);
+  // The record the forces pass stages for this particle: 1 / density taken once here.
+  const float inv = 1.0f / rr;
+  StagedNbr e;
+  e.vr = make_float4(svx[s], svy[s], svz[s], inv);
+  e.pos = spos[s];
+  e.pinv = pr * inv;
+  e.pad = 0.0f;
+  frec[s] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -710,18 +732,12 @@ __global__ void __launch_bounds__(kB)
                   contacts + (size_t)scene * sk.contact_stride);
 }
 
-struct StagedNbr {
-  float4 vr;   // velocity and 1 / density
-  uint2 pos;   // x (f32) and the y, z offsets from its cell centre (f16), as pack_stage
-  float pinv;  // pressure / density
-  float pad;
-};
-
 template <int kB, int kS>
 __global__ void __launch_bounds__(kB)
     k_forces_integrate_lds(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
                            const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
-                           const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
+                           const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts,
+                           const StagedNbr* __restrict__ frec) {
 #pragma clang fp contract(fast)
   __shared__ StagedNbr stage[kB / 64][kS];
   const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
@@ -781,15 +797,7 @@ __global__ void __launch_bounds__(kB)
       const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
-        for (uint32_t t = ulo + lane; t < uhi; t += 64) {
-          StagedNbr e;
-          const float inv = 1.0f / srho[t];
-          e.vr = make_float4(svx[t], svy[t], svz[t], inv);
-          e.pos = pack_stage(k, mk(spx[t], spy[t], spz[t]));
-          e.pinv = spres[t] * inv;
-          e.pad = 0.0f;
-          stage[wave][t - ulo] = e;
-        }
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = frec[t];
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
@@ -968,10 +976,17 @@ inline unsigned grid_for(uint32_t n, int block = kBlock) { return (n + block - 1
 // Kernel family of a neighbour pass.
 enum NbrMode { kGlobal = 0, kHalfLds = 1, kExactLds = 2 };
 
+// What the FP16-LDS density pass reads and writes beside the plain kernel's arguments.
+struct HalfRecs {
+  const uint2* spos;
+  const float *svx, *svy, *svz;
+  StagedNbr* frec;
+};
+
 template <int kB, bool kCover>
 void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t n, const float* spx,
                       const float* spy, const float* spz, const uint32_t* skeys, const uint4* cells,
-                      float* srho, float* spres, int mode, int stage, CoverOut co) {
+                      float* srho, float* spres, int mode, int stage, CoverOut co, const HalfRecs& hr) {
   if (mode == kGlobal) {
     hipLaunchKernelGGL((k_density<kB, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx, spy, spz,
                        skeys, cells, srho, spres, co);
@@ -987,20 +1002,20 @@ void launch_density_t(hipStream_t s, const FluidK& k, const SceneK& sk, uint32_t
                          spy, spz, skeys, cells, srho, spres, co);
   } else if (stage == 64) {
     hipLaunchKernelGGL((k_density_lds<kB, 64, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
-                       spy, spz, skeys, cells, srho, spres, co);
+                       spy, spz, skeys, cells, srho, spres, co, hr.spos, hr.svx, hr.svy, hr.svz, hr.frec);
   } else if (stage == 256) {
     hipLaunchKernelGGL((k_density_lds<kB, 256, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
-                       spy, spz, skeys, cells, srho, spres, co);
+                       spy, spz, skeys, cells, srho, spres, co, hr.spos, hr.svx, hr.svy, hr.svz, hr.frec);
   } else {
     hipLaunchKernelGGL((k_density_lds<kB, 128, kCover>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, n, spx,
-                       spy, spz, skeys, cells, srho, spres, co);
+                       spy, spz, skeys, cells, srho, spres, co, hr.spos, hr.svx, hr.svy, hr.svz, hr.frec);
   }
 }
 
 template <int kB>
 void launch_forces_t(hipStream_t s, const FluidK& k, const SceneK& sk, const ColliderDev& col, uint32_t n,
                      const SortedIn& in, const uint32_t* svals, const uint32_t* skeys, const uint4* cells,
-                     const StateOut& o, uint32_t* contacts, int mode, int stage) {
+                     const StateOut& o, uint32_t* contacts, int mode, int stage, const StagedNbr* frec) {
   if (mode == kGlobal) {
     hipLaunchKernelGGL(k_forces_integrate<kB>, dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n, in, svals,
                        skeys, cells, o, contacts);
@@ -1016,13 +1031,13 @@ void launch_forces_t(hipStream_t s, const FluidK& k, const SceneK& sk, const Col
                          in, svals, skeys, cells, o, contacts);
   } else if (stage == 64) {
     hipLaunchKernelGGL((k_forces_integrate_lds<kB, 64>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col, n,
-                       in, svals, skeys, cells, o, contacts);
+                       in, svals, skeys, cells, o, contacts, frec);
   } else if (stage == 256) {
     hipLaunchKernelGGL((k_forces_integrate_lds<kB, 256>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col,
-                       n, in, svals, skeys, cells, o, contacts);
+                       n, in, svals, skeys, cells, o, contacts, frec);
   } else {
     hipLaunchKernelGGL((k_forces_integrate_lds<kB, 128>), dim3(grid_for(n, kB)), dim3(kB), 0, s, k, sk, col,
-                       n, in, svals, skeys, cells, o, contacts);
+                       n, in, svals, skeys, cells, o, contacts, frec);
   }
 }
 
@@ -1044,7 +1059,7 @@ int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float*
 
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
-                   float* const* dst, int planes) {
+                   float* const* dst, int planes, const FluidK& k, uint2* spos) {
   if (n == 0) return CLAY_OK;
   Planes8 pl{};
   for (int p = 0; p < planes && p < 8; p++) {
@@ -1053,28 +1068,32 @@ int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, con
   }
   LaunchTimer t(c, s, "fluid_reorder");
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, n, skeys, svals, sk, invalid_key, cells,
-                     pl, planes);
+                     pl, planes, k, spos);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
 int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune, bool lds,
                    uint32_t n, const float* spx, const float* spy, const float* spz, const uint32_t* skeys,
-                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover) {
+                   const uint32_t* cells, float* srho, float* spres, const uint32_t* svals, float* cover,
+                   const HalfState& half) {
   if (n == 0) return CLAY_OK;
   const uint4* cl = reinterpret_cast<const uint4*>(cells);
   const int mode = lds ? kHalfLds : (tune.exact_lds ? kExactLds : kGlobal);
+  if (lds && (!half.spos || !half.frec)) return fail(CLAY_ERR_INVALID, "FP16-LDS density: staging records missing");
   const CoverOut co{svals, cover};
+  const HalfRecs hr{reinterpret_cast<const uint2*>(half.spos), half.svx, half.svy, half.svz,
+                    reinterpret_cast<StagedNbr*>(half.frec)};
   LaunchTimer t(c, s, lds ? "fluid_density_lds" : "fluid_density");
   if (cover) {
     if (tune.block == 128)
-      launch_density_t<128, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
+      launch_density_t<128, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co, hr);
     else
-      launch_density_t<256, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
+      launch_density_t<256, true>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co, hr);
   } else if (tune.block == 128) {
-    launch_density_t<128, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
+    launch_density_t<128, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co, hr);
   } else {
-    launch_density_t<256, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co);
+    launch_density_t<256, false>(s, k, sk, n, spx, spy, spz, skeys, cl, srho, spres, mode, tune.stage, co, hr);
   }
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
@@ -1083,8 +1102,9 @@ int launch_density(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, con
 int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK& sk, const FluidTune& tune,
                             bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
                             const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
-                            float* const* out, uint32_t* contacts) {
+                            float* const* out, uint32_t* contacts, const float4* frec) {
   if (n == 0) return CLAY_OK;
+  if (lds && !frec) return fail(CLAY_ERR_INVALID, "FP16-LDS forces: staging records missing");
   SortedIn in{};
   for (int p = 0; p < 8; p++) in.p[p] = sorted[p];
   StateOut o{};
@@ -1093,9 +1113,9 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK
   const int mode = lds ? kHalfLds : (tune.exact_lds ? kExactLds : kGlobal);
   LaunchTimer t(c, s, lds ? "fluid_forces_integrate_lds" : "fluid_forces_integrate");
   if (tune.block == 128)
-    launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage);
+    launch_forces_t<128>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage, reinterpret_cast<const StagedNbr*>(frec));
   else
-    launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage);
+    launch_forces_t<256>(s, k, sk, col, n, in, svals, skeys, cl, o, contacts, mode, tune.stage, reinterpret_cast<const StagedNbr*>(frec));
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
