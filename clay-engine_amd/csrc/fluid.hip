@@ -455,39 +455,36 @@ __device__ __forceinline__ void walk_row(const FluidK& k, const RowRun& r, F&& v
   }
 }
 
-// The same visits as walk_row, cell by cell, passing the cell's x step (0..2).
-template <typename F>
-__device__ __forceinline__ void walk_cells(const FluidK& k, const RowRun& r, F&& visit) {
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    if (!r.ok[j]) continue;
-    const uint32_t cap = r.st[j] + (uint32_t)k.bucket;
-    const uint32_t e = r.en[j] < cap ? r.en[j] : cap;
-    for (uint32_t t = r.st[j]; t < e; t++) visit(t, j);
-  }
-}
-
 // Offset of p from the centre of its own cell b.
 __device__ __forceinline__ f3 cell_rel(const FluidK& k, f3 p, const int32_t b[3]) {
   return mk(p.x - (k.go[0] + ((float)b[0] + 0.5f) * k.h), p.y - (k.go[1] + ((float)b[1] + 0.5f) * k.h),
             p.z - (k.go[2] + ((float)b[2] + 0.5f) * k.h));
 }
 
-// One lane's share of a row (dy, dz): the sorted range of the row's cells that can
-// hold a particle within `reach` of p_i.  A neighbour cell is dropped when the gap
-// between p_i and the cell's box is at least reach (with 1e-4 of margin); the row's
-// three cells are consecutive keys, so what remains is one range.
-struct RowSpan {
-  uint32_t lo, hi;
-};
-__device__ __forceinline__ RowSpan row_span(const RowRun& r, f3 oi, int dy, int dz, float h, float reach2) {
+// Which of a row's three cells (dy, dz fixed, dx = -1, 0, 1) can hold a particle
+// within `reach` of p_i (o_i: its offset from its cell centre): a cell is dropped
+// when the gap between p_i and the cell's box is at least reach, by a relative
+// margin on the squared reach that covers the rounding of o_i and of the cell
+// assignment.  Every candidate in a dropped cell lies beyond reach, so dropping it
+// changes no sum.
+__device__ __forceinline__ void row_keep(f3 oi, int dy, int dz, float h, float reach2, float margin, bool keep[3]) {
   const float hh = 0.5f * h;
   const float gy = dy == 0 ? 0.0f : fmax_s(dy < 0 ? oi.y + hh : hh - oi.y, 0.0f);
   const float gz = dz == 0 ? 0.0f : fmax_s(dz < 0 ? oi.z + hh : hh - oi.z, 0.0f);
   const float gyz = gy * gy + gz * gz;
   const float gl = fmax_s(oi.x + hh, 0.0f), gr = fmax_s(hh - oi.x, 0.0f);
-  const float lim = reach2 * 1.0001f;
-  const bool keep[3] = {gl * gl + gyz < lim, gyz < lim, gr * gr + gyz < lim};
+  const float lim = reach2 * margin;
+  keep[0] = gl * gl + gyz < lim;
+  keep[1] = gyz < lim;
+  keep[2] = gr * gr + gyz < lim;
+}
+
+// One lane's share of a row: the sorted range of the kept cells.  The row's three
+// cells are consecutive keys, so what remains is one range.
+struct RowSpan {
+  uint32_t lo, hi;
+};
+__device__ __forceinline__ RowSpan kept_span(const RowRun& r, const bool keep[3]) {
   RowSpan sp{0xFFFFFFFFu, 0u};
 #pragma unroll
   for (int j = 0; j < 3; j++)
@@ -496,6 +493,11 @@ __device__ __forceinline__ RowSpan row_span(const RowRun& r, f3 oi, int dy, int 
       sp.hi = r.en[j] > sp.hi ? r.en[j] : sp.hi;
     }
   return sp;
+}
+__device__ __forceinline__ RowSpan row_span(const RowRun& r, f3 oi, int dy, int dz, float h, float reach2) {
+  bool keep[3];
+  row_keep(oi, dy, dz, h, reach2, 1.0001f, keep);
+  return kept_span(r, keep);
 }
 
 // base - (the staged f16 offset): each component is one v_fma_mix_f32, which widens
@@ -616,21 +618,29 @@ __global__ void __launch_bounds__(kB)
     }
     if (kCover) cov.add(d, r2, k.hh);
   };
+  // Cells beyond reach hold only terms that add nothing (the interior radius
+  // classes neighbours out to 1.5 h): skipping them keeps every sum bit for bit.
+  const f3 oi = cell_rel(k, pi, b);
+  const float reach2 = kCover ? 2.25f * k.hh : k.hh;
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
-      const bool any = r.hi > r.lo;
-      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      bool keep[3];
+      row_keep(oi, dy, dz, k.h, reach2, 1.002f, keep);
+      const RowSpan sp = kept_span(r, keep);
+      const bool any = sp.hi > sp.lo;
+      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = make_float4(spx[t], spy[t], spz[t], 0.0f);
       wave_sync();
       if (staged) {
-        walk_cells(k, r, [&](uint32_t t, int) {
+        // The kept cells' candidates in increasing sorted index: the plain walk's order.
+        for (uint32_t t = sp.lo; t < sp.hi; t++) {
           const float4 q = stage[wave][t - ulo];
           term(mk(pi.x - q.x, pi.y - q.y, pi.z - q.z));
-        });
+        }
       } else {
         walk_row(k, r, [&](uint32_t t) { term(mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t])); });
       }
@@ -698,12 +708,17 @@ __global__ void __launch_bounds__(kB)
       fp = mk(fp.x - dv.x * ir * s2, fp.y - dv.y * ir * s2, fp.z - dv.z * ir * s2);
     }
   };
+  // Cells beyond h hold only terms that add nothing: skipping them keeps every sum.
+  const f3 oi = cell_rel(k, pi, b);
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
-      const bool any = r.hi > r.lo;
-      const uint32_t ulo = wave_min_u32(any ? r.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? r.hi : 0u);
+      bool keep[3];
+      row_keep(oi, dy, dz, k.h, k.hh, 1.002f, keep);
+      const RowSpan sp = kept_span(r, keep);
+      const bool any = sp.hi > sp.lo;
+      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
+      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
@@ -714,10 +729,10 @@ __global__ void __launch_bounds__(kB)
         }
       wave_sync();
       if (staged) {
-        walk_cells(k, r, [&](uint32_t t, int) {
+        for (uint32_t t = sp.lo; t < sp.hi; t++) {
           const StagedX& e = stage[wave][t - ulo];
           term(t, mk(pi.x - e.pr.x, pi.y - e.pr.y, pi.z - e.pr.z), mk(e.vp.x, e.vp.y, e.vp.z), e.pr.w, e.vp.w);
-        });
+        }
       } else {
         walk_row(k, r, [&](uint32_t t) {
           term(t, mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t],
