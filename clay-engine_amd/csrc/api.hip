@@ -859,7 +859,9 @@ int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
     return rc;
   }
   const SceneK sk = f->scene_k();
-  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, f->keys, f->vals, invalid))) return rc;
+  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, f->keys, f->vals, invalid,
+                       f->scene_shift)))
+    return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
@@ -1103,12 +1105,19 @@ int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t s
     fluid_free(f);
     return fail(CLAY_ERR_INVALID, "FluidSim: worldSize / smoothing gives a hash grid beyond 2^32 cells");
   }
-  if (batch && G > (uint64_t)kCellMask) {
+  if (batch && G > (1ull << kMaxSceneShift) - 1) {
     fluid_free(f);
     return fail(CLAY_ERR_INVALID, "FluidBatch: worldSize / smoothing gives a hash grid beyond 2^24 - 1 cells");
   }
   f->grid_cells = (size_t)G;
-  const uint64_t max_key = batch ? ((uint64_t)(scenes - 1) << kSceneShift) | kCellMask : G;
+  // A batch key is scene << shift | cell, with the all-ones cell field kept free
+  // for particles outside the grid: the fewest bits with 2^shift - 1 >= G.
+  if (batch) {
+    uint32_t sh = 1;
+    while ((1ull << sh) - 1 < G) sh++;
+    f->scene_shift = sh;
+  }
+  const uint64_t max_key = batch ? ((uint64_t)(scenes - 1) << f->scene_shift) | ((1ull << f->scene_shift) - 1) : G;
   int bits = 1;
   while (bits < 32 && (1ull << bits) <= max_key) bits++;
   f->key_bits = bits;

@@ -59,16 +59,17 @@ struct ColliderDev {
   LeafDev leaf[CLAY_MAX_LEAVES];
 };
 
-// How one substep launch finds its scenes.  A batch sorts keys (scene << 24 | cell),
+// How one substep launch finds its scenes.  A batch sorts keys (scene << shift | cell),
 // so the sorted key of a particle names its scene; a single simulation is scene 0.
 // Each scene owns `cells_per_scene` cell records and `contact_stride` words of
 // contact list.  The substep's stamp is read from device memory so that a frame
 // captured once in a graph advances it on every replay.
-constexpr int kSceneShift = 24;
-constexpr uint32_t kCellMask = (1u << kSceneShift) - 1u;  // cell bits; all ones = outside the grid
+constexpr int kMaxSceneShift = 24;  // cell bits of a batch key at most
 struct SceneK {
   const uint32_t* epoch;
   uint32_t batched;
+  uint32_t shift;      // batch keys: scene << shift | cell; the cell field all ones = outside the grid
+  uint32_t cell_mask;  // (1 << shift) - 1
   uint32_t cells_per_scene;
   uint32_t contact_stride;
   uint32_t xcd;     // renumber workgroups XCD-contiguously (a launch-shape choice)

@@ -193,6 +193,7 @@ struct Fluid {
   uint32_t* epoch_dev = nullptr; // stamp of the current substep's cell records
   size_t grid_cells = 0;
   int key_bits = 0;
+  uint32_t scene_shift = 0;      // batches: cell bits of a key (scene << scene_shift | cell)
   uint32_t* contacts = nullptr;  // per scene: [0] = count, then 32 x float4 (as bits)
   uint32_t* cmask = nullptr;     // one bit per particle id: in contact this substep
   float4* cdata = nullptr;       // per particle id: contact point and speed
@@ -222,6 +223,8 @@ struct Fluid {
     sk.xcd = tune.xcd ? 1u : 0u;
     sk.epoch = epoch_dev;
     sk.batched = scene_of != nullptr;
+    sk.shift = scene_shift;
+    sk.cell_mask = scene_shift ? (1u << scene_shift) - 1u : 0xFFFFFFFFu;
     sk.cells_per_scene = (uint32_t)grid_cells;
     sk.contact_stride = 4 + 4 * CLAY_CONTACTS;
     sk.cmask = cmask;
@@ -259,7 +262,8 @@ int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, con
                           const float* const* pos, const float* const* rest, double* out);
 // fluid.hip
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
-                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key);
+                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
+                uint32_t shift);
 // With `spos` (FP16-LDS), also each sorted particle's staged position.
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,

@@ -46,7 +46,7 @@ __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) {
 
 // Scene of sorted slot s and the first of its cell records.
 __device__ __forceinline__ uint32_t scene_at(const SceneK& sk, const uint32_t* __restrict__ skeys, uint32_t s) {
-  return sk.batched ? skeys[s] >> kSceneShift : 0u;
+  return sk.batched ? skeys[s] >> sk.shift : 0u;
 }
 __device__ __forceinline__ const uint4* scene_cells(const SceneK& sk, const uint4* __restrict__ cells,
                                                     uint32_t scene) {
@@ -106,21 +106,22 @@ __device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __r
 }
 
 // Cell key per particle.  One scene: the cell index, or `invalid` outside the
-// grid.  A batch: (scene << 24) | cell, with all cell bits set outside the grid,
-// so a scene's outside particles sort after its cells.
+// grid.  A batch: (scene << shift) | cell, with all `shift` cell bits set outside
+// the grid, so a scene's outside particles sort after its cells; `shift` is the
+// fewest bits that leave that value free, so the keys stay short for the sort.
 __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const float* __restrict__ px,
                                                  const float* __restrict__ py,
                                                  const float* __restrict__ pz,
                                                  const uint32_t* __restrict__ scene_of,
                                                  uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, uint32_t invalid) {
+                                                 uint32_t* __restrict__ vals, uint32_t invalid, uint32_t shift) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   int32_t q[3];
   cell_of(k, mk(px[i], py[i], pz[i]), q);
   const bool in = in_grid(k, q[0], q[1], q[2]);
   if (scene_of)
-    keys[i] = (scene_of[i] << kSceneShift) | (in ? cell_index(k, q[0], q[1], q[2]) : kCellMask);
+    keys[i] = (scene_of[i] << shift) | (in ? cell_index(k, q[0], q[1], q[2]) : (1u << shift) - 1u);
   else
     keys[i] = in ? cell_index(k, q[0], q[1], q[2]) : invalid;
   vals[i] = i;
@@ -162,10 +163,10 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t key = skeys[s];
-  const uint32_t cell = sk.batched ? (key & kCellMask) : key;
-  const bool inside = sk.batched ? cell != kCellMask : key != invalid;
+  const uint32_t cell = sk.batched ? (key & sk.cell_mask) : key;
+  const bool inside = sk.batched ? cell != sk.cell_mask : key != invalid;
   if (inside) {
-    const size_t rec = 4 * ((size_t)(sk.batched ? key >> kSceneShift : 0u) * sk.cells_per_scene + cell);
+    const size_t rec = 4 * ((size_t)(sk.batched ? key >> sk.shift : 0u) * sk.cells_per_scene + cell);
     if (s == 0 || skeys[s - 1] != key) {
       cells[rec + 0] = s;
       cells[rec + 2] = *sk.epoch;
@@ -1063,11 +1064,12 @@ bool fluid_tune_valid(const FluidTune& t) {
 }
 
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
-                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key) {
+                const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
+                uint32_t shift) {
   if (n == 0) return CLAY_OK;
   LaunchTimer t(c, s, "fluid_keys");
   hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, px, py, pz, scene_of, keys, vals,
-                     invalid_key);
+                     invalid_key, shift);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

@@ -2,10 +2,10 @@
 //
 // Replaces the reference's insert pass (fluid.ts:308-331), whose bucket order is
 // the atomicAdd order.  Here the order is defined: ascending key, ties by
-// ascending id.  8-bit digits, three launches per digit:
+// ascending id.  Digits of 8 to 11 bits (the fewest passes), three launches per digit:
 //   k_hist      per-tile digit histogram in LDS             -> hist[digit][tile]
 //   k_scan_rows exclusive scan of each digit's row of tiles  -> row totals
-//   k_scatter   digit base (scan of 256 row totals, in LDS) + tile offset +
+//   k_scatter   digit base (scan of the row totals, in LDS) + tile offset +
 //               stable rank inside the tile from wave64 ballots
 // A tile is 1024 keys: 4 rounds of 256 threads, so rank order inside a tile is
 // (round, lane) = input order and the sort is stable.
@@ -19,21 +19,24 @@ constexpr int kThreads = 256;
 constexpr int kRounds = 4;
 constexpr int kTileKeys = kThreads * kRounds;
 constexpr int kWaves = kThreads / 64;
+constexpr int kMaxDigitBits = 11;
 
+template <int W>
 __global__ void __launch_bounds__(kThreads)
     k_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift, uint32_t* __restrict__ hist,
            uint32_t ntiles) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr uint32_t kBins = 1u << W;
+  __shared__ uint32_t h[kBins];
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kTileKeys;
 #pragma unroll
   for (int r = 0; r < kRounds; r++) {
     const uint32_t i = base + r * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kBins - 1u)], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) hist[b * ntiles + blockIdx.x] = h[b];
 }
 
 // One block per digit: exclusive scan of hist[d][0..ntiles) in place; total -> totals[d].
@@ -65,31 +68,42 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
+template <int W>
 __global__ void __launch_bounds__(kThreads)
     k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
               int shift, const uint32_t* __restrict__ hist, uint32_t ntiles,
               const uint32_t* __restrict__ totals, uint32_t* __restrict__ keys_out,
               uint32_t* __restrict__ vals_out) {
-  __shared__ uint32_t digit_base[256];
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wave_cnt[kWaves][256];
+  constexpr uint32_t kBins = 1u << W;
+  constexpr uint32_t kPer = kBins / kThreads;  // digits per thread
+  __shared__ uint32_t digit_base[kBins];
+  __shared__ uint32_t running[kBins];
+  __shared__ uint32_t wave_cnt[kWaves][kBins];
+  __shared__ uint32_t part[kThreads];
   const int t = threadIdx.x;
   const int wave = t >> 6;
-  // Exclusive scan of the 256 digit totals (Hillis-Steele in LDS).
+  // Exclusive scan of the digit totals: each thread sums its kPer digits, a
+  // Hillis-Steele scan over the 256 partial sums, then the digits in order.
   {
-    const uint32_t v = totals[t];
-    digit_base[t] = v;
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; q++) sum += totals[t * kPer + q];
+    part[t] = sum;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const uint32_t add = t >= off ? digit_base[t - off] : 0u;
+    for (int off = 1; off < kThreads; off <<= 1) {
+      const uint32_t add = t >= off ? part[t - off] : 0u;
       __syncthreads();
-      digit_base[t] += add;
+      part[t] += add;
       __syncthreads();
     }
-    const uint32_t incl = digit_base[t];
-    __syncthreads();
-    digit_base[t] = incl - v + hist[(size_t)t * ntiles + blockIdx.x];
-    running[t] = 0;
+    uint32_t acc = part[t] - sum;
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; q++) {
+      const uint32_t d = t * kPer + q;
+      digit_base[d] = acc + hist[(size_t)d * ntiles + blockIdx.x];
+      acc += totals[d];
+      running[d] = 0;
+    }
   }
   const uint64_t lt = __lanemask_lt();
   const uint32_t base = blockIdx.x * kTileKeys;
@@ -100,15 +114,17 @@ __global__ void __launch_bounds__(kThreads)
     if (active) {
       key = keys[i];
       val = vals[i];
-      d = (key >> shift) & 255u;
+      d = (key >> shift) & (kBins - 1u);
     }
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) wave_cnt[w][t] = 0;
+    for (int w = 0; w < kWaves; w++)
+#pragma unroll
+      for (uint32_t q = 0; q < kPer; q++) wave_cnt[w][t * kPer + q] = 0;
     __syncthreads();
     // Lanes of this wave holding the same digit.
     uint64_t same = __ballot(active);
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
+    for (int b = 0; b < W; b++) {
       const bool bit = (d >> b) & 1u;
       const uint64_t ones = __ballot(bit);
       same &= bit ? ones : ~ones;
@@ -117,15 +133,17 @@ __global__ void __launch_bounds__(kThreads)
     if (active && rank == 0) wave_cnt[wave][d] = (uint32_t)__popcll(same);
     __syncthreads();
     // Per digit: offsets of each wave's group after the earlier waves and rounds.
-    {
-      uint32_t acc = running[t];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; q++) {
+      const uint32_t dd = t * kPer + q;
+      uint32_t acc = running[dd];
 #pragma unroll
       for (int w = 0; w < kWaves; w++) {
-        const uint32_t c = wave_cnt[w][t];
-        wave_cnt[w][t] = acc;
+        const uint32_t c = wave_cnt[w][dd];
+        wave_cnt[w][dd] = acc;
         acc += c;
       }
-      running[t] = acc;
+      running[dd] = acc;
     }
     __syncthreads();
     if (active) {
@@ -137,33 +155,55 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+template <int W>
+void sort_pass(Ctx* c, hipStream_t s, uint32_t n, uint32_t ntiles, int shift, const uint32_t* keys,
+               const uint32_t* vals, uint32_t* hist, uint32_t* totals, uint32_t* keys_out, uint32_t* vals_out) {
+  {
+    LaunchTimer t(c, s, "sort_hist");
+    hipLaunchKernelGGL(k_hist<W>, dim3(ntiles), dim3(kThreads), 0, s, keys, n, shift, hist, ntiles);
+  }
+  {
+    LaunchTimer t(c, s, "sort_scan");
+    hipLaunchKernelGGL(k_scan_rows, dim3(1u << W), dim3(kThreads), 0, s, hist, ntiles, totals);
+  }
+  {
+    LaunchTimer t(c, s, "sort_scatter");
+    hipLaunchKernelGGL(k_scatter<W>, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, n, shift, hist, ntiles,
+                       totals, keys_out, vals_out);
+  }
+}
+
 }  // namespace
 
 size_t radix_hist_size(uint32_t n) {
   const size_t ntiles = (n + kTileKeys - 1) / kTileKeys;
-  return 256 * (ntiles > 0 ? ntiles : 1) + 256;
+  return ((size_t)1 << kMaxDigitBits) * ((ntiles > 0 ? ntiles : 1) + 1);
 }
 
+// Fewest passes of at most 11-bit digits, the digit width spread evenly: 24-bit keys
+// take three 8-bit passes, 27-bit keys three 9-bit passes, 32-bit keys three 11-bit.
 int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
                      uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
   if (n == 0) return CLAY_OK;
   const uint32_t ntiles = (n + kTileKeys - 1) / kTileKeys;
-  uint32_t* totals = hist + 256 * (size_t)ntiles;
-  const int passes = (key_bits + 7) / 8;
+  const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const int w = std::max(8, (key_bits + passes - 1) / passes);
+  uint32_t* totals = hist + ((size_t)1 << w) * ntiles;
   for (int p = 0; p < passes; p++) {
-    const int shift = 8 * p;
-    {
-      LaunchTimer t(c, s, "sort_hist");
-      hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kThreads), 0, s, keys, n, shift, hist, ntiles);
-    }
-    {
-      LaunchTimer t(c, s, "sort_scan");
-      hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kThreads), 0, s, hist, ntiles, totals);
-    }
-    {
-      LaunchTimer t(c, s, "sort_scatter");
-      hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, n, shift, hist,
-                         ntiles, totals, keys_alt, vals_alt);
+    const int shift = w * p;
+    switch (w) {
+      case 8:
+        sort_pass<8>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        break;
+      case 9:
+        sort_pass<9>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        break;
+      case 10:
+        sort_pass<10>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        break;
+      default:
+        sort_pass<11>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        break;
     }
     CLAY_HIP_CHECK(hipGetLastError());
     uint32_t* tk = keys;
