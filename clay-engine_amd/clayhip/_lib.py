@@ -217,6 +217,7 @@ class EditorDesc(C.Structure):
 
 
 REDUCE_F32_TREE, REDUCE_F64 = 0, 1
+QUERY_SAMPLE, QUERY_FIELD = 0, 1
 
 class SolverDesc(C.Structure):
     _fields_ = [
@@ -311,6 +312,9 @@ SIGNATURES = {
     "clay_field_destroy": ([P], C.c_int),
     "clay_field_max_mip": ([P, C.POINTER(I32)], C.c_int),
     "clay_field_eval": ([P, P, I32, P, P], C.c_int),
+    "clay_field_query": ([P, I32, I32, P, I32, P], C.c_int),
+    "clay_field_entry": ([P, P, I32, P], C.c_int),
+    "clay_field_scale": ([P, I32, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
     "clay_splat_create": ([P, C.POINTER(CloudDesc), C.POINTER(SplatDesc), PP], C.c_int),
     "clay_splat_destroy": ([P], C.c_int),
     "clay_splat_set_live": ([P, U32, U32], C.c_int),

@@ -41,6 +41,44 @@ class TracedField:
         L.check(L.lib().clay_field_eval(self._h, pts.ctypes.data, n, dist.ctypes.data, nrm.ctypes.data))
         return dist, nrm
 
+    def _query(self, what: int, points, mip: int) -> np.ndarray:
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((pts.shape[0], 2), dtype=np.float32)
+        L.check(L.lib().clay_field_query(self._h, what, int(mip), pts.ctypes.data, pts.shape[0], out.ctypes.data))
+        return out
+
+    def sample(self, points, mip: int = 0) -> np.ndarray:
+        """TracedField.sample(p, mip) (field.ts:35-41): (n, 2) world distance and
+        band-normalised value, computed on the GPU."""
+        return self._query(L.QUERY_SAMPLE, points, mip)
+
+    def field(self, points, mip: int = 0) -> np.ndarray:
+        """TracedField.field(p, mip) (field.ts:42-43): (n, 2) world distance and material."""
+        return self._query(L.QUERY_FIELD, points, mip)
+
+    def entry(self, ro, rd, tMin=0.0, radius=0.0, aperture=0.0) -> np.ndarray:
+        """TracedField.entry (field.ts:55-74) per ray: the first t at which a swept
+        shape could touch the field, or 1e9 (NEVER)."""
+        ro = np.asarray(ro, dtype=np.float32).reshape(-1, 3)
+        rd = np.asarray(rd, dtype=np.float32).reshape(-1, 3)
+        n = ro.shape[0]
+        rays = np.empty((n, 9), dtype=np.float32)
+        rays[:, 0:3], rays[:, 3:6] = ro, rd
+        rays[:, 6], rays[:, 7], rays[:, 8] = tMin, radius, aperture
+        out = np.empty(n, dtype=np.float32)
+        L.check(L.lib().clay_field_entry(self._h, rays.ctypes.data, n, out.ctypes.data))
+        return out
+
+    def bandWorld(self, mip: int = 0) -> float:
+        b = C.c_float()
+        L.check(L.lib().clay_field_scale(self._h, int(mip), C.byref(b), None))
+        return b.value
+
+    def voxelWorld(self, mip: int = 0) -> float:
+        v = C.c_float()
+        L.check(L.lib().clay_field_scale(self._h, int(mip), None, C.byref(v)))
+        return v.value
+
     def destroy(self) -> None:
         if self._owned and self._h is not None:
             L.lib().clay_field_destroy(self._h)

@@ -221,6 +221,9 @@ Field* make_volume_field(Ctx* ctx, Volume* v) {
   l.vol = v->dev;
   l.dist0 = v->dist;
   l.mat0 = v->mat;
+  f->vols = {v};
+  f->bands = {0.0};
+  f->eps = {0.0};
   // One instance per scene: the collider reads the scene's own record and planes.
   l.scene_stride = v->instances > 1 ? (uint32_t)v->total_voxels : 0u;
   return f;
@@ -475,6 +478,9 @@ extern "C" int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int
   l.brushes = f->brushes_dev;
   l.eps = (float)(opts && opts->epsilon > 0 ? opts->epsilon : 1e-3);
   l.material = (float)(opts ? opts->material : 0.0);
+  f->vols = {nullptr};
+  f->bands = {opts && opts->band > 0 ? opts->band : 64.0};
+  f->eps = {opts && opts->epsilon > 0 ? opts->epsilon : 1e-3};
   *out = reinterpret_cast<clay_field*>(f);
   return CLAY_OK;
 }
@@ -500,6 +506,11 @@ extern "C" int clay_field_union(clay_field* a, clay_field* b, clay_field** out) 
   f->col.nleaf = 0;
   for (int i = 0; i < fa->col.nleaf; i++) f->col.leaf[f->col.nleaf++] = fa->col.leaf[i];
   for (int i = 0; i < fb->col.nleaf; i++) f->col.leaf[f->col.nleaf++] = fb->col.leaf[i];
+  for (const Field* c : {fa, fb}) {
+    f->vols.insert(f->vols.end(), c->vols.begin(), c->vols.end());
+    f->bands.insert(f->bands.end(), c->bands.begin(), c->bands.end());
+    f->eps.insert(f->eps.end(), c->eps.begin(), c->eps.end());
+  }
   f->children = {fa, fb};
   *out = reinterpret_cast<clay_field*>(f);
   return CLAY_OK;

@@ -112,6 +112,10 @@ struct Field {
   ColliderDev col{};             // leaves point at device memory owned below / by volumes
   clay_brush* brushes_dev = nullptr;
   std::vector<Field*> children;  // union operands (not owned)
+  // Per leaf, for the reads at every mip (clay_field_query / entry): the volume of a
+  // volume leaf (else null), and an analytic leaf's band and epsilon options.
+  std::vector<Volume*> vols;
+  std::vector<double> bands, eps;
 };
 
 struct Splat {

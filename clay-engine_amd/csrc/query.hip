@@ -1,0 +1,226 @@
+// query.hip -- what a tracer reads from a TracedField (lib/src/trace/field.ts:20-315),
+// at any mip: sample (world distance, band-normalised value), field (world distance,
+// material), the box entry of a swept ray, bandWorld and voxelWorld.  The fluid's
+// collider reads mip 0 only (clay_device.h); these read every level of a volume
+// leaf, so their per-leaf description is built on the host for each call and does
+// not enlarge the collider that every fluid kernel takes.  A volume with several
+// instances is read at instance 0.
+#include <cmath>
+
+#include "clay_internal.h"
+
+namespace clay {
+
+namespace {
+
+constexpr float kNever = 1e9f;  // NEVER (field.ts:82)
+
+struct QLeaf {
+  int32_t kind;    // 0 analytic, 1 volume
+  int32_t mips;
+  float inv_band;  // analytic: 1 / band
+  const VolDev* vol;
+  const uint16_t* dist;
+  const uint16_t* mat;
+  uint32_t off[CLAY_MAX_MIPS];
+  int32_t res[CLAY_MAX_MIPS];
+};
+
+struct QueryK {
+  int32_t nleaf;
+  QLeaf leaf[CLAY_MAX_LEAVES];
+};
+
+// textureSampleLevel(linear, clamp-to-edge) of one channel at a level of res^3
+// texels (volume.ts:262-267), with f32 weights.
+__device__ inline float sample_level(const VolDev& v, const uint16_t* d, int res, f3 p) {
+  const float u = (p.x - v.origin[0]) / v.extent;
+  const float w = (p.y - v.origin[1]) / v.extent;
+  const float s = (p.z - v.origin[2]) / v.extent;
+  const float fn = (float)res;
+  const float tx = u * fn - 0.5f, ty = w * fn - 0.5f, tz = s * fn - 0.5f;
+  const float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
+  const int x0 = sat_i32(fx0), y0 = sat_i32(fy0), z0 = sat_i32(fz0);
+  const float fx = tx - fx0, fy = ty - fy0, fz = tz - fz0;
+  const float c00 = fmix(texel0(d, res, x0, y0, z0), texel0(d, res, x0 + 1, y0, z0), fx);
+  const float c10 = fmix(texel0(d, res, x0, y0 + 1, z0), texel0(d, res, x0 + 1, y0 + 1, z0), fx);
+  const float c01 = fmix(texel0(d, res, x0, y0, z0 + 1), texel0(d, res, x0 + 1, y0, z0 + 1), fx);
+  const float c11 = fmix(texel0(d, res, x0, y0 + 1, z0 + 1), texel0(d, res, x0 + 1, y0 + 1, z0 + 1), fx);
+  return fmix(fmix(c00, c10, fy), fmix(c01, c11, fy), fz);
+}
+
+// sample(p, mip) or field(p, mip) of one leaf.  Volume: outside the box (dBox, 1) or
+// (dBox, 0); inside, the level min(mip, maxMip) read as (raw * bandWorld, raw) or
+// (dist * bandWorld, material) (field.ts:135-149, volume.ts:277-296).  Analytic:
+// (v, clamp(v / band, -1, 1)) or (v, material) (field.ts:208-216).
+__device__ inline float2 leaf_read(const QLeaf& q, const LeafDev& l, f3 p, int mip, bool field) {
+  if (q.kind == 0) {
+    const float v = analytic_dist(l, p);
+    return field ? make_float2(v, l.material) : make_float2(v, fclamp_s(v * q.inv_band, -1.0f, 1.0f));
+  }
+  const VolDev& v = *q.vol;
+  const f3 rel = mk(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
+  const float dbox = sd_box(rel, mk(v.half, v.half, v.half));
+  if (dbox > 0.0f) return make_float2(dbox, field ? 0.0f : 1.0f);
+  const int m = mip < q.mips - 1 ? mip : q.mips - 1;
+  const float band_world = v.band * v.voxel * ldexpf(1.0f, m);
+  const float raw = sample_level(v, q.dist + q.off[m], q.res[m], p);
+  if (!field) return make_float2(raw * band_world, raw);
+  return make_float2(raw * band_world, sample_level(v, q.mat + q.off[m], q.res[m], p));
+}
+
+// slabRange (field.ts:90-100): (tNear, tFar) against the box rel +- half.
+__device__ inline float2 slab_range(f3 rel, f3 half, f3 inv) {
+  const f3 a = mk((-half.x - rel.x) * inv.x, (-half.y - rel.y) * inv.y, (-half.z - rel.z) * inv.z);
+  const f3 b = mk((half.x - rel.x) * inv.x, (half.y - rel.y) * inv.y, (half.z - rel.z) * inv.z);
+  const f3 lo = mk(fmin_s(a.x, b.x), fmin_s(a.y, b.y), fmin_s(a.z, b.z));
+  const f3 hi = mk(fmax_s(a.x, b.x), fmax_s(a.y, b.y), fmax_s(a.z, b.z));
+  return make_float2(fmax_s(fmax_s(lo.x, lo.y), lo.z), fmin_s(fmin_s(hi.x, hi.y), hi.z));
+}
+
+// volumeField.entry (field.ts:150-177).
+__device__ inline float vol_entry(const VolDev& v, f3 ro, f3 rd, float tmin, float radius, float aperture) {
+  const f3 rel = mk(ro.x - v.center[0], ro.y - v.center[1], ro.z - v.center[2]);
+  const f3 half = mk(v.half, v.half, v.half);
+  const f3 r = mk(fabsf(rd.x) < 1e-8f ? 1e-8f : rd.x, fabsf(rd.y) < 1e-8f ? 1e-8f : rd.y,
+                  fabsf(rd.z) < 1e-8f ? 1e-8f : rd.z);
+  const f3 inv = mk(1.0f / r.x, 1.0f / r.y, 1.0f / r.z);
+  const float2 bare = slab_range(rel, half, inv);
+  const float g = radius + aperture * fmax_s(bare.x, tmin);
+  const float2 grown = slab_range(rel, mk(half.x + g, half.y + g, half.z + g), inv);
+  const float t = fmax_s(grown.x, tmin);
+  return grown.y >= t ? t : kNever;
+}
+
+__global__ void __launch_bounds__(256)
+    k_field_query(QueryK q, ColliderDev col, int n, int mip, int field, const float* __restrict__ pts,
+                  float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const f3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+  // unionField: the whole pair from the nearer child, left unless right is strictly nearer.
+  float2 best = leaf_read(q.leaf[0], col.leaf[0], p, mip, field != 0);
+  for (int l = 1; l < q.nleaf; l++) {
+    const float2 r = leaf_read(q.leaf[l], col.leaf[l], p, mip, field != 0);
+    if (!(best.x < r.x)) best = r;
+  }
+  out[2 * i] = best.x;
+  out[2 * i + 1] = best.y;
+}
+
+// unionField.entry (field.ts:255-264): the nearest of the children's entries.
+__global__ void __launch_bounds__(256)
+    k_field_entry(QueryK q, int n, const float* __restrict__ rays, float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + 9 * (size_t)i;
+  const f3 ro = mk(r[0], r[1], r[2]), rd = mk(r[3], r[4], r[5]);
+  float t = vol_entry(*q.leaf[0].vol, ro, rd, r[6], r[7], r[8]);
+  for (int l = 1; l < q.nleaf; l++) t = fmin_s(t, vol_entry(*q.leaf[l].vol, ro, rd, r[6], r[7], r[8]));
+  out[i] = t;
+}
+
+QueryK query_k(const Field* f) {
+  QueryK q{};
+  q.nleaf = f->col.nleaf;
+  for (int l = 0; l < q.nleaf; l++) {
+    QLeaf& d = q.leaf[l];
+    const Volume* v = l < (int)f->vols.size() ? f->vols[l] : nullptr;
+    d.kind = v ? 1 : 0;
+    if (v) {
+      d.mips = v->info.mip_levels;
+      d.vol = v->dev;
+      d.dist = v->dist;
+      d.mat = v->mat;
+      for (int m = 0; m < d.mips; m++) {
+        d.off[m] = (uint32_t)v->voxel_off[m];
+        d.res[m] = v->mip_res(m);
+      }
+    } else {
+      d.mips = 1;
+      d.inv_band = (float)(1.0 / (l < (int)f->bands.size() ? f->bands[l] : 64.0));
+    }
+  }
+  return q;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() { dfree(p); }
+};
+
+}  // namespace
+
+}  // namespace clay
+
+using namespace clay;
+
+extern "C" int clay_field_query(clay_field* fh, int32_t what, int32_t mip, const float* pts, int32_t n, float* out2) {
+  Field* f = reinterpret_cast<Field*>(fh);
+  if (!f || (n > 0 && (!pts || !out2))) return fail(CLAY_ERR_INVALID, "clay_field_query: NULL argument");
+  if (what != CLAY_QUERY_SAMPLE && what != CLAY_QUERY_FIELD) return fail(CLAY_ERR_INVALID, "clay_field_query: unknown read");
+  if (mip < 0) return fail(CLAY_ERR_INVALID, "clay_field_query: negative mip");
+  if (n <= 0) return CLAY_OK;
+  DevBuf<float> dp, dout;
+  int rc;
+  if ((rc = dalloc(&dp.p, (size_t)n * 3)) || (rc = dalloc(&dout.p, (size_t)n * 2))) return rc;
+  hipStream_t s = f->ctx->stream;
+  CLAY_HIP_CHECK(hipMemcpyAsync(dp.p, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+  {
+    LaunchTimer t(f->ctx, s, "field_query");
+    hipLaunchKernelGGL(k_field_query, dim3((n + 255) / 256), dim3(256), 0, s, query_k(f), f->col, n, (int)mip,
+                       what == CLAY_QUERY_FIELD ? 1 : 0, dp.p, dout.p);
+  }
+  CLAY_HIP_CHECK(hipGetLastError());
+  CLAY_HIP_CHECK(hipMemcpyAsync(out2, dout.p, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  return CLAY_OK;
+}
+
+extern "C" int clay_field_entry(clay_field* fh, const float* rays9, int32_t n, float* t_out) {
+  Field* f = reinterpret_cast<Field*>(fh);
+  if (!f || (n > 0 && (!rays9 || !t_out))) return fail(CLAY_ERR_INVALID, "clay_field_entry: NULL argument");
+  for (const Volume* v : f->vols)
+    if (!v) return fail(CLAY_ERR_INVALID, "TracedField.entry: an analytic leaf bounds nothing, so the field has no entry");
+  if (n <= 0) return CLAY_OK;
+  DevBuf<float> dr, dout;
+  int rc;
+  if ((rc = dalloc(&dr.p, (size_t)n * 9)) || (rc = dalloc(&dout.p, (size_t)n))) return rc;
+  hipStream_t s = f->ctx->stream;
+  CLAY_HIP_CHECK(hipMemcpyAsync(dr.p, rays9, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+  {
+    LaunchTimer t(f->ctx, s, "field_entry");
+    hipLaunchKernelGGL(k_field_entry, dim3((n + 255) / 256), dim3(256), 0, s, query_k(f), n, dr.p, dout.p);
+  }
+  CLAY_HIP_CHECK(hipGetLastError());
+  CLAY_HIP_CHECK(hipMemcpyAsync(t_out, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  return CLAY_OK;
+}
+
+// bandWorld / voxelWorld (field.ts:46-48, 150-158, 218-224, 300-317), in f32 as a
+// shader evaluates them: volume band * voxel * 2^m and voxel * 2^m at m = min(mip,
+// maxMip); analytic band and epsilon; a union the smaller band and the larger voxel.
+extern "C" int clay_field_scale(const clay_field* fh, int32_t mip, float* band_world, float* voxel_world) {
+  const Field* f = reinterpret_cast<const Field*>(fh);
+  if (!f || mip < 0) return fail(CLAY_ERR_INVALID, "clay_field_scale: NULL field or negative mip");
+  float band = 0.0f, voxel = 0.0f;
+  for (size_t l = 0; l < f->vols.size(); l++) {
+    float b, vx;
+    if (const Volume* v = f->vols[l]) {
+      const int m = std::min<int>(mip, v->info.mip_levels - 1);
+      const float p2 = std::ldexp(1.0f, m);
+      b = (float)v->info.band * (float)(v->info.world_size / v->info.resolution) * p2;
+      vx = (float)(v->info.world_size / v->info.resolution) * p2;
+    } else {
+      b = (float)f->bands[l];
+      vx = (float)f->eps[l];
+    }
+    band = l == 0 ? b : std::min(band, b);
+    voxel = l == 0 ? vx : std::max(voxel, vx);
+  }
+  if (band_world) *band_world = band;
+  if (voxel_world) *voxel_world = voxel;
+  return CLAY_OK;
+}

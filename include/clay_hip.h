@@ -198,6 +198,22 @@ int clay_field_max_mip(const clay_field* f, int32_t* out);
 /* sample(p, 0).x and normal(p) at n host points (xyz interleaved); runs on the GPU. */
 int clay_field_eval(clay_field* f, const float* points_host, int32_t n, float* dist_host,
                     float* normal_host);
+/* What a tracer reads (field.ts:20-73), at n host points (xyz interleaved), on the
+ * GPU: CLAY_QUERY_SAMPLE -> sample(p, mip) = (world distance, band-normalised value);
+ * CLAY_QUERY_FIELD -> field(p, mip) = (world distance, material).  A volume leaf reads
+ * level min(mip, maxMip) (instance 0 of a multi-instance volume); a union returns
+ * the nearer child's pair.  out2: 2 floats per point. */
+#define CLAY_QUERY_SAMPLE 0
+#define CLAY_QUERY_FIELD 1
+int clay_field_query(clay_field* f, int32_t what, int32_t mip, const float* points_host, int32_t n,
+                     float* out2_host);
+/* entry(ro, rd, tMin, radius, aperture) per ray (9 floats: ro xyz, rd xyz, tMin,
+ * radius, aperture; field.ts:150-177 and 255-264): the first t >= tMin at which a
+ * shape of radius radius + aperture * t could touch the field's box, or 1e9 (NEVER).
+ * Fails for a field with an analytic leaf, which has no entry. */
+int clay_field_entry(clay_field* f, const float* rays9_host, int32_t n, float* t_host);
+/* bandWorld(mip) and voxelWorld(mip) in f32 (either pointer may be NULL). */
+int clay_field_scale(const clay_field* f, int32_t mip, float* band_world, float* voxel_world);
 
 /* ---- ParticleCloud + SplatField (lib/src/sim/splat.ts) ----------------- */
 #define CLAY_LIVE_ALL 0     /* occluder.gpu.ts:98-101 */

@@ -38,6 +38,7 @@ extern "C" {
 
 #define ORC_INVALID_KEY 0xFFFFFFFFu
 #define ORC_MAX_LEAVES 4
+#define ORC_MAX_MIPS 8
 #define ORC_CONTACTS 32
 
 /* Brush record: same field order and 80-byte size as the reference's `Brush`
@@ -109,6 +110,13 @@ typedef struct {
   double grad_voxels;
   const uint16_t* mat16;   /* volume: mip-0 `.g` channel (f16 bits), NULL reads 0 */
   double material;         /* analytic: the material `field` reports (field.ts:213-216) */
+  /* Every mip, for the reads at any level (orc_field_query): level m has res >> m
+   * texels per axis; NULL planes beyond `mips`.  Analytic: the band option. */
+  int32_t mips;
+  int32_t pad2;
+  const uint16_t* dist_m[ORC_MAX_MIPS];
+  const uint16_t* mat_m[ORC_MAX_MIPS];
+  double analytic_band;
 } orc_leaf;
 
 /* unionField over leaves, folded left: union(union(a, b), c) (field.ts:246-315). */
@@ -244,6 +252,11 @@ typedef struct {
    * indices: 6 * quad_capacity; counts: particles and quads found; draw: 4 u32.      \
    * Slots are taken in ascending cell index, quads in ascending lattice point then   \
    * edge axis. */                                                                    \
+  /* TracedField.sample (what 0) or field (what 1) at `mip`, and entry per ray      \
+   * (ro, rd, tMin, radius, aperture): field.ts:135-177, 208-216, 255-296. */          \
+  void orc_field_query_##S(const orc_collider* col, int what, int mip, int n,           \
+                           const T* p3, T* out2);                                      \
+  void orc_field_entry_##S(const orc_collider* col, int n, const T* rays9, T* out);     \
   void orc_extract_##S(const orc_extract_params* k, const orc_collider* col,          \
                        int capacity, uint32_t* body3, T* planes, T* aux,               \
                        uint32_t* owner, T* state, T* motion, uint32_t* ids,            \

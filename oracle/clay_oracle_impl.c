@@ -1414,3 +1414,101 @@ void FN(orc_extract)(const orc_extract_params* k, const orc_collider* col, int c
   draw[2] = 0;
   draw[3] = 0;
 }
+
+
+/* ---------------------------------------------------------------------------
+ * What a tracer reads from a field (lib/src/trace/field.ts:20-315) at any mip.
+ * ------------------------------------------------------------------------- */
+static T sample_mip(const orc_leaf* l, const uint16_t* plane, int res, const volp* v, v3 p) {
+  orc_leaf lm = *l;
+  volp vm = *v;
+  lm.res = res;
+  vm.res = res;
+  return sample_plane(&lm, plane, &vm, p);
+}
+
+/* sample(p, mip) (what 0) or field(p, mip) (what 1) of one leaf: field.ts:135-149,
+ * volume.ts:277-296; analytic field.ts:208-216. */
+static void leaf_read(const orc_leaf* l, int what, int mip, v3 p, T* x, T* y) {
+  if (l->kind == 0) {
+    const T d = analytic_dist(l, p);
+    *x = d;
+    *y = what ? rnd(l->material) : t_clamp(d * rnd(1.0 / l->analytic_band), -1, 1);
+    return;
+  }
+  volp v = vol_params(l);
+  v3 rel = V(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
+  T dbox = sd_box(rel, V(v.half, v.half, v.half));
+  if (dbox > 0) {
+    *x = dbox;
+    *y = what ? 0 : 1;
+    return;
+  }
+  const int mips = l->mips > 0 ? l->mips : 1;
+  const int m = mip < mips - 1 ? mip : mips - 1;
+  const T band_world = v.band * v.voxel * (T)ldexp(1.0, m);
+  const uint16_t* dp = m == 0 && !l->dist_m[0] ? l->dist16 : l->dist_m[m];
+  const T raw = sample_mip(l, dp, l->res >> m, &v, p);
+  *x = raw * band_world;
+  if (!what) {
+    *y = raw;
+  } else {
+    const uint16_t* mp = m == 0 && !l->mat_m[0] ? l->mat16 : l->mat_m[m];
+    *y = mp ? sample_mip(l, mp, l->res >> m, &v, p) : 0;
+  }
+}
+
+void FN(orc_field_query)(const orc_collider* col, int what, int mip, int n, const T* p3, T* out2) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    v3 p = V(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
+    T bx, by;
+    leaf_read(&col->leaf[0], what, mip, p, &bx, &by);
+    for (int k = 1; k < col->nleaf; k++) {
+      T x, y;
+      leaf_read(&col->leaf[k], what, mip, p, &x, &y);
+      if (!(bx < x)) {
+        bx = x;
+        by = y;
+      }
+    }
+    out2[2 * i] = bx;
+    out2[2 * i + 1] = by;
+  }
+}
+
+/* slabRange (field.ts:90-100) */
+static void slab_range(v3 rel, v3 half, v3 inv, T* lo_out, T* hi_out) {
+  v3 a = V((-half.x - rel.x) * inv.x, (-half.y - rel.y) * inv.y, (-half.z - rel.z) * inv.z);
+  v3 b = V((half.x - rel.x) * inv.x, (half.y - rel.y) * inv.y, (half.z - rel.z) * inv.z);
+  v3 lo = V(t_min(a.x, b.x), t_min(a.y, b.y), t_min(a.z, b.z));
+  v3 hi = V(t_max(a.x, b.x), t_max(a.y, b.y), t_max(a.z, b.z));
+  *lo_out = t_max(t_max(lo.x, lo.y), lo.z);
+  *hi_out = t_min(t_min(hi.x, hi.y), hi.z);
+}
+
+/* volumeField.entry (field.ts:150-177) */
+static T vol_entry(const orc_leaf* l, v3 ro, v3 rd, T tmin, T radius, T aperture) {
+  volp v = vol_params(l);
+  v3 rel = V(ro.x - v.center[0], ro.y - v.center[1], ro.z - v.center[2]);
+  v3 half = V(v.half, v.half, v.half);
+  const T eps = KF(1e-8);
+  v3 r = V(t_fabs(rd.x) < eps ? eps : rd.x, t_fabs(rd.y) < eps ? eps : rd.y, t_fabs(rd.z) < eps ? eps : rd.z);
+  v3 inv = V((T)1 / r.x, (T)1 / r.y, (T)1 / r.z);
+  T blo, bhi, glo, ghi;
+  slab_range(rel, half, inv, &blo, &bhi);
+  const T g = radius + aperture * t_max(blo, tmin);
+  slab_range(rel, V(half.x + g, half.y + g, half.z + g), inv, &glo, &ghi);
+  const T t = t_max(glo, tmin);
+  return ghi >= t ? t : KF(1e9);
+}
+
+void FN(orc_field_entry)(const orc_collider* col, int n, const T* rays9, T* out) {
+  for (int i = 0; i < n; i++) {
+    const T* r = rays9 + 9 * (size_t)i;
+    v3 ro = V(r[0], r[1], r[2]), rd = V(r[3], r[4], r[5]);
+    T t = vol_entry(&col->leaf[0], ro, rd, r[6], r[7], r[8]);
+    for (int k = 1; k < col->nleaf; k++) t = t_min(t, vol_entry(&col->leaf[k], ro, rd, r[6], r[7], r[8]));
+    out[i] = t;
+  }
+}

@@ -61,6 +61,8 @@ class OrcLeaf(C.Structure):
         ("dist16", C.c_void_p), ("res", C.c_int32), ("pad1", C.c_int32), ("origin", C.c_double * 3),
         ("world_size", C.c_double), ("band", C.c_double), ("grad_voxels", C.c_double),
         ("mat16", C.c_void_p), ("material", C.c_double),
+        ("mips", C.c_int32), ("pad2", C.c_int32), ("dist_m", C.c_void_p * 8), ("mat_m", C.c_void_p * 8),
+        ("analytic_band", C.c_double),
     ]
 
 
@@ -137,6 +139,8 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_body_centroids_{s}").argtypes = [I, P, P, P, P, P, P, P, P]
             getattr(L, f"orc_pbd_step_{s}").argtypes = [C.POINTER(OrcPbdParams), COL, I, I, I, P, P, P, P, P, I, I]
             getattr(L, f"orc_sin_cos_{s}").argtypes = [C.c_float if s == "f32" else C.c_double, P, P]
+            getattr(L, f"orc_field_query_{s}").argtypes = [COL, I, I, I, P, P]
+            getattr(L, f"orc_field_entry_{s}").argtypes = [COL, I, P, P]
             getattr(L, f"orc_extract_{s}").argtypes = [C.POINTER(OrcExtractParams), COL, I, P, P, P, P, P, P, P,
                                                        P, P, P]
         L.orc_tile_dilate.argtypes = [I, I, I, P]
@@ -247,7 +251,7 @@ class Collider:
         self.c = OrcCollider()
         self._keep = []
 
-    def add_analytic(self, brush_records, eps: float = 1e-3, material: float = 0.0) -> "Collider":
+    def add_analytic(self, brush_records, eps: float = 1e-3, material: float = 0.0, band: float = 64.0) -> "Collider":
         arr = (OrcBrush * len(brush_records))()
         for i, b in enumerate(brush_records):
             C.memmove(C.byref(arr[i]), C.byref(b), C.sizeof(OrcBrush))
@@ -258,12 +262,15 @@ class Collider:
         leaf.brushes = C.cast(arr, C.c_void_p)
         leaf.eps = eps
         leaf.material = material
+        leaf.analytic_band = band
+        leaf.mips = 1
         self.c.nleaf += 1
         return self
 
     def add_volume(self, dist16: np.ndarray, res: int, origin, world_size: float, band: float,
-                   grad_voxels: float = 1.0, mat16: np.ndarray | None = None) -> "Collider":
-        """`mat16`: the mip-0 material channel (f16 bits); None reads material 0."""
+                   grad_voxels: float = 1.0, mat16: np.ndarray | None = None, levels=None) -> "Collider":
+        """`mat16`: the mip-0 material channel (f16 bits); None reads material 0.
+        `levels`: every mip as (dist16, mat16) pairs, for the reads at any level."""
         d = np.ascontiguousarray(dist16, dtype=np.uint16).reshape(-1)
         assert d.size == res ** 3
         self._keep.append(d)
@@ -281,8 +288,34 @@ class Collider:
             assert m.size == res ** 3
             self._keep.append(m)
             leaf.mat16 = m.ctypes.data
+        leaf.mips = 1
+        if levels is not None:
+            leaf.mips = len(levels)
+            for lv, (dl, ml) in enumerate(levels):
+                dl = np.ascontiguousarray(dl, dtype=np.uint16).reshape(-1)
+                ml = np.ascontiguousarray(ml, dtype=np.uint16).reshape(-1)
+                assert dl.size == (res >> lv) ** 3 and ml.size == dl.size
+                self._keep += [dl, ml]
+                leaf.dist_m[lv] = dl.ctypes.data
+                leaf.mat_m[lv] = ml.ctypes.data
         self.c.nleaf += 1
         return self
+
+    def query(self, points: np.ndarray, mip: int = 0, what: str = "sample", precision: str = "f32"):
+        """TracedField.sample / field at `mip`: (n, 2)."""
+        dt = np.float32 if precision == "f32" else np.float64
+        p = np.ascontiguousarray(points, dtype=dt).reshape(-1, 3)
+        out = np.empty((p.shape[0], 2), dtype=dt)
+        getattr(lib(), f"orc_field_query_{precision}")(C.byref(self.c), 1 if what == "field" else 0, mip,
+                                                       p.shape[0], p.ctypes.data, out.ctypes.data)
+        return out
+
+    def entry(self, rays9: np.ndarray, precision: str = "f32") -> np.ndarray:
+        dt = np.float32 if precision == "f32" else np.float64
+        r = np.ascontiguousarray(rays9, dtype=dt).reshape(-1, 9)
+        out = np.empty(r.shape[0], dtype=dt)
+        getattr(lib(), f"orc_field_entry_{precision}")(C.byref(self.c), r.shape[0], r.ctypes.data, out.ctypes.data)
+        return out
 
     def eval(self, points: np.ndarray, precision: str = "f32"):
         dt = np.float32 if precision == "f32" else np.float64
