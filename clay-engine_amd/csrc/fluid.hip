@@ -262,6 +262,28 @@ struct StateOut {
   float* p[11];  // px py pz vx vy vz rho pres nvx nvy nvz
 };
 
+// The reference's contact counter (fluid.ts:469-477), advanced once per wave and
+// counter: the wave's contacting lanes are grouped by the counter they add to (a
+// wave can straddle scenes), and one lane of each group adds the group's size.
+// One atomic per particle on a scene's single counter serialises in L2 once a
+// scene holds 1e5 contacts.  Called by every active lane of the wave.
+__device__ __forceinline__ void add_contacts(uint32_t* counter, bool hit) {
+  // readlane returns a signed int: both halves are zero-extended before they are
+  // joined, and the leader's bit is cleared whatever the compare gives, so every
+  // iteration retires at least one lane.
+  const uint64_t addr = (uint64_t)(uintptr_t)counter;
+  uint64_t pending = __ballot(hit);
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)addr, leader);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(addr >> 32), leader);
+    const uint64_t want = (uint64_t)lo | ((uint64_t)hi << 32);
+    const uint64_t same = __ballot(hit && addr == want) | (1ull << leader);
+    if ((int)__lane_id() == leader) atomicAdd(counter, (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+}
+
 // The end of the force pass (fluid.ts:430-437) and the integrate pass
 // (fluid.ts:449-491) for one particle, writing its id-ordered state.
 __device__ __forceinline__ void finish_particle(const FluidK& k, const SceneK& sk, const ColliderDev& col,
@@ -284,6 +306,7 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const SceneK& s
   f3 p = mk(pi.x + v.x * k.dt, pi.y + v.y * k.dt, pi.z + v.z * k.dt);
   if (col.nleaf > 0) {
     float dw;
+    bool hit = false;
     const int leaf = collider_pick(col, scene, p, &dw);
     if (dw < k.radius) {
       const f3 nr = collider_normal(col, scene, leaf, p);
@@ -292,13 +315,14 @@ __device__ __forceinline__ void finish_particle(const FluidK& k, const SceneK& s
       const float vn = fmin_s(dot3(v, nr), 0.0f);
       v = mk(v.x - nr.x * vn, v.y - nr.y * vn, v.z - nr.z * vn);
       if (dw < k.cdepth) {
-        // The count is the reference's atomic counter; which 32 contacts the list
-        // keeps is decided afterwards, by particle id (k_contacts_collect).
-        atomicAdd(contacts, 1u);
+        // Which 32 contacts the list keeps is decided afterwards, by particle id
+        // (k_contacts_collect).
+        hit = true;
         sk.cdata[j] = make_float4(p.x, p.y, p.z, len3(v));
         atomicOr(&sk.cmask[j >> 5], 1u << (j & 31u));
       }
     }
+    add_contacts(contacts, hit);
   }
   const float clx = fmin_s(fmax_s(p.x, k.lo[0]), k.hi[0]);
   const float cly = fmin_s(fmax_s(p.y, k.lo[1]), k.hi[1]);
