@@ -7,7 +7,7 @@
 //   k_scan_rows exclusive scan of each digit's row of tiles  -> row totals
 //   k_scatter   digit base (scan of the row totals, in LDS) + tile offset +
 //               stable rank inside the tile from wave64 ballots
-// A tile is 1024 keys: 4 rounds of 256 threads, so rank order inside a tile is
+// A tile is 2048 keys: 8 rounds of 256 threads, so rank order inside a tile is
 // (round, lane) = input order and the sort is stable.
 #include "clay_internal.h"
 
@@ -16,7 +16,7 @@ namespace clay {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRounds = 4;
+constexpr int kRounds = 8;
 constexpr int kTileKeys = kThreads * kRounds;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxDigitBits = 11;
