@@ -61,6 +61,11 @@ constexpr int kHalo = 5;                    // owner schedule: voxels staged aro
 constexpr int kExt = kBrick + 2 * kHalo;    // staged edge (26)
 constexpr int kExtVox = kExt * kExt * kExt;
 constexpr int kPart = 2048;                 // owner schedule: particles partitioned per LDS pass
+// Owner schedule workgroup: the staged region takes ~78 KB of LDS, so two workgroups
+// fit a CU; 512 threads each keep 4 waves per SIMD in flight (1024-thread
+// workgroups leave most lanes idle on the small bricks of following volumes).
+constexpr int kOwnerBits = 9;
+constexpr int kOwnerThreads = 1 << kOwnerBits;
 
 struct SplatMipK {
   int32_t r;          // res >> m
@@ -542,7 +547,7 @@ __global__ void __launch_bounds__(kThreads)
 // wave's lanes busy.  The staged region is merged into the code grid with
 // atomicMin, because neighbouring bricks' shells overlap it.
 template <bool kNormals>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kOwnerThreads)
     k_owner_splat(MipsK mk_, CloudK c, const VolDev* __restrict__ vol, const uint32_t* __restrict__ start,
                   const uint32_t* __restrict__ count, const uint32_t* __restrict__ item_start,
                   const ScanOut* __restrict__ scan, uint32_t nbricks, const uint32_t* __restrict__ list,
@@ -553,7 +558,7 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ uint32_t sh_brick;
   const int t = threadIdx.x;
   const uint32_t init = kNormals ? kEncSat : kSInit;
-  const uint32_t rev = __brev((uint32_t)t) >> 24;
+  const uint32_t rev = __brev((uint32_t)t) >> (32 - kOwnerBits);
   const uint32_t items = scan->items;
   for (uint32_t item = mk_.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x; item < items; item += gridDim.x) {
     if (t == 0) {
@@ -578,7 +583,7 @@ __global__ void __launch_bounds__(kThreads)
     const int32_t ez1 = q.oz + kBrick + kHalo < m.r ? q.oz + kBrick + kHalo : m.r;
     const int32_t nx = ex1 - ex0, ny = ey1 - ey0, nz = ez1 - ez0;
     const int32_t nvox = nx * ny * nz;
-    for (int v = t; v < nvox; v += kThreads) lds[v] = init;
+    for (int v = t; v < nvox; v += kOwnerThreads) lds[v] = init;
     __syncthreads();
     const uint32_t n_b = count[b];
     const uint32_t chunk = item - item_start[b];
@@ -628,7 +633,7 @@ __global__ void __launch_bounds__(kThreads)
         const uint32_t mm = n_e - sub < (uint32_t)kPart ? n_e - sub : (uint32_t)kPart;
         if (t < 2) part_n[t] = 0;
         __syncthreads();
-        for (uint32_t e = t; e < mm; e += kThreads) {
+        for (uint32_t e = t; e < mm; e += kOwnerThreads) {
           const uint32_t i = list[base + sub + e];
           if (c.cover[i] < m.cull2)
             part[mm - 1u - atomicAdd(&part_n[1], 1u)] = i;
@@ -637,24 +642,24 @@ __global__ void __launch_bounds__(kThreads)
         }
         __syncthreads();
         const uint32_t ns = part_n[0], ni = mm - ns;
-        for (uint32_t e0 = 0; e0 < ns; e0 += kThreads) {
+        for (uint32_t e0 = 0; e0 < ns; e0 += kOwnerThreads) {
           const uint32_t e = e0 + rev;
           if (e < ns) walk(part[e]);
         }
-        for (uint32_t e0 = 0; e0 < ni; e0 += kThreads) {
+        for (uint32_t e0 = 0; e0 < ni; e0 += kOwnerThreads) {
           const uint32_t e = e0 + rev;
           if (e < ni) walk(part[ns + e]);
         }
         __syncthreads();
       }
     } else {
-      for (uint32_t pass = 0; pass * kThreads < n_e; pass++) {
-        const uint32_t e = pass * kThreads + rev;
+      for (uint32_t pass = 0; pass * kOwnerThreads < n_e; pass++) {
+        const uint32_t e = pass * kOwnerThreads + rev;
         if (e < n_e) walk(list[base + e]);
       }
     }
     __syncthreads();
-    for (int v = t; v < nvox; v += kThreads) {
+    for (int v = t; v < nvox; v += kOwnerThreads) {
       const uint32_t val = lds[v];
       if (val == init) continue;
       const int32_t x = ex0 + v % nx, y = ey0 + (v / nx) % ny, z = ez0 + v / (nx * ny);
@@ -957,7 +962,8 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_brick");
     if (mk_.owner)
-      hipLaunchKernelGGL(k_owner_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
+      hipLaunchKernelGGL(k_owner_splat<kNormals>,
+                         dim3(std::max(1, sp->persistent_blocks * kThreads / kOwnerThreads)), dim3(kOwnerThreads), 0, s, mk_, c,
                          sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
                          (uint32_t)nbricks, sp->brick_list, sp->grid);
     else
