@@ -552,7 +552,9 @@ __global__ void __launch_bounds__(kOwnerThreads)
                   const uint32_t* __restrict__ count, const uint32_t* __restrict__ item_start,
                   const ScanOut* __restrict__ scan, uint32_t nbricks, const uint32_t* __restrict__ list,
                   uint32_t* __restrict__ grid) {
-  __shared__ uint32_t lds[kExtVox];
+  // The staged region, sized at launch for the largest brick-plus-shell of these
+  // levels (owner_stage_voxels): at most kExtVox.
+  extern __shared__ uint32_t lds[];
   __shared__ uint32_t part[kPart];    // interior-radius schedule: the sub-chunk, surface particles first
   __shared__ uint32_t part_n[2];      // surface and interior counts
   __shared__ uint32_t sh_brick;
@@ -927,6 +929,23 @@ static CloudK cloud_k(const Splat* sp, const uint32_t* order, const float* cover
   return c;
 }
 
+// Voxels of the largest region the owner schedule stages for these levels: per
+// axis a brick and its kHalo shell, clipped to the level.  A 32^3 level has two
+// bricks per axis, each clipped on one side (21 voxels), so its workgroups need
+// half the LDS of a brick inside a large level (26).
+static uint32_t owner_stage_voxels(const MipsK& mk_) {
+  int32_t ext = 1;
+  for (int mi = 0; mi < mk_.mips; mi++) {
+    const int32_t r = mk_.m[mi].r;
+    for (int32_t o = 0; o < r; o += kBrick) {
+      const int32_t lo = o - kHalo > 0 ? o - kHalo : 0;
+      const int32_t hi = o + kBrick + kHalo < r ? o + kBrick + kHalo : r;
+      ext = std::max(ext, hi - lo);
+    }
+  }
+  return (uint32_t)(ext * ext * ext);
+}
+
 template <bool kNormals>
 static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
   Ctx* ctx = sp->ctx;
@@ -963,7 +982,8 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
     LaunchTimer t(ctx, s, "splat_brick");
     if (mk_.owner)
       hipLaunchKernelGGL(k_owner_splat<kNormals>,
-                         dim3(std::max(1, sp->persistent_blocks * kThreads / kOwnerThreads)), dim3(kOwnerThreads), 0, s, mk_, c,
+                         dim3(std::max(1, sp->persistent_blocks * kThreads / kOwnerThreads)), dim3(kOwnerThreads),
+                         sizeof(uint32_t) * owner_stage_voxels(mk_), s, mk_, c,
                          sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
                          (uint32_t)nbricks, sp->brick_list, sp->grid);
     else
