@@ -8,7 +8,7 @@
 //   voxel to f16.
 //
 // TILED:
-//   1. k_bin<count>, k_brick_scan, k_bin<fill>: particles are binned to 16^3-voxel
+//   1. k_bin<count>, k_brick_sum/scan/cursor, k_bin<fill>: particles are binned to 16^3-voxel
 //      bricks of every mip (a particle lands in each brick its voxel box overlaps;
 //      the counters are hit once per wave and distinct brick).  The scan also
 //      cuts every brick's list into chunks (2048 particles unless tuned): a chunk
@@ -332,60 +332,98 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Binning pass 2: per brick, the total of its slice counters (-> count) and, over all
-// bricks, exclusive scans of the totals (-> list starts; slice cursors) and of the
-// per-brick chunk counts (-> first work item of each brick).  One block.
+// Binning pass 2, in three launches: per brick, the total of its slice counters
+// (-> count); over all bricks, exclusive scans of the totals (-> list starts) and of
+// the per-brick chunk counts (-> first work item of each brick); per slice, its fill
+// cursor.  A brick's kSlices counters are read and written by kSlices lanes at once.
+static_assert(kSlices == 32, "a brick's slices are one half-wave");
+
+__device__ __forceinline__ uint32_t half_wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Totals: two bricks per wave.
+__global__ void __launch_bounds__(kThreads)
+    k_brick_sum(const uint32_t* __restrict__ sub, uint32_t nbricks, uint32_t* __restrict__ count) {
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t b = g / kSlices;
+  const uint32_t v = b < nbricks ? sub[g] : 0u;
+  const uint32_t tot = half_wave_sum(v);
+  if (b < nbricks && (g & (kSlices - 1)) == 0) count[b] = tot;
+}
+
+// Inclusive scan of v over the wave's 64 lanes.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// The scans over bricks: one block, 1024 bricks per round, wave scans joined by a
+// scan of the 16 wave totals.
 __global__ void __launch_bounds__(1024)
-    k_brick_scan(const uint32_t* __restrict__ sub, uint32_t nbricks, uint32_t chunk, uint32_t* __restrict__ count,
-                 uint32_t* __restrict__ start, uint32_t* __restrict__ cursor,
-                 uint32_t* __restrict__ item_start, ScanOut* __restrict__ out) {
-  __shared__ uint32_t sa[1024], sb[1024];
+    k_brick_scan(uint32_t nbricks, uint32_t chunk, const uint32_t* __restrict__ count,
+                 uint32_t* __restrict__ start, uint32_t* __restrict__ item_start, ScanOut* __restrict__ out) {
+  __shared__ uint32_t wa[16], wb[16];
   __shared__ uint32_t carry_a, carry_b;
-  if (threadIdx.x == 0) {
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  if (t == 0) {
     carry_a = 0;
     carry_b = 0;
   }
   __syncthreads();
   for (uint32_t c0 = 0; c0 < nbricks; c0 += 1024) {
-    const uint32_t i = c0 + threadIdx.x;
-    uint32_t va = 0;
-    if (i < nbricks)
-      for (int j = 0; j < kSlices; j++) va += sub[i * kSlices + j];
+    const uint32_t i = c0 + (uint32_t)t;
+    const uint32_t va = i < nbricks ? count[i] : 0u;
     const uint32_t vb = (va + chunk - 1) / chunk;
-    sa[threadIdx.x] = va;
-    sb[threadIdx.x] = vb;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const uint32_t xa = threadIdx.x >= (unsigned)off ? sa[threadIdx.x - off] : 0u;
-      const uint32_t xb = threadIdx.x >= (unsigned)off ? sb[threadIdx.x - off] : 0u;
-      __syncthreads();
-      sa[threadIdx.x] += xa;
-      sb[threadIdx.x] += xb;
-      __syncthreads();
+    const uint32_t ia = wave_incl_scan(va, lane), ib = wave_incl_scan(vb, lane);
+    if (lane == 63) {
+      wa[wave] = ia;
+      wb[wave] = ib;
     }
-    const uint32_t ia = sa[threadIdx.x], ib = sb[threadIdx.x];
+    __syncthreads();
+    uint32_t pa = carry_a, pb = carry_b;
+    for (int w = 0; w < wave; w++) {
+      pa += wa[w];
+      pb += wb[w];
+    }
     if (i < nbricks) {
-      const uint32_t s0 = carry_a + ia - va;
-      count[i] = va;
-      start[i] = s0;
-      uint32_t run = s0;
-      for (int j = 0; j < kSlices; j++) {
-        cursor[i * kSlices + j] = run;
-        run += sub[i * kSlices + j];
-      }
-      item_start[i] = carry_b + ib - vb;
+      start[i] = pa + ia - va;
+      item_start[i] = pb + ib - vb;
     }
     __syncthreads();
-    if (threadIdx.x == 1023) {
-      carry_a += ia;
-      carry_b += ib;
+    if (t == 1023) {
+      carry_a = pa + ia;
+      carry_b = pb + ib;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     out->entries = carry_a;
     out->items = carry_b;
   }
+}
+
+// Fill cursors: each slice starts after its brick's earlier slices.
+__global__ void __launch_bounds__(kThreads)
+    k_brick_cursor(const uint32_t* __restrict__ sub, uint32_t nbricks, const uint32_t* __restrict__ start,
+                   uint32_t* __restrict__ cursor) {
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t b = g / kSlices;
+  const int l = (int)(g & (kSlices - 1));
+  const uint32_t v = b < nbricks ? sub[g] : 0u;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < kSlices; d <<= 1) {
+    const uint32_t u = __shfl_up(x, d, kSlices);
+    if (l >= d) x += u;
+  }
+  if (b < nbricks) cursor[g] = start[b] + x - v;
 }
 
 struct BrickPos {
@@ -962,8 +1000,13 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   }
   {
     LaunchTimer t(ctx, s, "splat_brick_scan");
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, sp->brick_sub, (uint32_t)nbricks, sp->chunk,
-                       sp->brick_count, sp->brick_start, sp->brick_cursor, sp->item_start, scan);
+    const unsigned sgrid = (unsigned)(((size_t)nbricks * kSlices + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_brick_sum, dim3(sgrid), dim3(kThreads), 0, s, sp->brick_sub, (uint32_t)nbricks,
+                       sp->brick_count);
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, (uint32_t)nbricks, sp->chunk, sp->brick_count,
+                       sp->brick_start, sp->item_start, scan);
+    hipLaunchKernelGGL(k_brick_cursor, dim3(sgrid), dim3(kThreads), 0, s, sp->brick_sub, (uint32_t)nbricks,
+                       sp->brick_start, sp->brick_cursor);
     CLAY_HIP_CHECK(hipGetLastError());
   }
   if (c.count) {
