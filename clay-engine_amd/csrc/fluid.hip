@@ -970,42 +970,59 @@ __global__ void k_bump(uint32_t* epoch) {
 __global__ void __launch_bounds__(kBlock)
     k_contacts_collect(uint32_t* __restrict__ contacts, uint32_t stride, const uint32_t* __restrict__ cmask,
                        const float4* __restrict__ cdata, const uint32_t* __restrict__ base, uint32_t n) {
-  __shared__ uint32_t sh[kBlock];
+  constexpr int kPer = 4;  // flag words per thread and round
+  __shared__ uint32_t wsum[kBlock / 64];
   __shared__ uint32_t found;
   const uint32_t sc = blockIdx.x;
   const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
   float4* pts = reinterpret_cast<float4*>(contacts + (size_t)sc * stride + 4);
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   if (t == 0) found = 0;
   __syncthreads();
   if (lo >= hi) return;
   const uint32_t w0 = lo >> 5, w1 = (hi + 31u) >> 5;
-  for (uint32_t wb = w0; wb < w1; wb += kBlock) {
-    const uint32_t w = wb + (uint32_t)t;
-    uint32_t bits = 0;
-    if (w < w1) {
-      bits = cmask[w];
-      if (w == w0) bits &= ~0u << (lo & 31u);
-      const uint32_t last = hi - (w << 5);  // ids of this word below hi
-      if (last < 32u) bits &= (1u << last) - 1u;
+  for (uint32_t wb = w0; wb < w1; wb += kPer * kBlock) {
+    uint32_t bits[kPer];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const uint32_t w = wb + (uint32_t)(kPer * t + q);
+      uint32_t m = 0;
+      if (w < w1) {
+        m = cmask[w];
+        if (w == w0) m &= ~0u << (lo & 31u);
+        const uint32_t last = hi - (w << 5);  // ids of this word below hi
+        if (last < 32u) m &= (1u << last) - 1u;
+      }
+      bits[q] = m;
+      cnt += (uint32_t)__popc(m);
     }
-    const uint32_t cnt = (uint32_t)__popc(bits);
-    sh[t] = cnt;
+    // Exclusive rank of this thread's first flag: a wave scan, then the earlier waves.
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(incl, d);
+      if (lane >= d) incl += u;
+    }
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-      const uint32_t add = t >= off ? sh[t - off] : 0u;
-      __syncthreads();
-      sh[t] += add;
-      __syncthreads();
+    uint32_t r = found + incl - cnt, total = found;
+    for (int w = 0; w < kBlock / 64; w++) {
+      if (w < wave) r += wsum[w];
+      total += wsum[w];
     }
-    uint32_t r = found + sh[t] - cnt;
-    while (bits && r < (uint32_t)CLAY_CONTACTS) {
-      const uint32_t b = (uint32_t)__ffs(bits) - 1u;
-      pts[r++] = cdata[(w << 5) + b];
-      bits &= bits - 1u;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      uint32_t m = bits[q];
+      const uint32_t w = wb + (uint32_t)(kPer * t + q);
+      while (m && r < (uint32_t)CLAY_CONTACTS) {
+        const uint32_t bit = (uint32_t)__ffs(m) - 1u;
+        pts[r++] = cdata[(w << 5) + bit];
+        m &= m - 1u;
+      }
     }
     __syncthreads();
-    if (t == kBlock - 1) found += sh[t];
+    if (t == 0) found = total;
     __syncthreads();
     if (found >= (uint32_t)CLAY_CONTACTS) break;
   }
