@@ -7,8 +7,10 @@
 //   k_scan_rows exclusive scan of each digit's row of tiles  -> row totals
 //   k_scatter   digit base (scan of the row totals, in LDS) + tile offset +
 //               stable rank inside the tile from wave64 ballots
-// A tile is 2048 keys: 8 rounds of 256 threads, so rank order inside a tile is
-// (round, lane) = input order and the sort is stable.
+// A tile is R rounds of 256 threads, so rank order inside a tile is (round, lane) =
+// input order and the sort is stable.  R = 8 (2048 keys) up to 8M keys and 32 above:
+// a pass reads and writes its per-tile histogram column at scattered addresses, and
+// 33.5M keys in 2048-key tiles make that a 64 MB matrix at 10-bit digits.
 #include "clay_internal.h"
 
 namespace clay {
@@ -16,16 +18,18 @@ namespace clay {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRounds = 8;
-constexpr int kTileKeys = kThreads * kRounds;
+constexpr int kMinRounds = 8;
+constexpr int kMaxRounds = 32;
+constexpr uint32_t kBigSort = 1u << 23;  // keys from which tiles take kMaxRounds rounds
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxDigitBits = 11;
 
-template <int W>
+template <int W, int kRounds>
 __global__ void __launch_bounds__(kThreads)
     k_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift, uint32_t* __restrict__ hist,
            uint32_t ntiles) {
   constexpr uint32_t kBins = 1u << W;
+  constexpr int kTileKeys = kThreads * kRounds;
   __shared__ uint32_t h[kBins];
   for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) h[b] = 0;
   __syncthreads();
@@ -68,7 +72,7 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-template <int W>
+template <int W, int kRounds>
 __global__ void __launch_bounds__(kThreads)
     k_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
               int shift, const uint32_t* __restrict__ hist, uint32_t ntiles,
@@ -76,6 +80,7 @@ __global__ void __launch_bounds__(kThreads)
               uint32_t* __restrict__ vals_out) {
   constexpr uint32_t kBins = 1u << W;
   constexpr uint32_t kPer = kBins / kThreads;  // digits per thread
+  constexpr int kTileKeys = kThreads * kRounds;
   __shared__ uint32_t digit_base[kBins];
   __shared__ uint32_t running[kBins];
   __shared__ uint32_t wave_cnt[kWaves][kBins];
@@ -155,12 +160,12 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-template <int W>
+template <int W, int R>
 void sort_pass(Ctx* c, hipStream_t s, uint32_t n, uint32_t ntiles, int shift, const uint32_t* keys,
                const uint32_t* vals, uint32_t* hist, uint32_t* totals, uint32_t* keys_out, uint32_t* vals_out) {
   {
     LaunchTimer t(c, s, "sort_hist");
-    hipLaunchKernelGGL(k_hist<W>, dim3(ntiles), dim3(kThreads), 0, s, keys, n, shift, hist, ntiles);
+    hipLaunchKernelGGL((k_hist<W, R>), dim3(ntiles), dim3(kThreads), 0, s, keys, n, shift, hist, ntiles);
   }
   {
     LaunchTimer t(c, s, "sort_scan");
@@ -168,7 +173,7 @@ void sort_pass(Ctx* c, hipStream_t s, uint32_t n, uint32_t ntiles, int shift, co
   }
   {
     LaunchTimer t(c, s, "sort_scatter");
-    hipLaunchKernelGGL(k_scatter<W>, dim3(ntiles), dim3(kThreads), 0, s, keys, vals, n, shift, hist, ntiles,
+    hipLaunchKernelGGL((k_scatter<W, R>), dim3(ntiles), dim3(kThreads), 0, s, keys, vals, n, shift, hist, ntiles,
                        totals, keys_out, vals_out);
   }
 }
@@ -176,36 +181,35 @@ void sort_pass(Ctx* c, hipStream_t s, uint32_t n, uint32_t ntiles, int shift, co
 }  // namespace
 
 size_t radix_hist_size(uint32_t n) {
-  const size_t ntiles = (n + kTileKeys - 1) / kTileKeys;
+  // Sized for the smallest tiles, the most of them.
+  const size_t tile = (size_t)kThreads * kMinRounds;
+  const size_t ntiles = (n + tile - 1) / tile;
   return ((size_t)1 << kMaxDigitBits) * ((ntiles > 0 ? ntiles : 1) + 1);
 }
 
-// Fewest passes of at most 11-bit digits, the digit width spread evenly: 24-bit keys
-// take three 8-bit passes, 27-bit keys three 9-bit passes, 32-bit keys three 11-bit.
-int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
-                     uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
-  if (n == 0) return CLAY_OK;
-  const uint32_t ntiles = (n + kTileKeys - 1) / kTileKeys;
-  const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
-  const int w = std::max(8, (key_bits + passes - 1) / passes);
+namespace {
+template <int R>
+void sort_passes(Ctx* c, hipStream_t s, uint32_t n, int passes, int w, uint32_t*& keys, uint32_t*& vals,
+                 uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
+  const uint32_t tile = (uint32_t)kThreads * R;
+  const uint32_t ntiles = (n + tile - 1) / tile;
   uint32_t* totals = hist + ((size_t)1 << w) * ntiles;
   for (int p = 0; p < passes; p++) {
     const int shift = w * p;
     switch (w) {
       case 8:
-        sort_pass<8>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<8, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
         break;
       case 9:
-        sort_pass<9>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<9, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
         break;
       case 10:
-        sort_pass<10>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<10, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
         break;
       default:
-        sort_pass<11>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<11, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
         break;
     }
-    CLAY_HIP_CHECK(hipGetLastError());
     uint32_t* tk = keys;
     keys = keys_alt;
     keys_alt = tk;
@@ -213,6 +217,21 @@ int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*&
     vals = vals_alt;
     vals_alt = tv;
   }
+}
+}  // namespace
+
+// Fewest passes of at most 11-bit digits, the digit width spread evenly: 24-bit keys
+// take three 8-bit passes, 27-bit keys three 9-bit passes, 32-bit keys three 11-bit.
+int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
+                     uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
+  if (n == 0) return CLAY_OK;
+  const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const int w = std::max(8, (key_bits + passes - 1) / passes);
+  if (n >= kBigSort)
+    sort_passes<kMaxRounds>(c, s, n, passes, w, keys, vals, keys_alt, vals_alt, hist);
+  else
+    sort_passes<kMinRounds>(c, s, n, passes, w, keys, vals, keys_alt, vals_alt, hist);
+  CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
 
