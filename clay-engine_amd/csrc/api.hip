@@ -844,6 +844,9 @@ void fluid_free(Fluid* f) {
   dfree(f->scene_of);
   dfree(f->scene_base_dev);
   if (f->graph) (void)hipGraphExecDestroy(f->graph);
+  if (f->ev_fork) (void)hipEventDestroy(f->ev_fork);
+  if (f->ev_join) (void)hipEventDestroy(f->ev_join);
+  if (f->side) (void)hipStreamDestroy(f->side);
   delete f;
 }
 
@@ -852,7 +855,8 @@ size_t contact_words(const Fluid* f) { return (size_t)f->scenes * (4 + 4 * CLAY_
 // First half of a substep (fluid.ts:293-368): cell stamp, keys, sort, cell
 // table and reordered state, then densities and pressures.  With `cover`, the
 // density pass also writes each particle's interior radius for the surface bake.
-int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
+// The cell sort of a substep: stamp, keys, sort, reorder into the sorted planes.
+int fluid_sort(Fluid* f, hipStream_t s) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
@@ -877,17 +881,29 @@ int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
   const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
-  if ((rc = launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6, f->k,
-                           lds ? f->spos : nullptr)))
-    return rc;
+  return launch_reorder(c, s, n, f->keys, f->vals, sk, invalid, f->cells, src, dst, 6, f->k,
+                        lds ? f->spos : nullptr);
+}
+
+// The density pass over the sorted planes (with `cover`, also the interior radii).
+int fluid_density(Fluid* f, hipStream_t s, float* cover) {
+  const uint32_t n = f->live;
+  if (n == 0) return CLAY_OK;
+  const SceneK sk = f->scene_k();
+  const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
   HalfState half;
   half.spos = f->spos;
   half.svx = f->svx;
   half.svy = f->svy;
   half.svz = f->svz;
   half.frec = f->frec;
-  return launch_density(c, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells, f->srho,
-                        f->spres, f->vals, cover, half);
+  return launch_density(f->ctx, s, f->k, sk, f->tune, lds, n, f->spx, f->spy, f->spz, f->keys, f->cells,
+                        f->srho, f->spres, f->vals, cover, half);
+}
+
+int fluid_sort_density(Fluid* f, hipStream_t s, float* cover) {
+  const int rc = fluid_sort(f, s);
+  return rc ? rc : fluid_density(f, s, cover);
 }
 
 // Second half of a substep (fluid.ts:370-491): clearGrid's contacts reset
@@ -926,7 +942,6 @@ int fluid_frame_impl(Fluid* f, hipStream_t s) {
     if (rc) return rc;
   }
   if (emit_count > 0) f->presorted = false;
-  bool sorted = false;
   if (f->live > 0 && f->consts.substeps > 0) {
     // Substep i sorts and computes densities from the positions substep i - 1 left;
     // the first one's were computed at the end of the previous frame unless the
@@ -939,14 +954,23 @@ int fluid_frame_impl(Fluid* f, hipStream_t s) {
       if (i > 0 && (rc = fluid_sort_density(f, s, nullptr))) return rc;
       if ((rc = fluid_forces(f, s))) return rc;
     }
-    if ((rc = fluid_sort_density(f, s, f->cover))) return rc;
+    // The frame-end sort, then its density pass beside the bake's binning, which
+    // reads positions and the sort's permutation but not the interior radii.  The
+    // permutation is a spatially coherent visiting order for the splat; the codes
+    // do not depend on it.
+    if ((rc = fluid_sort(f, s))) return rc;
+    f->surface->cloud.live_count = f->live;
+    CLAY_HIP_CHECK(hipEventRecord(f->ev_fork, s));
+    CLAY_HIP_CHECK(hipStreamWaitEvent(f->side, f->ev_fork, 0));
+    if ((rc = splat_bake_prepare(f->surface, f->side, f->vals))) return rc;
+    if ((rc = fluid_density(f, s, f->cover))) return rc;
+    CLAY_HIP_CHECK(hipEventRecord(f->ev_join, f->side));
+    CLAY_HIP_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));
     f->presorted = true;
-    sorted = true;
+    return splat_bake_finish(f->surface, s, f->vals, f->cover);
   }
   f->surface->cloud.live_count = f->live;
-  // The permutation of that sort is a spatially coherent visiting order for the
-  // splat; the codes do not depend on it.
-  return splat_bake_ordered(f->surface, s, sorted ? f->vals : nullptr, sorted ? f->cover : nullptr);
+  return splat_bake_ordered(f->surface, s, nullptr, nullptr);
 }
 
 // Shared by FluidSim (one scene, `counts` null) and a batch of `scenes` scenes
@@ -1147,6 +1171,12 @@ int fluid_create_impl(Ctx* c, Field* world, const clay_fluid_desc* d, uint32_t s
   if (hipStreamSynchronize(s) != hipSuccess) {
     fluid_free(f);
     return fail(CLAY_ERR_HIP, "FluidSim: initialisation failed");
+  }
+  if (hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&f->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fluid_free(f);
+    return fail(CLAY_ERR_HIP, "FluidSim: stream creation failed");
   }
   if (batch) f->live = P;
   fluid_build_k(f);

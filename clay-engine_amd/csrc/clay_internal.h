@@ -222,6 +222,10 @@ struct Fluid {
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
+  // The bake's binning runs beside the frame-end density pass: forked after the
+  // sort onto `side`, joined before the walk (default priority).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   SceneK scene_k() const {
     SceneK sk{};
     sk.xcd = tune.xcd ? 1u : 0u;
@@ -305,6 +309,10 @@ int splat_bake(Splat* sp, hipStream_t s);
 // a squared radius beyond which the particle holds no voxel's minimum (a fluid
 // surface's interior radius), or null.
 int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
+// splat_bake_ordered in two parts: the binning (positions and order only), then the
+// walk and the resolve (which read `cover`).
+int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order);
+int splat_bake_finish(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
 int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
                          std::vector<size_t>* brick_off, size_t* brick_total,
                          std::vector<size_t>* tile_off, size_t* tile_total);

@@ -984,8 +984,10 @@ static uint32_t owner_stage_voxels(const MipsK& mk_) {
   return (uint32_t)(ext * ext * ext);
 }
 
+// Binning: everything before the walk, which reads positions and the visiting
+// order only (not the interior radii).
 template <bool kNormals>
-static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
+static int tiled_prepare(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
   Ctx* ctx = sp->ctx;
   const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
   const unsigned nbricks = (unsigned)(sp->brick_total * sp->scenes);
@@ -1021,6 +1023,15 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
                        sp->grid);
     CLAY_HIP_CHECK(hipGetLastError());
   }
+  return CLAY_OK;
+}
+
+// The walk and the resolve, after tiled_prepare.
+template <bool kNormals>
+static int tiled_finish(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& c) {
+  Ctx* ctx = sp->ctx;
+  const unsigned nbricks = (unsigned)(sp->brick_total * sp->scenes);
+  ScanOut* scan = reinterpret_cast<ScanOut*>(sp->brick_total_dev);
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_brick");
     if (mk_.owner)
@@ -1044,12 +1055,11 @@ static int tiled_bake(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK& 
   return CLAY_OK;
 }
 
-int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover) {
+int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order) {
   Ctx* ctx = sp->ctx;
   MipsK mk_{};
   fill_mips(sp, &mk_);
-  const CloudK c = cloud_k(sp, order, cover);
-  const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
+  const CloudK c = cloud_k(sp, order, nullptr);
   if (sp->follow) {
     LaunchTimer t(ctx, s, "splat_follow");
     const double half = sp->vol->info.world_size / 2;
@@ -1068,6 +1078,16 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const fl
     }
     CLAY_HIP_CHECK(hipGetLastError());
   }
+  if (sp->desc.mode == CLAY_SPLAT_PLAIN) return CLAY_OK;
+  return c.nrm[0] ? tiled_prepare<true>(sp, s, mk_, c) : tiled_prepare<false>(sp, s, mk_, c);
+}
+
+int splat_bake_finish(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover) {
+  Ctx* ctx = sp->ctx;
+  MipsK mk_{};
+  fill_mips(sp, &mk_);
+  const CloudK c = cloud_k(sp, order, cover);
+  const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
   if (sp->desc.mode == CLAY_SPLAT_PLAIN) {
     int rc = launch_fill_u32(ctx, s, sp->grid, sp->grid_total, kEncSat);
     if (rc) return rc;
@@ -1089,10 +1109,15 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const fl
     sp->has_baked = true;
     return CLAY_OK;
   }
-  const int rc = c.nrm[0] ? tiled_bake<true>(sp, s, mk_, c) : tiled_bake<false>(sp, s, mk_, c);
+  const int rc = c.nrm[0] ? tiled_finish<true>(sp, s, mk_, c) : tiled_finish<false>(sp, s, mk_, c);
   if (rc) return rc;
   sp->has_baked = true;
   return CLAY_OK;
+}
+
+int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover) {
+  const int rc = splat_bake_prepare(sp, s, order);
+  return rc ? rc : splat_bake_finish(sp, s, order, cover);
 }
 
 int splat_bake(Splat* sp, hipStream_t s) { return splat_bake_ordered(sp, s, nullptr, nullptr); }
