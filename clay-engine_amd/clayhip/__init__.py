@@ -9,7 +9,7 @@ from ._lib import ClayError, LIB_PATH, lib
 from .batch import FluidBatch
 from .brush import compile_brushes, make_brush
 from .edit import SdfEditor, body_centroids, body_centroids_device
-from .field import TracedField, analyticField, unionField, volumeField
+from .field import TracedField, analyticField, lerpField, offsetField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
 from .pbd import ClaySolver, SurfaceExtractor
 from .splat import ParticleCloud, SplatField
@@ -37,5 +37,7 @@ __all__ = [
     "lib",
     "make_brush",
     "unionField",
+    "offsetField",
+    "lerpField",
     "volumeField",
 ]

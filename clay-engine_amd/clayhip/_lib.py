@@ -309,6 +309,8 @@ SIGNATURES = {
     "clay_field_analytic": ([P, C.POINTER(Brush), I32, C.POINTER(AnalyticOpts), PP], C.c_int),
     "clay_field_volume": ([P, P, PP], C.c_int),
     "clay_field_union": ([P, P, PP], C.c_int),
+    "clay_field_offset": ([P, P, PP], C.c_int),
+    "clay_field_lerp": ([P, P, P, PP], C.c_int),
     "clay_field_destroy": ([P], C.c_int),
     "clay_field_max_mip": ([P, C.POINTER(I32)], C.c_int),
     "clay_field_eval": ([P, P, I32, P, P], C.c_int),

@@ -111,3 +111,38 @@ def unionField(a: TracedField, b: TracedField) -> TracedField:
     h = C.c_void_p()
     L.check(L.lib().clay_field_union(a.handle, b.handle, C.byref(h)))
     return TracedField(a.root, h, keep=(a, b))
+
+
+def _device_floats(value, n: int):
+    """A CUDA float32 tensor of n values: `value` itself when it is one (the caller
+    may then change it in place, as the reference's GPU-side closure reads a
+    uniform), else a new one holding it."""
+    import torch
+
+    if isinstance(value, torch.Tensor):
+        if not (value.is_cuda and value.dtype == torch.float32 and value.numel() == n and value.is_contiguous()):
+            raise ValueError(f"expected a contiguous CUDA float32 tensor of {n} values")
+        return value
+    return torch.tensor(np.asarray(value, dtype=np.float32).reshape(n), device="cuda")
+
+
+def offsetField(field: TracedField, offset) -> TracedField:
+    """offsetField (field.ts:388-429): `field` read at p - offset.  `offset` is a
+    CUDA tensor of 3 floats read on the device at every evaluation, or 3 numbers."""
+    dev = _device_floats(offset, 3)
+    h = C.c_void_p()
+    L.check(L.lib().clay_field_offset(field.handle, C.c_void_p(dev.data_ptr()), C.byref(h)))
+    out = TracedField(field.root, h, keep=(field, dev))
+    out.offset = dev
+    return out
+
+
+def lerpField(a: TracedField, b: TracedField, t) -> TracedField:
+    """lerpField (field.ts:316-386) of a morph pair of volume fields, mixed by
+    clamp(t, 0, 1); `t` is a CUDA tensor of one float read on the device, or a number."""
+    dev = _device_floats(t, 1)
+    h = C.c_void_p()
+    L.check(L.lib().clay_field_lerp(a.handle, b.handle, C.c_void_p(dev.data_ptr()), C.byref(h)))
+    out = TracedField(a.root, h, keep=(a, b, dev))
+    out.t = dev
+    return out

@@ -222,6 +222,7 @@ Field* make_volume_field(Ctx* ctx, Volume* v) {
   l.dist0 = v->dist;
   l.mat0 = v->mat;
   f->vols = {v};
+  f->vols_b = {nullptr};
   f->bands = {0.0};
   f->eps = {0.0};
   // One instance per scene: the collider reads the scene's own record and planes.
@@ -479,6 +480,7 @@ extern "C" int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int
   l.eps = (float)(opts && opts->epsilon > 0 ? opts->epsilon : 1e-3);
   l.material = (float)(opts ? opts->material : 0.0);
   f->vols = {nullptr};
+  f->vols_b = {nullptr};
   f->bands = {opts && opts->band > 0 ? opts->band : 64.0};
   f->eps = {opts && opts->epsilon > 0 ? opts->epsilon : 1e-3};
   *out = reinterpret_cast<clay_field*>(f);
@@ -508,9 +510,70 @@ extern "C" int clay_field_union(clay_field* a, clay_field* b, clay_field** out) 
   for (int i = 0; i < fb->col.nleaf; i++) f->col.leaf[f->col.nleaf++] = fb->col.leaf[i];
   for (const Field* c : {fa, fb}) {
     f->vols.insert(f->vols.end(), c->vols.begin(), c->vols.end());
+    f->vols_b.insert(f->vols_b.end(), c->vols_b.begin(), c->vols_b.end());
     f->bands.insert(f->bands.end(), c->bands.begin(), c->bands.end());
     f->eps.insert(f->eps.end(), c->eps.begin(), c->eps.end());
   }
+  f->children = {fa, fb};
+  *out = reinterpret_cast<clay_field*>(f);
+  return CLAY_OK;
+}
+
+// offsetField (field.ts:388-429): every leaf of `a` reads p - *offset_dev.  The
+// offset is read on the device at each evaluation, so a caller can move the field
+// without a host round trip.  One offset per leaf.
+extern "C" int clay_field_offset(clay_field* a, const float* offset_dev, clay_field** out) {
+  Field* fa = reinterpret_cast<Field*>(a);
+  if (!fa || !offset_dev || !out) return fail(CLAY_ERR_INVALID, "clay_field_offset: NULL argument");
+  for (int i = 0; i < fa->col.nleaf; i++)
+    if (fa->col.leaf[i].offset)
+      return fail(CLAY_ERR_INVALID, "offsetField: the field is already offset (add the offsets on the device)");
+  Field* f = new (std::nothrow) Field();
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  f->ctx = fa->ctx;
+  f->max_mip = fa->max_mip;
+  f->col = fa->col;
+  for (int i = 0; i < f->col.nleaf; i++) f->col.leaf[i].offset = offset_dev;
+  f->vols = fa->vols;
+  f->vols_b = fa->vols_b;
+  f->bands = fa->bands;
+  f->eps = fa->eps;
+  f->children = {fa};
+  *out = reinterpret_cast<clay_field*>(f);
+  return CLAY_OK;
+}
+
+// lerpField (field.ts:316-386) of a morph pair: two volume fields of the same
+// resolution, mips and instances, blended by clamp(*t_dev, 0, 1) read on the device.
+extern "C" int clay_field_lerp(clay_field* a, clay_field* b, const float* t_dev, clay_field** out) {
+  Field* fa = reinterpret_cast<Field*>(a);
+  Field* fb = reinterpret_cast<Field*>(b);
+  if (!fa || !fb || !t_dev || !out) return fail(CLAY_ERR_INVALID, "clay_field_lerp: NULL argument");
+  if (fa->col.nleaf != 1 || fb->col.nleaf != 1 || fa->col.leaf[0].kind != 1 || fb->col.leaf[0].kind != 1 ||
+      fa->col.leaf[0].offset || fb->col.leaf[0].offset)
+    return fail(CLAY_ERR_INVALID, "lerpField: both fields must be plain volume fields (a morph pair)");
+  const Volume* va = fa->vols[0];
+  const Volume* vb = fb->vols[0];
+  if (va->info.resolution != vb->info.resolution || va->info.mip_levels != vb->info.mip_levels ||
+      va->instances != vb->instances)
+    return fail(CLAY_ERR_INVALID, "lerpField: the two volumes must have the same resolution, mips and instances");
+  Field* f = new (std::nothrow) Field();
+  if (!f) return fail(CLAY_ERR_INVALID, "out of host memory");
+  f->ctx = fa->ctx;
+  f->max_mip = std::max(fa->max_mip, fb->max_mip);
+  f->col.nleaf = 1;
+  LeafDev& l = f->col.leaf[0];
+  l = fa->col.leaf[0];
+  const LeafDev& lb = fb->col.leaf[0];
+  l.kind = 2;
+  l.vol_b = lb.vol;
+  l.dist0_b = lb.dist0;
+  l.mat0_b = lb.mat0;
+  l.lerp_t = t_dev;
+  f->vols = {fa->vols[0]};
+  f->vols_b = {fb->vols[0]};
+  f->bands = {0.0};
+  f->eps = {0.0};
   f->children = {fa, fb};
   *out = reinterpret_cast<clay_field*>(f);
   return CLAY_OK;

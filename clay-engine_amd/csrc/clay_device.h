@@ -41,7 +41,7 @@ struct VolDev {
 
 // One collider leaf: analytic brush fold or volume (mip 0).
 struct LeafDev {
-  int32_t kind;  // 0 analytic, 1 volume
+  int32_t kind;  // 0 analytic, 1 volume, 2 lerp of two volumes
   int32_t nbrush;
   const clay_brush* brushes;
   float eps;
@@ -50,6 +50,15 @@ struct LeafDev {
   const uint16_t* dist0;  // mip-0 distance plane (f16 bits), x fastest
   const uint16_t* mat0;   // mip-0 material plane (f16 bits), same layout
   float material;         // analytic: the material `field` reports (field.ts:213-216)
+  // offsetField (field.ts:388-429): a device vec3 subtracted from p before the leaf
+  // is read (the caller may move it on the device between launches), or null.
+  const float* offset;
+  // lerpField (field.ts:316-386), kind 2: the second volume of the morph pair (same
+  // resolution, mips and instances as the first) and the device blend t.
+  const VolDev* vol_b;
+  const uint16_t* dist0_b;
+  const uint16_t* mat0_b;
+  const float* lerp_t;
 };
 
 // unionField folded left over up to CLAY_MAX_LEAVES leaves (field.ts:246-315).
@@ -296,12 +305,47 @@ __device__ __forceinline__ const uint16_t* leaf_plane(const LeafDev& l, uint32_t
   return l.dist0 + (size_t)l.scene_stride * scene;
 }
 
+__device__ __forceinline__ const VolDev& leaf_vol_b(const LeafDev& l, uint32_t scene) {
+  return l.scene_stride ? l.vol_b[scene] : *l.vol_b;
+}
+
+// offsetField: the child reads p - offset (field.ts:405-427).
+__device__ __forceinline__ f3 leaf_point(const LeafDev& l, f3 p) {
+  return l.offset ? mk(p.x - l.offset[0], p.y - l.offset[1], p.z - l.offset[2]) : p;
+}
+
+// lerpField's k = clamp(t(), 0, 1) (field.ts:348).
+__device__ __forceinline__ float lerp_k(const LeafDev& l) { return fclamp_s(*l.lerp_t, 0.0f, 1.0f); }
+
 __device__ inline float leaf_dist(const LeafDev& l, uint32_t scene, f3 p) {
+  p = leaf_point(l, p);
   if (l.kind == 0) return analytic_dist(l, p);
-  return vol_dist(leaf_vol(l, scene), leaf_plane(l, scene), p);
+  const float a = vol_dist(leaf_vol(l, scene), leaf_plane(l, scene), p);
+  if (l.kind == 1) return a;
+  return fmix(a, vol_dist(leaf_vol_b(l, scene), l.dist0_b + (size_t)l.scene_stride * scene, p), lerp_k(l));
+}
+
+// volumeField.normal (field.ts:178-180): central differences of sampleDist.
+__device__ inline f3 vol_normal(const VolDev& v, const uint16_t* d, f3 p) {
+  const float h = v.grad_h;
+  const f3 g = mk(vol_dist(v, d, mk(p.x + h, p.y, p.z)) - vol_dist(v, d, mk(p.x - h, p.y, p.z)),
+                  vol_dist(v, d, mk(p.x, p.y + h, p.z)) - vol_dist(v, d, mk(p.x, p.y - h, p.z)),
+                  vol_dist(v, d, mk(p.x, p.y, p.z + h)) - vol_dist(v, d, mk(p.x, p.y, p.z - h)));
+  const float len = len3(g);
+  return len > 1e-8f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
 }
 
 __device__ inline f3 leaf_normal(const LeafDev& l, uint32_t scene, f3 p) {
+  p = leaf_point(l, p);
+  if (l.kind == 2) {
+    // normalize(mix(a.normal, b.normal, k)) (field.ts:379-383); a zero mix reads up.
+    const float k = lerp_k(l);
+    const f3 na = vol_normal(leaf_vol(l, scene), leaf_plane(l, scene), p);
+    const f3 nb = vol_normal(leaf_vol_b(l, scene), l.dist0_b + (size_t)l.scene_stride * scene, p);
+    const f3 g = mk(fmix(na.x, nb.x, k), fmix(na.y, nb.y, k), fmix(na.z, nb.z, k));
+    const float len = len3(g);
+    return len > 1e-9f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
+  }
   if (l.kind == 0) {
     float e = l.eps;
     f3 g = mk(analytic_dist(l, mk(p.x + e, p.y, p.z)) - analytic_dist(l, mk(p.x - e, p.y, p.z)),
@@ -310,24 +354,25 @@ __device__ inline f3 leaf_normal(const LeafDev& l, uint32_t scene, f3 p) {
     float len = len3(g);
     return len > 1e-9f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
   }
-  const VolDev& v = leaf_vol(l, scene);
-  const uint16_t* d = leaf_plane(l, scene);
-  float h = v.grad_h;
-  f3 g = mk(vol_dist(v, d, mk(p.x + h, p.y, p.z)) - vol_dist(v, d, mk(p.x - h, p.y, p.z)),
-            vol_dist(v, d, mk(p.x, p.y + h, p.z)) - vol_dist(v, d, mk(p.x, p.y - h, p.z)),
-            vol_dist(v, d, mk(p.x, p.y, p.z + h)) - vol_dist(v, d, mk(p.x, p.y, p.z - h)));
-  float len = len3(g);
-  return len > 1e-8f ? g * (1.0f / len) : mk(0.0f, 1.0f, 0.0f);
+  return vol_normal(leaf_vol(l, scene), leaf_plane(l, scene), p);
 }
 
-// field(p, 0).y of one leaf: analyticField's constant material, or volumeField's
-// trilinear material at mip 0 and 0 outside the volume box (volume.ts:288-296).
-__device__ inline float leaf_material(const LeafDev& l, uint32_t scene, f3 p) {
-  if (l.kind == 0) return l.material;
-  const VolDev& v = leaf_vol(l, scene);
+// volumeField.field(p, 0).y: the trilinear material at mip 0, 0 outside the box
+// (volume.ts:288-296).
+__device__ inline float vol_material(const VolDev& v, const uint16_t* m, f3 p) {
   f3 rel = mk(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
   if (sd_box(rel, mk(v.half, v.half, v.half)) > 0.0f) return 0.0f;
-  return sample_raw0(v, l.mat0 + (size_t)l.scene_stride * scene, p);
+  return sample_raw0(v, m, p);
+}
+
+// field(p, 0).y of one leaf: analyticField's constant material, volumeField's, or
+// lerpField's mix of its two volumes' (field.ts:355-359).
+__device__ inline float leaf_material(const LeafDev& l, uint32_t scene, f3 p) {
+  p = leaf_point(l, p);
+  if (l.kind == 0) return l.material;
+  const float a = vol_material(leaf_vol(l, scene), l.mat0 + (size_t)l.scene_stride * scene, p);
+  if (l.kind == 1) return a;
+  return fmix(a, vol_material(leaf_vol_b(l, scene), l.mat0_b + (size_t)l.scene_stride * scene, p), lerp_k(l));
 }
 
 // Returns the leaf the union selects (right child wins unless left is strictly nearer).

@@ -115,6 +115,7 @@ struct Field {
   // Per leaf, for the reads at every mip (clay_field_query / entry): the volume of a
   // volume leaf (else null), and an analytic leaf's band and epsilon options.
   std::vector<Volume*> vols;
+  std::vector<Volume*> vols_b;  // a lerp leaf's second volume, else null
   std::vector<double> bands, eps;
 };
 

@@ -16,12 +16,15 @@ namespace {
 constexpr float kNever = 1e9f;  // NEVER (field.ts:82)
 
 struct QLeaf {
-  int32_t kind;    // 0 analytic, 1 volume
+  int32_t kind;    // 0 analytic, 1 volume, 2 lerp of two volumes
   int32_t mips;
   float inv_band;  // analytic: 1 / band
   const VolDev* vol;
   const uint16_t* dist;
   const uint16_t* mat;
+  const VolDev* vol_b;  // lerp: the second volume (same levels as the first)
+  const uint16_t* dist_b;
+  const uint16_t* mat_b;
   uint32_t off[CLAY_MAX_MIPS];
   int32_t res[CLAY_MAX_MIPS];
 };
@@ -53,20 +56,31 @@ __device__ inline float sample_level(const VolDev& v, const uint16_t* d, int res
 // (dBox, 0); inside, the level min(mip, maxMip) read as (raw * bandWorld, raw) or
 // (dist * bandWorld, material) (field.ts:135-149, volume.ts:277-296).  Analytic:
 // (v, clamp(v / band, -1, 1)) or (v, material) (field.ts:208-216).
-__device__ inline float2 leaf_read(const QLeaf& q, const LeafDev& l, f3 p, int mip, bool field) {
-  if (q.kind == 0) {
-    const float v = analytic_dist(l, p);
-    return field ? make_float2(v, l.material) : make_float2(v, fclamp_s(v * q.inv_band, -1.0f, 1.0f));
-  }
-  const VolDev& v = *q.vol;
+__device__ inline float2 vol_read(const QLeaf& q, const VolDev& v, const uint16_t* dist, const uint16_t* mat,
+                                  f3 p, int mip, bool field) {
   const f3 rel = mk(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
   const float dbox = sd_box(rel, mk(v.half, v.half, v.half));
   if (dbox > 0.0f) return make_float2(dbox, field ? 0.0f : 1.0f);
   const int m = mip < q.mips - 1 ? mip : q.mips - 1;
   const float band_world = v.band * v.voxel * ldexpf(1.0f, m);
-  const float raw = sample_level(v, q.dist + q.off[m], q.res[m], p);
+  const float raw = sample_level(v, dist + q.off[m], q.res[m], p);
   if (!field) return make_float2(raw * band_world, raw);
-  return make_float2(raw * band_world, sample_level(v, q.mat + q.off[m], q.res[m], p));
+  return make_float2(raw * band_world, sample_level(v, mat + q.off[m], q.res[m], p));
+}
+
+// offsetField reads its child at p - offset (field.ts:405-423); lerpField mixes its two
+// volumes' reads, keeping the smaller band value for sample (field.ts:343-359).
+__device__ inline float2 leaf_read(const QLeaf& q, const LeafDev& l, f3 p, int mip, bool field) {
+  p = leaf_point(l, p);
+  if (q.kind == 0) {
+    const float v = analytic_dist(l, p);
+    return field ? make_float2(v, l.material) : make_float2(v, fclamp_s(v * q.inv_band, -1.0f, 1.0f));
+  }
+  const float2 a = vol_read(q, *q.vol, q.dist, q.mat, p, mip, field);
+  if (q.kind == 1) return a;
+  const float2 b = vol_read(q, *q.vol_b, q.dist_b, q.mat_b, p, mip, field);
+  const float k = lerp_k(l);
+  return make_float2(fmix(a.x, b.x, k), field ? fmix(a.y, b.y, k) : fmin_s(a.y, b.y));
 }
 
 // slabRange (field.ts:90-100): (tNear, tFar) against the box rel +- half.
@@ -108,15 +122,24 @@ __global__ void __launch_bounds__(256)
   out[2 * i + 1] = best.y;
 }
 
+// entry of one leaf: offsetField moves the ray origin (field.ts:401-404), lerpField takes
+// the nearer of its two volumes' entries (field.ts:336-342).
+__device__ inline float leaf_entry(const QLeaf& q, const LeafDev& l, f3 ro, f3 rd, float tmin, float radius,
+                                   float aperture) {
+  ro = leaf_point(l, ro);
+  const float t = vol_entry(*q.vol, ro, rd, tmin, radius, aperture);
+  return q.kind == 2 ? fmin_s(t, vol_entry(*q.vol_b, ro, rd, tmin, radius, aperture)) : t;
+}
+
 // unionField.entry (field.ts:255-264): the nearest of the children's entries.
 __global__ void __launch_bounds__(256)
-    k_field_entry(QueryK q, int n, const float* __restrict__ rays, float* __restrict__ out) {
+    k_field_entry(QueryK q, ColliderDev col, int n, const float* __restrict__ rays, float* __restrict__ out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float* r = rays + 9 * (size_t)i;
   const f3 ro = mk(r[0], r[1], r[2]), rd = mk(r[3], r[4], r[5]);
-  float t = vol_entry(*q.leaf[0].vol, ro, rd, r[6], r[7], r[8]);
-  for (int l = 1; l < q.nleaf; l++) t = fmin_s(t, vol_entry(*q.leaf[l].vol, ro, rd, r[6], r[7], r[8]));
+  float t = leaf_entry(q.leaf[0], col.leaf[0], ro, rd, r[6], r[7], r[8]);
+  for (int l = 1; l < q.nleaf; l++) t = fmin_s(t, leaf_entry(q.leaf[l], col.leaf[l], ro, rd, r[6], r[7], r[8]));
   out[i] = t;
 }
 
@@ -126,12 +149,18 @@ QueryK query_k(const Field* f) {
   for (int l = 0; l < q.nleaf; l++) {
     QLeaf& d = q.leaf[l];
     const Volume* v = l < (int)f->vols.size() ? f->vols[l] : nullptr;
-    d.kind = v ? 1 : 0;
+    const Volume* vb = l < (int)f->vols_b.size() ? f->vols_b[l] : nullptr;
+    d.kind = v ? (vb ? 2 : 1) : 0;
     if (v) {
       d.mips = v->info.mip_levels;
       d.vol = v->dev;
       d.dist = v->dist;
       d.mat = v->mat;
+      if (vb) {
+        d.vol_b = vb->dev;
+        d.dist_b = vb->dist;
+        d.mat_b = vb->mat;
+      }
       for (int m = 0; m < d.mips; m++) {
         d.off[m] = (uint32_t)v->voxel_off[m];
         d.res[m] = v->mip_res(m);
@@ -191,7 +220,8 @@ extern "C" int clay_field_entry(clay_field* fh, const float* rays9, int32_t n, f
   CLAY_HIP_CHECK(hipMemcpyAsync(dr.p, rays9, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
   {
     LaunchTimer t(f->ctx, s, "field_entry");
-    hipLaunchKernelGGL(k_field_entry, dim3((n + 255) / 256), dim3(256), 0, s, query_k(f), n, dr.p, dout.p);
+    hipLaunchKernelGGL(k_field_entry, dim3((n + 255) / 256), dim3(256), 0, s, query_k(f), f->col, n, dr.p,
+                       dout.p);
   }
   CLAY_HIP_CHECK(hipGetLastError());
   CLAY_HIP_CHECK(hipMemcpyAsync(t_out, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
@@ -213,6 +243,13 @@ extern "C" int clay_field_scale(const clay_field* fh, int32_t mip, float* band_w
       const float p2 = std::ldexp(1.0f, m);
       b = (float)v->info.band * (float)(v->info.world_size / v->info.resolution) * p2;
       vx = (float)(v->info.world_size / v->info.resolution) * p2;
+      if (const Volume* w = l < f->vols_b.size() ? f->vols_b[l] : nullptr) {
+        // lerpField: the smaller band and the larger voxel of the pair (field.ts:366-377).
+        const int mb = std::min<int>(mip, w->info.mip_levels - 1);
+        const float q2 = std::ldexp(1.0f, mb);
+        b = std::min(b, (float)w->info.band * (float)(w->info.world_size / w->info.resolution) * q2);
+        vx = std::max(vx, (float)(w->info.world_size / w->info.resolution) * q2);
+      }
     } else {
       b = (float)f->bands[l];
       vx = (float)f->eps[l];

@@ -193,6 +193,15 @@ int clay_field_analytic(clay_ctx* ctx, const clay_brush* brushes, int32_t n,
 int clay_field_volume(clay_ctx* ctx, clay_volume* vol, clay_field** out);
 /* unionField (field.ts:246-315); a and b must outlive the union. */
 int clay_field_union(clay_field* a, clay_field* b, clay_field** out);
+/* offsetField (field.ts:388-429): the field read at p - offset, with offset a device
+ * vec3 (3 floats) read at every evaluation, so the caller can move the field on the
+ * device.  a and offset_dev must outlive the result.  A field offset once already is
+ * refused. */
+int clay_field_offset(clay_field* a, const float* offset_dev, clay_field** out);
+/* lerpField (field.ts:316-386) of a morph pair: two volumeFields of the same resolution,
+ * mips and instances, mixed by clamp(*t_dev, 0, 1) (a device float read at every
+ * evaluation).  a, b and t_dev must outlive the result. */
+int clay_field_lerp(clay_field* a, clay_field* b, const float* t_dev, clay_field** out);
 int clay_field_destroy(clay_field* f);
 int clay_field_max_mip(const clay_field* f, int32_t* out);
 /* sample(p, 0).x and normal(p) at n host points (xyz interleaved); runs on the GPU. */

@@ -95,8 +95,8 @@ typedef struct {
 } orc_fluid_k;
 
 /* One collider leaf: an analytic brush fold or a baked volume (mip 0). */
-typedef struct {
-  int32_t kind;            /* 0 = analytic brush list, 1 = volume */
+typedef struct orc_leaf_s {
+  int32_t kind;            /* 0 = analytic brush list, 1 = volume, 2 = lerp of two volumes */
   int32_t nbrush;
   const orc_brush* brushes;
   double eps;              /* analytic: central-difference epsilon (field.ts:201) */
@@ -117,6 +117,14 @@ typedef struct {
   const uint16_t* dist_m[ORC_MAX_MIPS];
   const uint16_t* mat_m[ORC_MAX_MIPS];
   double analytic_band;
+  /* offsetField (field.ts:388-429): when has_offset, p - offset is read. */
+  int32_t has_offset;
+  int32_t pad3;
+  double offset[3];
+  /* lerpField (field.ts:316-386), kind 2: this leaf's volume is `a`, lerp_b the
+   * second volume (same resolution, mips and box), mixed by clamp(lerp_t, 0, 1). */
+  const struct orc_leaf_s* lerp_b;
+  double lerp_t;
 } orc_leaf;
 
 /* unionField over leaves, folded left: union(union(a, b), c) (field.ts:246-315). */

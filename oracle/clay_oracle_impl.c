@@ -224,21 +224,21 @@ static T vol_dist(const orc_leaf* l, const volp* v, v3 p) {
   return sample_raw(l, v, p) * band_world;
 }
 
-static T leaf_dist(const orc_leaf* l, v3 p) {
-  if (l->kind == 0) return analytic_dist(l, p);
+/* offsetField: the child reads p - offset (field.ts:405-427). */
+static v3 leaf_point(const orc_leaf* l, v3 p) {
+  return l->has_offset ? V(p.x - rnd(l->offset[0]), p.y - rnd(l->offset[1]), p.z - rnd(l->offset[2])) : p;
+}
+
+/* lerpField's k = clamp(t(), 0, 1) (field.ts:348). */
+static T lerp_k(const orc_leaf* l) { return t_clamp(rnd(l->lerp_t), 0, 1); }
+
+static T vol_leaf_dist(const orc_leaf* l, v3 p) {
   volp v = vol_params(l);
   return vol_dist(l, &v, p);
 }
 
-static v3 leaf_normal(const orc_leaf* l, v3 p) {
-  if (l->kind == 0) {
-    T e = rnd(l->eps);
-    v3 g = V(analytic_dist(l, V(p.x + e, p.y, p.z)) - analytic_dist(l, V(p.x - e, p.y, p.z)),
-             analytic_dist(l, V(p.x, p.y + e, p.z)) - analytic_dist(l, V(p.x, p.y - e, p.z)),
-             analytic_dist(l, V(p.x, p.y, p.z + e)) - analytic_dist(l, V(p.x, p.y, p.z - e)));
-    T len = vlen(g);
-    return len > KF(1e-9) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
-  }
+/* volumeField.normal (field.ts:178-180): central differences of sampleDist. */
+static v3 vol_normal(const orc_leaf* l, v3 p) {
   volp v = vol_params(l);
   T h = v.grad_h;
   v3 g = V(vol_dist(l, &v, V(p.x + h, p.y, p.z)) - vol_dist(l, &v, V(p.x - h, p.y, p.z)),
@@ -248,10 +248,46 @@ static v3 leaf_normal(const orc_leaf* l, v3 p) {
   return len > KF(1e-8) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
 }
 
+static T leaf_dist(const orc_leaf* l, v3 p) {
+  p = leaf_point(l, p);
+  if (l->kind == 0) return analytic_dist(l, p);
+  const T a = vol_leaf_dist(l, p);
+  return l->kind == 1 ? a : t_mix(a, vol_leaf_dist(l->lerp_b, p), lerp_k(l));
+}
+
+static v3 leaf_normal(const orc_leaf* l, v3 p) {
+  p = leaf_point(l, p);
+  if (l->kind == 2) {
+    /* normalize(mix(a.normal, b.normal, k)) (field.ts:379-383); a zero mix reads up. */
+    const T k = lerp_k(l);
+    const v3 na = vol_normal(l, p), nb = vol_normal(l->lerp_b, p);
+    const v3 g = V(t_mix(na.x, nb.x, k), t_mix(na.y, nb.y, k), t_mix(na.z, nb.z, k));
+    const T len = vlen(g);
+    return len > KF(1e-9) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
+  }
+  if (l->kind == 0) {
+    T e = rnd(l->eps);
+    v3 g = V(analytic_dist(l, V(p.x + e, p.y, p.z)) - analytic_dist(l, V(p.x - e, p.y, p.z)),
+             analytic_dist(l, V(p.x, p.y + e, p.z)) - analytic_dist(l, V(p.x, p.y - e, p.z)),
+             analytic_dist(l, V(p.x, p.y, p.z + e)) - analytic_dist(l, V(p.x, p.y, p.z - e)));
+    T len = vlen(g);
+    return len > KF(1e-9) ? vmuls(g, (T)1 / len) : V(0, 1, 0);
+  }
+  return vol_normal(l, p);
+}
+
 /* field(p, 0).y of one leaf: analyticField's constant material, or volumeField's
  * trilinear material at mip 0 and 0 outside the volume box (volume.ts:288-296). */
+static T vol_material(const orc_leaf* l, v3 p);
+
 static T leaf_material(const orc_leaf* l, v3 p) {
+  p = leaf_point(l, p);
   if (l->kind == 0) return rnd(l->material);
+  const T a = vol_material(l, p);
+  return l->kind == 1 ? a : t_mix(a, vol_material(l->lerp_b, p), lerp_k(l));
+}
+
+static T vol_material(const orc_leaf* l, v3 p) {
   volp v = vol_params(l);
   v3 rel = V(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
   if (sd_box(rel, V(v.half, v.half, v.half)) > 0) return 0;
@@ -1429,13 +1465,31 @@ static T sample_mip(const orc_leaf* l, const uint16_t* plane, int res, const vol
 
 /* sample(p, mip) (what 0) or field(p, mip) (what 1) of one leaf: field.ts:135-149,
  * volume.ts:277-296; analytic field.ts:208-216. */
+static void vol_read(const orc_leaf* l, int what, int mip, v3 p, T* x, T* y);
+
 static void leaf_read(const orc_leaf* l, int what, int mip, v3 p, T* x, T* y) {
+  p = leaf_point(l, p);
+  if (l->kind == 2) {
+    /* lerpField (field.ts:343-365): sample mixes x and keeps the smaller y; field
+     * mixes both. */
+    T ax, ay, bx, by;
+    vol_read(l, what, mip, p, &ax, &ay);
+    vol_read(l->lerp_b, what, mip, p, &bx, &by);
+    const T k = lerp_k(l);
+    *x = t_mix(ax, bx, k);
+    *y = what ? t_mix(ay, by, k) : t_min(ay, by);
+    return;
+  }
   if (l->kind == 0) {
     const T d = analytic_dist(l, p);
     *x = d;
     *y = what ? rnd(l->material) : t_clamp(d * rnd(1.0 / l->analytic_band), -1, 1);
     return;
   }
+  vol_read(l, what, mip, p, x, y);
+}
+
+static void vol_read(const orc_leaf* l, int what, int mip, v3 p, T* x, T* y) {
   volp v = vol_params(l);
   v3 rel = V(p.x - v.center[0], p.y - v.center[1], p.z - v.center[2]);
   T dbox = sd_box(rel, V(v.half, v.half, v.half));
@@ -1503,12 +1557,20 @@ static T vol_entry(const orc_leaf* l, v3 ro, v3 rd, T tmin, T radius, T aperture
   return ghi >= t ? t : KF(1e9);
 }
 
+/* entry of one leaf: offsetField moves the ray origin (field.ts:401-404), lerpField
+ * takes the nearer of its two volumes' entries (field.ts:336-342). */
+static T leaf_entry(const orc_leaf* l, v3 ro, v3 rd, T tmin, T radius, T aperture) {
+  ro = leaf_point(l, ro);
+  const T t = vol_entry(l, ro, rd, tmin, radius, aperture);
+  return l->kind == 2 ? t_min(t, vol_entry(l->lerp_b, ro, rd, tmin, radius, aperture)) : t;
+}
+
 void FN(orc_field_entry)(const orc_collider* col, int n, const T* rays9, T* out) {
   for (int i = 0; i < n; i++) {
     const T* r = rays9 + 9 * (size_t)i;
     v3 ro = V(r[0], r[1], r[2]), rd = V(r[3], r[4], r[5]);
-    T t = vol_entry(&col->leaf[0], ro, rd, r[6], r[7], r[8]);
-    for (int k = 1; k < col->nleaf; k++) t = t_min(t, vol_entry(&col->leaf[k], ro, rd, r[6], r[7], r[8]));
+    T t = leaf_entry(&col->leaf[0], ro, rd, r[6], r[7], r[8]);
+    for (int k = 1; k < col->nleaf; k++) t = t_min(t, leaf_entry(&col->leaf[k], ro, rd, r[6], r[7], r[8]));
     out[i] = t;
   }
 }

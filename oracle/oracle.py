@@ -63,6 +63,8 @@ class OrcLeaf(C.Structure):
         ("mat16", C.c_void_p), ("material", C.c_double),
         ("mips", C.c_int32), ("pad2", C.c_int32), ("dist_m", C.c_void_p * 8), ("mat_m", C.c_void_p * 8),
         ("analytic_band", C.c_double),
+        ("has_offset", C.c_int32), ("pad3", C.c_int32), ("offset", C.c_double * 3),
+        ("lerp_b", C.c_void_p), ("lerp_t", C.c_double),
     ]
 
 
@@ -298,6 +300,31 @@ class Collider:
                 self._keep += [dl, ml]
                 leaf.dist_m[lv] = dl.ctypes.data
                 leaf.mat_m[lv] = ml.ctypes.data
+        self.c.nleaf += 1
+        return self
+
+    def offset(self, o) -> "Collider":
+        """offsetField over every leaf so far (field.ts:388-429): each reads p - o."""
+        for i in range(self.c.nleaf):
+            leaf = self.c.leaf[i]
+            assert not leaf.has_offset, "one offset per leaf"
+            leaf.has_offset = 1
+            for a in range(3):
+                leaf.offset[a] = float(o[a])
+        return self
+
+    def add_lerp(self, a: "Collider", b: "Collider", t: float) -> "Collider":
+        """lerpField(volumeField(A), volumeField(B), t) (field.ts:316-386) as one leaf,
+        from two one-volume colliders."""
+        assert a.c.nleaf == 1 and b.c.nleaf == 1 and a.c.leaf[0].kind == 1 and b.c.leaf[0].kind == 1
+        second = OrcLeaf()
+        C.memmove(C.byref(second), C.byref(b.c.leaf[0]), C.sizeof(OrcLeaf))
+        self._keep += [a, b, second]
+        leaf = self.c.leaf[self.c.nleaf]
+        C.memmove(C.byref(leaf), C.byref(a.c.leaf[0]), C.sizeof(OrcLeaf))
+        leaf.kind = 2
+        leaf.lerp_b = C.cast(C.pointer(second), C.c_void_p)
+        leaf.lerp_t = float(t)
         self.c.nleaf += 1
         return self
 

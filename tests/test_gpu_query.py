@@ -87,3 +87,67 @@ def test_entry_of_a_union_of_volumes(root):
     u.destroy()
     va.destroy()
     vb.destroy()
+
+
+def test_offset_and_lerp_fields_follow_device_values(root):
+    """offsetField (field.ts:388-429) over lerpField (field.ts:316-386) of a morph pair,
+    in a union with an analytic field, as SoftBody builds its body field
+    (game/softbody.ts:209-223): sample and field at every mip, the mip-0 distance and
+    normal the colliders read, and entry, equal the oracle's f32 mode bit for bit;
+    then again after the blend and the offset are changed in place on the device."""
+    import torch
+
+    va = baked(root)
+    vb = clayhip.SdfVolume(root, resolution=64, mipLevels=3, worldSize=8.0, origin=(-4.0, -2.0, -4.0), band=4)
+    vb.bakeBrushes([{"kind": "sphere", "size": 1.4, "pos": (-0.8, -0.4, 0.5), "material": 5.0},
+                    {"kind": "sphere", "size": 0.9, "pos": (1.5, 0.2, -0.6), "material": 6.0}])
+    t = torch.tensor([0.35], dtype=torch.float32, device="cuda")
+    off = torch.tensor([0.3, -0.2, 0.45], dtype=torch.float32, device="cuda")
+    fa, fb = clayhip.volumeField(va), clayhip.volumeField(vb)
+    lerp = clayhip.lerpField(fa, fb, t)
+    moved = clayhip.offsetField(lerp, off)
+    ana = clayhip.analyticField(root, [SPHERE], {"material": 7.0, "band": 3.0})
+    u = clayhip.unionField(ana, moved)
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-4.5, 4.5, size=(20000, 3)).astype(np.float32)
+    rays = np.concatenate([rng.uniform(-12, 12, size=(5000, 3)), rng.normal(size=(5000, 3)),
+                           rng.uniform(0, 0.5, size=(5000, 1)), rng.uniform(0, 0.4, size=(5000, 1)),
+                           rng.uniform(0, 0.05, size=(5000, 1))], axis=1).astype(np.float32)
+
+    def oracles(tv, ov):
+        pair = lambda: (oracle_volume(O.Collider(), va, (-4.0, -2.0, -4.0), 8.0),
+                        oracle_volume(O.Collider(), vb, (-4.0, -2.0, -4.0), 8.0))
+        ol = O.Collider().add_lerp(*pair(), tv)
+        om = O.Collider().add_lerp(*pair(), tv).offset(ov)
+        ou = O.Collider().add_analytic([make_brush(SPHERE)], material=7.0, band=3.0).add_lerp(*pair(), tv)
+        ou.c.leaf[1].has_offset = 1
+        for a in range(3):
+            ou.c.leaf[1].offset[a] = float(ov[a])
+        return ol, om, ou
+
+    def check(tv, ov):
+        ol, om, ou = oracles(tv, ov)
+        for f, o, name in ((lerp, ol, "lerp"), (moved, om, "offset lerp"), (u, ou, "union")):
+            for mip in (0, 1, 2, 3):
+                for what in ("sample", "field"):
+                    got, want = getattr(f, what)(p, mip), o.query(p, mip, what, "f32")
+                    assert bits_equal(got, want), f"{name} {what} mip {mip}: {(got != want).any(1).sum()} differ"
+            d, n = f.evaluate(p)
+            od, on = o.eval(p)
+            assert bits_equal(d, od) and bits_equal(n, on), f"{name} mip-0 distance or normal differs"
+        got = moved.entry(rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7], rays[:, 8])
+        assert bits_equal(got, om.entry(rays, "f32")), "offset lerp entry differs"
+
+    check(np.float32(0.35), np.array([0.3, -0.2, 0.45], np.float32))
+    t.fill_(0.8)
+    off.copy_(torch.tensor([-0.6, 0.1, 0.2]))
+    torch.cuda.synchronize()
+    check(np.float32(0.8), np.array([-0.6, 0.1, 0.2], np.float32))
+    with pytest.raises(L.ClayError, match="morph pair"):
+        clayhip.lerpField(u, fb, t)
+    with pytest.raises(L.ClayError, match="already offset"):
+        clayhip.offsetField(moved, off)
+    for x in (u, ana, moved, lerp, fa, fb):
+        x.destroy()
+    va.destroy()
+    vb.destroy()

@@ -138,3 +138,59 @@ def test_entry_against_the_slab_intersection():
     rays_a[:, 8] = 0.2
     ta = c.entry(rays_a, "f64")
     assert (ta <= t + 1e-12).all() and (ta < t).any()
+
+
+def test_offset_reads_the_child_at_the_moved_point():
+    """offsetField (field.ts:388-429): every read of the offset field at p is the
+    child's read at the f32 point p - o, bit for bit."""
+    lv = levels(7)
+    o = np.array([0.37, -0.21, 1.13], np.float32)
+    moved = volume_collider(lv).offset(o)
+    plain = volume_collider(lv)
+    rng = np.random.default_rng(8)
+    p = (ORIGIN + rng.uniform(-0.5, WORLD + 1.5, size=(2000, 3))).astype(np.float32)
+    q = (p - o).astype(np.float32)
+    for mip in (0, 1, 2):
+        for what in ("sample", "field"):
+            assert np.array_equal(moved.query(p, mip, what, "f32"), plain.query(q, mip, what, "f32"))
+    d0, n0 = moved.eval(p, "f32")
+    d1, n1 = plain.eval(q, "f32")
+    assert np.array_equal(d0, d1) and np.array_equal(n0, n1)
+    rays = np.concatenate([rng.uniform(-8, 8, size=(500, 3)), rng.normal(size=(500, 3)),
+                           np.full((500, 1), 0.05), np.full((500, 1), 0.2), np.zeros((500, 1))], axis=1).astype(np.float32)
+    shifted = rays.copy()
+    shifted[:, :3] = (rays[:, :3] - o).astype(np.float32)
+    assert np.array_equal(moved.entry(rays, "f32"), plain.entry(shifted, "f32"))
+
+
+def test_lerp_of_a_morph_pair():
+    """lerpField (field.ts:316-386): sample mixes the distance and keeps the smaller band
+    value, field mixes both, the normal is the normalized mix, entry the nearer one;
+    t is clamped, so t <= 0 and t >= 1 give the two volumes back exactly."""
+    la, lb = levels(11), levels(12)
+    rng = np.random.default_rng(9)
+    p = (ORIGIN + rng.uniform(-0.3, WORLD + 0.3, size=(2000, 3))).astype(np.float32)
+    for t in (0.3, 0.75):
+        lerp = O.Collider().add_lerp(volume_collider(la), volume_collider(lb), t)
+        k = np.float32(t)
+        for mip in (0, 1, 2):
+            a = volume_collider(la).query(p, mip, "sample", "f32")
+            b = volume_collider(lb).query(p, mip, "sample", "f32")
+            s = lerp.query(p, mip, "sample", "f32")
+            assert np.array_equal(s[:, 0], a[:, 0] * (np.float32(1) - k) + b[:, 0] * k)
+            assert np.array_equal(s[:, 1], np.minimum(a[:, 1], b[:, 1]))
+            fa = volume_collider(la).query(p, mip, "field", "f32")
+            fb = volume_collider(lb).query(p, mip, "field", "f32")
+            assert np.array_equal(lerp.query(p, mip, "field", "f32"), fa * (np.float32(1) - k) + fb * k)
+        _, na = volume_collider(la).eval(p, "f32")
+        _, nb = volume_collider(lb).eval(p, "f32")
+        _, nl = lerp.eval(p, "f32")
+        g = na * (np.float32(1) - k) + nb * k
+        assert np.allclose(nl, g / np.linalg.norm(g, axis=1, keepdims=True), atol=2e-6)
+    for t, src in ((-0.5, la), (0.0, la), (1.0, lb), (1.7, lb)):
+        lerp = O.Collider().add_lerp(volume_collider(la), volume_collider(lb), t)
+        assert np.array_equal(lerp.query(p, 1, "sample", "f32")[:, 0], volume_collider(src).query(p, 1, "sample", "f32")[:, 0])
+    rays = np.concatenate([rng.uniform(-8, 8, size=(500, 3)), rng.normal(size=(500, 3)),
+                           np.full((500, 1), 0.05), np.full((500, 1), 0.2), np.zeros((500, 1))], axis=1).astype(np.float32)
+    lerp = O.Collider().add_lerp(volume_collider(la), volume_collider(lb), 0.5)
+    assert np.array_equal(lerp.entry(rays, "f32"), volume_collider(la).entry(rays, "f32"))
