@@ -541,8 +541,9 @@ __device__ __forceinline__ float sub_f16_hi(float base, uint32_t h2) {
 __device__ __forceinline__ float dot3_fma(f3 d) { return __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)); }
 
 // Occupancy hints: the FP16 density kernel fits 64 VGPRs (8 waves per SIMD) with no
-// spill; the forces kernels are held to 7 waves, at the cost of a few scratch words
-// outside the neighbour loop (FP16 forces: 1.11 -> 1.02 ms per batched frame).
+// spill; the forces kernels are held to 7 waves, at the cost of 48 bytes of scratch
+// per lane, which the extra wave more than pays for (FP16 forces: 1.11 -> 1.02 ms per
+// batched frame).
 template <int kB, int kS, bool kCover>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8)))
     k_density_lds(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
