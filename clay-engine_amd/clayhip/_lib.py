@@ -336,6 +336,7 @@ SIGNATURES = {
     "clay_fluid_configure": ([P, C.POINTER(FluidParams)], C.c_int),
     "clay_fluid_get_params": ([P, C.POINTER(FluidParams)], C.c_int),
     "clay_fluid_step": ([P, P], C.c_int),
+    "clay_fluid_set_graph": ([P, C.c_int32], C.c_int),
     "clay_fluid_substep": ([P, P], C.c_int),
     "clay_fluid_read_contacts": ([P, P, C.POINTER(U32), C.POINTER(U32)], C.c_int),
     "clay_fluid_conserved": ([P, DP], C.c_int),

@@ -140,6 +140,10 @@ class FluidSim:
         """fluid.ts:536-558: emit, `substeps` substeps, bake every mip."""
         L.check(L.lib().clay_fluid_step(self._h, L.stream_handle(stream)))
 
+    def set_graph(self, enabled: bool) -> None:
+        """Replay each frame from a captured HIP graph (no reference counterpart)."""
+        L.check(L.lib().clay_fluid_set_graph(self._h, 1 if enabled else 0))
+
     def substep(self, stream=None) -> None:
         L.check(L.lib().clay_fluid_substep(self._h, L.stream_handle(stream)))
 

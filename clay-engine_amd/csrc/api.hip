@@ -1272,6 +1272,11 @@ extern "C" int clay_fluid_constants(const clay_fluid* fh, clay_fluid_consts* out
   return CLAY_OK;
 }
 
+namespace {
+void drop_graph(Fluid* f);
+int batch_frame(Fluid* f, hipStream_t s);
+}  // namespace
+
 extern "C" int clay_fluid_set_state(clay_fluid* fh, const float* pos, const float* vel, uint32_t n) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || (n > 0 && !pos)) return fail(CLAY_ERR_INVALID, "clay_fluid_set_state: NULL argument");
@@ -1328,6 +1333,8 @@ extern "C" int clay_fluid_spawn(clay_fluid* fh, uint32_t n, const double pos[3],
 extern "C" int clay_fluid_configure(clay_fluid* fh, const clay_fluid_params* p) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_fluid_configure: NULL argument");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);  // the parameters are launch arguments of a captured frame
   f->params = *p;
   fluid_build_k(f);
   f->presorted = false;  // densities of the next substep depend on the parameters
@@ -1351,11 +1358,33 @@ extern "C" int clay_fluid_substep(clay_fluid* fh, clay_stream stream) {
   return fluid_forces(f, s);
 }
 
+// With a graph (clay_fluid_set_graph), a frame is captured once and replayed, as a
+// batch's is.  Launch sizes follow the live count, so a frame that emits runs
+// eagerly and a changed live count recaptures.
 extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_step: NULL fluid");
   fluid_build_k(f);
-  return fluid_frame_impl(f, pick(f->ctx, stream));
+  hipStream_t s = pick(f->ctx, stream);
+  if (f->use_graph && f->pending_emit == 0 && f->live > 0) {
+    if (f->graph && f->graph_live != f->live) {
+      CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+      drop_graph(f);
+    }
+    const int rc = batch_frame(f, s);
+    if (!rc) f->graph_live = f->live;
+    return rc;
+  }
+  return fluid_frame_impl(f, s);
+}
+
+extern "C" int clay_fluid_set_graph(clay_fluid* fh, int32_t enabled) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_set_graph: NULL fluid");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);
+  f->use_graph = enabled != 0;
+  return CLAY_OK;
 }
 
 extern "C" int clay_fluid_read_contacts(clay_fluid* fh, float* out4, uint32_t* n, uint32_t* raw) {
@@ -1456,6 +1485,8 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
 extern "C" int clay_fluid_set_tuning(clay_fluid* fh, const clay_tuning* t) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || !t) return fail(CLAY_ERR_INVALID, "clay_fluid_set_tuning: NULL argument");
+  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  drop_graph(f);
   return apply_tuning(f, t);
 }
 

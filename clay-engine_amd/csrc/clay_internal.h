@@ -223,6 +223,7 @@ struct Fluid {
   bool use_graph = false;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
+  uint32_t graph_live = 0;             // a FluidSim's live count when its graph was captured
   // The bake's binning runs beside the frame-end density pass: forked after the
   // sort onto `side`, joined before the walk (default priority).
   hipStream_t side = nullptr;

@@ -363,6 +363,11 @@ int clay_fluid_configure(clay_fluid* f, const clay_fluid_params* p);
 int clay_fluid_get_params(const clay_fluid* f, clay_fluid_params* out);
 /* FluidSim.step: emit, substeps x (bin, density, forces, integrate), bake (fluid.ts:536-558). */
 int clay_fluid_step(clay_fluid* f, clay_stream stream);
+/* Capture a frame once as a HIP graph and replay it on later steps (off by default).
+ * Frames that emit run eagerly; a changed live count, configure and set_tuning
+ * recapture.  No counterpart in the reference (WebGPU has no graphs); for callers
+ * that step one FluidSim many times with the same settings. */
+int clay_fluid_set_graph(clay_fluid* f, int32_t enabled);
 /* One substep without emit or bake (for per-substep parity checks). */
 int clay_fluid_substep(clay_fluid* f, clay_stream stream);
 /* FluidSim.readContacts: fluid.ts:565-569.  out4 holds min(raw, 32) points
