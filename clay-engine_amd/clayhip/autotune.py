@@ -20,7 +20,7 @@ SPACE = {
     "block": (128, 256),
     "stage": (64, 128, 256),
     "chunk": (1024, 2048, 4096),
-    "groups_per_cu": (4, 8, 16),
+    "groups_per_cu": (4, 8, 16, 32),
     "fluid_xcd": (0, 1),
     "splat_xcd": (0, 1),
     "splat_owner": (0, 1),
