@@ -915,6 +915,16 @@ void fluid_free(Fluid* f) {
 
 size_t contact_words(const Fluid* f) { return (size_t)f->scenes * (4 + 4 * CLAY_CONTACTS); }
 
+// Waits for everything a fluid may still have in flight: the context stream, the
+// stream of the latest step (a caller's stream) and the stream a graph replays on.
+hipError_t fluid_quiesce(const Fluid* f) {
+  hipError_t e = hipStreamSynchronize(f->ctx->stream);
+  if (e == hipSuccess && f->last_stream && f->last_stream != f->ctx->stream) e = hipStreamSynchronize(f->last_stream);
+  if (e == hipSuccess && f->graph_stream && f->graph_stream != f->ctx->stream && f->graph_stream != f->last_stream)
+    e = hipStreamSynchronize(f->graph_stream);
+  return e;
+}
+
 // First half of a substep (fluid.ts:293-368): cell stamp, keys, sort, cell
 // table and reordered state, then densities and pressures.  With `cover`, the
 // density pass also writes each particle's interior radius for the surface bake.
@@ -937,8 +947,9 @@ int fluid_sort(Fluid* f, hipStream_t s) {
     return rc;
   }
   const SceneK sk = f->scene_k();
-  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, f->keys, f->vals, invalid,
-                       f->scene_shift)))
+  const bool in_tmp = radix_input_in_tmp(f->key_bits);
+  if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, in_tmp ? f->keys2 : f->keys,
+                        in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift)))
     return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
@@ -1261,7 +1272,7 @@ extern "C" int clay_fluid_create(clay_ctx* ctx, clay_field* world, const clay_fl
 
 extern "C" int clay_fluid_destroy(clay_fluid* fh) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
-  if (f) (void)hipStreamSynchronize(f->ctx->stream);
+  if (f) (void)fluid_quiesce(f);
   fluid_free(f);
   return CLAY_OK;
 }
@@ -1281,8 +1292,7 @@ extern "C" int clay_fluid_set_state(clay_fluid* fh, const float* pos, const floa
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || (n > 0 && !pos)) return fail(CLAY_ERR_INVALID, "clay_fluid_set_state: NULL argument");
   if (n > f->capacity) return fail(CLAY_ERR_INVALID, "clay_fluid_set_state: more particles than capacity");
-  hipStream_t s = f->ctx->stream;
-  CLAY_HIP_CHECK(hipStreamSynchronize(s));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   float* dp[3] = {f->px, f->py, f->pz};
   float* dv[3] = {f->vx, f->vy, f->vz};
   for (int a = 0; a < 3; a++) {
@@ -1307,7 +1317,7 @@ extern "C" int clay_fluid_get_state(clay_fluid* fh, int32_t what, float* host) {
                                {f->pres, nullptr, nullptr},
                                {f->nvx, f->nvy, f->nvz}};
   if (what < 0 || what > 4) return fail(CLAY_ERR_INVALID, "clay_fluid_get_state: unknown field");
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   for (int a = 0; a < 3; a++) {
     if (!planes[what][a] || !n) continue;
     CLAY_HIP_CHECK(hipMemcpy(host + (size_t)a * n, planes[what][a], sizeof(float) * n, hipMemcpyDeviceToHost));
@@ -1333,7 +1343,7 @@ extern "C" int clay_fluid_spawn(clay_fluid* fh, uint32_t n, const double pos[3],
 extern "C" int clay_fluid_configure(clay_fluid* fh, const clay_fluid_params* p) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_fluid_configure: NULL argument");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);  // the parameters are launch arguments of a captured frame
   f->params = *p;
   fluid_build_k(f);
@@ -1352,6 +1362,7 @@ extern "C" int clay_fluid_substep(clay_fluid* fh, clay_stream stream) {
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_substep: NULL fluid");
   fluid_build_k(f);
   hipStream_t s = pick(f->ctx, stream);
+  f->last_stream = s;
   int rc;
   if (f->live > 0 && !f->presorted && (rc = fluid_sort_density(f, s, nullptr))) return rc;
   f->presorted = false;  // the forces pass moves the particles
@@ -1366,6 +1377,7 @@ extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_step: NULL fluid");
   fluid_build_k(f);
   hipStream_t s = pick(f->ctx, stream);
+  f->last_stream = s;
   if (f->use_graph && f->pending_emit == 0 && f->live > 0) {
     if (f->graph && f->graph_live != f->live) {
       CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
@@ -1381,7 +1393,7 @@ extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
 extern "C" int clay_fluid_set_graph(clay_fluid* fh, int32_t enabled) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_set_graph: NULL fluid");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);
   f->use_graph = enabled != 0;
   return CLAY_OK;
@@ -1391,7 +1403,7 @@ extern "C" int clay_fluid_read_contacts(clay_fluid* fh, float* out4, uint32_t* n
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_read_contacts: NULL fluid");
   uint32_t buf[4 + 4 * CLAY_CONTACTS];
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(buf, f->contacts, sizeof(buf), hipMemcpyDeviceToHost));
   const uint32_t cnt = std::min<uint32_t>(buf[0], CLAY_CONTACTS);
   if (out4) memcpy(out4, buf + 4, sizeof(float) * 4 * cnt);
@@ -1485,7 +1497,7 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
 extern "C" int clay_fluid_set_tuning(clay_fluid* fh, const clay_tuning* t) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f || !t) return fail(CLAY_ERR_INVALID, "clay_fluid_set_tuning: NULL argument");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);
   return apply_tuning(f, t);
 }
@@ -1581,8 +1593,7 @@ extern "C" int clay_batch_create(clay_ctx* ctx, clay_field* world, const clay_ba
 
 extern "C" int clay_batch_destroy(clay_batch* b) {
   Fluid* f = as_batch(b);
-  if (f) (void)hipStreamSynchronize(f->ctx->stream);
-  if (f && f->graph_stream) (void)hipStreamSynchronize(f->graph_stream);
+  if (f) (void)fluid_quiesce(f);
   fluid_free(f);
   return CLAY_OK;
 }
@@ -1591,8 +1602,7 @@ extern "C" int clay_batch_set_state(clay_batch* b, const float* pos, const float
   Fluid* f = as_batch(b);
   if (!f || !pos) return fail(CLAY_ERR_INVALID, "clay_batch_set_state: NULL argument");
   const uint32_t n = f->capacity;
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   float* dp[3] = {f->px, f->py, f->pz};
   float* dv[3] = {f->vx, f->vy, f->vz};
   for (int a = 0; a < 3; a++) {
@@ -1608,20 +1618,21 @@ extern "C" int clay_batch_set_state(clay_batch* b, const float* pos, const float
 
 extern "C" int clay_batch_get_state(clay_batch* b, int32_t what, float* host) {
   Fluid* f = as_batch(b);
-  if (f && f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  if (f) CLAY_HIP_CHECK(fluid_quiesce(f));
   return clay_fluid_get_state(reinterpret_cast<clay_fluid*>(b), what, host);
 }
 
 extern "C" int clay_batch_step(clay_batch* b, clay_stream stream) {
   Fluid* f = as_batch(b);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_step: NULL batch");
-  return batch_frame(f, pick(f->ctx, stream));
+  f->last_stream = pick(f->ctx, stream);
+  return batch_frame(f, f->last_stream);
 }
 
 extern "C" int clay_batch_set_graph(clay_batch* b, int32_t enabled) {
   Fluid* f = as_batch(b);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_set_graph: NULL batch");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);
   f->use_graph = enabled != 0;
   return CLAY_OK;
@@ -1630,7 +1641,7 @@ extern "C" int clay_batch_set_graph(clay_batch* b, int32_t enabled) {
 extern "C" int clay_batch_configure(clay_batch* b, const clay_fluid_params* p) {
   Fluid* f = as_batch(b);
   if (!f || !p) return fail(CLAY_ERR_INVALID, "clay_batch_configure: NULL argument");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);
   f->params = *p;
   fluid_build_k(f);
@@ -1641,7 +1652,7 @@ extern "C" int clay_batch_configure(clay_batch* b, const clay_fluid_params* p) {
 extern "C" int clay_batch_set_tuning(clay_batch* b, const clay_tuning* t) {
   Fluid* f = as_batch(b);
   if (!f || !t) return fail(CLAY_ERR_INVALID, "clay_batch_set_tuning: NULL argument");
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   drop_graph(f);
   return apply_tuning(f, t);
 }
@@ -1673,8 +1684,8 @@ extern "C" int clay_batch_read_contacts(clay_batch* b, uint32_t scene, float* ou
   if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_read_contacts: NULL batch");
   if (int rc = check_scene(f, scene, "clay_batch_read_contacts")) return rc;
   uint32_t buf[4 + 4 * CLAY_CONTACTS];
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(buf, f->contacts + (size_t)scene * (4 + 4 * CLAY_CONTACTS), sizeof(buf),
                            hipMemcpyDeviceToHost));
   const uint32_t cnt = std::min<uint32_t>(buf[0], CLAY_CONTACTS);
@@ -1693,8 +1704,7 @@ extern "C" int clay_batch_volume_download(clay_batch* b, uint32_t scene, int32_t
   const size_t r = (size_t)v->mip_res(mip), n = r * r * r;
   const size_t off = (size_t)scene * v->total_voxels + v->voxel_off[mip];
   std::vector<uint16_t> d(n), m(n);
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(d.data(), v->dist + off, n * 2, hipMemcpyDeviceToHost));
   CLAY_HIP_CHECK(hipMemcpy(m.data(), v->mat + off, n * 2, hipMemcpyDeviceToHost));
   const uint16_t one = f2h_host(1.0f);
@@ -1714,8 +1724,7 @@ extern "C" int clay_batch_grid_download(clay_batch* b, uint32_t scene, int32_t m
   Splat* sp = f->surface;
   if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_batch_grid_download: bad mip");
   const size_t r = (size_t)sp->vol->mip_res(mip), n = r * r * r;
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(host, sp->grid + (size_t)scene * sp->grid_total + sp->grid_off[mip], n * 4,
                            hipMemcpyDeviceToHost));
   return CLAY_OK;
@@ -1729,8 +1738,7 @@ extern "C" int clay_batch_tiles_download(clay_batch* b, uint32_t scene, int32_t 
   if (mip < 0 || mip >= sp->mips) return fail(CLAY_ERR_INVALID, "clay_batch_tiles_download: bad mip");
   const int tr = sp->vol->mip_res(mip) / kTile;
   const size_t nt = (size_t)tr * tr * tr;
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(host, sp->tile_live + (size_t)scene * sp->tile_total + sp->tile_off[mip], nt,
                            hipMemcpyDeviceToHost));
   return CLAY_OK;
@@ -1741,8 +1749,7 @@ extern "C" int clay_batch_origin(clay_batch* b, uint32_t scene, float origin[3])
   if (!f || !origin) return fail(CLAY_ERR_INVALID, "clay_batch_origin: NULL argument");
   if (int rc = check_scene(f, scene, "clay_batch_origin")) return rc;
   VolDev rec;
-  CLAY_HIP_CHECK(hipStreamSynchronize(f->ctx->stream));
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   CLAY_HIP_CHECK(hipMemcpy(&rec, f->surface->vol->dev + scene, sizeof(VolDev), hipMemcpyDeviceToHost));
   for (int a = 0; a < 3; a++) origin[a] = rec.origin[a];
   return CLAY_OK;
@@ -1753,7 +1760,7 @@ extern "C" int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]
   if (!f || !out) return fail(CLAY_ERR_INVALID, "clay_batch_conserved: NULL argument");
   if (int rc = check_scene(f, scene, "clay_batch_conserved")) return rc;
   hipStream_t s = f->ctx->stream;
-  if (f->graph_stream) CLAY_HIP_CHECK(hipStreamSynchronize(f->graph_stream));
+  CLAY_HIP_CHECK(fluid_quiesce(f));
   const size_t first = f->scene_base[scene];
   const uint32_t n = f->scene_base[scene + 1] - f->scene_base[scene];
   const float* pv[6] = {f->px + first, f->py + first, f->pz + first, f->vx + first, f->vy + first, f->vz + first};

@@ -224,6 +224,7 @@ struct Fluid {
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;  // the stream the graph was captured on
   uint32_t graph_live = 0;             // a FluidSim's live count when its graph was captured
+  hipStream_t last_stream = nullptr;   // the stream of the latest step or substep
   // The bake's binning runs beside the frame-end density pass: forked after the
   // sort onto `side`, joined before the walk (default priority).
   hipStream_t side = nullptr;
@@ -303,8 +304,12 @@ int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, 
                      double* partial, uint32_t partial_blocks, double* out);
 // sort.hip
 size_t radix_hist_size(uint32_t n);
-int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
-                     uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist);
+// Stable sort of n (key, value) pairs of `key_bits`-bit keys; the result always
+// lands in (keys, vals).  The input must be written to (keys_tmp, vals_tmp) when
+// radix_input_in_tmp(key_bits), else to (keys, vals).
+bool radix_input_in_tmp(int key_bits);
+int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t* keys, uint32_t* vals,
+                     uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist);
 // splat.hip
 int splat_bake(Splat* sp, hipStream_t s);
 // `order`: visiting order (codes do not depend on it).  `cover`: per particle id,

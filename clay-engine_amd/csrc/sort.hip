@@ -188,49 +188,58 @@ size_t radix_hist_size(uint32_t n) {
 }
 
 namespace {
+int digit_passes(int key_bits) { return (key_bits + kMaxDigitBits - 1) / kMaxDigitBits; }
+
+// Ping-pongs between the two buffer pairs, starting from the one holding the input
+// (the tmp pair for an odd number of passes), so that the last pass writes `keys`/`vals`.
 template <int R>
-void sort_passes(Ctx* c, hipStream_t s, uint32_t n, int passes, int w, uint32_t*& keys, uint32_t*& vals,
-                 uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
+void sort_passes(Ctx* c, hipStream_t s, uint32_t n, int passes, int w, uint32_t* keys, uint32_t* vals,
+                 uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist) {
   const uint32_t tile = (uint32_t)kThreads * R;
   const uint32_t ntiles = (n + tile - 1) / tile;
   uint32_t* totals = hist + ((size_t)1 << w) * ntiles;
+  const bool odd = passes & 1;
+  uint32_t* ik = odd ? keys_tmp : keys;
+  uint32_t* iv = odd ? vals_tmp : vals;
+  uint32_t* ok = odd ? keys : keys_tmp;
+  uint32_t* ov = odd ? vals : vals_tmp;
   for (int p = 0; p < passes; p++) {
     const int shift = w * p;
     switch (w) {
       case 8:
-        sort_pass<8, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<8, R>(c, s, n, ntiles, shift, ik, iv, hist, totals, ok, ov);
         break;
       case 9:
-        sort_pass<9, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<9, R>(c, s, n, ntiles, shift, ik, iv, hist, totals, ok, ov);
         break;
       case 10:
-        sort_pass<10, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<10, R>(c, s, n, ntiles, shift, ik, iv, hist, totals, ok, ov);
         break;
       default:
-        sort_pass<11, R>(c, s, n, ntiles, shift, keys, vals, hist, totals, keys_alt, vals_alt);
+        sort_pass<11, R>(c, s, n, ntiles, shift, ik, iv, hist, totals, ok, ov);
         break;
     }
-    uint32_t* tk = keys;
-    keys = keys_alt;
-    keys_alt = tk;
-    uint32_t* tv = vals;
-    vals = vals_alt;
-    vals_alt = tv;
+    std::swap(ik, ok);
+    std::swap(iv, ov);
   }
 }
 }  // namespace
 
+bool radix_input_in_tmp(int key_bits) { return digit_passes(key_bits) & 1; }
+
 // Fewest passes of at most 11-bit digits, the digit width spread evenly: 24-bit keys
 // take three 8-bit passes, 27-bit keys three 9-bit passes, 32-bit keys three 11-bit.
-int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t*& keys,
-                     uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* hist) {
+// The buffers never change roles, so a captured graph of a sort stays valid however
+// many sorts run between its replays.
+int radix_sort_pairs(Ctx* c, hipStream_t s, uint32_t n, int key_bits, uint32_t* keys, uint32_t* vals,
+                     uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* hist) {
   if (n == 0) return CLAY_OK;
-  const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const int passes = digit_passes(key_bits);
   const int w = std::max(8, (key_bits + passes - 1) / passes);
   if (n >= kBigSort)
-    sort_passes<kMaxRounds>(c, s, n, passes, w, keys, vals, keys_alt, vals_alt, hist);
+    sort_passes<kMaxRounds>(c, s, n, passes, w, keys, vals, keys_tmp, vals_tmp, hist);
   else
-    sort_passes<kMinRounds>(c, s, n, passes, w, keys, vals, keys_alt, vals_alt, hist);
+    sort_passes<kMinRounds>(c, s, n, passes, w, keys, vals, keys_tmp, vals_tmp, hist);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
