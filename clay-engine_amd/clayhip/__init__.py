@@ -12,7 +12,7 @@ from .edit import SdfEditor, body_centroids, body_centroids_device
 from .field import TracedField, analyticField, lerpField, offsetField, unionField, volumeField
 from .fluid import FluidSim, fluid_desc
 from .pbd import ClaySolver, SurfaceExtractor
-from .splat import ParticleCloud, SplatField
+from .splat import ParticleCloud, SplatField, bake_static
 from .volume import Root, SdfVolume, derive
 
 __all__ = [
@@ -29,6 +29,7 @@ __all__ = [
     "SplatField",
     "TracedField",
     "analyticField",
+    "bake_static",
     "body_centroids",
     "body_centroids_device",
     "compile_brushes",

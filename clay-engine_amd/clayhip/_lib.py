@@ -321,6 +321,7 @@ SIGNATURES = {
     "clay_splat_destroy": ([P], C.c_int),
     "clay_splat_set_live": ([P, U32, U32], C.c_int),
     "clay_splat_bake": ([P, P], C.c_int),
+    "clay_splat_bake_static": ([P, P, P, P, P], C.c_int),
     "clay_splat_set_center": ([P, DP, P], C.c_int),
     "clay_splat_volume": ([P, PP], C.c_int),
     "clay_splat_field": ([P, PP], C.c_int),

@@ -51,6 +51,42 @@ class ParticleCloud:
         self.desc = d
 
 
+def splat_desc(opts: dict) -> L.SplatDesc:
+    """SplatFieldOptions (splat.ts:80-106) as the C descriptor."""
+    if "radius" not in opts:
+        raise ValueError("SplatField: `radius` is required")
+    d = L.SplatDesc()
+    d.radius = float(opts["radius"])
+    d.material = float(opts.get("material", 0))
+    d.resolution = int(opts.get("resolution", 64))
+    d.mip_levels = int(opts.get("mipLevels", 0) or 0)
+    d.band = float(opts.get("band", 2))
+    origin = opts.get("origin", (-12, -4, -12))
+    for i in range(3):
+        d.origin[i] = float(origin[i])
+    d.world_size = float(opts.get("worldSize", 24))
+    mode = opts.get("mode", "plain")
+    d.mode = {"plain": L.SPLAT_PLAIN, "tiled": L.SPLAT_TILED}[mode]
+    d.keep_codes = 1 if opts.get("keepCodes", False) else 0
+    return d
+
+
+def bake_static(root: Root, cloud: ParticleCloud, options: dict, dist, stream=None) -> None:
+    """The plain bake of `cloud` with no allocation (clay_splat_bake_static): writes
+    every mip's f16 distance plane, concatenated mip-major and x fastest, into `dist`,
+    a contiguous CUDA int16/uint16 tensor of sum_m (res >> m)^3 elements."""
+    d = splat_desc(dict(options, mode="plain"))
+    res = d.resolution if d.resolution > 0 else 64
+    mips = d.mip_levels if d.mip_levels > 0 else max(1, int(np.log2(res / 8)))
+    need = sum((res >> m) ** 3 for m in range(mips))
+    if not getattr(dist, "is_cuda", False) or dist.element_size() != 2 or not dist.is_contiguous():
+        raise ValueError("bake_static: dist must be a contiguous 16-bit CUDA tensor")
+    if dist.numel() < need:
+        raise ValueError(f"bake_static: dist holds {dist.numel()} texels, the bake writes {need}")
+    L.check(L.lib().clay_splat_bake_static(root.handle, C.byref(cloud.desc), C.byref(d), C.c_void_p(int(dist.data_ptr())),
+                                           L.stream_handle(stream)))
+
+
 class SplatField:
     """Turns a ParticleCloud into a mip-mapped volume by a fixed-point atomicMin
     union of spheres (splat.ts:136-309).  `mode` 'plain' keeps the reference's
@@ -69,22 +105,7 @@ class SplatField:
             self._h = _handle
             self._owned = False
         else:
-            opts = dict(options or {}, **kw)
-            if "radius" not in opts:
-                raise ValueError("SplatField: `radius` is required")
-            d = L.SplatDesc()
-            d.radius = float(opts["radius"])
-            d.material = float(opts.get("material", 0))
-            d.resolution = int(opts.get("resolution", 64))
-            d.mip_levels = int(opts.get("mipLevels", 0) or 0)
-            d.band = float(opts.get("band", 2))
-            origin = opts.get("origin", (-12, -4, -12))
-            for i in range(3):
-                d.origin[i] = float(origin[i])
-            d.world_size = float(opts.get("worldSize", 24))
-            mode = opts.get("mode", "plain")
-            d.mode = {"plain": L.SPLAT_PLAIN, "tiled": L.SPLAT_TILED}[mode]
-            d.keep_codes = 1 if opts.get("keepCodes", False) else 0
+            d = splat_desc(dict(options or {}, **kw))
             h = C.c_void_p()
             if scenes is None:
                 L.check(L.lib().clay_splat_create(root.handle, C.byref(cloud.desc), C.byref(d), C.byref(h)))

@@ -294,6 +294,11 @@ extern "C" int clay_ctx_create(int device, clay_ctx** out) {
     delete c;
     return fail(CLAY_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
+  if (const int rc = splat_static_init(device)) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = reinterpret_cast<clay_ctx*>(c);
   return CLAY_OK;
 }
@@ -778,6 +783,13 @@ extern "C" int clay_splat_bake(clay_splat* s, clay_stream stream) {
   Splat* sp = reinterpret_cast<Splat*>(s);
   if (!sp) return fail(CLAY_ERR_INVALID, "clay_splat_bake: NULL splat");
   return splat_bake(sp, pick(sp->ctx, stream));
+}
+
+extern "C" int clay_splat_bake_static(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                                      uint16_t* dist_dev, clay_stream stream) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !cloud || !desc || !dist_dev) return fail(CLAY_ERR_INVALID, "clay_splat_bake_static: NULL argument");
+  return splat_bake_static(c, cloud, desc, dist_dev, pick(c, stream));
 }
 
 extern "C" int clay_splat_set_center(clay_splat* s, const double c[3], clay_stream stream) {

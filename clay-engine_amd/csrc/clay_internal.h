@@ -325,6 +325,12 @@ int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
                          std::vector<size_t>* tile_off, size_t* tile_total);
 size_t splat_max_bricks_per_particle(double radius, double band, double voxel0, int mips);
 size_t splat_bin_slices();
+// The static scratch of clay_splat_bake_static: looks up this device's copy
+// (the device must be current), once per device.
+int splat_static_init(int device);
+size_t splat_static_bytes();
+int splat_bake_static(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, uint16_t* dist,
+                      hipStream_t s);
 // volume.hip
 int launch_bake_brushes(Ctx* c, hipStream_t s, Volume* v, const clay_brush* brushes_dev, int n);
 int launch_field_eval(Ctx* c, hipStream_t s, const ColliderDev& col, int n, const float* pts,

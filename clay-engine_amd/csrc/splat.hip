@@ -46,6 +46,8 @@
 // scene-major (scene * bricks-per-scene + brick), a particle's scene comes from a
 // per-particle table, and each scene has its own origin record, code grid block,
 // texel planes and tile mask.  A one-volume splat is the batch of one scene.
+#include <mutex>
+
 #include "clay_internal.h"
 
 namespace clay {
@@ -1141,6 +1143,154 @@ int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res, std::ve
 }
 
 size_t splat_bin_slices() { return kSlices; }
+
+// ---------------------------------------------------------------------------
+// A bake that allocates nothing (clay_splat_bake_static)
+// ---------------------------------------------------------------------------
+// The plain schedule's working memory (the origin record and the dense u32 code
+// grid of every mip) lives in one file-scope device array.  Each device holds its
+// own copy for the life of the process.  Host code never passes `scratch` itself
+// to a kernel: it passes the address hipGetSymbolAddress returned for the current
+// device, looked up once per device when a context is created.
+constexpr size_t kStaticScratchBytes = (size_t)32 << 20;
+constexpr uint32_t kStaticHeadWords = 64;  // the origin record, padded to 256 bytes
+static_assert(sizeof(VolDev) <= 4 * kStaticHeadWords, "the origin record must fit the scratch head");
+static __device__ __align__(256) uint32_t scratch[kStaticScratchBytes / 4];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+uint32_t* g_static_ptr[kMaxDevices] = {};
+
+__global__ void k_put_voldev(VolDev* __restrict__ dst, VolDev v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_fill_codes(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) p[i] = v;
+}
+}  // namespace
+
+int splat_static_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(CLAY_ERR_INVALID, "clay_splat_bake_static: device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (g_static_ptr[device]) return CLAY_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(CLAY_ERR_HIP, std::string("clay_splat_bake_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+  g_static_ptr[device] = static_cast<uint32_t*>(p);
+  return CLAY_OK;
+}
+
+size_t splat_static_bytes() { return kStaticScratchBytes; }
+
+int splat_bake_static(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, uint16_t* dist,
+                      hipStream_t s) {
+  if (!(desc->radius > 0)) return fail(CLAY_ERR_INVALID, "SplatField: radius must be positive");
+  if (desc->mode != CLAY_SPLAT_PLAIN)
+    return fail(CLAY_ERR_INVALID, "clay_splat_bake_static: only the plain schedule runs without allocation");
+  if (cloud->live_rule != CLAY_LIVE_ALL && cloud->live_rule != CLAY_LIVE_PREFIX && cloud->live_rule != CLAY_LIVE_RANGE)
+    return fail(CLAY_ERR_INVALID, "ParticleCloud: unknown live rule");
+  if (cloud->capacity > 0 && (!cloud->pos[0] || !cloud->pos[1] || !cloud->pos[2]))
+    return fail(CLAY_ERR_INVALID, "ParticleCloud: positions are required");
+  const bool nrm = cloud->nrm[0] != nullptr;
+  if (nrm && (!cloud->nrm[1] || !cloud->nrm[2]))
+    return fail(CLAY_ERR_INVALID, "ParticleCloud: normals must give all three axes or none");
+  // Defaults and geometry as clay_splat_create derives them (splat.ts:147-160).
+  const int res = desc->resolution > 0 ? desc->resolution : 64;
+  const int mips_req = desc->mip_levels > 0 ? desc->mip_levels : std::max(1, (int)log2((double)res / kTile));
+  clay_volume_info info;
+  int rc = clay_volume_derive(res, mips_req, &info);
+  if (rc) return rc;
+  const double band = desc->band > 0 ? desc->band : 2.0;
+  const double world = desc->world_size > 0 ? desc->world_size : 24.0;
+  size_t cells = 0;
+  for (int m = 0; m < info.mip_levels; m++) {
+    const size_t r = (size_t)(res >> m);
+    cells += r * r * r;
+  }
+  const size_t need = 4 * (kStaticHeadWords + cells);
+  if (need > kStaticScratchBytes) {
+    char buf[200];
+    snprintf(buf, sizeof(buf),
+             "clay_splat_bake_static: a %d^3 bake in %d mip levels needs %zu bytes of scratch; the static "
+             "scratch holds %zu",
+             res, info.mip_levels, need, kStaticScratchBytes);
+    return fail(CLAY_ERR_INVALID, buf);
+  }
+  uint32_t* base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_static_ptr[ctx->device] : nullptr;
+  if (!base) return fail(CLAY_ERR_STATE, "clay_splat_bake_static: the static scratch of this device is not set up");
+  VolDev* vdev = reinterpret_cast<VolDev*>(base);
+  uint32_t* grid = base + kStaticHeadWords;
+  // The origin record the kernels read (volume.ts:221-230), as SdfVolume writes it.
+  VolDev vh{};
+  const double half = world / 2;
+  for (int a = 0; a < 3; a++) {
+    vh.origin[a] = (float)desc->origin[a];
+    vh.center[a] = (float)(desc->origin[a] + half);
+  }
+  vh.voxel = (float)(world / res);
+  vh.extent = (float)world;
+  vh.half = (float)half;
+  vh.band = (float)band;
+  vh.grad_h = vh.voxel;
+  vh.res = res;
+  MipsK mk_{};
+  mk_.mips = info.mip_levels;
+  mk_.inv_enc = (float)(1.0 / 32767.5);
+  const double voxel0 = world / res;
+  size_t off = 0;
+  for (int m = 0; m < info.mip_levels; m++) {
+    SplatMipK& k = mk_.m[m];
+    k.r = res >> m;
+    k.off = (uint32_t)off;
+    k.voff = (uint32_t)off;
+    const double voxel = voxel0 * (double)(1 << m);
+    const double bw = band * voxel;
+    k.voxel = (float)voxel;
+    k.bw = (float)bw;
+    k.reach = (float)(desc->radius + bw);
+    k.inv = (float)(1.0 / voxel);
+    k.radius = (float)desc->radius;
+    const double cr = (double)k.reach * 1.001 + 1e-6;
+    k.cull2 = (float)(cr * cr);
+    off += (size_t)k.r * k.r * k.r;
+  }
+  CloudK c{};
+  for (int a = 0; a < 3; a++) {
+    c.pos[a] = cloud->pos[a];
+    c.nrm[a] = cloud->nrm[a];
+  }
+  uint32_t first = 0, count = cloud->capacity;
+  if (cloud->live_rule == CLAY_LIVE_PREFIX) {
+    count = cloud->live_count;
+  } else if (cloud->live_rule == CLAY_LIVE_RANGE) {
+    first = cloud->live_first;
+    count = cloud->live_count;
+  }
+  if (first > cloud->capacity) first = cloud->capacity;
+  if (count > cloud->capacity - first) count = cloud->capacity - first;
+  c.first = first;
+  c.end = first + count;
+  c.count = count;
+  // Every call rewrites the record and refills the whole grid: the array keeps the
+  // previous call's contents.
+  hipLaunchKernelGGL(k_put_voldev, dim3(1), dim3(64), 0, s, vdev, vh);
+  const unsigned fill_blocks = (unsigned)std::min<size_t>((cells + kThreads - 1) / kThreads, 16384);
+  hipLaunchKernelGGL(k_fill_codes, dim3(fill_blocks), dim3(kThreads), 0, s, grid, (uint32_t)cells, kEncSat);
+  const unsigned pgrid = (c.count + kThreads - 1) / kThreads;
+  for (int m = 0; m < info.mip_levels; m++) {
+    if (c.count)
+      hipLaunchKernelGGL(k_splat_plain, dim3(pgrid), dim3(kThreads), 0, s, mk_.m[m], c, vdev, grid);
+    const uint32_t mc = (uint32_t)mk_.m[m].r * mk_.m[m].r * mk_.m[m].r;
+    const unsigned rg = std::min<unsigned>((mc + kThreads - 1) / kThreads, 16384u);
+    hipLaunchKernelGGL(k_resolve_plain, dim3(rg), dim3(kThreads), 0, s, mk_.m[m], mk_.inv_enc, grid, dist);
+  }
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
 
 size_t splat_max_bricks_per_particle(double radius, double band, double voxel0, int mips) {
   size_t worst = 1;

@@ -271,6 +271,22 @@ int clay_splat_grid_download(clay_splat* s, int32_t mip, uint32_t* codes_host);
 /* Exact live-tile mask of one mip after the last bake: 1 byte per 8^3 tile. */
 int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* live_host);
 
+/* A bake that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It is clay_splat_bake with
+ * the plain schedule (desc->mode must be CLAY_SPLAT_PLAIN) of a splat created
+ * from the same cloud and desc, but nothing is created: the caller passes the
+ * cloud (device SoA) and the output, the f16 distance plane (.r of the
+ * reference's rgba16float texels, distance / band) of every mip concatenated,
+ * mip-major and x fastest: sum over m of (res >> m)^3 uint16_t in device memory.
+ * The dense u32 code grid and the origin record live in a static device array
+ * of 32 MiB per GPU, held until the process ends; a bake needing more (above
+ * 192^3 in 3 mips, for example) is refused with CLAY_ERR_INVALID.  Enqueued only.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two threads) race, and must be ordered by the caller. */
+int clay_splat_bake_static(clay_ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc,
+                           uint16_t* dist_dev, clay_stream stream);
+
 /* ---- FluidSim (lib/src/sim/fluid.ts:111-582) --------------------------- */
 typedef struct {
   uint32_t capacity;        /* 8192 */
