@@ -399,6 +399,37 @@ class Fluid:
             C.byref(self.k), col, self.n, C.byref(st), C.byref(self.contacts),
             self.keys.ctypes.data_as(C.c_void_p), self.perm.ctypes.data_as(C.c_void_p))
 
+    def density(self) -> None:
+        """The density pass alone (fluid.ts:333-368) over the current positions."""
+        st = self._state()
+        getattr(lib(), f"orc_density_{self.precision}")(C.byref(self.k), self.n, C.byref(st))
+
+    def forces(self) -> None:
+        """The force pass alone (fluid.ts:370-439): writes nvel from the current
+        positions, velocities, densities and pressures."""
+        st = self._state()
+        getattr(lib(), f"orc_forces_{self.precision}")(C.byref(self.k), self.n, C.byref(st))
+
+    def integrate(self) -> None:
+        """The integrate pass alone (fluid.ts:444-491): advances by nvel, collides,
+        records contacts and clamps to the walls."""
+        st = self._state()
+        col = C.byref(self.collider.c) if self.collider else None
+        self.contacts = OrcContacts()
+        getattr(lib(), f"orc_integrate_{self.precision}")(C.byref(self.k), col, self.n, C.byref(st),
+                                                          C.byref(self.contacts))
+
+    def set(self, what: str, values: np.ndarray) -> None:
+        """Overwrites the first n entries of a field: 'pos', 'vel', 'nvel' (n, 3) or
+        'density', 'pressure' (n,)."""
+        n = self.n
+        if what in ("pos", "vel", "nvel"):
+            pre = {"pos": "p", "vel": "v", "nvel": "nv"}[what]
+            for i, ax in enumerate("xyz"):
+                self.a[pre + ax][:n] = values[:, i]
+        else:
+            self.a[{"density": "rho", "pressure": "pres"}[what]][:n] = values
+
     def spawn(self, count: int, pos, vel, spread: float | None = None) -> None:
         """FluidSim.spawn (fluid.ts:511-525)."""
         spacing = self.options.get("spacing", DEFAULTS["spacing"])
