@@ -735,7 +735,7 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
       return fail(CLAY_ERR_HIP, "SplatField: tile mask initialisation failed");
     }
   }
-  if (sc.follow && (rc = dalloc(&sp->follow_slots, 6 * S))) {
+  if (sc.follow && (rc = dalloc(&sp->follow_slots, (size_t)kBoundsStride * S))) {
     splat_free(sp);
     return rc;
   }
@@ -941,7 +941,7 @@ hipError_t fluid_quiesce(const Fluid* f) {
 // table and reordered state, then densities and pressures.  With `cover`, the
 // density pass also writes each particle's interior radius for the surface bake.
 // The cell sort of a substep: stamp, keys, sort, reorder into the sorted planes.
-int fluid_sort(Fluid* f, hipStream_t s) {
+int fluid_sort(Fluid* f, hipStream_t s, uint32_t* bounds = nullptr) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
@@ -961,7 +961,7 @@ int fluid_sort(Fluid* f, hipStream_t s) {
   const SceneK sk = f->scene_k();
   const bool in_tmp = radix_input_in_tmp(f->key_bits);
   if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, in_tmp ? f->keys2 : f->keys,
-                        in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift)))
+                        in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift, bounds)))
     return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
@@ -1044,11 +1044,13 @@ int fluid_frame_impl(Fluid* f, hipStream_t s) {
     // reads positions and the sort's permutation but not the interior radii.  The
     // permutation is a spatially coherent visiting order for the splat; the codes
     // do not depend on it.
-    if ((rc = fluid_sort(f, s))) return rc;
+    // The keys pass of this sort also folds the bounds a following surface reads.
+    uint32_t* bounds = splat_bounds_begin(f->surface, s);
+    if ((rc = fluid_sort(f, s, bounds))) return rc;
     f->surface->cloud.live_count = f->live;
     CLAY_HIP_CHECK(hipEventRecord(f->ev_fork, s));
     CLAY_HIP_CHECK(hipStreamWaitEvent(f->side, f->ev_fork, 0));
-    if ((rc = splat_bake_prepare(f->surface, f->side, f->vals))) return rc;
+    if ((rc = splat_bake_prepare(f->surface, f->side, f->vals, bounds != nullptr))) return rc;
     if ((rc = fluid_density(f, s, f->cover))) return rc;
     CLAY_HIP_CHECK(hipEventRecord(f->ev_join, f->side));
     CLAY_HIP_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));

@@ -112,6 +112,19 @@ struct FluidK {
 // ---------------------------------------------------------------------------
 // Scalar helpers
 // ---------------------------------------------------------------------------
+// Words between two scenes' bound slots (six used): one 128-byte line per scene,
+// so the atomics of different scenes do not queue on one cache line.
+constexpr uint32_t kBoundsStride = 32;
+
+// An order-preserving u32 encoding of a float (and back): integer min/max of the
+// encodings is the float min/max, so atomics fold bounds exactly.
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
 __device__ __forceinline__ float fmin_s(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmax_s(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float fclamp_s(float x, float lo, float hi) {

@@ -272,9 +272,10 @@ int launch_contacts_to_edits(Ctx* c, hipStream_t s, Editor* e, const uint32_t* c
 int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, const uint32_t* first_count,
                           const float* const* pos, const float* const* rest, double* out);
 // fluid.hip
+// With `bounds`, also folds each scene's particle bounds into its six slots.
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
-                uint32_t shift);
+                uint32_t shift, uint32_t* bounds = nullptr);
 // With `spos` (FP16-LDS), also each sorted particle's staged position.
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
@@ -318,7 +319,12 @@ int splat_bake(Splat* sp, hipStream_t s);
 int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
 // splat_bake_ordered in two parts: the binning (positions and order only), then the
 // walk and the resolve (which read `cover`).
-int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order);
+// `bounds_given`: the slots splat_bounds_begin returned were filled with this
+// bake's particle bounds (by the frame-end k_keys), so the follow reads them.
+int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order, bool bounds_given);
+// Resets and returns the per-scene bounds slots of a splat that follows its
+// particles' bounds (else null), for a producer to fill before the bake.
+uint32_t* splat_bounds_begin(Splat* sp, hipStream_t s);
 int splat_bake_finish(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
 int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,
                          std::vector<size_t>* brick_off, size_t* brick_total,

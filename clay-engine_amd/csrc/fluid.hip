@@ -109,22 +109,96 @@ __device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __r
 // grid.  A batch: (scene << shift) | cell, with all `shift` cell bits set outside
 // the grid, so a scene's outside particles sort after its cells; `shift` is the
 // fewest bits that leave that value free, so the keys stay short for the sort.
+//
+// With `bounds` (the frame-end sort of a fluid whose surface follows its
+// particles), the same position reads also fold each scene's bounds into its six
+// slots (min x, y, z, max x, y, z as ordered bits), so the bake needs no second
+// pass over the positions.  A workgroup inside one scene (the usual case: ids run
+// scene-major and scenes hold >= 1024 particles) reduces to one set of six
+// atomics.  Otherwise the wave's lanes are grouped by scene and each group takes a
+// butterfly min/max, its first lane doing the atomics.
 __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const float* __restrict__ px,
                                                  const float* __restrict__ py,
                                                  const float* __restrict__ pz,
                                                  const uint32_t* __restrict__ scene_of,
                                                  uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals, uint32_t invalid, uint32_t shift) {
+                                                 uint32_t* __restrict__ vals, uint32_t invalid, uint32_t shift,
+                                                 uint32_t* __restrict__ bounds) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  int32_t q[3];
-  cell_of(k, mk(px[i], py[i], pz[i]), q);
-  const bool in = in_grid(k, q[0], q[1], q[2]);
-  if (scene_of)
-    keys[i] = (scene_of[i] << shift) | (in ? cell_index(k, q[0], q[1], q[2]) : (1u << shift) - 1u);
-  else
-    keys[i] = in ? cell_index(k, q[0], q[1], q[2]) : invalid;
-  vals[i] = i;
+  const bool active = i < n;
+  f3 p = mk(0.0f, 0.0f, 0.0f);
+  uint32_t sc = 0;
+  if (active) {
+    p = mk(px[i], py[i], pz[i]);
+    if (scene_of) sc = scene_of[i];
+    int32_t q[3];
+    cell_of(k, p, q);
+    const bool in = in_grid(k, q[0], q[1], q[2]);
+    if (scene_of)
+      keys[i] = (sc << shift) | (in ? cell_index(k, q[0], q[1], q[2]) : (1u << shift) - 1u);
+    else
+      keys[i] = in ? cell_index(k, q[0], q[1], q[2]) : invalid;
+    vals[i] = i;
+  }
+  if (!bounds) return;  // uniform over the launch
+  const float inf = __uint_as_float(0x7f800000u);
+  const int lane = (int)__lane_id();
+  const uint32_t first = blockIdx.x * kBlock;  // < n
+  const uint32_t last = first + kBlock - 1 < n ? first + kBlock - 1 : n - 1;
+  if (!scene_of || scene_of[first] == scene_of[last]) {  // uniform over the workgroup
+    __shared__ float part[kBlock / 64][6];
+    float a[6] = {active ? p.x : inf, active ? p.y : inf, active ? p.z : inf,
+                  active ? p.x : -inf, active ? p.y : -inf, active ? p.z : -inf};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        a[q] = fmin_s(a[q], __shfl_xor(a[q], off));
+        a[q + 3] = fmax_s(a[q + 3], __shfl_xor(a[q + 3], off));
+      }
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 6; q++) part[threadIdx.x >> 6][q] = a[q];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int q = threadIdx.x;
+      float v = part[0][q];
+      for (int w = 1; w < kBlock / 64; w++) v = q < 3 ? fmin_s(v, part[w][q]) : fmax_s(v, part[w][q]);
+      uint32_t* slot = bounds + (size_t)kBoundsStride * (scene_of ? scene_of[first] : 0u) + q;
+      if (q < 3)
+        atomicMin(slot, ordered_bits(v));
+      else
+        atomicMax(slot, ordered_bits(v));
+    }
+    return;
+  }
+  uint64_t pending = __ballot(active);
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)sc, leader);
+    const uint64_t same = __ballot(active && sc == lead);
+    const bool mine = (same >> lane) & 1ull;
+    float a[6] = {mine ? p.x : inf, mine ? p.y : inf, mine ? p.z : inf,
+                  mine ? p.x : -inf, mine ? p.y : -inf, mine ? p.z : -inf};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        a[q] = fmin_s(a[q], __shfl_xor(a[q], off));
+        a[q + 3] = fmax_s(a[q + 3], __shfl_xor(a[q + 3], off));
+      }
+    }
+    if (lane == leader) {
+      uint32_t* slot = bounds + (size_t)kBoundsStride * lead;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        atomicMin(slot + q, ordered_bits(a[q]));
+        atomicMax(slot + 3 + q, ordered_bits(a[q + 3]));
+      }
+    }
+    pending &= ~same;
+  }
 }
 
 struct Planes8 {
@@ -1110,11 +1184,11 @@ bool fluid_tune_valid(const FluidTune& t) {
 
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
-                uint32_t shift) {
+                uint32_t shift, uint32_t* bounds) {
   if (n == 0) return CLAY_OK;
   LaunchTimer t(c, s, "fluid_keys");
   hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, px, py, pz, scene_of, keys, vals,
-                     invalid_key, shift);
+                     invalid_key, shift, bounds);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

@@ -839,13 +839,6 @@ __global__ void __launch_bounds__(kThreads)
 // grid: every workgroup reduces a slice of one scene (blockIdx.y) in LDS, then
 // folds it into the scene's six slots with integer atomics on order-preserving
 // encodings of the floats (exact, so the result does not depend on timing).
-__device__ __forceinline__ uint32_t ordered_bits(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float from_ordered(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
-}
 
 __global__ void __launch_bounds__(kThreads)
     k_bounds_partial(const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
@@ -874,14 +867,14 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
   }
-  if (t < 3) atomicMin(&slots[6 * sc + t], ordered_bits(sh[t][0]));
-  else if (t < 6) atomicMax(&slots[6 * sc + t], ordered_bits(sh[t][0]));
+  if (t < 3) atomicMin(&slots[kBoundsStride * sc + t], ordered_bits(sh[t][0]));
+  else if (t < 6) atomicMax(&slots[kBoundsStride * sc + t], ordered_bits(sh[t][0]));
 }
 
 // Slots start at (max, max, max, 0, 0, 0) encoded: an empty scene keeps its origin.
 __global__ void k_bounds_init(uint32_t* __restrict__ slots, uint32_t scenes) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 6 * scenes) slots[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+  if (i < 6 * scenes) slots[kBoundsStride * (i / 6) + i % 6] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
 }
 
 __global__ void k_bounds_follow(const uint32_t* __restrict__ slots, const uint32_t* __restrict__ base, uint32_t n,
@@ -891,7 +884,8 @@ __global__ void k_bounds_follow(const uint32_t* __restrict__ slots, const uint32
   const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
   if (lo >= hi) return;
   for (int q = 0; q < 3; q++) {
-    const float c = (from_ordered(slots[6 * sc + q]) + from_ordered(slots[6 * sc + 3 + q])) * 0.5f;
+    const float c =
+        (from_ordered(slots[kBoundsStride * sc + q]) + from_ordered(slots[kBoundsStride * sc + 3 + q])) * 0.5f;
     const double o = (double)c - half;
     vols[sc].origin[q] = (float)o;
     vols[sc].center[q] = (float)(o + half);
@@ -1057,7 +1051,14 @@ static int tiled_finish(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK
   return CLAY_OK;
 }
 
-int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order) {
+uint32_t* splat_bounds_begin(Splat* sp, hipStream_t s) {
+  if (!sp->follow || sp->follow_centers || !sp->follow_slots) return nullptr;
+  const unsigned S = (unsigned)sp->scenes;
+  hipLaunchKernelGGL(k_bounds_init, dim3((6 * S + 255) / 256), dim3(256), 0, s, sp->follow_slots, S);
+  return hipGetLastError() == hipSuccess ? sp->follow_slots : nullptr;
+}
+
+int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order, bool bounds_given) {
   Ctx* ctx = sp->ctx;
   MipsK mk_{};
   fill_mips(sp, &mk_);
@@ -1065,7 +1066,11 @@ int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order) {
   if (sp->follow) {
     LaunchTimer t(ctx, s, "splat_follow");
     const double half = sp->vol->info.world_size / 2;
-    if (sp->follow_centers) {
+    if (!sp->follow_centers && bounds_given) {
+      const unsigned S = (unsigned)sp->scenes;
+      hipLaunchKernelGGL(k_bounds_follow, dim3((S + 63) / 64), dim3(64), 0, s, sp->follow_slots, sp->scene_base,
+                         c.end, sp->vol->dev, half, S);
+    } else if (sp->follow_centers) {
       hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
                          sp->scene_base, c.end, sp->follow_centers, sp->vol->dev, half);
     } else {
@@ -1118,7 +1123,7 @@ int splat_bake_finish(Splat* sp, hipStream_t s, const uint32_t* order, const flo
 }
 
 int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover) {
-  const int rc = splat_bake_prepare(sp, s, order);
+  const int rc = splat_bake_prepare(sp, s, order, false);
   return rc ? rc : splat_bake_finish(sp, s, order, cover);
 }
 
