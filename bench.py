@@ -74,8 +74,31 @@ def committed_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
+# VALU issue: a gfx950 SIMD is 16 lanes wide, so a wave64 VALU instruction takes 4
+# cycles and a CU's 4 SIMDs together issue one wave instruction per cycle.
+CUS = 256
+CLOCK_HZ = 2.4e9
+VALU_PEAK = CUS * CLOCK_HZ  # wave64 VALU instructions per second
+
+
+def committed_valu(kernel: str, workload: str) -> float | None:
+    """SQ_INSTS_VALU per launch of `kernel` (wave instructions) from the committed
+    rocprofv3 counter summary of this workload, or None."""
+    path = ROOT / "profiles" / f"counters_{workload.replace('-', '_')}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "valu_insts_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def roofline_of(stats: dict, particles: int, mips: int, live_voxels: int, workload: str) -> dict | None:
-    """Roofline object of the dominant kernel of a timed pass."""
+    """Roofline object of the dominant kernel of a timed pass: its algorithmic HBM
+    bytes per launch over its average launch time against the HBM peak, and beside
+    it the fraction of the VALU issue rate its counted instructions take,
+    SQ_INSTS_VALU * 4 cycles / (CUs * 4 SIMDs * clock * time)."""
     if not stats:
         return None
     name, (tot_ms, launches) = max(stats.items(), key=lambda kv: kv[1][0])
@@ -84,9 +107,14 @@ def roofline_of(stats: dict, particles: int, mips: int, live_voxels: int, worklo
     if nbytes is None:
         return None
     achieved = nbytes / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": committed_traffic(name, workload),
-            "kernel": name, "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes}
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": committed_traffic(name, workload),
+           "kernel": name, "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes}
+    valu = committed_valu(name, workload)
+    if valu is not None:
+        out["valu"] = {"insts_per_launch": valu, "peak_insts_per_s": VALU_PEAK,
+                       "frac": round(valu / (VALU_PEAK * avg_s), 4)}
+    return out
 
 
 def step_roofline(particles: int, substeps: int, mips: int, live_voxels: int, ms_per_step: float) -> dict:
@@ -174,12 +202,14 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
     ms = timed(sim.step, steps, warmup, stream)
     ks = min(steps, 5)
     stats = kernel_stats(root, sim.step, ks, stream)
+    atomics = splat_atomics(root, sim.splat_atomics(), n, stats) if splat == "tiled" else None
     live = sum(int(sim.surface.liveTiles(m).sum()) * 512 for m in range(mips))
     out = {
         "workload": scene["name"], "particles": n, "ms_per_step": ms, "value": n / (ms / 1e3),
         "substeps": scene["options"]["substeps"], "mips": mips, "live_voxels": live,
         "kernels_ms_per_step": per_step(stats, ks),
         "roofline": roofline_of(stats, n, mips, live, name) if splat == "tiled" else None,
+        "splat_atomics": atomics,
         "tuning": dict(sim.tuning, source=tune_src),
         "scene": scene,
     }
@@ -188,6 +218,29 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
     for k in keep:
         k.destroy()
     return out
+
+
+def cpu_protocol(step, budget_s: float) -> dict:
+    """BASELINE.md's timing protocol within a time budget: warm-up steps (5, or
+    fewer when one step takes more than 3% of the budget), then 3 runs of K timed
+    steps (100, or as many as a third of the budget holds), wall clock around each
+    run; the median run's seconds per step."""
+    import statistics
+
+    t0 = time.perf_counter()
+    step()
+    first = time.perf_counter() - t0
+    warm = min(5, max(1, int(0.15 * budget_s / max(first, 1e-9))))
+    for _ in range(warm - 1):
+        step()
+    k = max(1, min(100, int(budget_s / (3 * max(first, 1e-9)))))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(k):
+            step()
+        runs.append((time.perf_counter() - t0) / k)
+    return {"s_per_step": statistics.median(runs), "warmup": warm, "timed": k, "runs": [round(r, 4) for r in runs]}
 
 
 def cpu_baseline_single(scene, budget_s: float, threads: int) -> dict:
@@ -210,22 +263,20 @@ def cpu_baseline_single(scene, budget_s: float, threads: int) -> dict:
     fl = O.Fluid(opts, scene["pos"], collider=col)
     so = O.splat_opts(opts["bakeResolution"], opts["bakeMipLevels"], opts["worldSize"], opts["origin"],
                       opts["bakeBand"], opts["surfaceRadius"])
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
+
+    def step():
         for _ in range(opts["substeps"]):
             fl.substep()
-        grids = O.splat_bake(so, fl.get("pos"))
-        O.splat_resolve(so, grids)
-        steps += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
+        O.splat_resolve(so, O.splat_bake(so, fl.get("pos")))
+
+    pr = cpu_protocol(step, budget_s)
     n = scene["pos"].shape[0]
     return {
-        "value": n * steps / dt, "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
-        "sample": f"{steps} full step(s) of the same workload ({n} particles, 2 substeps + "
-                  f"{opts['bakeMipLevels']}-mip bake), {dt:.1f} s wall, oracle f32 mode with OpenMP",
+        "value": n / pr["s_per_step"], "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
+        "sample": f"the whole workload ({n} particles, 2 substeps + {opts['bakeMipLevels']}-mip bake per step): "
+                  f"{pr['warmup']} warm-up step(s), then 3 runs of {pr['timed']} timed step(s) (s/step "
+                  f"{pr['runs']}), median run; oracle f32 mode with OpenMP. Fewer than BASELINE.md's 5 + 100 "
+                  f"steps where the {budget_s:.0f} s budget holds fewer",
     }
 
 
@@ -246,29 +297,29 @@ def cpu_baseline_batched(scene, budget_s: float, threads: int) -> dict:
     counts = scene["counts"]
     base = np.concatenate([np.zeros(1, np.int64), np.cumsum(counts, dtype=np.int64)])
     half = opts["followSize"] / 2
-    done = 0
-    used = 0
-    t0 = time.perf_counter()
-    for s in range(len(counts)):
-        pos = scene["pos"][base[s]:base[s + 1]]
-        fl = O.Fluid(dict(opts, capacity=int(counts[s])), pos, collider=col)
-        for _ in range(opts["substeps"]):
-            fl.substep()
-        p = fl.get("pos")
-        centre = (p.min(axis=0) + p.max(axis=0)) * np.float32(0.5)
-        origin = [float(np.float32(c - half)) for c in centre.astype(np.float64)]
-        so = O.splat_opts(opts["bakeResolution"], opts["bakeMipLevels"], opts["followSize"], origin,
-                          opts["bakeBand"], opts["surfaceRadius"])
-        O.splat_resolve(so, O.splat_bake(so, p))
-        done += int(counts[s])
-        used += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
+    picked = list(range(0, len(counts), max(1, len(counts) // 8)))[:8]  # spread over the count range
+    fluids = [O.Fluid(dict(opts, capacity=int(counts[s])), scene["pos"][base[s]:base[s + 1]], collider=col)
+              for s in picked]
+
+    def step():
+        for fl in fluids:
+            for _ in range(opts["substeps"]):
+                fl.substep()
+            p = fl.get("pos")
+            centre = (p.min(axis=0) + p.max(axis=0)) * np.float32(0.5)
+            origin = [float(np.float32(c - half)) for c in centre.astype(np.float64)]
+            so = O.splat_opts(opts["bakeResolution"], opts["bakeMipLevels"], opts["followSize"], origin,
+                              opts["bakeBand"], opts["surfaceRadius"])
+            O.splat_resolve(so, O.splat_bake(so, p))
+
+    pr = cpu_protocol(step, budget_s)
+    done = int(sum(int(counts[s]) for s in picked))
     return {
-        "value": done / dt, "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
-        "sample": f"one full step of each of the first {used} scenes ({done} particles, 2 substeps + "
-                  f"{opts['bakeMipLevels']}-mip following bake), {dt:.1f} s wall, oracle f32 mode with OpenMP",
+        "value": done / pr["s_per_step"], "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
+        "sample": f"scenes {picked} of the batch ({done} particles; a step is 2 substeps + the "
+                  f"{opts['bakeMipLevels']}-mip following bake of each): {pr['warmup']} warm-up step(s), then 3 "
+                  f"runs of {pr['timed']} timed step(s) (s/step {pr['runs']}), median run; oracle f32 mode with "
+                  f"OpenMP. Fewer than BASELINE.md's 5 + 100 steps where the {budget_s:.0f} s budget holds fewer",
     }
 
 
@@ -296,6 +347,44 @@ def measure_per_scene(root, world, scene, steps: int, stream) -> dict:
     n = int(counts.sum())
     return {"path": "one f32 FluidSim per scene, same scenes", "ms_per_step": ms, "value": n / (ms / 1e3),
             "steps": steps}
+
+
+def splat_atomics(root, counts: tuple[int, int], particles: int, stats: dict) -> dict:
+    """The owner walk's atomics per bake (counted by a tallying re-bake of the same
+    state) against the measured issue rate of ds_min_u32 (SURVEY's splat roof)."""
+    lds, glob = counts
+    tot_ms, launches = stats.get("splat_brick", (0.0, 0))
+    avg_s = tot_ms / launches / 1e3 if launches else float("nan")
+    peak_lds = root.atomic_peak("lds_min")
+    peak_glob = root.atomic_peak("global_min")
+    achieved = lds / avg_s
+    return {"lds_min_per_bake": lds, "global_min_per_bake": glob, "lds_min_per_particle": lds / particles,
+            "splat_brick_avg_ms": round(avg_s * 1e3, 4), "achieved_lds_min_per_s": achieved,
+            "peak_lds_min_per_s": peak_lds, "frac": round(achieved / peak_lds, 4),
+            "peak_global_min_per_s": peak_glob,
+            "note": "peaks measured in this run by clay_atomic_peak (conflict-free ds_min_u32; global atomicMin, "
+                    "a wave per 256-byte span)"}
+
+
+def measure_batched_1m(root, args, stream, tuning: dict) -> dict:
+    """The north-star comparison at its stated size: 64 scenes of 16 384 particles
+    (1 048 576 in total), batched with FP16-LDS staging and a captured graph, with
+    the batched workload's tuned launch shapes."""
+    import clayhip
+    from clayhip import scenes
+
+    scene = scenes.batched_even(64, 16384)
+    world = clayhip.analyticField(root, scene["collider"]["brushes"])
+    b = clayhip.FluidBatch(root, world, dict(scene["options"], precision=args.precision), scene["counts"])
+    b.set_graph(not args.no_graph)
+    b.tuning = tuning
+    b.set_state(scene["pos"])
+    ms = timed(b.step, args.steps, args.warmup, stream)
+    n = int(scene["counts"].sum())
+    b.destroy()
+    world.destroy()
+    return {"path": f"batched, {args.precision}, graph, " + scene["name"], "particles": n, "ms_per_step": ms,
+            "value": n / (ms / 1e3)}
 
 
 def measure_batched(root, args, stream) -> dict:
@@ -327,6 +416,7 @@ def measure_batched(root, args, stream) -> dict:
     # Per-kernel breakdown from eager frames (a graph replay has no per-launch events).
     b.set_graph(False)
     stats = kernel_stats(root, b.step, 3, stream)
+    atomics = splat_atomics(root, b.splat_atomics(), n, stats)
     live = sum(int(b.live_tiles(s, m).sum()) * 512 for s in range(b.scenes) for m in range(mips))
     # Error of the measured path against the f32 batch over one step from its current state.
     err = None
@@ -349,11 +439,14 @@ def measure_batched(root, args, stream) -> dict:
         "tuning": dict(b.tuning, source=tune_src), "graph": not args.no_graph,
         "kernels_ms_per_step": per_step(stats, 3),
         "roofline": roofline_of(stats, n, mips, live, "batched-256"),
+        "splat_atomics": atomics,
         "error": err, "scene": scene,
     }
+    tuning = {k: v for k, v in b.tuning.items()}
     b.destroy()
     if not args.no_compare:
         out["plain_per_scene"] = measure_per_scene(root, world, scene, min(args.steps, 5), stream)
+        out["batched_1m"] = measure_batched_1m(root, args, stream, tuning)
     world.destroy()
     return out
 
@@ -566,6 +659,8 @@ def main() -> int:
                                     "ms_per_step": single["ms_per_step"], "value": single["value"]},
                 "batched_over_plain_per_scene": r["value"] / r["plain_per_scene"]["value"],
                 "batched_over_plain_single_1m": r["value"] / single["value"],
+                "batched_1m_total": r["batched_1m"],
+                "batched_1m_total_over_plain_single_1m": r["batched_1m"]["value"] / single["value"],
             }
     elif coupled:
         r = measure_coupled(root, args, stream)
@@ -598,6 +693,7 @@ def main() -> int:
         },
         "step_roofline": step_roofline(r["particles"], r["substeps"], r["mips"], r["live_voxels"], r["ms_per_step"]),
         "roofline": r["roofline"],
+        "splat_atomics": r.get("splat_atomics"),
         "kernels_ms_per_step": r["kernels_ms_per_step"],
     }
     if batched:

@@ -120,6 +120,13 @@ class FluidBatch:
         L.check(L.lib().clay_batch_conserved(self._h, int(scene), out))
         return {"mass": out[0], "momentum": tuple(out[1:4]), "com": tuple(out[4:7])}
 
+    def splat_atomics(self) -> tuple[int, int]:
+        """Bakes every scene's surface of the current state again and counts its
+        owner walk's atomics: (LDS ds_min_u32, global atomicMin)."""
+        out = (C.c_uint64 * 2)()
+        L.check(L.lib().clay_batch_splat_atomics(self._h, out))
+        return int(out[0]), int(out[1])
+
     def destroy(self) -> None:
         if self._h is not None:
             L.lib().clay_batch_destroy(self._h)

@@ -169,6 +169,13 @@ class FluidSim:
         t = L.Tuning(**cur)
         L.check(L.lib().clay_fluid_set_tuning(self._h, C.byref(t)))
 
+    def splat_atomics(self) -> tuple[int, int]:
+        """Bakes the surface of the current state again and counts its owner walk's
+        atomics: (LDS ds_min_u32, global atomicMin)."""
+        out = (C.c_uint64 * 2)()
+        L.check(L.lib().clay_fluid_splat_atomics(self._h, out))
+        return int(out[0]), int(out[1])
+
     def conserved(self) -> dict:
         """Total mass, momentum and centre of mass of the live particles, reduced in FP64."""
         out = (C.c_double * 7)()

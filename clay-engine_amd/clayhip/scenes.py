@@ -170,3 +170,23 @@ def batched(scenes: int = 256, seed: int = 7) -> dict:
         "pos": np.concatenate(blocks, axis=0),
         "steps": 100,
     }
+
+
+def batched_even(scenes: int = 64, count: int = 16384, seed: int = 7) -> dict:
+    """`scenes` independent scenes of `count` particles each (64 x 16 384 = 1 048 576
+    in total, the particle count of the single-scene 1M dam break), drop heights
+    uniform in [0.5, 3.0] from the reference hash, with the batched workload's
+    options."""
+    counts = np.full(scenes, count, dtype=np.int64)
+    u = hash1(np.arange(scenes, dtype=np.uint32) + np.uint32(seed * 7919 + 104729)).astype(np.float64)
+    drops = 0.5 + 2.5 * u
+    blocks = [drop_block(count, float(drops[i]), seed=seed * 1000 + i) for i in range(scenes)]
+    return {
+        "name": f"batched {scenes} scenes x {count} particles ({scenes * count} in total), following 32^3 x 3 mips each",
+        "options": batch_options(),
+        "collider": {"kind": "analytic", "brushes": [floor_brush()]},
+        "counts": counts.astype(np.uint32),
+        "drops": drops,
+        "pos": np.concatenate(blocks, axis=0),
+        "steps": 100,
+    }

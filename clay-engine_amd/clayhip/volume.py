@@ -29,6 +29,14 @@ class Root:
     def synchronize(self) -> None:
         L.check(L.lib().clay_ctx_synchronize(self.handle))
 
+    def atomic_peak(self, kind: str, rounds: int = 4096) -> float:
+        """Measured issue rate (ops/s) of 'lds_min' (ds_min_u32, conflict-free) or
+        'global_min' (global atomicMin, a wave per 256-byte span)."""
+        out = C.c_double()
+        k = {"lds_min": 0, "global_min": 1}[kind]
+        L.check(L.lib().clay_atomic_peak(self.handle, k, int(rounds), C.byref(out)))
+        return out.value
+
     def set_timing(self, enabled: bool) -> None:
         L.check(L.lib().clay_ctx_set_timing(self.handle, 1 if enabled else 0))
 

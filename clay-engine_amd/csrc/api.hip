@@ -1404,6 +1404,51 @@ extern "C" int clay_fluid_step(clay_fluid* fh, clay_stream stream) {
   return fluid_frame_impl(f, s);
 }
 
+namespace {
+// The surface bake of the current state again, with the owner walk tallying its
+// atomics.  Needs the frame-end pass of a step (its order and interior radii).
+int fluid_splat_atomics(Fluid* f, uint64_t out[2], const char* who) {
+  if (!out) return fail(CLAY_ERR_INVALID, std::string(who) + ": NULL argument");
+  CLAY_HIP_CHECK(fluid_quiesce(f));
+  if (f->surface->desc.mode != CLAY_SPLAT_TILED)
+    return fail(CLAY_ERR_INVALID, std::string(who) + ": the counts come from the tiled schedule's owner walk");
+  if (!f->presorted || f->live == 0)
+    return fail(CLAY_ERR_STATE, std::string(who) + ": count after a step (the bake reads its frame-end pass)");
+  unsigned long long* d = nullptr;
+  int rc = dalloc(&d, 2);
+  if (rc) return rc;
+  hipStream_t s = f->ctx->stream;
+  hipError_t e = hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), s);
+  f->surface->atomic_counts = d;
+  if (e == hipSuccess) {
+    rc = splat_bake_prepare(f->surface, s, f->vals, false);
+    if (!rc) rc = splat_bake_finish(f->surface, s, f->vals, f->cover);
+  }
+  f->surface->atomic_counts = nullptr;
+  unsigned long long h[2] = {0, 0};
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+  dfree(d);
+  if (e != hipSuccess) return fail(CLAY_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
+  if (rc) return rc;
+  out[0] = h[0];
+  out[1] = h[1];
+  return CLAY_OK;
+}
+}  // namespace
+
+extern "C" int clay_fluid_splat_atomics(clay_fluid* fh, uint64_t out[2]) {
+  Fluid* f = reinterpret_cast<Fluid*>(fh);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_splat_atomics: NULL fluid");
+  return fluid_splat_atomics(f, out, "clay_fluid_splat_atomics");
+}
+
+extern "C" int clay_batch_splat_atomics(clay_batch* b, uint64_t out[2]) {
+  Fluid* f = reinterpret_cast<Fluid*>(b);
+  if (!f) return fail(CLAY_ERR_INVALID, "clay_batch_splat_atomics: NULL batch");
+  return fluid_splat_atomics(f, out, "clay_batch_splat_atomics");
+}
+
 extern "C" int clay_fluid_set_graph(clay_fluid* fh, int32_t enabled) {
   Fluid* f = reinterpret_cast<Fluid*>(fh);
   if (!f) return fail(CLAY_ERR_INVALID, "clay_fluid_set_graph: NULL fluid");

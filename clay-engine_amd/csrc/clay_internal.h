@@ -154,6 +154,7 @@ struct Splat {
   bool follow = false;                   // centre each volume before a bake:
   const double* follow_centers = nullptr;  // on these (6 doubles per scene, xyz first), else on the AABB
   uint32_t* follow_slots = nullptr;      // per scene: the AABB's six ordered-float slots
+  unsigned long long* atomic_counts = nullptr;  // set for a bake that tallies its atomics
   // tuning
   uint32_t chunk = 1024;                 // particles per splat work item
   int persistent_blocks = 2048;          // grid of the splat work-item kernel
@@ -335,6 +336,8 @@ size_t splat_bin_slices();
 // (the device must be current), once per device.
 int splat_static_init(int device);
 size_t splat_static_bytes();
+// atomic_peak.hip
+int atomic_peak(Ctx* c, int kind, uint32_t rounds, double* ops_per_s);
 int splat_bake_static(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, uint16_t* dist,
                       hipStream_t s);
 // volume.hip

@@ -586,12 +586,16 @@ __global__ void __launch_bounds__(kThreads)
 // clipped by the brick: all lanes walk boxes of the same size, which keeps the
 // wave's lanes busy.  The staged region is merged into the code grid with
 // atomicMin, because neighbouring bricks' shells overlap it.
-template <bool kNormals>
+// kCount: also tallies the atomics the walk issues (ds_min_u32 into LDS, one per
+// walked voxel) and the merge issues (global atomicMin, one per voxel below the
+// initial value) into counts[0] and counts[1]; the codes are the same.
+template <bool kNormals, bool kCount = false>
 __global__ void __launch_bounds__(kOwnerThreads)
     k_owner_splat(MipsK mk_, CloudK c, const VolDev* __restrict__ vol, const uint32_t* __restrict__ start,
                   const uint32_t* __restrict__ count, const uint32_t* __restrict__ item_start,
                   const ScanOut* __restrict__ scan, uint32_t nbricks, const uint32_t* __restrict__ list,
-                  uint32_t* __restrict__ grid) {
+                  uint32_t* __restrict__ grid, unsigned long long* __restrict__ counts = nullptr) {
+  unsigned long long n_lds = 0, n_glob = 0;
   // The staged region, sized at launch for the largest brick-plus-shell of these
   // levels (owner_stage_voxels): at most kExtVox.
   extern __shared__ uint32_t lds[];
@@ -660,6 +664,7 @@ __global__ void __launch_bounds__(kOwnerThreads)
           int32_t xa, xb;
           axis_span(p.x, __builtin_amdgcn_sqrtf(rem), m.inv, x0, x1, xa, xb);
           uint32_t* row = lds + (y - ey0) * nx + (z - ez0) * nx * ny - ex0;
+          if (kCount && xb >= xa) n_lds += (unsigned long long)(xb - xa + 1);
           splat_row<kNormals>(m, p, nr, dy, dz, dy2, dz2, row + xa, row + xb, (float)xa + 0.5f);
         }
       }
@@ -706,8 +711,20 @@ __global__ void __launch_bounds__(kOwnerThreads)
       if (val == init) continue;
       const int32_t x = ex0 + v % nx, y = ey0 + (v / nx) % ny, z = ez0 + v / (nx * ny);
       atomicMin(sgrid + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z), val);
+      if (kCount) n_glob++;
     }
     __syncthreads();
+  }
+  if (kCount) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      n_lds += __shfl_xor(n_lds, off);
+      n_glob += __shfl_xor(n_glob, off);
+    }
+    if ((t & 63) == 0) {
+      atomicAdd(&counts[0], n_lds);
+      atomicAdd(&counts[1], n_glob);
+    }
   }
 }
 
@@ -1030,12 +1047,18 @@ static int tiled_finish(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK
   ScanOut* scan = reinterpret_cast<ScanOut*>(sp->brick_total_dev);
   if (c.count) {
     LaunchTimer t(ctx, s, "splat_brick");
-    if (mk_.owner)
-      hipLaunchKernelGGL(k_owner_splat<kNormals>,
+    if (mk_.owner && sp->atomic_counts)
+      hipLaunchKernelGGL((k_owner_splat<kNormals, true>),
                          dim3(std::max(1, sp->persistent_blocks * kThreads / kOwnerThreads)), dim3(kOwnerThreads),
                          sizeof(uint32_t) * owner_stage_voxels(mk_), s, mk_, c,
                          sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
-                         (uint32_t)nbricks, sp->brick_list, sp->grid);
+                         (uint32_t)nbricks, sp->brick_list, sp->grid, sp->atomic_counts);
+    else if (mk_.owner)
+      hipLaunchKernelGGL((k_owner_splat<kNormals, false>),
+                         dim3(std::max(1, sp->persistent_blocks * kThreads / kOwnerThreads)), dim3(kOwnerThreads),
+                         sizeof(uint32_t) * owner_stage_voxels(mk_), s, mk_, c,
+                         sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,
+                         (uint32_t)nbricks, sp->brick_list, sp->grid, nullptr);
     else
       hipLaunchKernelGGL(k_brick_splat<kNormals>, dim3(sp->persistent_blocks), dim3(kThreads), 0, s, mk_, c,
                          sp->vol->dev, sp->brick_start, sp->brick_count, sp->item_start, scan,

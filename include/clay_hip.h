@@ -271,6 +271,23 @@ int clay_splat_grid_download(clay_splat* s, int32_t mip, uint32_t* codes_host);
 /* Exact live-tile mask of one mip after the last bake: 1 byte per 8^3 tile. */
 int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* live_host);
 
+/* ---- Atomic throughput of the splat ------------------------------------
+ * The bake's walk is a stream of unsigned min-atomics.  clay_atomic_peak measures
+ * the issue rate of each kind on this device (ops/s): LDS ds_min_u32 with every
+ * lane on its own bank, or global atomicMin with a wave on one 256-byte span of a
+ * 256 MiB buffer; `rounds` (1..65536) atomics per lane of 8 512-lane workgroups per
+ * CU.  Synchronous.  No counterpart in the reference (splat.ts:241-253 issues the
+ * atomics; nothing there measures them). */
+#define CLAY_ATOMIC_LDS_MIN 0
+#define CLAY_ATOMIC_GLOBAL_MIN 1
+int clay_atomic_peak(clay_ctx* ctx, int32_t kind, uint32_t rounds, double* ops_per_s);
+/* Bakes the surface of the current state again (SplatField.bake, splat.ts:296-304,
+ * with the same codes) and counts the atomics of its owner walk: out[0] the LDS
+ * ds_min_u32 (one per walked voxel), out[1] the global atomicMin of the merge.
+ * Call after a step; the surface must use the tiled schedule (zeros when the
+ * brick walk runs instead of the owner walk).  Synchronous. */
+int clay_fluid_splat_atomics(clay_fluid* f, uint64_t out[2]);
+
 /* A bake that allocates nothing.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  It is clay_splat_bake with
@@ -549,6 +566,8 @@ int clay_batch_tiles_download(clay_batch* b, uint32_t scene, int32_t mip, uint8_
 int clay_batch_origin(clay_batch* b, uint32_t scene, float origin[3]);
 /* clay_fluid_conserved over one scene's particles. */
 int clay_batch_conserved(clay_batch* b, uint32_t scene, double out[7]);
+/* clay_fluid_splat_atomics over every scene's following bake. */
+int clay_batch_splat_atomics(clay_batch* b, uint64_t out[2]);
 
 /* ---- ClaySolver: shape-matched particle bodies (lib/src/sim/pbd.ts:78-424) ----
  * A ParticleSet (lib/src/sim/particles.ts:18-206) of `capacity` particles and up

@@ -1,0 +1,53 @@
+"""The splat's atomic counts and the measured atomic peaks.
+
+The tallying re-bake must leave the codes exactly as the plain bake of the same
+state wrote them, and count at least one LDS min per voxel the walk lowered (every
+voxel below the saturated code got its value from some particle's ds_min_u32).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import clayhip
+from clayhip import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tallying_bake_keeps_codes_and_counts_every_lowered_voxel(root):
+    sc = scenes.dam_break_small()
+    world = clayhip.analyticField(root, sc["collider"]["brushes"])
+    sim = clayhip.FluidSim(root, world, dict(sc["options"], splatMode="tiled"))
+    sim.set_state(sc["pos"])
+    for _ in range(3):
+        sim.step()
+    mips = sc["options"]["bakeMipLevels"]
+    before = [sim.surface.codes(m) for m in range(mips)]
+    lds, glob = sim.splat_atomics()
+    after = [sim.surface.codes(m) for m in range(mips)]
+    for m in range(mips):
+        assert np.array_equal(before[m], after[m]), f"mip {m}"
+    lowered = sum(int((c < 65535).sum()) for c in after)
+    assert lowered > 0
+    assert lds >= lowered and glob >= lowered
+    assert lds < 2000 * sc["pos"].shape[0]  # about 1 100 per particle per bake in the reference's walk
+    sim.destroy()
+    world.destroy()
+
+
+def test_splat_atomics_need_a_step(root):
+    sc = scenes.dam_break_small()
+    world = clayhip.analyticField(root, sc["collider"]["brushes"])
+    sim = clayhip.FluidSim(root, world, dict(sc["options"], splatMode="tiled"))
+    sim.set_state(sc["pos"])
+    with pytest.raises(clayhip.ClayError, match="after a step"):
+        sim.splat_atomics()
+    sim.destroy()
+    world.destroy()
+
+
+@pytest.mark.parametrize("kind", ["lds_min", "global_min"])
+def test_atomic_peaks_are_measured(root, kind):
+    rate = root.atomic_peak(kind, rounds=1024)
+    assert math.isfinite(rate) and 1e9 < rate < 1e15
