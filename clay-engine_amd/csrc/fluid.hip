@@ -637,7 +637,10 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8)))
   int32_t b[3] = {0, 0, 0};
   if (active) cell_of(k, pi, b);
   const f3 oi = cell_rel(k, pi, b);
-  float rho = 0.0f;
+  // Compensated density: each row's terms (at most a few dozen) are summed plainly,
+  // and the nine row sums are added with Neumaier's compensation, so the rounding
+  // of the total is that of one row rather than of every term.
+  float rho_hi = 0.0f, rho_lo = 0.0f, rho = 0.0f;
   Cover cov;
   auto term = [&](f3 d) {
     const float r2 = dot3_fma(d);
@@ -646,6 +649,12 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8)))
       rho = rho + k.mass_poly6 * (w * w * w);
     }
     if (kCover) cov.add(d, r2, k.hh);
+  };
+  auto fold_row = [&]() {
+    const float t = rho_hi + rho;
+    rho_lo += fabsf(rho_hi) >= fabsf(rho) ? (rho_hi - t) + rho : (rho - t) + rho_hi;
+    rho_hi = t;
+    rho = 0.0f;
   };
   // The frame-end pass classes neighbours out to 1.5 h for the interior radius.
   const float reach2 = kCover ? 2.25f * k.hh : k.hh;
@@ -669,10 +678,12 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8)))
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
+      fold_row();
       wave_sync();
     }
   }
   if (!active) return;
+  rho = rho_hi + rho_lo;
   const float rr = fmax_s(rho, k.rest_floor), pr = fmax_s(k.stiff * (rho - k.rest), 0.0f);
   srho[s] = rr;
   spres[s] = pr;
