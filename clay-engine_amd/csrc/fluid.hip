@@ -480,22 +480,27 @@ __global__ void __launch_bounds__(kB)
 // each lane reads its own scene's cell records, and the union of ranges is still
 // one span of the sorted order.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
-    v = w < v ? w : v;
-  }
-  return v;
+// Wave-wide min / max of a u32, returned to every lane: DPP row shifts build each
+// 16-lane row's result in its last lane, two row broadcasts carry it across the
+// rows, and lane 63 holds the wave's result (no LDS round trips).  Lanes a shift
+// or broadcast does not reach keep their own value.  All 64 lanes must be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t keep, uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)keep, (int)x, kCtrl, kRowMask, 0xF, false);
 }
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
-    v = w > v ? w : v;
-  }
-  return v;
+template <bool kMax>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v) {
+  auto pick = [](uint32_t a, uint32_t b) { return kMax ? (a > b ? a : b) : (a < b ? a : b); };
+  v = pick(v, dpp_u32<0x111, 0xF>(v, v));  // row_shr:1
+  v = pick(v, dpp_u32<0x112, 0xF>(v, v));  // row_shr:2
+  v = pick(v, dpp_u32<0x114, 0xF>(v, v));  // row_shr:4
+  v = pick(v, dpp_u32<0x118, 0xF>(v, v));  // row_shr:8
+  v = pick(v, dpp_u32<0x142, 0xA>(v, v));  // row_bcast:15 into rows 1 and 3
+  v = pick(v, dpp_u32<0x143, 0xC>(v, v));  // row_bcast:31 into rows 2 and 3
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return wave_reduce_u32<false>(v); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return wave_reduce_u32<true>(v); }
 
 // Orders a wave's LDS accesses around this point (writes before it are seen by
 // every lane after it), without waiting for the workgroup's other waves: each
@@ -613,6 +618,13 @@ __device__ __forceinline__ float sub_f16_hi(float base, uint32_t h2) {
   return r;
 }
 __device__ __forceinline__ float dot3_fma(f3 d) { return __builtin_fmaf(d.x, d.x, __builtin_fmaf(d.y, d.y, d.z * d.z)); }
+
+// Waves per SIMD the forces kernels are compiled for.  Their staging slice takes
+// 32 bytes per neighbour per wave: at 256 neighbours two 64-lane waves hold 16 KB,
+// so LDS (160 KB per CU) admits 5 waves per SIMD whatever the registers allow, and
+// asking for 7 would only make the register allocator spill.
+template <int kS>
+constexpr int kForcesWaves = kS >= 256 ? 5 : 7;
 
 // Occupancy hints: the FP16 density kernel fits 64 VGPRs (8 waves per SIMD) with no
 // spill; the forces kernels are held to 7 waves, at the cost of 48 bytes of scratch
@@ -773,7 +785,7 @@ struct StagedX {
 };
 
 template <int kB, int kS>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(7)))
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForcesWaves<kS>)))
     k_forces_integrate_x(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
                          const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
                          const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts) {
@@ -862,7 +874,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(7)))
 }
 
 template <int kB, int kS>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(7)))
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForcesWaves<kS>)))
     k_forces_integrate_lds(FluidK k, SceneK sk, ColliderDev col, uint32_t n, SortedIn in,
                            const uint32_t* __restrict__ svals, const uint32_t* __restrict__ skeys,
                            const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts,
