@@ -190,7 +190,8 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
     if splat == "tiled":
         key = autotune.cache_key(name, precision)
         if autotune_now:
-            autotune.autotune(sim, key, frames=3, state=lambda: sim.set_state(scene["pos"]))
+            autotune.autotune(sim, key, frames=3, space=autotune.space_for(precision),
+                              state=lambda: sim.set_state(scene["pos"]))
             tune_src = "autotuned in this run"
         else:
             cached = autotune.load(autotune.DEFAULT_CACHE, key)
@@ -402,7 +403,8 @@ def measure_batched(root, args, stream) -> dict:
     key = autotune.cache_key("batched-256", args.precision)
     tune_src = "default"
     if args.autotune:
-        autotune.autotune(b, key, frames=3, state=lambda: b.set_state(scene["pos"]))
+        autotune.autotune(b, key, frames=3, space=autotune.space_for(args.precision),
+                          state=lambda: b.set_state(scene["pos"]))
         tune_src = "autotuned in this run"
     else:
         cached = autotune.load(autotune.DEFAULT_CACHE, key)

@@ -4,8 +4,10 @@ Tries each candidate clay_tuning on the live workload -- a few captured frames
 per candidate, timed with HIP events on one stream -- keeps the fastest, and
 caches the choice in a JSON file keyed by device and workload.  The knobs only
 change launch shapes (workgroup size, LDS staging slice, splat work-item size,
-persistent grid, XCD-contiguous workgroup numbering), never results; the cell size is not a knob because it would
-change every neighbour's summation order.
+persistent grid, XCD-contiguous workgroup numbering) and, for the f32 path, whether
+neighbour rows are staged in LDS (exact_lds: the same sums in the same order), never
+results.  The cell size is not a knob: it would change every neighbour's summation
+order, and a finer grid costs more per-row work than it saves in candidates.
 """
 from __future__ import annotations
 
@@ -25,6 +27,12 @@ SPACE = {
     "splat_xcd": (0, 1),
     "splat_owner": (0, 1),
 }
+
+
+def space_for(precision: str) -> dict:
+    """The search space of a workload: f32 also tries the global-memory neighbour
+    walk against the LDS-staged one; FP16-LDS always stages."""
+    return dict(SPACE, exact_lds=(0, 1)) if precision == "f32" else dict(SPACE)
 
 
 def candidates(space: dict = SPACE) -> list[dict]:
@@ -61,7 +69,9 @@ def autotune(obj, key: str, frames: int = 3, space: dict = SPACE, cache: Path | 
              state=None) -> dict:
     """Tunes `obj` in place and returns {"tuning", "ms", "table"}.  `state`, if
     given, is a callable that restores the workload's initial state before each
-    candidate, so every candidate times the same frames."""
+    candidate, so every candidate times the same frames.  The cached "ms" is this
+    comparison's own figure (the first frames after the initial state), not a
+    steady-state frame time."""
     import torch
 
     stream = torch.cuda.Stream()

@@ -62,6 +62,7 @@ constexpr uint32_t kSInit = 0xFFFFFFFFu;    // "no particle" for the squared dis
 constexpr int kHalo = 5;                    // owner schedule: voxels staged around a brick
 constexpr int kExt = kBrick + 2 * kHalo;    // staged edge (26)
 constexpr int kExtVox = kExt * kExt * kExt;
+static_assert(kExtVox < (1 << 15) && kExt <= 26, "the owner merge's multiply-high division needs these bounds");
 constexpr int kPart = 2048;                 // owner schedule: particles partitioned per LDS pass
 // Owner schedule workgroup: the staged region takes ~78 KB of LDS, so two workgroups
 // fit a CU; 512 threads each keep 4 waves per SIMD in flight (1024-thread
@@ -706,10 +707,14 @@ __global__ void __launch_bounds__(kOwnerThreads)
       }
     }
     __syncthreads();
+    // Voxel coordinates from the flat index by multiply-high with ceil(2^32 / d):
+    // exact for indices below 2^15 and divisors up to 26 (checked exhaustively).
+    const uint32_t mx = 0xFFFFFFFFu / (uint32_t)nx + 1u, my = 0xFFFFFFFFu / (uint32_t)ny + 1u;
     for (int v = t; v < nvox; v += kOwnerThreads) {
       const uint32_t val = lds[v];
       if (val == init) continue;
-      const int32_t x = ex0 + v % nx, y = ey0 + (v / nx) % ny, z = ez0 + v / (nx * ny);
+      const uint32_t qx = __umulhi((uint32_t)v, mx), qy = __umulhi(qx, my);
+      const int32_t x = ex0 + (v - (int32_t)qx * nx), y = ey0 + ((int32_t)qx - (int32_t)qy * ny), z = ez0 + (int32_t)qy;
       atomicMin(sgrid + (uint32_t)x + (uint32_t)m.r * ((uint32_t)y + (uint32_t)m.r * (uint32_t)z), val);
       if (kCount) n_glob++;
     }
