@@ -627,11 +627,12 @@ template <int kS>
 constexpr int kForcesWaves = kS >= 256 ? 5 : 7;
 
 // Occupancy hints: the FP16 density kernel fits 64 VGPRs (8 waves per SIMD) with no
-// spill; the forces kernels are held to 7 waves, at the cost of 48 bytes of scratch
-// per lane, which the extra wave more than pays for (FP16 forces: 1.11 -> 1.02 ms per
-// batched frame).
+// spill; its frame-end form (interior-radius classes) needs a few more and runs at
+// 7; the forces kernels with 128-entry staging are held to 7 waves, at the cost of
+// 48 bytes of scratch per lane, which the extra wave more than pays for (FP16
+// forces: 1.11 -> 1.02 ms per batched frame).
 template <int kB, int kS, bool kCover>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover ? 7 : 8)))
     k_density_lds(FluidK k, SceneK sk, uint32_t n, const float* __restrict__ spx, const float* __restrict__ spy,
                   const float* __restrict__ spz, const uint32_t* __restrict__ skeys,
                   const uint4* __restrict__ cells, float* __restrict__ srho, float* __restrict__ spres, CoverOut co,
