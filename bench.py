@@ -74,11 +74,23 @@ def committed_traffic(kernel: str, workload: str) -> float | None:
     return None
 
 
-# VALU issue: a gfx950 SIMD is 16 lanes wide, so a wave64 VALU instruction takes 4
-# cycles and a CU's 4 SIMDs together issue one wave instruction per cycle.
-CUS = 256
-CLOCK_HZ = 2.4e9
-VALU_PEAK = CUS * CLOCK_HZ  # wave64 VALU instructions per second
+# VALU issue: MI355X's published FP32 vector peak, 157.3 TFLOP/s, is two wave64
+# FMA instructions (128 flops) per CU per cycle at 2.4 GHz.  gfx950 reaches that
+# issue rate only for two-operand VGPR forms (v_fmac_f32, v_add_f32, v_mul_f32,
+# bitwise, moves); three-operand, compare/select and max forms issue at about half
+# and transcendentals at a quarter of it.  clay_valu_peak measures the full-rate
+# ceiling on the device this run is on, reported beside the published figure.
+VALU_PEAK_PUBLISHED = 157.3e12 / 128  # wave64 VALU instructions per second
+_VALU_MEASURED: dict = {}
+
+
+def valu_measured(root) -> dict:
+    """Wave64 instructions per second this device retires for each instruction form
+    clay_valu_peak knows ("fmac" is the full-rate ceiling), measured once per run."""
+    if not _VALU_MEASURED:
+        for f in root.VALU_FORMS:
+            _VALU_MEASURED[f] = max(root.valu_peak(f) for _ in range(2))
+    return _VALU_MEASURED
 
 
 def committed_valu(kernel: str, workload: str) -> float | None:
@@ -94,11 +106,12 @@ def committed_valu(kernel: str, workload: str) -> float | None:
     return None
 
 
-def roofline_of(stats: dict, particles: int, mips: int, live_voxels: int, workload: str) -> dict | None:
+def roofline_of(root, stats: dict, particles: int, mips: int, live_voxels: int, workload: str) -> dict | None:
     """Roofline object of the dominant kernel of a timed pass: its algorithmic HBM
     bytes per launch over its average launch time against the HBM peak, and beside
-    it the fraction of the VALU issue rate its counted instructions take,
-    SQ_INSTS_VALU * 4 cycles / (CUs * 4 SIMDs * clock * time)."""
+    it the fraction of the VALU issue peak its counted wave instructions
+    (SQ_INSTS_VALU per launch) take over the same average time, against both the
+    published FP32 issue rate and the full-rate ceiling measured in this run."""
     if not stats:
         return None
     name, (tot_ms, launches) = max(stats.items(), key=lambda kv: kv[1][0])
@@ -112,8 +125,12 @@ def roofline_of(stats: dict, particles: int, mips: int, live_voxels: int, worklo
            "kernel": name, "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes}
     valu = committed_valu(name, workload)
     if valu is not None:
-        out["valu"] = {"insts_per_launch": valu, "peak_insts_per_s": VALU_PEAK,
-                       "frac": round(valu / (VALU_PEAK * avg_s), 4)}
+        forms = valu_measured(root)
+        out["valu"] = {"insts_per_launch": valu, "peak_insts_per_s": VALU_PEAK_PUBLISHED,
+                       "frac": round(valu / (VALU_PEAK_PUBLISHED * avg_s), 4),
+                       "measured_fmac_insts_per_s": forms["fmac"],
+                       "frac_of_measured": round(valu / (forms["fmac"] * avg_s), 4),
+                       "measured_insts_per_s_by_form": {f: float(f"{r:.4g}") for f, r in forms.items()}}
     return out
 
 
@@ -209,7 +226,7 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
         "workload": scene["name"], "particles": n, "ms_per_step": ms, "value": n / (ms / 1e3),
         "substeps": scene["options"]["substeps"], "mips": mips, "live_voxels": live,
         "kernels_ms_per_step": per_step(stats, ks),
-        "roofline": roofline_of(stats, n, mips, live, name) if splat == "tiled" else None,
+        "roofline": roofline_of(root, stats, n, mips, live, name) if splat == "tiled" else None,
         "splat_atomics": atomics,
         "tuning": dict(sim.tuning, source=tune_src),
         "scene": scene,
@@ -440,7 +457,7 @@ def measure_batched(root, args, stream) -> dict:
         "value": n / (ms / 1e3), "substeps": opts["substeps"], "mips": mips, "live_voxels": live,
         "tuning": dict(b.tuning, source=tune_src), "graph": not args.no_graph,
         "kernels_ms_per_step": per_step(stats, 3),
-        "roofline": roofline_of(stats, n, mips, live, "batched-256"),
+        "roofline": roofline_of(root, stats, n, mips, live, "batched-256"),
         "splat_atomics": atomics,
         "error": err, "scene": scene,
     }
@@ -478,7 +495,7 @@ def measure_coupled(root, args, stream) -> dict:
             out.update(entry)
             out["live_voxels"] = int(sum(int(cb.fluid.live_tiles(s, m).sum()) * 512 for s in range(args.scenes)
                                          for m in range(out["mips"])))
-            out["roofline"] = roofline_of(stats, out["particles"], out["mips"], out["live_voxels"], "coupled")
+            out["roofline"] = roofline_of(root, stats, out["particles"], out["mips"], out["live_voxels"], "coupled")
         else:
             out["plain_f32"] = entry
         cb.destroy()

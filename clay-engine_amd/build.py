@@ -21,7 +21,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libclay_hip.so"
-SOURCES = ["api.hip", "atomic_peak.hip", "edit.hip", "fluid.hip", "pbd.hip", "query.hip", "sort.hip", "splat.hip", "volume.hip", "wire.hip"]
+SOURCES = ["api.hip", "peaks.hip", "edit.hip", "fluid.hip", "pbd.hip", "query.hip", "sort.hip", "splat.hip", "volume.hip", "wire.hip"]
 ARCH = "gfx950"
 
 

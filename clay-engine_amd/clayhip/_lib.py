@@ -323,6 +323,7 @@ SIGNATURES = {
     "clay_splat_bake": ([P, P], C.c_int),
     "clay_splat_bake_static": ([P, P, P, P, P], C.c_int),
     "clay_atomic_peak": ([P, I32, U32, DP], C.c_int),
+    "clay_valu_peak": ([P, I32, U32, DP], C.c_int),
     "clay_fluid_splat_atomics": ([P, P], C.c_int),
     "clay_batch_splat_atomics": ([P, P], C.c_int),
     "clay_splat_set_center": ([P, DP, P], C.c_int),

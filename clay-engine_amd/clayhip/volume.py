@@ -29,6 +29,17 @@ class Root:
     def synchronize(self) -> None:
         L.check(L.lib().clay_ctx_synchronize(self.handle))
 
+    VALU_FORMS = ("fmac", "fmac_sgpr", "fma_vop3", "fma_mix", "max", "cndmask", "rsq", "add_u32", "mul_f32")
+
+    def valu_peak(self, form: str = "fmac", rounds: int = 16384) -> float:
+        """Measured wave64 VALU instructions per second of one instruction form
+        (independent register chains at full occupancy); "fmac" is the full-rate ceiling."""
+        if form not in self.VALU_FORMS:
+            raise ValueError(f"unknown VALU form {form!r}; one of {self.VALU_FORMS}")
+        out = C.c_double()
+        L.check(L.lib().clay_valu_peak(self.handle, self.VALU_FORMS.index(form), int(rounds), C.byref(out)))
+        return out.value
+
     def atomic_peak(self, kind: str, rounds: int = 4096) -> float:
         """Measured issue rate (ops/s) of 'lds_min' (ds_min_u32, conflict-free) or
         'global_min' (global atomicMin, a wave per 256-byte span)."""

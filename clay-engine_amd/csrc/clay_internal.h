@@ -336,8 +336,9 @@ size_t splat_bin_slices();
 // (the device must be current), once per device.
 int splat_static_init(int device);
 size_t splat_static_bytes();
-// atomic_peak.hip
+// peaks.hip
 int atomic_peak(Ctx* c, int kind, uint32_t rounds, double* ops_per_s);
+int valu_peak(Ctx* c, int form, uint32_t rounds, double* wave_insts_per_s);
 int splat_bake_static(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_desc* desc, uint16_t* dist,
                       hipStream_t s);
 // volume.hip

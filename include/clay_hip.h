@@ -281,6 +281,22 @@ int clay_splat_tiles_download(clay_splat* s, int32_t mip, uint8_t* live_host);
 #define CLAY_ATOMIC_LDS_MIN 0
 #define CLAY_ATOMIC_GLOBAL_MIN 1
 int clay_atomic_peak(clay_ctx* ctx, int32_t kind, uint32_t rounds, double* ops_per_s);
+/* Measured VALU issue rate of this device for one instruction form: wave64
+ * instructions retired per second, from eight independent register chains per lane
+ * at full occupancy, `rounds` (1..2^20) per chain.  Synchronous.  No counterpart in
+ * the reference: CLAY_VALU_FMAC (v_fmac_f32, all-VGPR operands) is the full-rate
+ * ceiling the neighbour and splat kernels' instruction counts are set against, and
+ * the other forms are the slower encodings those kernels' loops contain. */
+#define CLAY_VALU_FMAC 0      /* v_fmac_f32 v, v, v (compiled from fmaf) */
+#define CLAY_VALU_FMAC_SGPR 1 /* v_fmac_f32 v, s, v */
+#define CLAY_VALU_FMA_VOP3 2  /* v_fma_f32 v, v, v, v */
+#define CLAY_VALU_FMA_MIX 3   /* v_fma_mix_f32 (f16 operand widened) */
+#define CLAY_VALU_MAX 4       /* v_max_f32 */
+#define CLAY_VALU_CNDMASK 5   /* v_cndmask_b32 with an SGPR-pair mask */
+#define CLAY_VALU_RSQ 6       /* v_rsq_f32 (transcendental unit) */
+#define CLAY_VALU_ADD_U32 7   /* v_add_u32 */
+#define CLAY_VALU_MUL_F32 8   /* v_mul_f32 */
+int clay_valu_peak(clay_ctx* ctx, int32_t form, uint32_t rounds, double* wave_insts_per_s);
 /* Bakes the surface of the current state again (SplatField.bake, splat.ts:296-304,
  * with the same codes) and counts the atomics of its owner walk: out[0] the LDS
  * ds_min_u32 (one per walked voxel), out[1] the global atomicMin of the merge.
