@@ -272,18 +272,32 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
 // the diagonals within a few frames); the smallest level whose 26 classes are
 // all filled gives the radius.
 // ---------------------------------------------------------------------------
+// Direction classes of a particle's neighbours for the interior radius: a
+// neighbour with 0.1875 h < |d| < 1.5 h sets the bit of its lattice direction
+// (per axis, the sign of d_a when d_a^2 > (0.35 |d|)^2, else 0) in the masks of
+// the distance levels it is within (h, 1.25 h, 1.5 h).  Candidates outside the
+// shell are branched over (they are mostly outside for the whole wave at once);
+// inside it, every test is a sign bit of an IEEE difference (neg_mask), which
+// gives the comparisons' outcome bit for bit from full-rate integer and VGPR
+// operations.
 struct Cover {
   uint32_t m0 = 0, m1 = 0, m2 = 0;
-  __device__ __forceinline__ void add(f3 d, float r2, float hh) {
-    if (!(r2 > 0.03515625f * hh) || !(r2 < 2.25f * hh)) return;  // 0.1875 h < |d| < 1.5 h
-    const float t2 = 0.1225f * r2;         // (0.35 |d|)^2
-    const int sx = d.x * d.x > t2 ? (d.x > 0.0f ? 1 : -1) : 0;
-    const int sy = d.y * d.y > t2 ? (d.y > 0.0f ? 1 : -1) : 0;
-    const int sz = d.z * d.z > t2 ? (d.z > 0.0f ? 1 : -1) : 0;
-    const uint32_t bit = 1u << (13 + sx + 3 * sy + 9 * sz);
-    if (r2 < hh) m0 |= bit;
-    if (r2 < 1.5625f * hh) m1 |= bit;
-    m2 |= bit;
+  float lo, hi, l0, l1, c35;
+  uint32_t one;
+  __device__ __forceinline__ explicit Cover(float hh)
+      : lo(vreg(0.03515625f * hh)), hi(vreg(2.25f * hh)), l0(vreg(hh)), l1(vreg(1.5625f * hh)),
+        c35(vreg(0.1225f)), one(vreg_u(1u)) {}
+  __device__ __forceinline__ void add(f3 d, float r2) {
+    if (!(r2 > lo) || !(r2 < hi)) return;
+    const float t2 = c35 * r2;  // (0.35 |d|)^2
+    const int32_t sx = neg_mask(t2 - d.x * d.x) & (neg_mask(d.x) | 1);             // -1, 0, 1
+    const int32_t sy = neg_mask(t2 - d.y * d.y) & ((neg_mask(d.y) & -6) + 3);       // -3, 0, 3
+    const int32_t sz1 = neg_mask(t2 - d.z * d.z) & (neg_mask(d.z) | 1);
+    const int32_t sz = sz1 + (int32_t)((uint32_t)sz1 << 3);                         // -9, 0, 9
+    const uint32_t b = one << (uint32_t)(13 + sx + sy + sz);
+    m0 |= b & (uint32_t)neg_mask(r2 - l0);
+    m1 |= b & (uint32_t)neg_mask(r2 - l1);
+    m2 |= b;
   }
   // Squared radius (1.118 dmax)^2 = 1.25 dmax^2, or +inf when no level is full.
   __device__ __forceinline__ float radius2(float hh) const {
@@ -314,7 +328,7 @@ __global__ void __launch_bounds__(kB)
   int32_t b[3];
   cell_of(k, pi, b);
   float rho = 0.0f;
-  Cover cov;
+  Cover cov(k.hh);
   for_neighbours(k, cs, epoch, b, [&](uint32_t t) {
     const f3 d = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
     const float r2 = dot3(d, d);
@@ -322,7 +336,7 @@ __global__ void __launch_bounds__(kB)
       const float w = k.hh - r2;
       rho = rho + k.mass_poly6 * (w * w * w);
     }
-    if (kCover) cov.add(d, r2, k.hh);
+    if (kCover) cov.add(d, r2);
   });
   srho[s] = fmax_s(rho, k.rest_floor);
   spres[s] = fmax_s(k.stiff * (rho - k.rest), 0.0f);
@@ -652,16 +666,17 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
   const f3 oi = cell_rel(k, pi, b);
   // Compensated density: each row's terms (at most a few dozen) are summed plainly,
   // and the nine row sums are added with Neumaier's compensation, so the rounding
-  // of the total is that of one row rather than of every term.
+  // of the total is that of one row rather than of every term.  The sums are of
+  // (hh - r2)^3, clamped at zero beyond h instead of branched on; the mass and
+  // kernel factor multiply the total once.
   float rho_hi = 0.0f, rho_lo = 0.0f, rho = 0.0f;
-  Cover cov;
+  const float hh_v = vreg(k.hh);
+  Cover cov(k.hh);
   auto term = [&](f3 d) {
     const float r2 = dot3_fma(d);
-    if (r2 < k.hh) {
-      const float w = k.hh - r2;
-      rho = rho + k.mass_poly6 * (w * w * w);
-    }
-    if (kCover) cov.add(d, r2, k.hh);
+    const float w = __builtin_fmaxf(hh_v - r2, 0.0f);
+    rho = __builtin_fmaf(w * w, w, rho);
+    if (kCover) cov.add(d, r2);
   };
   auto fold_row = [&]() {
     const float t = rho_hi + rho;
@@ -684,10 +699,19 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
-        for (uint32_t t = sp.lo; t < sp.hi; t++) {
-          const uint2 u = stage[wave][t - ulo];
+        auto staged_term = [&](uint2 u) {
           term(mk(pi.x - __uint_as_float(u.x), sub_f16_lo(by, u.y), sub_f16_hi(bz, u.y)));
+        };
+        // Two candidates per trip: half the loop counting and exit tests.
+        const uint2* q = &stage[wave][0] + (any ? sp.lo - ulo : 0u);
+        const uint32_t cnt = any ? sp.hi - sp.lo : 0u;
+        uint32_t j = 0;
+        for (; j + 2 <= cnt; j += 2) {
+          const uint2 u0 = q[j], u1 = q[j + 1];
+          staged_term(u0);
+          staged_term(u1);
         }
+        if (j < cnt) staged_term(q[j]);
       } else {
         walk_row(k, r, [&](uint32_t t) { term(pi - mk(spx[t], spy[t], spz[t])); });
       }
@@ -696,7 +720,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
     }
   }
   if (!active) return;
-  rho = rho_hi + rho_lo;
+  rho = (rho_hi + rho_lo) * k.mass_poly6;
   const float rr = fmax_s(rho, k.rest_floor), pr = fmax_s(k.stiff * (rho - k.rest), 0.0f);
   srho[s] = rr;
   spres[s] = pr;
@@ -736,14 +760,14 @@ __global__ void __launch_bounds__(kB)
   int32_t b[3] = {0, 0, 0};
   if (active) cell_of(k, pi, b);
   float rho = 0.0f;
-  Cover cov;
+  Cover cov(k.hh);
   auto term = [&](f3 d) {
     const float r2 = dot3(d, d);
     if (r2 < k.hh) {
       const float w = k.hh - r2;
       rho = rho + k.mass_poly6 * (w * w * w);
     }
-    if (kCover) cov.add(d, r2, k.hh);
+    if (kCover) cov.add(d, r2);
   };
   // Cells beyond reach hold only terms that add nothing (the interior radius
   // classes neighbours out to 1.5 h): skipping them keeps every sum bit for bit.
@@ -911,23 +935,25 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
   f3 vsum = vi * k.h6;
   float wsum = k.h6;
   // The neighbour's 1 / density and pressure / density come in precomputed (once
-  // per staged particle, not once per pair), the constant factor m spiky / 2 is
-  // taken once, and r, 1 / r come from the hardware reciprocal square root (1 ulp):
-  // r > 1e-6 is tested as r^2 > 1e-12.
-  const float cpress = k.mass * 0.5f * k.spiky;
-  // The self pair (t == s) is within h: it is left out inside the interacting branch.
-  auto term = [&](uint32_t t, f3 dv, f3 vj, float inv, float pinv) {
+  // per staged particle, not once per pair), r and 1 / r come from the hardware
+  // reciprocal square root (1 ulp), r > 1e-6 is tested as r^2 > 1e-12, and the
+  // constant factor -m spiky / 2 multiplies the summed pressure force once.  The
+  // constants sit in VGPRs (full-rate operands).  The branch on r2 < hh is kept:
+  // consecutive sorted lanes walk the same candidates in step, so a candidate
+  // beyond h is usually beyond h for the whole wave and its work is skipped.
+  const float hh_v = vreg(k.hh), h_v = vreg(k.h), r2_min = vreg(1e-12f);
+  auto term = [&](f3 dv, f3 vj, float inv, float pinv) {
     const float r2 = dot3_fma(dv);
-    if (r2 >= k.hh || t == s) return;
-    const float w0 = k.hh - r2;
+    if (!(r2 < hh_v)) return;
+    const float w0 = hh_v - r2;
     const float w = w0 * w0 * w0;
     vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
     wsum = wsum + w;
-    if (r2 > 1e-12f) {
+    if (r2 > r2_min) {
       const float ir = __builtin_amdgcn_rsqf(r2);
-      const float hr = k.h - r2 * ir;
-      const float f = cpress * (pri * inv + pinv) * (hr * hr) * ir;
-      fp = mk(fp.x - dv.x * f, fp.y - dv.y * f, fp.z - dv.z * f);
+      const float hr = h_v - r2 * ir;
+      const float f = (pri * inv + pinv) * (hr * hr) * ir;
+      fp = mk(fp.x + dv.x * f, fp.y + dv.y * f, fp.z + dv.z * f);
     }
   };
   for (int dz = -1; dz <= 1; dz++) {
@@ -943,21 +969,29 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
-        for (uint32_t t = sp.lo; t < sp.hi; t++) {
-          const StagedNbr& e = stage[wave][t - ulo];
-          term(t, mk(pi.x - __uint_as_float(e.pos.x), sub_f16_lo(by, e.pos.y), sub_f16_hi(bz, e.pos.y)),
-               mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
-        }
+        auto walk = [&](uint32_t a, uint32_t e_) {
+          for (uint32_t t = a; t < e_; t++) {
+            const StagedNbr& e = stage[wave][t - ulo];
+            term(mk(pi.x - __uint_as_float(e.pos.x), sub_f16_lo(by, e.pos.y), sub_f16_hi(bz, e.pos.y)),
+                 mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+          }
+        };
+        // The self pair is left out by the bounds: [lo, s) and (s, hi).  A row that
+        // does not hold s has s outside [lo, hi), and one of the two ranges is empty.
+        walk(sp.lo, s < sp.hi ? s : sp.hi);
+        walk(s + 1 > sp.lo ? s + 1 : sp.lo, sp.hi);
       } else {
         walk_row(k, r, [&](uint32_t t) {
+          if (t == s) return;
           const float inv = 1.0f / srho[t];
-          term(t, pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), inv, spres[t] * inv);
+          term(pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), inv, spres[t] * inv);
         });
       }
       wave_sync();
     }
   }
   if (!active) return;
+  fp = fp * -(k.mass * 0.5f * k.spiky);
   finish_particle(k, sk, col, scene, s, pi, vi, rhoi, pri, fp, vsum, wsum, svals, out,
                   contacts + (size_t)scene * sk.contact_stride);
 }

@@ -5,9 +5,9 @@
 // v_add_f32, v_mul_f32, v_xor_b32, v_mov_b32) issue at close to two per CU per
 // cycle; the same operations with an SGPR operand, three-operand forms, and
 // v_cndmask/v_cmp/v_max/v_fma_mix at about one; the transcendental unit (v_exp,
-// v_rsq, v_sqrt, v_rcp) at one half.  k_valu_peak measures the full-rate ceiling
-// with independent v_fmac_f32 chains at full occupancy, k_valu_form the rate of
-// each of the other encodings the neighbour loops are made of.
+// v_rsq, v_sqrt, v_rcp) at one half.  k_valu_form measures the rate of one form
+// (v_fmac_f32 with VGPR operands among them, and the slower encodings the
+// neighbour loops are made of) with independent chains at full occupancy.
 //
 // The bake's walk is a stream of unsigned min-atomics: ds_min_u32 into an LDS copy
 // of a brick (the tiled schedules) and global atomicMin into the code grid (the
@@ -61,31 +61,10 @@ __global__ void __launch_bounds__(kPeakThreads)
   }
 }
 
-// Eight independent accumulators per lane, each a chain of a += x * y with x and y
-// per-lane registers, so every step is the two-operand VGPR multiply-accumulate
-// (v_fmac_f32), the FP32 form gfx950 issues at its full rate; 8 x `rounds`
-// instructions per lane, and the result is stored so nothing is folded away.
-__global__ void __launch_bounds__(256) k_valu_peak(uint32_t rounds, float x0, float y0, float* __restrict__ sink) {
-  const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-  const float x = x0 * (1.0f + (float)(g & 3) * 0x1p-20f), y = y0 * (1.0f + (float)(g & 7) * 0x1p-20f);
-  float a0 = (float)g, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f, a4 = a0 + 4.0f, a5 = a0 + 5.0f,
-        a6 = a0 + 6.0f, a7 = a0 + 7.0f;
-  for (uint32_t r = 0; r < rounds; r++) {
-    a0 = __builtin_fmaf(x, y, a0);
-    a1 = __builtin_fmaf(x, y, a1);
-    a2 = __builtin_fmaf(x, y, a2);
-    a3 = __builtin_fmaf(x, y, a3);
-    a4 = __builtin_fmaf(x, y, a4);
-    a5 = __builtin_fmaf(x, y, a5);
-    a6 = __builtin_fmaf(x, y, a6);
-    a7 = __builtin_fmaf(x, y, a7);
-  }
-  sink[g] = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
-}
-
-// The other encodings, one instruction written out per step so the compiler can
-// neither fuse nor re-encode it: eight independent register chains per lane, all
-// register-to-register (no memory access inside the loop).
+// One instruction form per kernel, written out per step so the compiler can neither
+// fuse nor re-encode it: eight independent register chains per lane, all
+// register-to-register (no memory access inside the loop), and the result stored
+// so nothing is folded away.
 #define CLAY_RATE_OP(form, text, ...)                                                    \
   if constexpr (kForm == form) {                                                       \
     asm volatile(text : "+v"(a0) : __VA_ARGS__);                                       \
@@ -107,6 +86,7 @@ __global__ void __launch_bounds__(256) k_valu_form(uint32_t rounds, float xs, fl
   float a0 = 1e-3f * (float)g, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f, a4 = a0 + 4.0f, a5 = a0 + 5.0f,
         a6 = a0 + 6.0f, a7 = a0 + 7.0f;
   for (uint32_t r = 0; r < rounds; r++) {
+    CLAY_RATE_OP(CLAY_VALU_FMAC, "v_fmac_f32_e32 %0, %1, %2", "v"(x), "v"(y))
     CLAY_RATE_OP(CLAY_VALU_FMAC_SGPR, "v_fmac_f32_e32 %0, %1, %2", "s"(xs), "v"(y))
     CLAY_RATE_OP(CLAY_VALU_FMA_VOP3, "v_fma_f32 %0, %0, %1, %2", "v"(x), "v"(y))
     CLAY_RATE_OP(CLAY_VALU_FMA_MIX, "v_fma_mix_f32 %0, %1, -1.0, %0 op_sel_hi:[1,0,0]", "v"(h2))
@@ -138,7 +118,7 @@ int valu_peak(Ctx* c, int form, uint32_t rounds, double* wave_insts_per_s) {
     switch (form) {
 #define CLAY_RATE_CASE(f, kern) \
   case f: hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, rounds, x, y, sink); break;
-      CLAY_RATE_CASE(CLAY_VALU_FMAC, k_valu_peak)
+      CLAY_RATE_CASE(CLAY_VALU_FMAC, k_valu_form<CLAY_VALU_FMAC>)
       CLAY_RATE_CASE(CLAY_VALU_FMAC_SGPR, k_valu_form<CLAY_VALU_FMAC_SGPR>)
       CLAY_RATE_CASE(CLAY_VALU_FMA_VOP3, k_valu_form<CLAY_VALU_FMA_VOP3>)
       CLAY_RATE_CASE(CLAY_VALU_FMA_MIX, k_valu_form<CLAY_VALU_FMA_MIX>)

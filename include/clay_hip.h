@@ -284,10 +284,10 @@ int clay_atomic_peak(clay_ctx* ctx, int32_t kind, uint32_t rounds, double* ops_p
 /* Measured VALU issue rate of this device for one instruction form: wave64
  * instructions retired per second, from eight independent register chains per lane
  * at full occupancy, `rounds` (1..2^20) per chain.  Synchronous.  No counterpart in
- * the reference: CLAY_VALU_FMAC (v_fmac_f32, all-VGPR operands) is the full-rate
- * ceiling the neighbour and splat kernels' instruction counts are set against, and
- * the other forms are the slower encodings those kernels' loops contain. */
-#define CLAY_VALU_FMAC 0      /* v_fmac_f32 v, v, v (compiled from fmaf) */
+ * the reference: the two-operand all-VGPR forms (CLAY_VALU_FMAC, _ADD_U32, _MUL_F32)
+ * give the full-rate ceiling the neighbour and splat kernels' instruction counts are
+ * set against, and the other forms are the slower encodings those loops contain. */
+#define CLAY_VALU_FMAC 0      /* v_fmac_f32 v, v, v */
 #define CLAY_VALU_FMAC_SGPR 1 /* v_fmac_f32 v, s, v */
 #define CLAY_VALU_FMA_VOP3 2  /* v_fma_f32 v, v, v, v */
 #define CLAY_VALU_FMA_MIX 3   /* v_fma_mix_f32 (f16 operand widened) */
