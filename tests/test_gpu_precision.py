@@ -78,6 +78,8 @@ def test_fp16lds_error_through_impact(root):
         assert rho_top > 2.0 * sim.constants.rest_density, "the column must pile up"
         sim.destroy()
     world.destroy()
+    print({p: dict(zip(("position_over_h_rms", "position_over_h_max", "density_rel_rms", "collision_flips"), w))
+           for p, w in worst.items()})
     rms, emax, rho_rms, _ = worst["f32"]
     assert rms < 2e-6 and emax < 2e-5 and rho_rms < 1e-6, worst
     rms, emax, rho_rms, _ = worst["fp16lds"]
