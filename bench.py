@@ -699,7 +699,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.precision == "f32" else "f32 (neighbour offsets staged as f16 in LDS)",
         "data": "synthetic (seeded jittered lattices)",
         "config": {
             "workload": r["workload"],

@@ -60,9 +60,10 @@ def test_step_and_bake_bit_exact(root, small, mode):
         codes = sim.surface.codes(m)
         assert np.array_equal(codes, grids[m]), f"mip {m}: {(codes != grids[m]).sum()} codes differ"
         assert np.array_equal(sim.surface.volume.download(m), tex[m]), f"mip {m}: texels differ"
-    n, got_pts = sim.readContacts(), None
+    got_pts = sim.readContacts()
     raw, want_pts = orc.read_contacts()
     assert sim.lastContactCount == raw
+    assert got_pts == [tuple(float(np.float32(v)) for v in p) for p in want_pts]
     sim.destroy()
 
 
