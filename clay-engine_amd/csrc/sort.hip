@@ -44,32 +44,35 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // One block per digit: exclusive scan of hist[d][0..ntiles) in place; total -> totals[d].
+// Each thread sums a contiguous segment of the row, one block-wide scan of the 256
+// segment sums (wave shuffles, then the four wave totals through LDS) gives each
+// segment its offset, and the thread rewrites its segment as running sums.
 __global__ void __launch_bounds__(kThreads)
     k_scan_rows(uint32_t* __restrict__ hist, uint32_t ntiles, uint32_t* __restrict__ totals) {
-  __shared__ uint32_t sh[kThreads];
-  __shared__ uint32_t carry;
+  __shared__ uint32_t wsum[kWaves];
   uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < ntiles; c0 += kThreads) {
-    const uint32_t i = c0 + threadIdx.x;
-    const uint32_t v = i < ntiles ? row[i] : 0u;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 256 entries.
-    for (int off = 1; off < kThreads; off <<= 1) {
-      const uint32_t add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0u;
-      __syncthreads();
-      sh[threadIdx.x] += add;
-      __syncthreads();
-    }
-    const uint32_t incl = sh[threadIdx.x];
-    if (i < ntiles) row[i] = carry + incl - v;
-    __syncthreads();
-    if (threadIdx.x == kThreads - 1) carry += incl;
-    __syncthreads();
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  const uint32_t per = (ntiles + kThreads - 1) / kThreads;
+  const uint32_t lo = t * per < ntiles ? t * per : ntiles;
+  const uint32_t hi = lo + per < ntiles ? lo + per : ntiles;
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; i++) sum += row[i];
+  uint32_t incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += v;
   }
-  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t acc = incl - sum;
+  for (uint32_t w = 0; w < wave; w++) acc += wsum[w];
+  for (uint32_t i = lo; i < hi; i++) {
+    const uint32_t v = row[i];
+    row[i] = acc;
+    acc += v;
+  }
+  if (t == kThreads - 1) totals[blockIdx.x] = acc;
 }
 
 template <int W, int kRounds>
