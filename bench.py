@@ -239,7 +239,8 @@ def measure_single(root, name: str, splat: str, precision: str, steps: int, warm
 
 
 def cpu_protocol(step, budget_s: float) -> dict:
-    """BASELINE.md's timing protocol within a time budget: warm-up steps (5, or
+    """The CPU timing protocol (5 warm-up steps, then 3 runs of 100 timed steps,
+    median run) within a time budget: warm-up steps (5, or
     fewer when one step takes more than 3% of the budget), then 3 runs of K timed
     steps (100, or as many as a third of the budget holds), wall clock around each
     run; the median run's seconds per step."""
@@ -293,7 +294,7 @@ def cpu_baseline_single(scene, budget_s: float, threads: int) -> dict:
         "value": n / pr["s_per_step"], "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
         "sample": f"the whole workload ({n} particles, 2 substeps + {opts['bakeMipLevels']}-mip bake per step): "
                   f"{pr['warmup']} warm-up step(s), then 3 runs of {pr['timed']} timed step(s) (s/step "
-                  f"{pr['runs']}), median run; oracle f32 mode with OpenMP. Fewer than BASELINE.md's 5 + 100 "
+                  f"{pr['runs']}), median run; oracle f32 mode with OpenMP. Fewer than the protocol's 5 + 100 "
                   f"steps where the {budget_s:.0f} s budget holds fewer",
     }
 
@@ -337,7 +338,7 @@ def cpu_baseline_batched(scene, budget_s: float, threads: int) -> dict:
         "sample": f"scenes {picked} of the batch ({done} particles; a step is 2 substeps + the "
                   f"{opts['bakeMipLevels']}-mip following bake of each): {pr['warmup']} warm-up step(s), then 3 "
                   f"runs of {pr['timed']} timed step(s) (s/step {pr['runs']}), median run; oracle f32 mode with "
-                  f"OpenMP. Fewer than BASELINE.md's 5 + 100 steps where the {budget_s:.0f} s budget holds fewer",
+                  f"OpenMP. Fewer than the protocol's 5 + 100 steps where the {budget_s:.0f} s budget holds fewer",
     }
 
 
@@ -369,7 +370,7 @@ def measure_per_scene(root, world, scene, steps: int, stream) -> dict:
 
 def splat_atomics(root, counts: tuple[int, int], particles: int, stats: dict) -> dict:
     """The owner walk's atomics per bake (counted by a tallying re-bake of the same
-    state) against the measured issue rate of ds_min_u32 (SURVEY's splat roof)."""
+    state) against the measured issue rate of ds_min_u32, the splat's roof."""
     lds, glob = counts
     tot_ms, launches = stats.get("splat_brick", (0.0, 0))
     avg_s = tot_ms / launches / 1e3 if launches else float("nan")
@@ -385,7 +386,7 @@ def splat_atomics(root, counts: tuple[int, int], particles: int, stats: dict) ->
 
 
 def measure_batched_1m(root, args, stream, tuning: dict) -> dict:
-    """The north-star comparison at its stated size: 64 scenes of 16 384 particles
+    """The batched path at 1M particles in total: 64 scenes of 16 384 particles
     (1 048 576 in total), batched with FP16-LDS staging and a captured graph, with
     the batched workload's tuned launch shapes."""
     import clayhip
