@@ -618,6 +618,31 @@ __device__ __forceinline__ RowSpan row_span(const RowRun& r, f3 oi, int dy, int 
   return kept_span(r, keep);
 }
 
+// The range of the sorted order a wave stages for one row: the union of its
+// lanes' kept spans.  For particles inside the grid a row's run [lo, hi) (the
+// first occupied cell's start, the last occupied cell's end) never decreases
+// along the sorted order: a later particle's cell has a key at least as large,
+// and so do the first and the last cell of its row.  Every kept span lies in
+// its lane's run, so the union lies in [lo of the first lane whose row is
+// occupied, hi of the last one): two readlanes instead of two wave reductions.
+// A wave holding a particle outside the grid (`mono` false: its rows can lie
+// anywhere) reduces the kept spans themselves.
+__device__ __forceinline__ void wave_union(const RowRun& r, const RowSpan& sp, bool any, bool mono, uint32_t& ulo,
+                                           uint32_t& uhi) {
+  if (mono) {
+    const uint64_t m = __ballot(r.hi > r.lo);
+    ulo = 0xFFFFFFFFu;
+    uhi = 0u;
+    if (m) {
+      ulo = (uint32_t)__builtin_amdgcn_readlane((int)r.lo, __builtin_ctzll(m));
+      uhi = (uint32_t)__builtin_amdgcn_readlane((int)r.hi, 63 - __builtin_clzll(m));
+    }
+  } else {
+    ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
+    uhi = wave_max_u32(any ? sp.hi : 0u);
+  }
+}
+
 // base - (the staged f16 offset): each component is one v_fma_mix_f32, which widens
 // the f16 half (exact) and computes base - x with one rounding -- the result of a
 // convert followed by a subtract, in one instruction instead of two.
@@ -664,6 +689,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
   int32_t b[3] = {0, 0, 0};
   if (active) cell_of(k, pi, b);
   const f3 oi = cell_rel(k, pi, b);
+  const bool mono = !__ballot(active && !in_grid(k, b[0], b[1], b[2]));  // wave_union
   // Compensated density: each row's terms (at most a few dozen) are summed plainly,
   // and the nine row sums are added with Neumaier's compensation, so the rounding
   // of the total is that of one row rather than of every term.  The sums are of
@@ -691,8 +717,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
       const RowSpan sp = row_span(r, oi, dy, dz, k.h, reach2);
       const bool any = sp.hi > sp.lo;
-      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
+      uint32_t ulo, uhi;
+      wave_union(r, sp, any, mono, ulo, uhi);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = spos[t];
@@ -772,6 +798,7 @@ __global__ void __launch_bounds__(kB)
   // Cells beyond reach hold only terms that add nothing (the interior radius
   // classes neighbours out to 1.5 h): skipping them keeps every sum bit for bit.
   const f3 oi = cell_rel(k, pi, b);
+  const bool mono = !__ballot(active && !in_grid(k, b[0], b[1], b[2]));  // wave_union
   const float reach2 = kCover ? 2.25f * k.hh : k.hh;
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
@@ -780,8 +807,8 @@ __global__ void __launch_bounds__(kB)
       row_keep(oi, dy, dz, k.h, reach2, 1.002f, keep);
       const RowSpan sp = kept_span(r, keep);
       const bool any = sp.hi > sp.lo;
-      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
+      uint32_t ulo, uhi;
+      wave_union(r, sp, any, mono, ulo, uhi);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = make_float4(spx[t], spy[t], spz[t], 0.0f);
@@ -861,6 +888,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
   };
   // Cells beyond h hold only terms that add nothing: skipping them keeps every sum.
   const f3 oi = cell_rel(k, pi, b);
+  const bool mono = !__ballot(active && !in_grid(k, b[0], b[1], b[2]));  // wave_union
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
@@ -868,8 +896,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
       row_keep(oi, dy, dz, k.h, k.hh, 1.002f, keep);
       const RowSpan sp = kept_span(r, keep);
       const bool any = sp.hi > sp.lo;
-      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
+      uint32_t ulo, uhi;
+      wave_union(r, sp, any, mono, ulo, uhi);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) {
@@ -931,6 +959,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
     cell_of(k, pi, b);
   }
   const f3 oi = cell_rel(k, pi, b);
+  const bool mono = !__ballot(active && !in_grid(k, b[0], b[1], b[2]));  // wave_union
   f3 fp = mk(0.0f, 0.0f, 0.0f);
   f3 vsum = vi * k.h6;
   float wsum = k.h6;
@@ -961,8 +990,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
       const RowRun r = row_run(k, cs, epoch, b, wrap_add(b[1], dy), wrap_add(b[2], dz), active);
       const RowSpan sp = row_span(r, oi, dy, dz, k.h, k.hh);
       const bool any = sp.hi > sp.lo;
-      const uint32_t ulo = wave_min_u32(any ? sp.lo : 0xFFFFFFFFu);
-      const uint32_t uhi = wave_max_u32(any ? sp.hi : 0u);
+      uint32_t ulo, uhi;
+      wave_union(r, sp, any, mono, ulo, uhi);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = frec[t];
