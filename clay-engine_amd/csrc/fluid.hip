@@ -814,8 +814,15 @@ __global__ void __launch_bounds__(kB)
         for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = make_float4(spx[t], spy[t], spz[t], 0.0f);
       wave_sync();
       if (staged) {
-        // The kept cells' candidates in increasing sorted index: the plain walk's order.
-        for (uint32_t t = sp.lo; t < sp.hi; t++) {
+        // The kept cells' candidates in increasing sorted index: the plain walk's
+        // order, two per trip with both loaded first.
+        uint32_t t = sp.lo;
+        for (; t + 2 <= sp.hi; t += 2) {
+          const float4 q0 = stage[wave][t - ulo], q1 = stage[wave][t + 1 - ulo];
+          term(mk(pi.x - q0.x, pi.y - q0.y, pi.z - q0.z));
+          term(mk(pi.x - q1.x, pi.y - q1.y, pi.z - q1.z));
+        }
+        if (t < sp.hi) {
           const float4 q = stage[wave][t - ulo];
           term(mk(pi.x - q.x, pi.y - q.y, pi.z - q.z));
         }
@@ -908,10 +915,18 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
         }
       wave_sync();
       if (staged) {
-        for (uint32_t t = sp.lo; t < sp.hi; t++) {
+        // In sorted order, two per trip with both positions loaded first.
+        auto cand = [&](uint32_t t, float4 pr) {
           const StagedX& e = stage[wave][t - ulo];
-          term(t, mk(pi.x - e.pr.x, pi.y - e.pr.y, pi.z - e.pr.z), mk(e.vp.x, e.vp.y, e.vp.z), e.pr.w, e.vp.w);
+          term(t, mk(pi.x - pr.x, pi.y - pr.y, pi.z - pr.z), mk(e.vp.x, e.vp.y, e.vp.z), pr.w, e.vp.w);
+        };
+        uint32_t t = sp.lo;
+        for (; t + 2 <= sp.hi; t += 2) {
+          const float4 p0 = stage[wave][t - ulo].pr, p1 = stage[wave][t + 1 - ulo].pr;
+          cand(t, p0);
+          cand(t + 1, p1);
         }
+        if (t < sp.hi) cand(t, stage[wave][t - ulo].pr);
       } else {
         walk_row(k, r, [&](uint32_t t) {
           term(t, mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]), mk(svx[t], svy[t], svz[t]), 1.0f / srho[t],
@@ -998,12 +1013,22 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
+        auto cand = [&](const StagedNbr& e, uint2 pos) {
+          term(mk(pi.x - __uint_as_float(pos.x), sub_f16_lo(by, pos.y), sub_f16_hi(bz, pos.y)),
+               mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+        };
+        // Two candidates per trip, both positions loaded before either is used:
+        // one LDS round trip and one loop test for two candidates.
         auto walk = [&](uint32_t a, uint32_t e_) {
-          for (uint32_t t = a; t < e_; t++) {
-            const StagedNbr& e = stage[wave][t - ulo];
-            term(mk(pi.x - __uint_as_float(e.pos.x), sub_f16_lo(by, e.pos.y), sub_f16_hi(bz, e.pos.y)),
-                 mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+          const StagedNbr* q = &stage[wave][0] + (a - ulo);
+          const uint32_t cnt = e_ > a ? e_ - a : 0u;
+          uint32_t j = 0;
+          for (; j + 2 <= cnt; j += 2) {
+            const uint2 p0 = q[j].pos, p1 = q[j + 1].pos;
+            cand(q[j], p0);
+            cand(q[j + 1], p1);
           }
+          if (j < cnt) cand(q[j], q[j].pos);
         };
         // The self pair is left out by the bounds: [lo, s) and (s, hi).  A row that
         // does not hold s has s outside [lo, hi), and one of the two ranges is empty.
