@@ -274,27 +274,29 @@ __global__ void __launch_bounds__(kBlock) k_reorder(uint32_t n, const uint32_t* 
 // ---------------------------------------------------------------------------
 // Direction classes of a particle's neighbours for the interior radius: a
 // neighbour with 0.1875 h < |d| < 1.5 h sets the bit of its lattice direction
-// (per axis, the sign of d_a when d_a^2 > (0.35 |d|)^2, else 0) in the masks of
+// (per axis, the sign of d_a when |d_a| >= 0.35 |d|, else 0) in the masks of
 // the distance levels it is within (h, 1.25 h, 1.5 h).  Candidates outside the
-// shell are branched over (they are mostly outside for the whole wave at once);
-// inside it, every test is a sign bit of an IEEE difference (neg_mask), which
-// gives the comparisons' outcome bit for bit from full-rate integer and VGPR
-// operations.
+// shell are branched over; inside it, the level tests are sign bits of IEEE
+// differences (neg_mask), exact and on full-rate integer operations.
 struct Cover {
   uint32_t m0 = 0, m1 = 0, m2 = 0;
-  float lo, hi, l0, l1, c35;
+  float lo, hi, l0, l1, c07, c15;
   uint32_t one;
   __device__ __forceinline__ explicit Cover(float hh)
       : lo(vreg(0.03515625f * hh)), hi(vreg(2.25f * hh)), l0(vreg(hh)), l1(vreg(1.5625f * hh)),
-        c35(vreg(0.1225f)), one(vreg_u(1u)) {}
+        c07(vreg(1.0f / 0.7f)), c15(vreg(1.5f)), one(vreg_u(1u)) {}
+  // Per axis, floor(d_a / (0.7 |d|) + 1.5) is 0, 1 or 2 as the unit component
+  // d_a / |d| is below -0.35, within [-0.35, 0.35) or at least 0.35.  Since
+  // |d_a / |d|| <= 1 the argument lies in [0.07, 2.93], where the truncating
+  // conversion is the floor.  1 / |d| comes from the hardware reciprocal square
+  // root, whose 1-ulp error only moves directions that lie on a class border.
   __device__ __forceinline__ void add(f3 d, float r2) {
     if (!(r2 > lo) || !(r2 < hi)) return;
-    const float t2 = c35 * r2;  // (0.35 |d|)^2
-    const int32_t sx = neg_mask(t2 - d.x * d.x) & (neg_mask(d.x) | 1);             // -1, 0, 1
-    const int32_t sy = neg_mask(t2 - d.y * d.y) & ((neg_mask(d.y) & -6) + 3);       // -3, 0, 3
-    const int32_t sz1 = neg_mask(t2 - d.z * d.z) & (neg_mask(d.z) | 1);
-    const int32_t sz = sz1 + (int32_t)((uint32_t)sz1 << 3);                         // -9, 0, 9
-    const uint32_t b = one << (uint32_t)(13 + sx + sy + sz);
+    const float k = __builtin_amdgcn_rsqf(r2) * c07;
+    const int32_t sx = (int32_t)__builtin_fmaf(d.x, k, c15);
+    const int32_t sy = (int32_t)__builtin_fmaf(d.y, k, c15);
+    const int32_t sz = (int32_t)__builtin_fmaf(d.z, k, c15);
+    const uint32_t b = one << (uint32_t)(sx + 3 * sy + 9 * sz);  // bit 13 + (sx - 1) + 3 (sy - 1) + 9 (sz - 1)
     m0 |= b & (uint32_t)neg_mask(r2 - l0);
     m1 |= b & (uint32_t)neg_mask(r2 - l1);
     m2 |= b;

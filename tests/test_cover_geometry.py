@@ -1,8 +1,8 @@
 """The surface bake's interior radius, checked against brute force on the CPU.
 
 The fluid's density pass classes each neighbour offset d (0.1875 h < |d| < 1.5 h)
-by the signs of its components above 0.35 |d| into one of the 26 lattice
-directions; when every class holds a neighbour closer than dmax (h, 1.25 h or
+by the signs of its components of magnitude at least 0.35 |d| into one of the 26
+lattice directions; when every class holds a neighbour closer than dmax (h, 1.25 h or
 1.5 h) the particle's squared interior radius is 1.25 dmax^2, and the splat skips
 its voxels beyond that radius, because a neighbour is then strictly nearer every
 such voxel centre (clay-engine_amd/csrc/fluid.hip, Cover).  The classes and the
@@ -21,9 +21,12 @@ T = 0.35                      # class threshold on |component| / |d|
 
 
 def classify(d: np.ndarray) -> np.ndarray:
-    r2 = (d * d).sum(1)
-    sg = np.where(d * d > (T * T) * r2[:, None], np.sign(d), 0).astype(np.int64)
-    return 13 + sg[:, 0] + 3 * sg[:, 1] + 9 * sg[:, 2]
+    """Per axis floor(d_a / (2 T |d|) + 1.5): 0, 1 or 2 as d_a / |d| is below -T,
+    within [-T, T) or at least T; the bit is s_x + 3 s_y + 9 s_z (Cover::add)."""
+    r = np.sqrt((d * d).sum(1))
+    s = np.floor(d / (2 * T * r[:, None]) + 1.5).astype(np.int64)
+    assert ((s >= 0) & (s <= 2)).all()
+    return s[:, 0] + 3 * s[:, 1] + 9 * s[:, 2]
 
 
 def interior(p: np.ndarray, nbrs: np.ndarray):
