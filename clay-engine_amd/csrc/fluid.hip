@@ -950,7 +950,12 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
                            const uint4* __restrict__ cells, StateOut out, uint32_t* __restrict__ contacts,
                            const StagedNbr* __restrict__ frec) {
 #pragma clang fp contract(fast)
-  __shared__ StagedNbr stage[kB / 64][kS];
+  // The staged records as three arrays (position, velocity and 1 / density,
+  // pressure / density): lanes reading different neighbours then hit different
+  // banks, where 32-byte records put every fourth record on the same banks.
+  __shared__ uint2 st_pos[kB / 64][kS];
+  __shared__ float4 st_vr[kB / 64][kS];
+  __shared__ float st_pinv[kB / 64][kS];
   const uint32_t s = (sk.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x) * kB + threadIdx.x;
   const bool active = s < n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1011,26 +1016,34 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
       wave_union(r, sp, any, mono, ulo, uhi);
       const bool staged = !__ballot(r.capped) && uhi > ulo && uhi - ulo <= (uint32_t)kS;
       if (staged)
-        for (uint32_t t = ulo + lane; t < uhi; t += 64) stage[wave][t - ulo] = frec[t];
+        for (uint32_t t = ulo + lane; t < uhi; t += 64) {
+          const StagedNbr e = frec[t];
+          st_pos[wave][t - ulo] = e.pos;
+          st_vr[wave][t - ulo] = e.vr;
+          st_pinv[wave][t - ulo] = e.pinv;
+        }
       wave_sync();
       if (staged) {
         const float by = oi.y - (float)dy * k.h, bz = oi.z - (float)dz * k.h;
-        auto cand = [&](const StagedNbr& e, uint2 pos) {
+        auto cand = [&](uint32_t j, uint2 pos) {
+          const float4 vr = st_vr[wave][j];
           term(mk(pi.x - __uint_as_float(pos.x), sub_f16_lo(by, pos.y), sub_f16_hi(bz, pos.y)),
-               mk(e.vr.x, e.vr.y, e.vr.z), e.vr.w, e.pinv);
+               mk(vr.x, vr.y, vr.z), vr.w, st_pinv[wave][j]);
         };
         // Two candidates per trip, both positions loaded before either is used:
         // one LDS round trip and one loop test for two candidates.
+        // [a, e_) lies in the staged [ulo, uhi) when it is not empty; an empty range
+        // (a lane whose row holds no candidate has a = 0xFFFFFFFF) walks nothing.
         auto walk = [&](uint32_t a, uint32_t e_) {
-          const StagedNbr* q = &stage[wave][0] + (a - ulo);
           const uint32_t cnt = e_ > a ? e_ - a : 0u;
+          const uint32_t base = cnt ? a - ulo : 0u;
           uint32_t j = 0;
           for (; j + 2 <= cnt; j += 2) {
-            const uint2 p0 = q[j].pos, p1 = q[j + 1].pos;
-            cand(q[j], p0);
-            cand(q[j + 1], p1);
+            const uint2 p0 = st_pos[wave][base + j], p1 = st_pos[wave][base + j + 1];
+            cand(base + j, p0);
+            cand(base + j + 1, p1);
           }
-          if (j < cnt) cand(q[j], q[j].pos);
+          if (j < cnt) cand(base + j, st_pos[wave][base + j]);
         };
         // The self pair is left out by the bounds: [lo, s) and (s, hi).  A row that
         // does not hold s has s outside [lo, hi), and one of the two ranges is empty.
