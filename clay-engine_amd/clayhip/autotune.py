@@ -14,6 +14,7 @@ from __future__ import annotations
 import itertools
 import json
 import os
+import sys
 from pathlib import Path
 
 DEFAULT_CACHE = Path(__file__).resolve().parent / "tuned.json"
@@ -76,11 +77,16 @@ def autotune(obj, key: str, frames: int = 3, space: dict = SPACE, cache: Path | 
 
     stream = torch.cuda.Stream()
     table = []
-    for cand in candidates(space):
+    cands = candidates(space)
+    for i, cand in enumerate(cands):
         if state is not None:
             state()
         obj.tuning = cand
         table.append({"tuning": cand, "ms": time_frames(obj, frames, stream)})
+        if (i + 1) % 32 == 0 or i + 1 == len(cands):
+            best = min(table, key=lambda r: r["ms"])
+            print(f"autotune {key}: {i + 1}/{len(cands)} candidates, best {best['ms']:.3f} ms",
+                  file=sys.stderr, flush=True)
     best = min(table, key=lambda r: r["ms"])
     obj.tuning = best["tuning"]
     if state is not None:
