@@ -115,13 +115,26 @@ __global__ void __launch_bounds__(kThreads)
   }
   const uint64_t lt = __lanemask_lt();
   const uint32_t base = blockIdx.x * kTileKeys;
+  // The keys and values of kAhead rounds are loaded together, before their rounds'
+  // barriers, so a group of rounds waits for memory once instead of once per round.
+  constexpr int kAhead = 8;
+  static_assert(kRounds % kAhead == 0, "rounds come in groups of kAhead");
+  uint32_t kk[kAhead], vv[kAhead];
   for (int r = 0; r < kRounds; r++) {
+    if (r % kAhead == 0) {
+#pragma unroll
+      for (int a = 0; a < kAhead; a++) {
+        const uint32_t ia = base + (r + a) * kThreads + t;
+        kk[a] = ia < n ? keys[ia] : 0u;
+        vv[a] = ia < n ? vals[ia] : 0u;
+      }
+    }
     const uint32_t i = base + r * kThreads + t;
     const bool active = i < n;
     uint32_t key = 0, val = 0, d = 0;
     if (active) {
-      key = keys[i];
-      val = vals[i];
+      key = kk[r % kAhead];
+      val = vv[r % kAhead];
       d = (key >> shift) & (kBins - 1u);
     }
 #pragma unroll
