@@ -992,10 +992,17 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
   // constants sit in VGPRs (full-rate operands).  The branch on r2 < hh is kept:
   // consecutive sorted lanes walk the same candidates in step, so a candidate
   // beyond h is usually beyond h for the whole wave and its work is skipped.
-  const float hh_v = vreg(k.hh), h_v = vreg(k.h), r2_min = vreg(1e-12f);
-  auto term = [&](f3 dv, f3 vj, float inv, float pinv) {
-    const float r2 = dot3_fma(dv);
+  const float hh_v = vreg(k.hh), h_v = vreg(k.h), r2_min = vreg(1e-12f), close2 = vreg(0.01f * k.hh);
+  auto term = [&](f3 dv, f3 vj, float inv, float pinv, uint32_t t) {
+    float r2 = dot3_fma(dv);
     if (!(r2 < hh_v)) return;
+    // A near-coincident pair (closer than h / 10, where the staged offsets' f16
+    // rounding, up to ~1.5e-4 h, would tilt the pair's direction by more than a
+    // thousandth) takes its offset from the f32 snapshot instead.
+    if (r2 < close2) {
+      dv = mk(pi.x - spx[t], pi.y - spy[t], pi.z - spz[t]);
+      r2 = dot3_fma(dv);
+    }
     const float w0 = hh_v - r2;
     const float w = w0 * w0 * w0;
     vsum = mk(vsum.x + vj.x * w, vsum.y + vj.y * w, vsum.z + vj.z * w);
@@ -1028,7 +1035,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
         auto cand = [&](uint32_t j, uint2 pos) {
           const float4 vr = st_vr[wave][j];
           term(mk(pi.x - __uint_as_float(pos.x), sub_f16_lo(by, pos.y), sub_f16_hi(bz, pos.y)),
-               mk(vr.x, vr.y, vr.z), vr.w, st_pinv[wave][j]);
+               mk(vr.x, vr.y, vr.z), vr.w, st_pinv[wave][j], ulo + j);
         };
         // Two candidates per trip, both positions loaded before either is used:
         // one LDS round trip and one loop test for two candidates.
@@ -1053,7 +1060,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
         walk_row(k, r, [&](uint32_t t) {
           if (t == s) return;
           const float inv = 1.0f / srho[t];
-          term(pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), inv, spres[t] * inv);
+          term(pi - mk(spx[t], spy[t], spz[t]), mk(svx[t], svy[t], svz[t]), inv, spres[t] * inv, t);
         });
       }
       wave_sync();
