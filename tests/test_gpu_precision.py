@@ -27,7 +27,7 @@ def test_per_substep_errors_against_f64(root):
     assert p["position_over_h"]["max"] < 1e-4
     assert q["density_rel"]["max"] < 2e-3
     assert q["position_over_h"]["rms"] < 2e-4
-    assert q["position_over_h"]["max"] < 5e-2
+    assert q["position_over_h"]["max"] < 1e-2
     assert q["density_rel"]["rms"] > p["density_rel"]["rms"]  # the staging is really f16
 
 
@@ -35,11 +35,10 @@ def test_fp16lds_error_through_impact(root):
     """Frames 16-47 of the 16k dam break, where the column hits the floor and piles
     up (densities to more than twice the rest density): every substep, the f64
     oracle takes one substep from the GPU's own state.  FP16-LDS keeps the RMS
-    position error below 1e-4 h and the density error below 1e-4 RMS.  Its
-    outliers come from pressure amplification in compressed piles (stiffness 220
-    times a density error of a few 1e-4), not from particles landing on the other
-    side of the collision threshold: the collision branch taken from the GPU's and
-    the oracle's velocities must agree for all but 0.05% of the particles."""
+    position error below 1e-4 h, the largest below 5e-3 h (near-coincident pairs
+    take their offsets in f32) and the density error below 1e-4 RMS; the collision
+    branch taken from the GPU's and the oracle's velocities must agree for all but
+    0.05% of the particles."""
     import oracle as O
 
     sc = scenes.dam_break_small()
@@ -83,7 +82,7 @@ def test_fp16lds_error_through_impact(root):
     rms, emax, rho_rms, _ = worst["f32"]
     assert rms < 2e-6 and emax < 2e-5 and rho_rms < 1e-6, worst
     rms, emax, rho_rms, _ = worst["fp16lds"]
-    assert rms < 1e-4 and emax < 2e-2 and rho_rms < 1e-4, worst
+    assert rms < 1e-4 and emax < 5e-3 and rho_rms < 1e-4, worst
 
 
 def test_conserved_quantities_in_fp64(root):
