@@ -5,9 +5,10 @@ f32, so its density agrees with f64 to 1e-5 (relative) and positions to 1e-4 h.
 The FP16-LDS path stages neighbour offsets from their own cell centre as f16
 (|o| <= h/2, rounding error <= 2^-16 ~ 1.5e-5 at h = 0.1), bounding the density
 error at 2e-3 relative and the RMS position error at 2e-4 h.  Its maximum
-position error is bounded loosely (0.05 h): the collision projection is
-discontinuous at the contact threshold, and a particle whose velocity differs in
-the last bits can land on the other side of it.
+position error is bounded at 1e-2 h: the collision projection is discontinuous
+at the contact threshold, and a particle whose velocity differs in the last bits
+can land on the other side of it.  A pair closer than h/10 takes its offset from
+the f32 positions, since f16 offsets would set its direction.
 """
 import numpy as np
 import pytest
@@ -102,3 +103,34 @@ def test_conserved_quantities_in_fp64(root):
     assert np.allclose(c["com"], pos.mean(axis=0), rtol=1e-12, atol=1e-12)
     sim.destroy()
     world.destroy()
+
+
+def test_near_coincident_pair_takes_its_direction_from_f32(root):
+    """Two particles 1.3e-3 h apart inside a block at rest, no gravity: their pair
+    pushes them apart along their offset.  Staged as f16 offsets from the cell
+    centre (rounding up to ~1.5e-4 h per component) that offset would be off by
+    tens of percent; the forces pass takes pairs closer than h/10 from the f32
+    positions, so both particles' staged velocities match the f64 oracle's to 1%."""
+    import oracle as O
+
+    pos = scenes.lattice(10, 10, 10, 0.05, (-0.25, 0.5, -0.25), seed=3)
+    k = int(np.argmin(np.linalg.norm(pos - pos.mean(axis=0), axis=1)))
+    extra = pos[k] + np.array([1.0e-4, 0.7e-4, 0.3e-4], dtype=np.float32)
+    pos = np.vstack([pos, extra[None, :]]).astype(np.float32)
+    brushes = [scenes.floor_brush()]
+    opts = dict(scenes.dam_break_options(pos.shape[0], 32, 1), precision="fp16lds", gravity=(0.0, 0.0, 0.0))
+    world = clayhip.analyticField(root, brushes)
+    sim = clayhip.FluidSim(root, world, opts)
+    sim.set_state(pos)
+    sim.substep()
+    got = sim.get_state("nvel").astype(np.float64)
+    col = O.Collider().add_analytic([clayhip.make_brush(b) for b in brushes])
+    ref = O.Fluid(opts, pos.astype(np.float64), collider=col, precision="f64")
+    ref.substep()
+    want = ref.get("nvel")
+    sim.destroy()
+    world.destroy()
+    for i in (k, pos.shape[0] - 1):
+        assert np.linalg.norm(want[i]) > 0.1, "the pair must repel"
+        err = np.linalg.norm(got[i] - want[i]) / np.linalg.norm(want[i])
+        assert err < 1e-2, f"particle {i}: relative nvel error {err:.3g}"
