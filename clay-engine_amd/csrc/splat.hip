@@ -605,7 +605,6 @@ __global__ void __launch_bounds__(kOwnerThreads)
   __shared__ uint32_t sh_brick;
   const int t = threadIdx.x;
   const uint32_t init = kNormals ? kEncSat : kSInit;
-  const uint32_t rev = __brev((uint32_t)t) >> (32 - kOwnerBits);
   const uint32_t items = scan->items;
   for (uint32_t item = mk_.xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x; item < items; item += gridDim.x) {
     if (t == 0) {
@@ -670,13 +669,27 @@ __global__ void __launch_bounds__(kOwnerThreads)
         }
       }
     };
+    // Entries [0, count) of a list, a pass of up to kOwnerThreads at a time: a pass
+    // of m entries goes to the fewest waves that hold it (W = ceil(m / 64)), lane l
+    // of wave w taking entry w + W rev6(l) (rev6: the 6-bit reversal).  The waves'
+    // lanes are full however short the list, and neighbouring lanes hold entries
+    // far apart (spatially apart in the binned order, so their ds_min_u32 rarely
+    // meet on one address); waves without work skip the pass.
+    auto packed = [&](uint32_t count, auto&& visit) {
+      const uint32_t w = (uint32_t)t >> 6, l = __brev((uint32_t)t & 63u) >> 26;
+      for (uint32_t e0 = 0; e0 < count; e0 += kOwnerThreads) {
+        const uint32_t m = count - e0 < (uint32_t)kOwnerThreads ? count - e0 : (uint32_t)kOwnerThreads;
+        const uint32_t W = (m + 63u) >> 6;
+        const uint32_t e = w + l * W;
+        if (w < W && e < m) visit(e0 + e);
+      }
+    };
     if (!kNormals && c.cover) {
       // A wave takes as long as its longest walk, so particles that walk their whole
       // reach (the fluid's surface) and interior ones (a voxel or so) go to different
       // waves: each sub-chunk is partitioned in LDS, surface particles first, and
-      // each part is walked in the spread lane order, so a wave's lanes hold
-      // particles of one part and far apart in it (distinct voxels for their
-      // ds_min_u32).
+      // each part is walked packed into the fewest waves, so a wave's lanes hold
+      // particles of one part.
       for (uint32_t sub = 0; sub < n_e; sub += kPart) {
         const uint32_t mm = n_e - sub < (uint32_t)kPart ? n_e - sub : (uint32_t)kPart;
         if (t < 2) part_n[t] = 0;
@@ -690,21 +703,12 @@ __global__ void __launch_bounds__(kOwnerThreads)
         }
         __syncthreads();
         const uint32_t ns = part_n[0], ni = mm - ns;
-        for (uint32_t e0 = 0; e0 < ns; e0 += kOwnerThreads) {
-          const uint32_t e = e0 + rev;
-          if (e < ns) walk(part[e]);
-        }
-        for (uint32_t e0 = 0; e0 < ni; e0 += kOwnerThreads) {
-          const uint32_t e = e0 + rev;
-          if (e < ni) walk(part[ns + e]);
-        }
+        packed(ns, [&](uint32_t e) { walk(part[e]); });
+        packed(ni, [&](uint32_t e) { walk(part[ns + e]); });
         __syncthreads();
       }
     } else {
-      for (uint32_t pass = 0; pass * kOwnerThreads < n_e; pass++) {
-        const uint32_t e = pass * kOwnerThreads + rev;
-        if (e < n_e) walk(list[base + e]);
-      }
+      packed(n_e, [&](uint32_t e) { walk(list[base + e]); });
     }
     __syncthreads();
     // Voxel coordinates from the flat index by multiply-high with ceil(2^32 / d):
