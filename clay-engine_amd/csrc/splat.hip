@@ -367,44 +367,56 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
   return v;
 }
 
-// The scans over bricks: one block, 1024 bricks per round, wave scans joined by a
-// scan of the 16 wave totals.
-__global__ void __launch_bounds__(1024)
+// The scans over bricks: one workgroup of four waves, each thread a run of
+// kScanRun consecutive bricks per round, run totals joined by wave scans and a scan
+// of the four wave totals.  The binning chain that ends in this launch runs on a
+// second stream beside the frame-end density pass, which holds every CU: a
+// four-wave workgroup is placed as soon as one of that pass's workgroups retires,
+// where a 1024-thread one waited for a CU with 16 free wave slots (5 us alone, about
+// 0.2 ms beside the density pass).
+constexpr int kScanThreads = 256, kScanRun = 8;
+__global__ void __launch_bounds__(kScanThreads)
     k_brick_scan(uint32_t nbricks, uint32_t chunk, const uint32_t* __restrict__ count,
                  uint32_t* __restrict__ start, uint32_t* __restrict__ item_start, ScanOut* __restrict__ out) {
-  __shared__ uint32_t wa[16], wb[16];
-  __shared__ uint32_t carry_a, carry_b;
+  __shared__ uint32_t wa[kScanThreads / 64], wb[kScanThreads / 64];
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  if (t == 0) {
-    carry_a = 0;
-    carry_b = 0;
-  }
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < nbricks; c0 += 1024) {
-    const uint32_t i = c0 + (uint32_t)t;
-    const uint32_t va = i < nbricks ? count[i] : 0u;
-    const uint32_t vb = (va + chunk - 1) / chunk;
-    const uint32_t ia = wave_incl_scan(va, lane), ib = wave_incl_scan(vb, lane);
+  uint32_t carry_a = 0, carry_b = 0;  // the same in every thread
+  for (uint32_t c0 = 0; c0 < nbricks; c0 += kScanThreads * kScanRun) {
+    const uint32_t i0 = c0 + (uint32_t)t * kScanRun;
+    uint32_t va[kScanRun], vb[kScanRun], sa = 0, sb = 0;
+#pragma unroll
+    for (int q = 0; q < kScanRun; q++) {
+      va[q] = i0 + q < nbricks ? count[i0 + q] : 0u;
+      vb[q] = (va[q] + chunk - 1) / chunk;
+      sa += va[q];
+      sb += vb[q];
+    }
+    const uint32_t ia = wave_incl_scan(sa, lane), ib = wave_incl_scan(sb, lane);
     if (lane == 63) {
       wa[wave] = ia;
       wb[wave] = ib;
     }
     __syncthreads();
-    uint32_t pa = carry_a, pb = carry_b;
-    for (int w = 0; w < wave; w++) {
-      pa += wa[w];
-      pb += wb[w];
+    uint32_t pa = carry_a + ia - sa, pb = carry_b + ib - sb;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; w++) {
+      if (w < wave) {
+        pa += wa[w];
+        pb += wb[w];
+      }
+      carry_a += wa[w];
+      carry_b += wb[w];
     }
-    if (i < nbricks) {
-      start[i] = pa + ia - va;
-      item_start[i] = pb + ib - vb;
+#pragma unroll
+    for (int q = 0; q < kScanRun; q++) {
+      if (i0 + q < nbricks) {
+        start[i0 + q] = pa;
+        item_start[i0 + q] = pb;
+      }
+      pa += va[q];
+      pb += vb[q];
     }
-    __syncthreads();
-    if (t == 1023) {
-      carry_a = pa + ia;
-      carry_b = pb + ib;
-    }
-    __syncthreads();
+    __syncthreads();  // wa and wb are written again by the next round
   }
   if (t == 0) {
     out->entries = carry_a;
@@ -1027,7 +1039,7 @@ static int tiled_prepare(Splat* sp, hipStream_t s, const MipsK& mk_, const Cloud
     const unsigned sgrid = (unsigned)(((size_t)nbricks * kSlices + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(k_brick_sum, dim3(sgrid), dim3(kThreads), 0, s, sp->brick_sub, (uint32_t)nbricks,
                        sp->brick_count);
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, s, (uint32_t)nbricks, sp->chunk, sp->brick_count,
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(kScanThreads), 0, s, (uint32_t)nbricks, sp->chunk, sp->brick_count,
                        sp->brick_start, sp->item_start, scan);
     hipLaunchKernelGGL(k_brick_cursor, dim3(sgrid), dim3(kThreads), 0, s, sp->brick_sub, (uint32_t)nbricks,
                        sp->brick_start, sp->brick_cursor);
