@@ -123,15 +123,23 @@ def roofline_of(root, stats: dict, particles: int, mips: int, live_voxels: int, 
     out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": committed_traffic(name, workload),
            "kernel": name, "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes}
-    valu = committed_valu(name, workload)
+    valu = valu_object(root, committed_valu(name, workload), avg_s)
     if valu is not None:
-        forms = valu_measured(root)
-        out["valu"] = {"insts_per_launch": valu, "peak_insts_per_s": VALU_PEAK_PUBLISHED,
-                       "frac": round(valu / (VALU_PEAK_PUBLISHED * avg_s), 4),
-                       "measured_peak_insts_per_s": max(forms.values()),
-                       "frac_of_measured": round(valu / (max(forms.values()) * avg_s), 4),
-                       "measured_insts_per_s_by_form": {f: float(f"{r:.4g}") for f, r in forms.items()}}
+        out["valu"] = valu
     return out
+
+
+def valu_object(root, valu: float | None, avg_s: float) -> dict | None:
+    """The VALU side of a roofline: `valu` wave instructions per launch over `avg_s`
+    against the published issue rate and the fastest form measured in this run."""
+    if valu is None:
+        return None
+    forms = valu_measured(root)
+    return {"insts_per_launch": valu, "peak_insts_per_s": VALU_PEAK_PUBLISHED,
+            "frac": round(valu / (VALU_PEAK_PUBLISHED * avg_s), 4),
+            "measured_peak_insts_per_s": max(forms.values()),
+            "frac_of_measured": round(valu / (max(forms.values()) * avg_s), 4),
+            "measured_insts_per_s_by_form": {f: float(f"{r:.4g}") for f, r in forms.items()}}
 
 
 def step_roofline(particles: int, substeps: int, mips: int, live_voxels: int, ms_per_step: float) -> dict:
@@ -620,13 +628,143 @@ def main_wire(root, args, stream) -> int:
     return 0
 
 
+SWEEP_METRIC = "tracer sweeps (primary cone rays) per second over a baked world (1 MI355X)"
+
+
+def camera_rays(width: int, height: int, fov_y: float, eye, at):
+    """Primary cone rays of a pinhole camera, 10 floats each (ro, rd, tMin 0, tMax 64,
+    radius 0, aperture = pixelAperture, march.ts:49-51), directions normalised in
+    f32, ordered by 8 x 8 pixel tiles so a wave's rays are neighbours."""
+    import numpy as np
+
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(at, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, (0.0, 1.0, 0.0))
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    th = np.tan(fov_y / 2)
+    ys, xs = np.mgrid[0:height, 0:width]
+    ys = ys.reshape(height // 8, 8, width // 8, 8).transpose(0, 2, 1, 3).reshape(-1)
+    xs = xs.reshape(height // 8, 8, width // 8, 8).transpose(0, 2, 1, 3).reshape(-1)
+    u = ((xs + 0.5) / width * 2 - 1) * th * width / height
+    v = (1 - (ys + 0.5) / height * 2) * th
+    d = (fwd[None] + u[:, None] * right[None] + v[:, None] * up[None]).astype(np.float32)
+    d /= np.sqrt((d * d).sum(1, dtype=np.float32))[:, None]
+    rays = np.zeros((d.shape[0], 10), np.float32)
+    rays[:, 0:3] = eye
+    rays[:, 3:6] = d
+    rays[:, 7] = 64.0
+    rays[:, 9] = np.float32(th / (0.5 * height))
+    return rays
+
+
+def measure_sweep(root, args, stream) -> dict:
+    """makeTracer(volumeField(world)).sweep of a 1920 x 1080 camera's primary cone
+    rays over the 288^3 x 3 ridge world (the dam-break-1m collider), rays and hits
+    resident in HBM."""
+    import numpy as np
+    import torch
+
+    import clayhip
+    from clayhip import scenes
+
+    vol = clayhip.SdfVolume(root, resolution=288, mipLevels=3, worldSize=scenes.WORLD, origin=scenes.ORIGIN,
+                            band=4)
+    vol.bakeBrushes([dict(b, material=float(i + 1)) for i, b in enumerate(scenes.ridge_world_brushes())])
+    field = clayhip.volumeField(vol)
+    rays_h = camera_rays(1920, 1080, np.radians(60.0), (0.0, 6.0, -16.0), (0.0, 0.0, 0.0))
+    n = rays_h.shape[0]
+    rays = torch.from_numpy(rays_h).cuda()
+    hits = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
+    step = lambda s: field.sweep_device(rays.data_ptr(), n, hits.data_ptr(), s.cuda_stream)
+    ms = timed(step, args.steps, args.warmup, stream)
+    stats = kernel_stats(root, step, args.steps, stream)
+    h = hits.cpu().numpy()
+    samples = int(h[:, 2].astype(np.int64).sum())
+    tot_ms, launches = stats["field_sweep"]
+    avg_s = tot_ms / launches / 1e3
+    # Compulsory bytes: per ray the ray (40 B) and its hit (16 B).  The texels each
+    # sample reads come through the caches (the counted fetch is the rays plus a
+    # fraction of the world's mip-0 plane), so the kernel's bound is instruction issue.
+    nb = 56 * n
+    levels = [(vol.download(m)[..., 0], vol.download(m)[..., 1]) for m in range(3)]
+    field.destroy()
+    vol.destroy()
+    roof = {"bound": "hbm", "achieved": round(nb / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(nb / avg_s / 1e9 / HBM_PEAK_GBS, 5),
+            "traffic": committed_traffic("field_sweep", "sweep-288"), "kernel": "field_sweep",
+            "avg_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nb, "samples_per_launch": samples}
+    valu = valu_object(root, committed_valu("field_sweep", "sweep-288"), avg_s)
+    if valu is not None:
+        roof["valu"] = valu
+    return {"ms": ms, "rays": n, "samples": samples, "hit_frac": float((h[:, 1] != 0).mean()),
+            "steps_mean": samples / n, "steps_max": int(h[:, 2].max()), "avg_ms": avg_s * 1e3, "roofline": roof,
+            "rays_host": rays_h, "levels": levels}
+
+
+def cpu_baseline_sweep(r: dict, budget_s: float) -> dict:
+    """The oracle's f32 sweep over every 64th ray of the same camera and world."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from clayhip import scenes
+
+    lv = r["levels"]
+    col = O.Collider().add_volume(lv[0][0], 288, scenes.ORIGIN, scenes.WORLD, 4.0, mat16=lv[0][1], levels=lv)
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    sample = r["rays_host"][::64]
+    t0 = time.perf_counter()
+    runs = 0
+    while True:
+        col.sweep(sample)
+        runs += 1
+        if time.perf_counter() - t0 > budget_s or runs >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": runs * sample.shape[0] / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{runs} passes of every 64th ray ({sample.shape[0]}) with the oracle's f32 sweep, "
+                      f"OpenMP over {threads} threads"}
+
+
+def main_sweep(root, args, stream) -> int:
+    t_wall = time.perf_counter()
+    r = measure_sweep(root, args, stream)
+    line = {
+        "metric": SWEEP_METRIC,
+        "value": r["rays"] / (r["ms"] / 1e3),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (the ridge world baked at 288^3 x 3 mips, a 1920 x 1080 camera)",
+        "config": {"workload": "sweep-288: 1920 x 1080 primary cone rays over one 288^3 x 3-mip world",
+                   "rays": r["rays"], "samples": r["samples"], "samples_per_s": r["samples"] / (r["ms"] / 1e3),
+                   "steps_mean": round(r["steps_mean"], 3), "steps_max": r["steps_max"],
+                   "hit_frac": round(r["hit_frac"], 4)},
+        "roofline": r["roofline"],
+    }
+    line["wall_s"] = round(time.perf_counter() - t_wall, 3)
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline_sweep(r, args.cpu_budget)
+    root.destroy()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="batched-256",
-                    choices=["batched-256", "dam-break-1m", "dam-break-16k", "coupled", "wire"])
+                    choices=["batched-256", "dam-break-1m", "dam-break-16k", "coupled", "wire", "sweep"])
     ap.add_argument("--scenes", type=int, default=8, help="coupled workload: scenes of 524 288 + 4096 particles")
     ap.add_argument("--splat", default="tiled", choices=["tiled", "plain"], help="single-scene workloads only")
     ap.add_argument("--precision", default=None, choices=["f32", "fp16lds"],
@@ -667,6 +805,8 @@ def main() -> int:
     stream = torch.cuda.Stream()
     if args.workload == "wire":
         return main_wire(root, args, stream)
+    if args.workload == "sweep":
+        return main_sweep(root, args, stream)
     t_wall = time.perf_counter()
     comparisons = None
     if batched:

@@ -56,6 +56,18 @@ class Brush(C.Structure):
     ]
 
 
+class TracerOpts(C.Structure):
+    """`clay_tracer_opts`: makeTracer's options (lib/src/trace/march.ts:22-40)."""
+
+    _fields_ = [("max_steps", C.c_int32), ("min_step", C.c_float), ("hit_eps", C.c_float), ("min_mip", C.c_int32)]
+
+
+class SweepHit(C.Structure):
+    """`clay_sweep_hit`: SweepHit (lib/src/trace/march.ts:4-12), 16 bytes."""
+
+    _fields_ = [("t", C.c_float), ("hit", C.c_uint32), ("steps", C.c_uint32), ("dist", C.c_float)]
+
+
 class VolumeDesc(C.Structure):
     _fields_ = [
         ("resolution", C.c_int32),
@@ -262,6 +274,8 @@ STRUCTS = {
     "clay_volume_desc": VolumeDesc,
     "clay_volume_info": VolumeInfo,
     "clay_analytic_opts": AnalyticOpts,
+    "clay_tracer_opts": TracerOpts,
+    "clay_sweep_hit": SweepHit,
     "clay_cloud_desc": CloudDesc,
     "clay_splat_desc": SplatDesc,
     "clay_fluid_desc": FluidDesc,
@@ -317,6 +331,9 @@ SIGNATURES = {
     "clay_field_query": ([P, I32, I32, P, I32, P], C.c_int),
     "clay_field_entry": ([P, P, I32, P], C.c_int),
     "clay_field_scale": ([P, I32, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+    "clay_tracer_opts_default": ([C.POINTER(TracerOpts)], None),
+    "clay_field_sweep": ([P, C.POINTER(TracerOpts), P, I32, P], C.c_int),
+    "clay_field_sweep_device": ([P, C.POINTER(TracerOpts), P, I32, P, P], C.c_int),
     "clay_splat_create": ([P, C.POINTER(CloudDesc), C.POINTER(SplatDesc), PP], C.c_int),
     "clay_splat_destroy": ([P], C.c_int),
     "clay_splat_set_live": ([P, U32, U32], C.c_int),

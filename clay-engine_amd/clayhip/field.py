@@ -69,6 +69,28 @@ class TracedField:
         L.check(L.lib().clay_field_entry(self._h, rays.ctypes.data, n, out.ctypes.data))
         return out
 
+    def sweep(self, ro, rd, tMin=0.0, tMax=1e9, radius=0.0, aperture=0.0, options: dict | None = None) -> dict:
+        """makeTracer(field, options).sweep (march.ts:97-218) per ray, on the GPU: ray
+        (radius and aperture 0), sphere sweep (radius) or cone (aperture).  The
+        per-ray arguments broadcast; `options` takes maxSteps, minStep, hitEps and
+        minMip.  Returns the SweepHit fields as arrays: t, hit, steps, dist."""
+        ro = np.asarray(ro, dtype=np.float32).reshape(-1, 3)
+        rd = np.asarray(rd, dtype=np.float32).reshape(-1, 3)
+        n = max(ro.shape[0], rd.shape[0])
+        rays = np.empty((n, 10), dtype=np.float32)
+        rays[:, 0:3], rays[:, 3:6] = ro, rd
+        rays[:, 6], rays[:, 7], rays[:, 8], rays[:, 9] = tMin, tMax, radius, aperture
+        hits = np.empty(n, dtype=[("t", np.float32), ("hit", np.uint32), ("steps", np.uint32), ("dist", np.float32)])
+        L.check(L.lib().clay_field_sweep(self._h, C.byref(tracer_opts(options)), rays.ctypes.data, n,
+                                         hits.ctypes.data))
+        return {k: hits[k].copy() for k in ("t", "hit", "steps", "dist")}
+
+    def sweep_device(self, rays_ptr: int, n: int, hits_ptr: int, stream: int = 0, options: dict | None = None) -> None:
+        """The sweep over device buffers (10 floats per ray in, 16-byte SweepHit out),
+        enqueued on `stream` (0: the context's own)."""
+        L.check(L.lib().clay_field_sweep_device(self._h, C.byref(tracer_opts(options)), C.c_void_p(rays_ptr), int(n),
+                                                C.c_void_p(hits_ptr), C.c_void_p(stream or None)))
+
     def bandWorld(self, mip: int = 0) -> float:
         b = C.c_float()
         L.check(L.lib().clay_field_scale(self._h, int(mip), C.byref(b), None))
@@ -83,6 +105,16 @@ class TracedField:
         if self._owned and self._h is not None:
             L.lib().clay_field_destroy(self._h)
         self._h = None
+
+
+def tracer_opts(options: dict | None = None) -> L.TracerOpts:
+    """makeTracer's options (march.ts:22-40) with the reference's defaults."""
+    o = L.TracerOpts()
+    L.lib().clay_tracer_opts_default(C.byref(o))
+    for key, field in (("maxSteps", "max_steps"), ("minStep", "min_step"), ("hitEps", "hit_eps"), ("minMip", "min_mip")):
+        if options and options.get(key) is not None:
+            setattr(o, field, options[key])
+    return o
 
 
 def analyticField(root: Root, brushes, options: dict | None = None) -> TracedField:

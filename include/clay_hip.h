@@ -224,6 +224,36 @@ int clay_field_entry(clay_field* f, const float* rays9_host, int32_t n, float* t
 /* bandWorld(mip) and voxelWorld(mip) in f32 (either pointer may be NULL). */
 int clay_field_scale(const clay_field* f, int32_t mip, float* band_world, float* voxel_world);
 
+/* makeTracer options (march.ts:22-40, 71-75): iteration cap, minimum forward step
+ * in mip-0 voxels, hit threshold in voxels of min_mip, and the finest level the
+ * trace may reach (clamped to the field's maxMip). */
+typedef struct clay_tracer_opts {
+  int32_t max_steps;  /* maxSteps, default 192 */
+  float min_step;     /* minStep, default 0.25 */
+  float hit_eps;      /* hitEps, default 0.5 */
+  int32_t min_mip;    /* minMip, default 0 */
+} clay_tracer_opts;
+void clay_tracer_opts_default(clay_tracer_opts* opts);
+/* SweepHit (march.ts:4-12): t, hit (0 = left the interval untouched), iterations
+ * consumed, and the field distance at the stop point. */
+typedef struct clay_sweep_hit {
+  float t;
+  uint32_t hit;
+  uint32_t steps;
+  float dist;
+} clay_sweep_hit;
+/* makeTracer(field, opts).sweep(ro, rd, tMin, tMax, radius, aperture) per ray
+ * (march.ts:97-218): radius 0 and aperture 0 is .ray, aperture alone .cone, radius
+ * alone .sphere.  rays10: 10 floats per ray (ro xyz, rd xyz, tMin, tMax, radius,
+ * aperture); rd is used as given.  The sweep starts at entry when every leaf
+ * bounds itself (volumes, lerps of volumes and their offsets), else at tMin.
+ * Host buffers; synchronous. */
+int clay_field_sweep(clay_field* f, const clay_tracer_opts* opts, const float* rays10_host, int32_t n,
+                     clay_sweep_hit* hits_host);
+/* The same over device buffers, enqueued on `stream`. */
+int clay_field_sweep_device(clay_field* f, const clay_tracer_opts* opts, const float* rays10_dev, int32_t n,
+                            clay_sweep_hit* hits_dev, clay_stream stream);
+
 /* ---- ParticleCloud + SplatField (lib/src/sim/splat.ts) ----------------- */
 #define CLAY_LIVE_ALL 0     /* occluder.gpu.ts:98-101 */
 #define CLAY_LIVE_PREFIX 1  /* i < count (fluid.ts:221-224) */

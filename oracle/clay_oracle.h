@@ -134,6 +134,14 @@ typedef struct {
   orc_leaf leaf[ORC_MAX_LEAVES];
 } orc_collider;
 
+/* makeTracer options (lib/src/trace/march.ts:22-40, 71-75). */
+typedef struct {
+  int32_t max_steps;
+  int32_t min_mip;
+  double min_step;   /* mip-0 voxels */
+  double hit_eps;    /* voxels of min_mip */
+} orc_tracer_opts;
+
 /* SplatField options (lib/src/sim/splat.ts:74-106, 147-162). `origin` is the
  * volume origin at bake time (SdfVolume.setOrigin). */
 typedef struct {
@@ -265,6 +273,10 @@ typedef struct {
   void orc_field_query_##S(const orc_collider* col, int what, int mip, int n,           \
                            const T* p3, T* out2);                                      \
   void orc_field_entry_##S(const orc_collider* col, int n, const T* rays9, T* out);     \
+  /* makeTracer(field, o).sweep per ray (march.ts:97-218); rays10: ro xyz, rd xyz,    \
+   * tMin, tMax, radius, aperture.  Out: (t, dist) and (hit, steps) per ray. */        \
+  void orc_field_sweep_##S(const orc_collider* col, const orc_tracer_opts* o, int n,    \
+                           const T* rays10, T* t_dist2, uint32_t* hit_steps2);         \
   void orc_extract_##S(const orc_extract_params* k, const orc_collider* col,          \
                        int capacity, uint32_t* body3, T* planes, T* aux,               \
                        uint32_t* owner, T* state, T* motion, uint32_t* ids,            \

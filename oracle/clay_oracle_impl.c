@@ -1574,3 +1574,120 @@ void FN(orc_field_entry)(const orc_collider* col, int n, const T* rays9, T* out)
     out[i] = t;
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * makeTracer(field, options).sweep (lib/src/trace/march.ts:71-218).
+ * ------------------------------------------------------------------------- */
+/* maxMip of the field: a volume's mipLevels - 1, 0 for an analytic leaf, the
+ * deeper of a lerp pair and of a union's children (field.ts:116, 206, 247, 324). */
+static int leaf_max_mip(const orc_leaf* l) {
+  if (l->kind == 0) return 0;
+  int m = (l->mips > 0 ? l->mips : 1) - 1;
+  if (l->kind == 2) {
+    const int b = (l->lerp_b->mips > 0 ? l->lerp_b->mips : 1) - 1;
+    m = b > m ? b : m;
+  }
+  return m;
+}
+
+/* voxelWorld(mip): voxelSize * exp2(min(mip, maxMip)) of a volume, the epsilon of
+ * an analytic field, the larger of a lerp pair's and of a union's children's
+ * (field.ts:150-153, 220-223, 300-305, 366-371). */
+static T vol_voxel_world(const orc_leaf* l, int mip) {
+  const int mips = l->mips > 0 ? l->mips : 1;
+  const int m = mip < mips - 1 ? mip : mips - 1;
+  return rnd(l->world_size / l->res) * (T)ldexp(1.0, m);
+}
+static T field_voxel_world(const orc_collider* col, int mip) {
+  T best = 0;
+  for (int k = 0; k < col->nleaf; k++) {
+    const orc_leaf* l = &col->leaf[k];
+    T v = l->kind == 0 ? rnd(l->eps) : vol_voxel_world(l, mip);
+    if (l->kind == 2) v = t_max(v, vol_voxel_world(l->lerp_b, mip));
+    best = k == 0 ? v : t_max(best, v);
+  }
+  return best;
+}
+
+/* sample(p, mip) of the union, folded left (field.ts:270-298). */
+static void field_sample(const orc_collider* col, int mip, v3 p, T* x, T* y) {
+  leaf_read(&col->leaf[0], 0, mip, p, x, y);
+  for (int k = 1; k < col->nleaf; k++) {
+    T a, b;
+    leaf_read(&col->leaf[k], 0, mip, p, &a, &b);
+    if (!(*x < a)) {
+      *x = a;
+      *y = b;
+    }
+  }
+}
+
+void FN(orc_field_sweep)(const orc_collider* col, const orc_tracer_opts* o, int n, const T* rays10, T* t_dist2,
+                         uint32_t* hit_steps2) {
+  int max_mip = 0;
+  for (int k = 0; k < col->nleaf; k++) {
+    const int m = leaf_max_mip(&col->leaf[k]);
+    max_mip = m > max_mip ? m : max_mip;
+  }
+  const int min_mip = o->min_mip < max_mip ? o->min_mip : max_mip;
+  const int start_mip = min_mip + (max_mip - min_mip - ((max_mip - min_mip) % 2));
+  const T hit_mip = (T)min_mip + (T)0.5;
+  int bounded = 1; /* entry exists only when every leaf bounds itself (field.ts:250-264) */
+  for (int k = 0; k < col->nleaf; k++)
+    if (col->leaf[k].kind == 0) bounded = 0;
+  const T slack = KF(0.866); /* INTERP_SLACK */
+  T vw[ORC_MAX_MIPS];
+  for (int m = 0; m <= max_mip && m < ORC_MAX_MIPS; m++) vw[m] = field_voxel_world(col, m);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; i++) {
+    const T* r = rays10 + 10 * (size_t)i;
+    const v3 ro = V(r[0], r[1], r[2]), rd = V(r[3], r[4], r[5]);
+    const T tmin = r[6], tmax = r[7], radius = r[8], aperture = r[9];
+    const T inv_growth = (T)1 / ((T)1 + aperture);
+    const T min_step = vw[0] * rnd(o->min_step);
+    const T hit_eps = vw[min_mip] * rnd(o->hit_eps);
+    const T inv_eps_ramp = (T)1 / t_max((T)2 * hit_eps, KF(1e-6));
+    T t_start = tmin;
+    if (bounded) {
+      t_start = leaf_entry(&col->leaf[0], ro, rd, tmin, radius, aperture);
+      for (int k = 1; k < col->nleaf; k++)
+        t_start = t_min(t_start, leaf_entry(&col->leaf[k], ro, rd, tmin, radius, aperture));
+    }
+    T t = t_start, mip = (T)start_mip, dist = 0;
+    uint32_t hit = 0, steps = 0;
+    for (int it = 0; it < o->max_steps; it++) {
+      steps = (uint32_t)it + 1;
+      T sx, sy;
+      field_sample(col, (int)mip, V(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t), &sx, &sy);
+      dist = sx;
+      const T safe = dist - slack * (vw[(int)mip] - vw[min_mip]);
+      const T target = radius + aperture * t;
+      const T eps = hit_eps * t_min((t - t_start) * inv_eps_ramp, 1);
+      if (safe <= 0 && mip >= hit_mip) {
+        mip = t_max(mip - 2, (T)min_mip);
+        t = t_max(t - (T)0.5 * vw[(int)mip], t_start);
+        continue;
+      }
+      if (safe < target + eps && sy < KF(0.999)) {
+        if (mip >= hit_mip) {
+          mip = t_max(mip - 2, (T)min_mip);
+          t = t_max(t - (T)0.5 * vw[(int)mip], t_start);
+          continue;
+        }
+        hit = 1;
+        t = t + t_max(dist - target, 0);
+        break;
+      }
+      t = t_max((t + safe - radius) * inv_growth, t + min_step);
+      if (sy >= KF(0.999)) {
+        const T up = t_min(mip + 2, (T)start_mip);
+        if (dist > slack * (vw[(int)up] - vw[min_mip])) mip = up;
+      }
+      if (t > tmax) break;
+    }
+    t_dist2[2 * i] = t_min(t, tmax);
+    t_dist2[2 * i + 1] = dist;
+    hit_steps2[2 * i] = hit;
+    hit_steps2[2 * i + 1] = steps;
+  }
+}

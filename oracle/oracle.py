@@ -72,6 +72,10 @@ class OrcCollider(C.Structure):
     _fields_ = [("nleaf", C.c_int32), ("pad", C.c_int32), ("leaf", OrcLeaf * 4)]
 
 
+class OrcTracerOpts(C.Structure):
+    _fields_ = [("max_steps", C.c_int32), ("min_mip", C.c_int32), ("min_step", C.c_double), ("hit_eps", C.c_double)]
+
+
 class OrcSplatOpts(C.Structure):
     _fields_ = [
         ("res", C.c_int32), ("mips", C.c_int32), ("world_size", C.c_double), ("origin", C.c_double * 3),
@@ -143,6 +147,7 @@ def lib() -> C.CDLL:
             getattr(L, f"orc_sin_cos_{s}").argtypes = [C.c_float if s == "f32" else C.c_double, P, P]
             getattr(L, f"orc_field_query_{s}").argtypes = [COL, I, I, I, P, P]
             getattr(L, f"orc_field_entry_{s}").argtypes = [COL, I, P, P]
+            getattr(L, f"orc_field_sweep_{s}").argtypes = [COL, C.POINTER(OrcTracerOpts), I, P, P, P]
             getattr(L, f"orc_extract_{s}").argtypes = [C.POINTER(OrcExtractParams), COL, I, P, P, P, P, P, P, P,
                                                        P, P, P]
         L.orc_tile_dilate.argtypes = [I, I, I, P]
@@ -343,6 +348,20 @@ class Collider:
         out = np.empty(r.shape[0], dtype=dt)
         getattr(lib(), f"orc_field_entry_{precision}")(C.byref(self.c), r.shape[0], r.ctypes.data, out.ctypes.data)
         return out
+
+    def sweep(self, rays10: np.ndarray, max_steps: int = 192, min_step: float = 0.25, hit_eps: float = 0.5,
+              min_mip: int = 0, precision: str = "f32") -> dict:
+        """makeTracer(field, options).sweep per ray (march.ts:97-218); rays10 rows are
+        ro xyz, rd xyz, tMin, tMax, radius, aperture.  Returns t, hit, steps, dist."""
+        dt = np.float32 if precision == "f32" else np.float64
+        r = np.ascontiguousarray(rays10, dtype=dt).reshape(-1, 10)
+        n = r.shape[0]
+        td = np.empty((n, 2), dtype=dt)
+        hs = np.empty((n, 2), dtype=np.uint32)
+        o = OrcTracerOpts(max_steps=max_steps, min_mip=min_mip, min_step=min_step, hit_eps=hit_eps)
+        getattr(lib(), f"orc_field_sweep_{precision}")(C.byref(self.c), C.byref(o), n, r.ctypes.data, td.ctypes.data,
+                                                       hs.ctypes.data)
+        return {"t": td[:, 0].copy(), "hit": hs[:, 0].copy(), "steps": hs[:, 1].copy(), "dist": td[:, 1].copy()}
 
     def eval(self, points: np.ndarray, precision: str = "f32"):
         dt = np.float32 if precision == "f32" else np.float64
