@@ -735,7 +735,7 @@ int splat_create_impl(Ctx* ctx, const clay_cloud_desc* cloud, const clay_splat_d
       return fail(CLAY_ERR_HIP, "SplatField: tile mask initialisation failed");
     }
   }
-  if (sc.follow && (rc = dalloc(&sp->follow_slots, (size_t)kBoundsStride * S))) {
+  if (sc.follow && ((rc = dalloc(&sp->follow_slots, (size_t)kBoundsStride * S)) || (rc = splat_bounds_reset(sp, s)))) {
     splat_free(sp);
     return rc;
   }
@@ -950,18 +950,19 @@ int fluid_sort(Fluid* f, hipStream_t s, uint32_t* bounds = nullptr) {
   // A new stamp per substep makes every older cell record stale at once.  After
   // 2^32 - 1 substeps the stamps restart, and the table is wiped once so that no
   // record from the previous cycle can carry a reused stamp.
+  // The keys pass advances the device stamp (except on the wrap, which sets it).
+  uint32_t* bump = f->epoch_dev;
   if (++f->epoch == 0) {
     if ((rc = launch_fill_u32(c, s, f->cells, 4 * f->grid_cells * f->scenes, 0u)) ||
         (rc = launch_fill_u32(c, s, f->epoch_dev, 1, 1u)))
       return rc;
     f->epoch = 1;
-  } else if ((rc = launch_bump(c, s, f->epoch_dev))) {
-    return rc;
+    bump = nullptr;
   }
   const SceneK sk = f->scene_k();
   const bool in_tmp = radix_input_in_tmp(f->key_bits);
   if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, in_tmp ? f->keys2 : f->keys,
-                        in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift, bounds)))
+                        in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift, bounds, bump)))
     return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
@@ -999,8 +1000,7 @@ int fluid_forces(Fluid* f, hipStream_t s) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   int rc;
-  if ((rc = launch_fill_u32(c, s, f->contacts, contact_words(f), 0u)) ||
-      (rc = launch_fill_u32(c, s, f->cmask, ((size_t)f->capacity + 31) / 32, 0u)))
+  if ((rc = launch_fill2_u32(c, s, f->contacts, contact_words(f), f->cmask, ((size_t)f->capacity + 31) / 32, 0u)))
     return rc;
   if (n == 0) return CLAY_OK;
   const SceneK sk = f->scene_k();

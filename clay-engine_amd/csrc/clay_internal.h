@@ -154,6 +154,7 @@ struct Splat {
   bool follow = false;                   // centre each volume before a bake:
   const double* follow_centers = nullptr;  // on these (6 doubles per scene, xyz first), else on the AABB
   uint32_t* follow_slots = nullptr;      // per scene: the AABB's six ordered-float slots
+  bool bounds_pending = false;           // slots handed to a keys pass, not yet read and reset
   unsigned long long* atomic_counts = nullptr;  // set for a bake that tallies its atomics
   // tuning
   uint32_t chunk = 1024;                 // particles per splat work item
@@ -276,7 +277,7 @@ int launch_body_centroids(Ctx* c, hipStream_t s, bool f64, uint32_t nbodies, con
 // With `bounds`, also folds each scene's particle bounds into its six slots.
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
-                uint32_t shift, uint32_t* bounds = nullptr);
+                uint32_t shift, uint32_t* bounds, uint32_t* bump = nullptr);
 // With `spos` (FP16-LDS), also each sorted particle's staged position.
 int launch_reorder(Ctx* c, hipStream_t s, uint32_t n, const uint32_t* skeys, const uint32_t* svals,
                    const SceneK& sk, uint32_t invalid_key, uint32_t* cells, const float* const* src,
@@ -298,9 +299,9 @@ int launch_forces_integrate(Ctx* c, hipStream_t s, const FluidK& k, const SceneK
                             bool lds, const ColliderDev& col, uint32_t n, const float* const* sorted,
                             const uint32_t* svals, const uint32_t* skeys, const uint32_t* cells,
                             float* const* out, uint32_t* contacts, const float4* frec);
-int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch);
 int launch_contacts_collect(Ctx* c, hipStream_t s, uint32_t* contacts, uint32_t stride, const uint32_t* cmask,
                             const float4* cdata, const uint32_t* base, uint32_t n, uint32_t scenes);
+int launch_fill2_u32(Ctx* c, hipStream_t s, uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t v);
 int launch_fill_u32(Ctx* c, hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 int launch_conserved(Ctx* c, hipStream_t s, uint32_t n, const float* const* pv, double mass,
                      double* partial, uint32_t partial_blocks, double* out);
@@ -325,6 +326,7 @@ int splat_bake_ordered(Splat* sp, hipStream_t s, const uint32_t* order, const fl
 int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order, bool bounds_given);
 // Resets and returns the per-scene bounds slots of a splat that follows its
 // particles' bounds (else null), for a producer to fill before the bake.
+int splat_bounds_reset(Splat* sp, hipStream_t s);
 uint32_t* splat_bounds_begin(Splat* sp, hipStream_t s);
 int splat_bake_finish(Splat* sp, hipStream_t s, const uint32_t* order, const float* cover);
 int splat_brick_geometry(int res, int mips, std::vector<int>* brick_res,

@@ -123,9 +123,12 @@ __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const flo
                                                  const uint32_t* __restrict__ scene_of,
                                                  uint32_t* __restrict__ keys,
                                                  uint32_t* __restrict__ vals, uint32_t invalid, uint32_t shift,
-                                                 uint32_t* __restrict__ bounds) {
+                                                 uint32_t* __restrict__ bounds, uint32_t* __restrict__ bump) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const bool active = i < n;
+  // The substep stamp advances here, not in a launch of its own: nothing in this
+  // pass or the sort reads it, and the reorder that stamps the cells runs after.
+  if (bump && i == 0) bump[0] = bump[0] + 1u;
   f3 p = mk(0.0f, 0.0f, 0.0f);
   uint32_t sc = 0;
   if (active) {
@@ -1169,9 +1172,15 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-// Advances the device-resident substep stamp.
-__global__ void k_bump(uint32_t* epoch) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) epoch[0] = epoch[0] + 1u;
+// Two fills in one launch: words [0, na) of a, then [0, nb) of b.
+__global__ void k_fill2_u32(uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < na + nb;
+       i += (size_t)gridDim.x * blockDim.x) {
+    if (i < na)
+      a[i] = v;
+    else
+      b[i - na] = v;
+  }
 }
 
 // The contact list (fluid.ts:54-58, 470-477) of each scene: its first 32 contacts
@@ -1318,11 +1327,11 @@ bool fluid_tune_valid(const FluidTune& t) {
 
 int launch_keys(Ctx* c, hipStream_t s, const FluidK& k, uint32_t n, const float* px, const float* py,
                 const float* pz, const uint32_t* scene_of, uint32_t* keys, uint32_t* vals, uint32_t invalid_key,
-                uint32_t shift, uint32_t* bounds) {
+                uint32_t shift, uint32_t* bounds, uint32_t* bump) {
   if (n == 0) return CLAY_OK;
   LaunchTimer t(c, s, "fluid_keys");
   hipLaunchKernelGGL(k_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, k, n, px, py, pz, scene_of, keys, vals,
-                     invalid_key, shift, bounds);
+                     invalid_key, shift, bounds, bump);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }
@@ -1398,13 +1407,6 @@ int launch_contacts_collect(Ctx* c, hipStream_t s, uint32_t* contacts, uint32_t 
   return CLAY_OK;
 }
 
-int launch_bump(Ctx* c, hipStream_t s, uint32_t* epoch) {
-  LaunchTimer t(c, s, "fluid_epoch");
-  hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, epoch);
-  CLAY_HIP_CHECK(hipGetLastError());
-  return CLAY_OK;
-}
-
 int launch_emit(Ctx* c, hipStream_t s, Fluid* f, uint32_t base, uint32_t count, uint32_t seed) {
   if (count == 0) return CLAY_OK;
   EmitK e{};
@@ -1422,6 +1424,16 @@ int launch_emit(Ctx* c, hipStream_t s, Fluid* f, uint32_t base, uint32_t count, 
   for (int p = 0; p < 11; p++) o.p[p] = planes[p];
   LaunchTimer t(c, s, "fluid_emit");
   hipLaunchKernelGGL(k_emit, dim3(grid_for(count)), dim3(kBlock), 0, s, e, o);
+  CLAY_HIP_CHECK(hipGetLastError());
+  return CLAY_OK;
+}
+
+int launch_fill2_u32(Ctx* c, hipStream_t s, uint32_t* a, size_t na, uint32_t* b, size_t nb, uint32_t v) {
+  if (na + nb == 0) return CLAY_OK;
+  size_t blocks = (na + nb + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  LaunchTimer t(c, s, "fill_u32");
+  hipLaunchKernelGGL(k_fill2_u32, dim3((unsigned)blocks), dim3(kBlock), 0, s, a, na, b, nb, v);
   CLAY_HIP_CHECK(hipGetLastError());
   return CLAY_OK;
 }

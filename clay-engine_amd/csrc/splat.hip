@@ -910,17 +910,19 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Slots start at (max, max, max, 0, 0, 0) encoded: an empty scene keeps its origin.
+// k_bounds_follow puts them back there once it has read them, so the keys pass of
+// the next frame-end sort folds into fresh slots with no reset launch before it.
 __global__ void k_bounds_init(uint32_t* __restrict__ slots, uint32_t scenes) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 6 * scenes) slots[kBoundsStride * (i / 6) + i % 6] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
 }
 
-__global__ void k_bounds_follow(const uint32_t* __restrict__ slots, const uint32_t* __restrict__ base, uint32_t n,
+__global__ void k_bounds_follow(uint32_t* __restrict__ slots, const uint32_t* __restrict__ base, uint32_t n,
                                 VolDev* __restrict__ vols, double half, uint32_t scenes) {
   const uint32_t sc = blockIdx.x * blockDim.x + threadIdx.x;
   if (sc >= scenes) return;
   const uint32_t lo = base ? base[sc] : 0u, hi = base ? base[sc + 1] : n;
-  if (lo >= hi) return;
+  if (lo >= hi) return;  // no particle folded into this scene's slots: still fresh
   for (int q = 0; q < 3; q++) {
     const float c =
         (from_ordered(slots[kBoundsStride * sc + q]) + from_ordered(slots[kBoundsStride * sc + 3 + q])) * 0.5f;
@@ -928,6 +930,7 @@ __global__ void k_bounds_follow(const uint32_t* __restrict__ slots, const uint32
     vols[sc].origin[q] = (float)o;
     vols[sc].center[q] = (float)(o + half);
   }
+  for (int q = 0; q < 6; q++) slots[kBoundsStride * sc + q] = q < 3 ? 0xFFFFFFFFu : 0u;
 }
 
 }  // namespace
@@ -1095,11 +1098,23 @@ static int tiled_finish(Splat* sp, hipStream_t s, const MipsK& mk_, const CloudK
   return CLAY_OK;
 }
 
-uint32_t* splat_bounds_begin(Splat* sp, hipStream_t s) {
-  if (!sp->follow || sp->follow_centers || !sp->follow_slots) return nullptr;
+int splat_bounds_reset(Splat* sp, hipStream_t s) {
+  if (!sp->follow_slots) return CLAY_OK;
   const unsigned S = (unsigned)sp->scenes;
   hipLaunchKernelGGL(k_bounds_init, dim3((6 * S + 255) / 256), dim3(256), 0, s, sp->follow_slots, S);
-  return hipGetLastError() == hipSuccess ? sp->follow_slots : nullptr;
+  CLAY_HIP_CHECK(hipGetLastError());
+  sp->bounds_pending = false;
+  return CLAY_OK;
+}
+
+// The slots a keys pass may fold into.  They are fresh (k_bounds_follow of the
+// previous bake reset them) unless a bake that was handed them never reached its
+// k_bounds_follow (an error between the two), which a reset here repairs.
+uint32_t* splat_bounds_begin(Splat* sp, hipStream_t s) {
+  if (!sp->follow || sp->follow_centers || !sp->follow_slots) return nullptr;
+  if (sp->bounds_pending && splat_bounds_reset(sp, s) != CLAY_OK) return nullptr;
+  sp->bounds_pending = true;
+  return sp->follow_slots;
 }
 
 int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order, bool bounds_given) {
@@ -1114,6 +1129,7 @@ int splat_bake_prepare(Splat* sp, hipStream_t s, const uint32_t* order, bool bou
       const unsigned S = (unsigned)sp->scenes;
       hipLaunchKernelGGL(k_bounds_follow, dim3((S + 63) / 64), dim3(64), 0, s, sp->follow_slots, sp->scene_base,
                          c.end, sp->vol->dev, half, S);
+      sp->bounds_pending = false;
     } else if (sp->follow_centers) {
       hipLaunchKernelGGL(k_follow, dim3(sp->scenes), dim3(kThreads), 0, s, c.pos[0], c.pos[1], c.pos[2],
                          sp->scene_base, c.end, sp->follow_centers, sp->vol->dev, half);
