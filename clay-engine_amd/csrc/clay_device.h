@@ -294,32 +294,42 @@ __device__ inline float analytic_dist(const LeafDev& l, f3 p) {
   return acc[0];
 }
 
-__device__ __forceinline__ float texel0(const uint16_t* d, int n, int x, int y, int z) {
-  x = x < 0 ? 0 : (x >= n ? n - 1 : x);
-  y = y < 0 ? 0 : (y >= n ? n - 1 : y);
-  z = z < 0 ? 0 : (z >= n ? n - 1 : z);
-  return h2f(d[(size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z)]);
-}
+__device__ __forceinline__ int clamp_texel(int x, int n) { return x < 0 ? 0 : (x >= n ? n - 1 : x); }
 
-// textureSampleLevel(linear, clamp-to-edge) at mip 0 with f32 weights (volume.ts:262-267).
-__device__ inline float sample_raw0(const VolDev& v, const uint16_t* d, f3 p) {
-  float u = (p.x - v.origin[0]) / v.extent;
-  float w = (p.y - v.origin[1]) / v.extent;
-  float s = (p.z - v.origin[2]) / v.extent;
-  float fn = (float)v.res;
-  float tx = u * fn - 0.5f, ty = w * fn - 0.5f, tz = s * fn - 0.5f;
-  float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
-  int x0 = sat_i32(fx0), y0 = sat_i32(fy0), z0 = sat_i32(fz0);
-  float fx = tx - fx0, fy = ty - fy0, fz = tz - fz0;
-  int n = v.res;
-  float c00 = fmix(texel0(d, n, x0, y0, z0), texel0(d, n, x0 + 1, y0, z0), fx);
-  float c10 = fmix(texel0(d, n, x0, y0 + 1, z0), texel0(d, n, x0 + 1, y0 + 1, z0), fx);
-  float c01 = fmix(texel0(d, n, x0, y0, z0 + 1), texel0(d, n, x0 + 1, y0, z0 + 1), fx);
-  float c11 = fmix(texel0(d, n, x0, y0 + 1, z0 + 1), texel0(d, n, x0 + 1, y0 + 1, z0 + 1), fx);
-  float c0 = fmix(c00, c10, fy);
-  float c1 = fmix(c01, c11, fy);
+// The trilinear blend of the 2 x 2 x 2 texels at (x0, y0, z0) .. (x0 + 1, y0 + 1, z0 + 1),
+// each coordinate clamped to the level of n^3 texels (clamp-to-edge), x then y then z.
+// The two planes' offsets are 64-bit (n^3 may pass 2^32), the offsets inside a plane
+// 32-bit (n^2 < 2^32 for n <= 65535).
+__device__ __forceinline__ float trilinear(const uint16_t* d, int n, int x0, int y0, int z0, float fx, float fy,
+                                           float fz) {
+  const uint32_t xa = (uint32_t)clamp_texel(x0, n), xb = (uint32_t)clamp_texel(x0 + 1, n);
+  const uint32_t ra = (uint32_t)clamp_texel(y0, n) * (uint32_t)n, rb = (uint32_t)clamp_texel(y0 + 1, n) * (uint32_t)n;
+  const uint32_t nn = (uint32_t)n * (uint32_t)n;
+  const uint16_t* pa = d + (size_t)clamp_texel(z0, n) * nn;
+  const uint16_t* pb = d + (size_t)clamp_texel(z0 + 1, n) * nn;
+  const float c00 = fmix(h2f(pa[ra + xa]), h2f(pa[ra + xb]), fx);
+  const float c10 = fmix(h2f(pa[rb + xa]), h2f(pa[rb + xb]), fx);
+  const float c01 = fmix(h2f(pb[ra + xa]), h2f(pb[ra + xb]), fx);
+  const float c11 = fmix(h2f(pb[rb + xa]), h2f(pb[rb + xb]), fx);
+  const float c0 = fmix(c00, c10, fy);
+  const float c1 = fmix(c01, c11, fy);
   return fmix(c0, c1, fz);
 }
+
+// textureSampleLevel(linear, clamp-to-edge) of one channel of a level of n^3 texels
+// with f32 weights (volume.ts:262-267).
+__device__ __forceinline__ float sample_level_n(const VolDev& v, const uint16_t* d, int n, f3 p) {
+  const float u = (p.x - v.origin[0]) / v.extent;
+  const float w = (p.y - v.origin[1]) / v.extent;
+  const float s = (p.z - v.origin[2]) / v.extent;
+  const float fn = (float)n;
+  const float tx = u * fn - 0.5f, ty = w * fn - 0.5f, tz = s * fn - 0.5f;
+  const float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
+  return trilinear(d, n, sat_i32(fx0), sat_i32(fy0), sat_i32(fz0), tx - fx0, ty - fy0, tz - fz0);
+}
+
+// The same at mip 0.
+__device__ inline float sample_raw0(const VolDev& v, const uint16_t* d, f3 p) { return sample_level_n(v, d, v.res, p); }
 
 // SdfVolume.sampleDist at mip 0 (volume.ts:277-286) == volumeField.sample(p, 0).x.
 __device__ inline float vol_dist(const VolDev& v, const uint16_t* d, f3 p) {

@@ -38,19 +38,7 @@ struct QueryK {
 // textureSampleLevel(linear, clamp-to-edge) of one channel at a level of res^3
 // texels (volume.ts:262-267), with f32 weights.
 __device__ inline float sample_level(const VolDev& v, const uint16_t* d, int res, f3 p) {
-  const float u = (p.x - v.origin[0]) / v.extent;
-  const float w = (p.y - v.origin[1]) / v.extent;
-  const float s = (p.z - v.origin[2]) / v.extent;
-  const float fn = (float)res;
-  const float tx = u * fn - 0.5f, ty = w * fn - 0.5f, tz = s * fn - 0.5f;
-  const float fx0 = floorf(tx), fy0 = floorf(ty), fz0 = floorf(tz);
-  const int x0 = sat_i32(fx0), y0 = sat_i32(fy0), z0 = sat_i32(fz0);
-  const float fx = tx - fx0, fy = ty - fy0, fz = tz - fz0;
-  const float c00 = fmix(texel0(d, res, x0, y0, z0), texel0(d, res, x0 + 1, y0, z0), fx);
-  const float c10 = fmix(texel0(d, res, x0, y0 + 1, z0), texel0(d, res, x0 + 1, y0 + 1, z0), fx);
-  const float c01 = fmix(texel0(d, res, x0, y0, z0 + 1), texel0(d, res, x0 + 1, y0, z0 + 1), fx);
-  const float c11 = fmix(texel0(d, res, x0, y0 + 1, z0 + 1), texel0(d, res, x0 + 1, y0 + 1, z0 + 1), fx);
-  return fmix(fmix(c00, c10, fy), fmix(c01, c11, fy), fz);
+  return sample_level_n(v, d, res, p);
 }
 
 // sample(p, mip) or field(p, mip) of one leaf.  Volume: outside the box (dBox, 1) or
