@@ -733,14 +733,23 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kCover 
         auto staged_term = [&](uint2 u) {
           term(mk(pi.x - __uint_as_float(u.x), sub_f16_lo(by, u.y), sub_f16_hi(bz, u.y)));
         };
-        // Two candidates per trip: half the loop counting and exit tests.
+        // Four candidates per trip (then two, then one): a quarter of the loop
+        // counting and exit tests.
         const uint2* q = &stage[wave][0] + (any ? sp.lo - ulo : 0u);
         const uint32_t cnt = any ? sp.hi - sp.lo : 0u;
         uint32_t j = 0;
-        for (; j + 2 <= cnt; j += 2) {
+        for (; j + 4 <= cnt; j += 4) {
+          const uint2 u0 = q[j], u1 = q[j + 1], u2 = q[j + 2], u3 = q[j + 3];
+          staged_term(u0);
+          staged_term(u1);
+          staged_term(u2);
+          staged_term(u3);
+        }
+        if (j + 2 <= cnt) {
           const uint2 u0 = q[j], u1 = q[j + 1];
           staged_term(u0);
           staged_term(u1);
+          j += 2;
         }
         if (j < cnt) staged_term(q[j]);
       } else {
@@ -1040,18 +1049,27 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(kForces
           term(mk(pi.x - __uint_as_float(pos.x), sub_f16_lo(by, pos.y), sub_f16_hi(bz, pos.y)),
                mk(vr.x, vr.y, vr.z), vr.w, st_pinv[wave][j], ulo + j);
         };
-        // Two candidates per trip, both positions loaded before either is used:
-        // one LDS round trip and one loop test for two candidates.
+        // Four candidates per trip (then two, then one), all positions loaded before
+        // any is used: one LDS round trip and one loop test for four candidates.
         // [a, e_) lies in the staged [ulo, uhi) when it is not empty; an empty range
         // (a lane whose row holds no candidate has a = 0xFFFFFFFF) walks nothing.
         auto walk = [&](uint32_t a, uint32_t e_) {
           const uint32_t cnt = e_ > a ? e_ - a : 0u;
           const uint32_t base = cnt ? a - ulo : 0u;
           uint32_t j = 0;
-          for (; j + 2 <= cnt; j += 2) {
+          for (; j + 4 <= cnt; j += 4) {
+            const uint2 p0 = st_pos[wave][base + j], p1 = st_pos[wave][base + j + 1];
+            const uint2 p2 = st_pos[wave][base + j + 2], p3 = st_pos[wave][base + j + 3];
+            cand(base + j, p0);
+            cand(base + j + 1, p1);
+            cand(base + j + 2, p2);
+            cand(base + j + 3, p3);
+          }
+          if (j + 2 <= cnt) {
             const uint2 p0 = st_pos[wave][base + j], p1 = st_pos[wave][base + j + 1];
             cand(base + j, p0);
             cand(base + j + 1, p1);
+            j += 2;
           }
           if (j < cnt) cand(base + j, st_pos[wave][base + j]);
         };
