@@ -105,6 +105,28 @@ __device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __r
   }
 }
 
+// Wave-wide min / max of a u32, returned to every lane: DPP row shifts build each
+// 16-lane row's result in its last lane, two row broadcasts carry it across the
+// rows, and lane 63 holds the wave's result (no LDS round trips).  Lanes a shift
+// or broadcast does not reach keep their own value.  All 64 lanes must be active.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t keep, uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)keep, (int)x, kCtrl, kRowMask, 0xF, false);
+}
+template <bool kMax>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v) {
+  auto pick = [](uint32_t a, uint32_t b) { return kMax ? (a > b ? a : b) : (a < b ? a : b); };
+  v = pick(v, dpp_u32<0x111, 0xF>(v, v));  // row_shr:1
+  v = pick(v, dpp_u32<0x112, 0xF>(v, v));  // row_shr:2
+  v = pick(v, dpp_u32<0x114, 0xF>(v, v));  // row_shr:4
+  v = pick(v, dpp_u32<0x118, 0xF>(v, v));  // row_shr:8
+  v = pick(v, dpp_u32<0x142, 0xA>(v, v));  // row_bcast:15 into rows 1 and 3
+  v = pick(v, dpp_u32<0x143, 0xC>(v, v));  // row_bcast:31 into rows 2 and 3
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return wave_reduce_u32<false>(v); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return wave_reduce_u32<true>(v); }
+
 // Cell key per particle.  One scene: the cell index, or `invalid` outside the
 // grid.  A batch: (scene << shift) | cell, with all `shift` cell bits set outside
 // the grid, so a scene's outside particles sort after its cells; `shift` is the
@@ -116,7 +138,7 @@ __device__ __forceinline__ void for_neighbours(const FluidK& k, const uint4* __r
 // pass over the positions.  A workgroup inside one scene (the usual case: ids run
 // scene-major and scenes hold >= 1024 particles) reduces to one set of six
 // atomics.  Otherwise the wave's lanes are grouped by scene and each group takes a
-// butterfly min/max, its first lane doing the atomics.
+// DPP min/max, its first lane doing the atomics.
 __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const float* __restrict__ px,
                                                  const float* __restrict__ py,
                                                  const float* __restrict__ pz,
@@ -144,21 +166,20 @@ __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const flo
     vals[i] = i;
   }
   if (!bounds) return;  // uniform over the launch
-  const float inf = __uint_as_float(0x7f800000u);
+  // Every lane of every wave reaches here (inactive lanes carry the identities), as
+  // the DPP reductions need.  The bounds are folded as ordered bits (ordered_bits),
+  // whose integer min / max is the float min / max.
   const int lane = (int)__lane_id();
   const uint32_t first = blockIdx.x * kBlock;  // < n
   const uint32_t last = first + kBlock - 1 < n ? first + kBlock - 1 : n - 1;
+  const uint32_t pb[3] = {ordered_bits(p.x), ordered_bits(p.y), ordered_bits(p.z)};
   if (!scene_of || scene_of[first] == scene_of[last]) {  // uniform over the workgroup
-    __shared__ float part[kBlock / 64][6];
-    float a[6] = {active ? p.x : inf, active ? p.y : inf, active ? p.z : inf,
-                  active ? p.x : -inf, active ? p.y : -inf, active ? p.z : -inf};
+    __shared__ uint32_t part[kBlock / 64][6];
+    uint32_t a[6];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        a[q] = fmin_s(a[q], __shfl_xor(a[q], off));
-        a[q + 3] = fmax_s(a[q + 3], __shfl_xor(a[q + 3], off));
-      }
+    for (int q = 0; q < 3; q++) {
+      a[q] = wave_min_u32(active ? pb[q] : 0xFFFFFFFFu);
+      a[q + 3] = wave_max_u32(active ? pb[q] : 0u);
     }
     if (lane == 0)
 #pragma unroll
@@ -166,13 +187,13 @@ __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const flo
     __syncthreads();
     if (threadIdx.x < 6) {
       const int q = threadIdx.x;
-      float v = part[0][q];
-      for (int w = 1; w < kBlock / 64; w++) v = q < 3 ? fmin_s(v, part[w][q]) : fmax_s(v, part[w][q]);
+      uint32_t v = part[0][q];
+      for (int w = 1; w < kBlock / 64; w++) v = q < 3 ? (part[w][q] < v ? part[w][q] : v) : (part[w][q] > v ? part[w][q] : v);
       uint32_t* slot = bounds + (size_t)kBoundsStride * (scene_of ? scene_of[first] : 0u) + q;
       if (q < 3)
-        atomicMin(slot, ordered_bits(v));
+        atomicMin(slot, v);
       else
-        atomicMax(slot, ordered_bits(v));
+        atomicMax(slot, v);
     }
     return;
   }
@@ -182,22 +203,18 @@ __global__ void __launch_bounds__(kBlock) k_keys(FluidK k, uint32_t n, const flo
     const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)sc, leader);
     const uint64_t same = __ballot(active && sc == lead);
     const bool mine = (same >> lane) & 1ull;
-    float a[6] = {mine ? p.x : inf, mine ? p.y : inf, mine ? p.z : inf,
-                  mine ? p.x : -inf, mine ? p.y : -inf, mine ? p.z : -inf};
+    uint32_t a[6];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        a[q] = fmin_s(a[q], __shfl_xor(a[q], off));
-        a[q + 3] = fmax_s(a[q + 3], __shfl_xor(a[q + 3], off));
-      }
+    for (int q = 0; q < 3; q++) {
+      a[q] = wave_min_u32(mine ? pb[q] : 0xFFFFFFFFu);
+      a[q + 3] = wave_max_u32(mine ? pb[q] : 0u);
     }
     if (lane == leader) {
       uint32_t* slot = bounds + (size_t)kBoundsStride * lead;
 #pragma unroll
       for (int q = 0; q < 3; q++) {
-        atomicMin(slot + q, ordered_bits(a[q]));
-        atomicMax(slot + 3 + q, ordered_bits(a[q + 3]));
+        atomicMin(slot + q, a[q]);
+        atomicMax(slot + 3 + q, a[q + 3]);
       }
     }
     pending &= ~same;
@@ -499,27 +516,6 @@ __global__ void __launch_bounds__(kB)
 // each lane reads its own scene's cell records, and the union of ranges is still
 // one span of the sorted order.
 // ---------------------------------------------------------------------------
-// Wave-wide min / max of a u32, returned to every lane: DPP row shifts build each
-// 16-lane row's result in its last lane, two row broadcasts carry it across the
-// rows, and lane 63 holds the wave's result (no LDS round trips).  Lanes a shift
-// or broadcast does not reach keep their own value.  All 64 lanes must be active.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ uint32_t dpp_u32(uint32_t keep, uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)keep, (int)x, kCtrl, kRowMask, 0xF, false);
-}
-template <bool kMax>
-__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v) {
-  auto pick = [](uint32_t a, uint32_t b) { return kMax ? (a > b ? a : b) : (a < b ? a : b); };
-  v = pick(v, dpp_u32<0x111, 0xF>(v, v));  // row_shr:1
-  v = pick(v, dpp_u32<0x112, 0xF>(v, v));  // row_shr:2
-  v = pick(v, dpp_u32<0x114, 0xF>(v, v));  // row_shr:4
-  v = pick(v, dpp_u32<0x118, 0xF>(v, v));  // row_shr:8
-  v = pick(v, dpp_u32<0x142, 0xA>(v, v));  // row_bcast:15 into rows 1 and 3
-  v = pick(v, dpp_u32<0x143, 0xC>(v, v));  // row_bcast:31 into rows 2 and 3
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return wave_reduce_u32<false>(v); }
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return wave_reduce_u32<true>(v); }
 
 // Orders a wave's LDS accesses around this point (writes before it are seen by
 // every lane after it), without waiting for the workgroup's other waves: each
