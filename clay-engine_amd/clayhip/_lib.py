@@ -177,6 +177,7 @@ class Tuning(C.Structure):
         ("splat_xcd", C.c_int32),
         ("splat_owner", C.c_int32),
         ("exact_lds", C.c_int32),
+        ("early_bin", C.c_int32),
     ]
 
 

@@ -27,6 +27,7 @@ SPACE = {
     "fluid_xcd": (0, 1),
     "splat_xcd": (0, 1),
     "splat_owner": (0, 1),
+    "early_bin": (0, 1),
 }
 
 

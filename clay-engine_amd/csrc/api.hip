@@ -941,7 +941,11 @@ hipError_t fluid_quiesce(const Fluid* f) {
 // table and reordered state, then densities and pressures.  With `cover`, the
 // density pass also writes each particle's interior radius for the surface bake.
 // The cell sort of a substep: stamp, keys, sort, reorder into the sorted planes.
-int fluid_sort(Fluid* f, hipStream_t s, uint32_t* bounds = nullptr) {
+int fluid_reorder(Fluid* f, hipStream_t s);
+
+// The cell sort of a substep: stamp, keys, sort, and (unless `reorder` is false, when
+// the caller runs fluid_reorder itself) the reorder into the sorted planes.
+int fluid_sort(Fluid* f, hipStream_t s, uint32_t* bounds = nullptr, bool reorder = true) {
   Ctx* c = f->ctx;
   const uint32_t n = f->live;
   const uint32_t invalid = (uint32_t)f->grid_cells;
@@ -959,12 +963,21 @@ int fluid_sort(Fluid* f, hipStream_t s, uint32_t* bounds = nullptr) {
     f->epoch = 1;
     bump = nullptr;
   }
-  const SceneK sk = f->scene_k();
   const bool in_tmp = radix_input_in_tmp(f->key_bits);
   if ((rc = launch_keys(c, s, f->k, n, f->px, f->py, f->pz, f->scene_of, in_tmp ? f->keys2 : f->keys,
                         in_tmp ? f->vals2 : f->vals, invalid, f->scene_shift, bounds, bump)))
     return rc;
   if ((rc = radix_sort_pairs(c, s, n, f->key_bits, f->keys, f->vals, f->keys2, f->vals2, f->hist))) return rc;
+  return reorder ? fluid_reorder(f, s) : CLAY_OK;
+}
+
+// Cell records and the sorted snapshot of the state, from the sort's permutation.
+int fluid_reorder(Fluid* f, hipStream_t s) {
+  Ctx* c = f->ctx;
+  const uint32_t n = f->live;
+  if (n == 0) return CLAY_OK;
+  const uint32_t invalid = (uint32_t)f->grid_cells;
+  const SceneK sk = f->scene_k();
   const float* src[6] = {f->px, f->py, f->pz, f->vx, f->vy, f->vz};
   float* dst[6] = {f->spx, f->spy, f->spz, f->svx, f->svy, f->svz};
   const bool lds = f->desc.precision == CLAY_PRECISION_FP16LDS;
@@ -1041,16 +1054,20 @@ int fluid_frame_impl(Fluid* f, hipStream_t s) {
       if ((rc = fluid_forces(f, s))) return rc;
     }
     // The frame-end sort, then its density pass beside the bake's binning, which
-    // reads positions and the sort's permutation but not the interior radii.  The
-    // permutation is a spatially coherent visiting order for the splat; the codes
-    // do not depend on it.
+    // reads the id-order positions and the sort's permutation but not the sorted
+    // planes or the interior radii: it forks after the reorder, or (tuning early_bin)
+    // as soon as the permutation is final, beside the reorder too.  The permutation
+    // is a spatially coherent visiting order for the splat; the codes do not depend
+    // on it.
     // The keys pass of this sort also folds the bounds a following surface reads.
     uint32_t* bounds = splat_bounds_begin(f->surface, s);
-    if ((rc = fluid_sort(f, s, bounds))) return rc;
+    const bool early = f->tune.early_bin;
+    if ((rc = fluid_sort(f, s, bounds, !early))) return rc;
     f->surface->cloud.live_count = f->live;
     CLAY_HIP_CHECK(hipEventRecord(f->ev_fork, s));
     CLAY_HIP_CHECK(hipStreamWaitEvent(f->side, f->ev_fork, 0));
     if ((rc = splat_bake_prepare(f->surface, f->side, f->vals, bounds != nullptr))) return rc;
+    if (early && (rc = fluid_reorder(f, s))) return rc;
     if ((rc = fluid_density(f, s, f->cover))) return rc;
     CLAY_HIP_CHECK(hipEventRecord(f->ev_join, f->side));
     CLAY_HIP_CHECK(hipStreamWaitEvent(s, f->ev_join, 0));
@@ -1524,10 +1541,13 @@ int apply_tuning(Fluid* f, const clay_tuning* t) {
   if (t->groups_per_cu < 1 || t->groups_per_cu > 32)
     return fail(CLAY_ERR_INVALID, "clay_tuning: groups_per_cu must be in [1, 32]");
   if ((t->fluid_xcd != 0 && t->fluid_xcd != 1) || (t->splat_xcd != 0 && t->splat_xcd != 1) ||
-      (t->splat_owner != 0 && t->splat_owner != 1) || (t->exact_lds != 0 && t->exact_lds != 1))
-    return fail(CLAY_ERR_INVALID, "clay_tuning: fluid_xcd, splat_xcd, splat_owner and exact_lds must be 0 or 1");
+      (t->splat_owner != 0 && t->splat_owner != 1) || (t->exact_lds != 0 && t->exact_lds != 1) ||
+      (t->early_bin != 0 && t->early_bin != 1))
+    return fail(CLAY_ERR_INVALID,
+                "clay_tuning: fluid_xcd, splat_xcd, splat_owner, exact_lds and early_bin must be 0 or 1");
   ft.xcd = t->fluid_xcd != 0;
   ft.exact_lds = t->exact_lds != 0;
+  ft.early_bin = t->early_bin != 0;
   f->tune = ft;
   f->surface->xcd = t->splat_xcd != 0;
   f->surface->owner = t->splat_owner != 0;
@@ -1549,6 +1569,7 @@ void read_tuning(const Fluid* f, clay_tuning* out) {
   out->splat_xcd = f->surface->xcd ? 1 : 0;
   out->splat_owner = f->surface->owner ? 1 : 0;
   out->exact_lds = f->tune.exact_lds ? 1 : 0;
+  out->early_bin = f->tune.early_bin ? 1 : 0;
 }
 
 }  // namespace

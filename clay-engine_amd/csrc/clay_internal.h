@@ -169,6 +169,7 @@ struct FluidTune {
   int stage = 128;         // LDS staging slice per wave: 64, 128 or 256 neighbours
   bool xcd = false;        // XCD-contiguous workgroup numbering in the neighbour kernels
   bool exact_lds = true;   // f32 precision: LDS-staged exact kernels (else global-memory walks)
+  bool early_bin = false;  // the bake's binning forks before the frame-end reorder
 };
 bool fluid_tune_valid(const FluidTune& t);
 

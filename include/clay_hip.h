@@ -564,6 +564,8 @@ typedef struct {
                              stages itself plus a 5-voxel shell in LDS (when reach <= 5 voxels) */
   int32_t exact_lds;      /* F32 precision: 1 stages neighbour rows in LDS as f32 (results are the
                              global-memory kernels' bit for bit), 0 walks them in global memory */
+  int32_t early_bin;      /* 1: the bake's binning forks onto its second stream before the frame-end
+                             reorder (beside it and the density pass), 0: after it */
 } clay_tuning;
 int clay_fluid_set_tuning(clay_fluid* f, const clay_tuning* t);
 /* The splat fields of clay_tuning (chunk, groups_per_cu, splat_xcd, splat_owner) for a

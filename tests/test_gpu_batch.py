@@ -146,7 +146,7 @@ def test_tuning_does_not_change_results(root, floor):
     a = make_batch(root, floor, opts, counts, pos)
     t = make_batch(root, floor, opts, counts, pos)
     want = {"block": 128, "stage": 64, "chunk": 512, "groups_per_cu": 3, "fluid_xcd": 1, "splat_xcd": 1,
-            "splat_owner": 1, "exact_lds": 0}
+            "splat_owner": 1, "exact_lds": 0, "early_bin": 1}
     t.tuning = want
     assert t.tuning == want
     for _ in range(2):
